@@ -1,0 +1,251 @@
+/*
+ * lensdiff_hip.h — C ABI of the MI355X (gfx950) LensDiff diffraction backend.
+ *
+ * Plain C, POD-only: no C++ types, exceptions or std::string cross this boundary.
+ * Every entry point returns an ldh_status (0 = LDH_OK). On failure the context keeps a
+ * short kebab-case error tag, readable with ldh_last_error(), mirroring the reference's
+ * `std::string* error` convention (source/src/core/LensDiffCpuReference.cpp:2077-2084).
+ *
+ * The plugin-facing C++ seam, RunLensDiffHip(request, params, cache, error), is a thin layer
+ * above these calls (lensdiff_amd/csrc/LensDiffHip.h); it mirrors
+ *   bool RunLensDiffCpuReference(const LensDiffRenderRequest&, const LensDiffParams&,
+ *                                LensDiffPsfBankCache&, std::string*)
+ *   (source/src/core/LensDiffCpuReference.h:64-67) and
+ *   bool RunLensDiffCuda(...)  (source/src/cuda/LensDiffCuda.h:7-10).
+ *
+ * Enum values and field meanings follow source/src/core/LensDiffTypes.h:14-208.
+ */
+#ifndef LENSDIFF_HIP_H
+#define LENSDIFF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LDH_ABI_VERSION 1
+/* Largest wavelength list a bank may carry (the reference caps fixed modes at 9 bins,
+ * LensDiffTypes.h:12; lists longer than 9 are an extension of this backend). */
+#define LDH_MAX_WAVELENGTHS 64
+
+/* ---- status codes ------------------------------------------------------------------- */
+typedef int32_t ldh_status;
+enum {
+    LDH_OK = 0,
+    LDH_ERR_INVALID_ARGUMENT = 1,
+    LDH_ERR_MISSING_BUFFER = 2,   /* "missing-source-or-destination-buffer" */
+    LDH_ERR_INVALID_BOUNDS = 3,   /* "invalid-source-bounds" */
+    LDH_ERR_UNSUPPORTED = 4,      /* a parameter this backend does not implement */
+    LDH_ERR_OUT_OF_MEMORY = 5,    /* device allocation failed or budget exceeded */
+    LDH_ERR_DEVICE = 6,           /* a HIP runtime call failed */
+    LDH_ERR_NOT_READY = 7,        /* ldh_poll: request still in flight */
+    LDH_ERR_TOO_LARGE = 8         /* a size beyond this backend's documented limits */
+};
+
+/* ---- enums (values identical to LensDiffTypes.h:14-91) ------------------------------ */
+enum { LDH_EXTRACT_MAX_RGB = 0, LDH_EXTRACT_LUMA = 1 };
+enum {
+    LDH_APERTURE_CIRCLE = 0, LDH_APERTURE_POLYGON = 1, LDH_APERTURE_STAR = 2,
+    LDH_APERTURE_SPIRAL = 3, LDH_APERTURE_CUSTOM = 4, LDH_APERTURE_HEXAGON = 5,
+    LDH_APERTURE_SQUARE_GRID = 6, LDH_APERTURE_SNOWFLAKE = 7
+};
+enum { LDH_APOD_FLAT = 0, LDH_APOD_COSINE = 1, LDH_APOD_GAUSSIAN = 2 };
+enum { LDH_SPECTRAL_MONO = 0, LDH_SPECTRAL_TRISTIMULUS = 1, LDH_SPECTRAL_5 = 2, LDH_SPECTRAL_9 = 3 };
+enum { LDH_STYLE_NATURAL = 0, LDH_STYLE_CYAN_MAGENTA = 1, LDH_STYLE_WARM_COOL = 2 };
+enum { LDH_LOOK_PHYSICAL = 0, LDH_LOOK_SPLIT = 1 };
+enum { LDH_ENERGY_PRESERVE = 0, LDH_ENERGY_AUGMENT = 1 };
+enum {
+    LDH_VIEW_FINAL = 0, LDH_VIEW_SELECTION = 1, LDH_VIEW_PUPIL = 2, LDH_VIEW_PSF = 3,
+    LDH_VIEW_OTF = 4, LDH_VIEW_CORE = 5, LDH_VIEW_STRUCTURE = 6, LDH_VIEW_EFFECT = 7,
+    LDH_VIEW_PHASE = 8, LDH_VIEW_PHASE_EDGE = 9, LDH_VIEW_FIELD_PSF = 10,
+    LDH_VIEW_CHROMATIC_SPLIT = 11, LDH_VIEW_CREATIVE_FRINGE = 12, LDH_VIEW_SCATTER = 13
+};
+enum { LDH_TRANSFER_LINEAR = 0, LDH_TRANSFER_DAVINCI_INTERMEDIATE = 1 };
+enum { LDH_PHASE_OFF = 0, LDH_PHASE_ENABLED = 1 };
+/* Which kernel of a bank bin (LensDiffPsfBin, LensDiffTypes.h:373-378). */
+enum { LDH_KERNEL_FULL = 0, LDH_KERNEL_CORE = 1, LDH_KERNEL_STRUCTURE = 2 };
+/* How the plain render convolves the spectral bins. */
+enum {
+    LDH_CONV_PER_BIN = 0,      /* one convolution per bin, then the per-pixel spectral map */
+    LDH_CONV_PRECOMBINED = 1   /* bins pre-combined into natural/style RGB kernels */
+};
+
+/* ---- image views (LensDiffImageRect / LensDiffImageView, LensDiffTypes.h:93-111) ---- */
+typedef struct ldh_rect {
+    int32_t x1, y1, x2, y2;
+} ldh_rect;
+
+typedef struct ldh_image_view {
+    void* data;                 /* host pointer (OFX has no HIP render suite) or device */
+    int64_t row_bytes;          /* may be negative */
+    ldh_rect bounds;
+    int32_t origin_x, origin_y;
+    int32_t components;         /* 4 (RGBA); 3 is accepted for reading */
+    int32_t bytes_per_component;/* 4 (float32) */
+} ldh_image_view;
+
+/* LensDiffRenderRequest (LensDiffTypes.h:113-126), host-pointer form. */
+typedef struct ldh_render_request {
+    ldh_rect render_window;
+    ldh_rect frame_bounds;
+    ldh_image_view src;
+    ldh_image_view dst;
+    int32_t frame_short_side_px;
+} ldh_render_request;
+
+/* LensDiffParams (LensDiffTypes.h:128-208) plus this backend's extensions (last block). */
+typedef struct ldh_params {
+    int32_t extraction_mode, aperture_mode, apodization_mode, spectral_mode;
+    int32_t spectrum_style, look_mode, energy_mode, debug_view;
+    int32_t input_transfer, resolution_aware;
+
+    double threshold, softness_stops, point_emphasis, core_preserve;
+
+    int32_t blade_count, vane_count, pupil_resolution, frame_short_side_px;
+    double roundness, rotation_deg, central_obstruction, vane_thickness;
+    double diffraction_scale_px;   /* percent of the optics short side */
+    double max_kernel_radius_px;   /* percent of the optics short side */
+
+    int32_t phase_mode, chromatic_affects_luma;
+    double phase_defocus, phase_astigmatism0, phase_astigmatism45, phase_coma_x, phase_coma_y;
+    double phase_spherical, phase_trefoil_x, phase_trefoil_y;
+    double phase_secondary_astigmatism0, phase_secondary_astigmatism45;
+    double phase_quadrafoil0, phase_quadrafoil45, phase_secondary_coma_x, phase_secondary_coma_y;
+    double pupil_decenter_x, pupil_decenter_y;
+    double phase_field_strength, phase_field_edge_bias, phase_field_defocus;
+    double phase_field_astig_radial, phase_field_astig_tangential;
+    double phase_field_coma_radial, phase_field_coma_tangential, phase_field_spherical;
+    double phase_field_trefoil_radial, phase_field_trefoil_tangential;
+    double phase_field_secondary_astig_radial, phase_field_secondary_astig_tangential;
+    double phase_field_quadrafoil_radial, phase_field_quadrafoil_tangential;
+    double phase_field_secondary_coma_radial, phase_field_secondary_coma_tangential;
+
+    double spectrum_force, spectrum_saturation;
+    double chromatic_focus, chromatic_spread, creative_fringe, scatter_amount, scatter_radius;
+    double effect_gain, core_compensation, anisotropy_emphasis;
+    double core_gain, structure_gain, core_shoulder, structure_shoulder;
+
+    /* ---- extensions (no counterpart in LensDiffParams) ---- */
+    /* Explicit wavelength list; 0 = the spectral mode's fixed list
+     * (LensDiffCpuReference.cpp:1602-1614). */
+    int32_t wavelength_count;
+    int32_t conv_mode;             /* LDH_CONV_* for spectral banks */
+    float wavelengths_nm[LDH_MAX_WAVELENGTHS];
+    /* Zernike terms the reference basis lacks (LensDiffPhase.cpp:182-197), in waves:
+     * Noll 2 (tilt x), 3 (tilt y), 18, 19, 20, 21. */
+    double zernike_extra[6];
+    /* Side of the square FFT tile used by overlap-save convolution; 0 = the reference's
+     * nextPow2(W+k-1) x nextPow2(H+k-1) padding (LensDiffCpuReference.cpp:1168-1169). */
+    int32_t fft_tile;
+    /* 1 = store cached kernel spectra as FP16 (FP32 accumulation). */
+    int32_t spectra_fp16;
+} ldh_params;
+
+/* Fill *p with LensDiffParams' member defaults (LensDiffTypes.h:128-208). */
+void ldh_params_default(ldh_params* p);
+
+/* ---- context ------------------------------------------------------------------------ */
+typedef struct ldh_context ldh_context;
+typedef struct ldh_bank ldh_bank;
+
+/* Create a context on HIP device `device`. `memory_budget_bytes` bounds the resident bank
+ * cache (0 = 64 GiB). Contexts are internally synchronised. */
+ldh_status ldh_context_create(int32_t device, uint64_t memory_budget_bytes, ldh_context** out);
+ldh_status ldh_context_destroy(ldh_context* ctx);
+const char* ldh_status_string(ldh_status status);
+/* Last kebab-case error tag recorded on this context ("" if none). */
+const char* ldh_last_error(const ldh_context* ctx);
+ldh_status ldh_synchronize(ldh_context* ctx);
+
+/* ---- sizes (LensDiffCpuReference.cpp:104-115, LensDiffTypes.h:292-320) ---------------- */
+typedef struct ldh_sizes {
+    int32_t pupil_size;            /* P = clamp(pupilResolution, 64, 1024) */
+    int32_t raw_psf_size;          /* R = clamp(max(2P, 512, 4r), 128, 16384) */
+    int32_t max_kernel_radius_px;  /* r = max(4, ceil(percent * 0.01 * S)) */
+    int32_t optics_short_side_px;  /* S */
+    double diffraction_scale_px;
+} ldh_sizes;
+ldh_status ldh_resolve_sizes(const ldh_params* p, ldh_sizes* out);
+
+/* ---- PSF synthesis stages (host buffers in, host buffers out; for parity and debug) ---- */
+/* BuildLensDiffPupilAmplitudeForPsf (LensDiffCpuReference.cpp:758-822): P*P floats. */
+ldh_status ldh_pupil_amplitude(ldh_context* ctx, const ldh_params* p, float* out);
+/* BuildLensDiffPupilPhaseForPsf (LensDiffPhase.cpp:215-243): P*P waves. */
+ldh_status ldh_pupil_phase(ldh_context* ctx, const ldh_params* p, float* out);
+/* BuildLensDiffShiftedRawPsf (LensDiffCpuReference.cpp:831-859): R*R floats, unit sum.
+ * `phase_waves` may be NULL (flat phase). */
+ldh_status ldh_raw_psf(ldh_context* ctx, const float* amplitude, const float* phase_waves,
+                       int32_t pupil_size, int32_t raw_size, float* out);
+/* EstimateLensDiffFirstMinimumRadius (LensDiffCpuReference.cpp:897-955) of a raw PSF. */
+ldh_status ldh_first_minimum_radius(ldh_context* ctx, const float* shifted_raw_psf,
+                                    int32_t raw_size, float* out_radius);
+/* makeResampledKernel (LensDiffCpuReference.cpp:1060-1086): K=2r+1, K*K floats. */
+ldh_status ldh_resample_kernel(ldh_context* ctx, const float* shifted_raw_psf, int32_t raw_size,
+                               float scale_factor, int32_t support_radius, float* out);
+/* populatePsfBankCacheFromBaseKernels for one bin (LensDiffCpuReference.cpp:1616-1656):
+ * base K*K kernel -> full/core/structure of side (2*radius+1). Each output needs K*K floats
+ * of capacity; *out_size receives the cropped side. */
+ldh_status ldh_shape_kernels(ldh_context* ctx, const ldh_params* p, const float* base_kernel,
+                             int32_t kernel_size, float* full, float* core, float* structure,
+                             int32_t* out_size);
+/* sumKernel (LensDiffCpuReference.cpp:436-438): the value of a sequential FP32 accumulation of
+ * x[0..n) (std::accumulate with a float accumulator), computed on the device. */
+ldh_status ldh_sequential_sum(ldh_context* ctx, const float* x, int64_t n, float* out);
+/* In-place 2D complex FFT of interleaved complex64 (host buffer), rows then columns,
+ * forward sign -1, inverse scaled by 1/(w*h) (LensDiffCpuReference.cpp:194-301). */
+ldh_status ldh_fft2d(ldh_context* ctx, float* data, int32_t width, int32_t height, int32_t inverse);
+
+/* ---- PSF banks (EnsureLensDiffPsfBank, LensDiffCpuReference.cpp:2068-2071) ------------- */
+typedef struct ldh_bank_info {
+    int32_t bin_count;
+    int32_t support_radius_px;     /* LensDiffPsfBankCache::supportRadiusPx */
+    int32_t pupil_size;
+    int32_t raw_psf_size;
+    float wavelengths_nm[LDH_MAX_WAVELENGTHS];
+    int32_t kernel_size[LDH_MAX_WAVELENGTHS];   /* cropped side per bin */
+} ldh_bank_info;
+
+/* Build or fetch the bank for `p` from the context's cache (refcounted). */
+ldh_status ldh_bank_acquire(ldh_context* ctx, const ldh_params* p, ldh_bank** out);
+ldh_status ldh_bank_release(ldh_context* ctx, ldh_bank* bank);
+ldh_status ldh_bank_info_get(ldh_context* ctx, const ldh_bank* bank, ldh_bank_info* out);
+/* Copy one kernel of one bin to host (capacity in floats). */
+ldh_status ldh_bank_download(ldh_context* ctx, const ldh_bank* bank, int32_t bin, int32_t which,
+                             float* out, int64_t capacity);
+
+/* ---- render --------------------------------------------------------------------------- */
+/* The plain per-frame path (RunLensDiffCpuReference semantics, LensDiffCpuReference.cpp:
+ * 2073-2261): host views in and out, synchronous, single-entry bank cache per context. */
+ldh_status ldh_render(ldh_context* ctx, const ldh_render_request* req, const ldh_params* p);
+
+/* Same path with a device-resident, tightly packed RGBA float32 frame (w*h*4 floats) in
+ * and out; runs on the context stream without host staging. */
+ldh_status ldh_render_device(ldh_context* ctx, const ldh_params* p, const float* d_src,
+                             float* d_dst, int32_t width, int32_t height);
+
+/* Device memory helpers for callers without a GPU framework. */
+ldh_status ldh_device_alloc(ldh_context* ctx, uint64_t bytes, void** out);
+ldh_status ldh_device_free(ldh_context* ctx, void* ptr);
+ldh_status ldh_memcpy_h2d(ldh_context* ctx, void* dst, const void* src, uint64_t bytes);
+ldh_status ldh_memcpy_d2h(ldh_context* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---- measurement ---------------------------------------------------------------------- */
+/* Per-kernel-family timing recorded with HIP events on the launch stream. */
+typedef struct ldh_kernel_stat {
+    char name[48];
+    int64_t launches;
+    double total_ms;
+    double algorithmic_bytes;      /* summed byte-model figure of those launches */
+} ldh_kernel_stat;
+ldh_status ldh_profiling_enable(ldh_context* ctx, int32_t enable);
+/* Returns the number of families written (<= capacity). */
+int32_t ldh_profiling_read(ldh_context* ctx, ldh_kernel_stat* out, int32_t capacity);
+ldh_status ldh_profiling_reset(ldh_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LENSDIFF_HIP_H */

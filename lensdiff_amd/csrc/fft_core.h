@@ -1,0 +1,264 @@
+// Device-side FFT building blocks for gfx950: register DFTs of radix 2/4/8/16, Stockham stages
+// over a tile held in LDS, and a tile kernel whose global loads and stores are supplied by an
+// IO policy (so pointwise work is fused into the first and last pass).
+//
+// A tile holds NL lines of N points. Element (line, i) lives at LDS index line*LS + i*IS,
+// padded by one float2 every 32 entries to keep the first stage's strided writes spread
+// across LDS banks. Row tiles use (LS, IS) = (N, 1); column tiles (lines adjacent in global
+// memory) use (1, NL).
+#pragma once
+
+#include "ldh_internal.h"
+
+namespace ldh {
+
+__device__ __forceinline__ int lds_pad(int x) { return x + (x >> 5); }
+
+// multiply by -i (forward) or +i (inverse)
+template <bool INV> __device__ __forceinline__ float2 rot90(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <bool INV> __device__ __forceinline__ void dft2(float2& a, float2& b) {
+    const float2 t = a;
+    a = make_float2(t.x + b.x, t.y + b.y);
+    b = make_float2(t.x - b.x, t.y - b.y);
+}
+
+template <bool INV> __device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+    const float2 t0 = make_float2(a0.x + a2.x, a0.y + a2.y);
+    const float2 t1 = make_float2(a0.x - a2.x, a0.y - a2.y);
+    const float2 t2 = make_float2(a1.x + a3.x, a1.y + a3.y);
+    const float2 t3 = rot90<INV>(make_float2(a1.x - a3.x, a1.y - a3.y));
+    a0 = make_float2(t0.x + t2.x, t0.y + t2.y);
+    a2 = make_float2(t0.x - t2.x, t0.y - t2.y);
+    a1 = make_float2(t1.x + t3.x, t1.y + t3.y);
+    a3 = make_float2(t1.x - t3.x, t1.y - t3.y);
+}
+
+// exp(-+2 pi i j / 16) for j = 0..15 (cos, sin of 2 pi j / 16)
+__device__ __forceinline__ float2 w16(int j, bool inv) {
+    constexpr float c[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
+                             0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f,
+                             -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508977f,
+                             0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f};
+    const float cs = c[j & 15];
+    const float sn = c[(j + 12) & 15];  // sin(2 pi j/16) = cos(2 pi (j-4)/16)
+    return make_float2(cs, inv ? sn : -sn);
+}
+
+template <bool INV> __device__ __forceinline__ void dft8(float2* v) {
+    // even/odd split: E = DFT4(v0,v2,v4,v6), O = DFT4(v1,v3,v5,v7)
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(o0, o1, o2, o3);
+    const float h = 0.70710678118654752f;
+    // W8^1, W8^2, W8^3 applied to o1, o2, o3
+    const float2 t1 = INV ? make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y))
+                          : make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
+    const float2 t2 = rot90<INV>(o2);
+    const float2 t3 = INV ? make_float2(-h * (o3.x + o3.y), h * (o3.x - o3.y))
+                          : make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+    v[0] = make_float2(e0.x + o0.x, e0.y + o0.y);
+    v[4] = make_float2(e0.x - o0.x, e0.y - o0.y);
+    v[1] = make_float2(e1.x + t1.x, e1.y + t1.y);
+    v[5] = make_float2(e1.x - t1.x, e1.y - t1.y);
+    v[2] = make_float2(e2.x + t2.x, e2.y + t2.y);
+    v[6] = make_float2(e2.x - t2.x, e2.y - t2.y);
+    v[3] = make_float2(e3.x + t3.x, e3.y + t3.y);
+    v[7] = make_float2(e3.x - t3.x, e3.y - t3.y);
+}
+
+template <bool INV> __device__ __forceinline__ void dft16(float2* v) {
+    // X[k1 + 4 k2] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 v[4 n1 + n2] W4^(n1 k1)
+    float2 y[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        float2 a0 = v[n2], a1 = v[4 + n2], a2 = v[8 + n2], a3 = v[12 + n2];
+        dft4<INV>(a0, a1, a2, a3);
+        y[n2 * 4 + 0] = a0;
+        y[n2 * 4 + 1] = n2 ? cmul(a1, w16(n2 * 1, INV)) : a1;
+        y[n2 * 4 + 2] = n2 ? cmul(a2, w16(n2 * 2, INV)) : a2;
+        y[n2 * 4 + 3] = n2 ? cmul(a3, w16(n2 * 3, INV)) : a3;
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        float2 b0 = y[0 * 4 + k1], b1 = y[1 * 4 + k1], b2 = y[2 * 4 + k1], b3 = y[3 * 4 + k1];
+        dft4<INV>(b0, b1, b2, b3);
+        v[k1 + 0] = b0;
+        v[k1 + 4] = b1;
+        v[k1 + 8] = b2;
+        v[k1 + 12] = b3;
+    }
+}
+
+template <int R, bool INV> __device__ __forceinline__ void dft(float2* v) {
+    if constexpr (R == 2) dft2<INV>(v[0], v[1]);
+    else if constexpr (R == 4) dft4<INV>(v[0], v[1], v[2], v[3]);
+    else if constexpr (R == 8) dft8<INV>(v);
+    else dft16<INV>(v);
+}
+
+constexpr int clog2(int n) { return n <= 1 ? 0 : 1 + clog2(n / 2); }
+
+// Radix of the stage that starts at sub-length Ns: 16 while at least 16 remains, then the rest.
+template <int N, int Ns> struct StageRadix {
+    static constexpr int rem = N / Ns;
+    static constexpr int value = rem >= 16 ? 16 : rem;
+};
+
+// One Stockham stage over all NL lines of the tile, executed by NT threads.
+template <int N, int Ns, int NL, int NT, bool INV, int LS, int IS>
+__device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict__ tw, int tid) {
+    constexpr int R = StageRadix<N, Ns>::value;
+    constexpr int BF = N / R;
+    constexpr int TOTAL = NL * BF;
+    constexpr int PER = (TOTAL + NT - 1) / NT;
+    float2 v[PER][R];
+    int line_of[PER], j_of[PER];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int u = tid + p * NT;
+        int line, j;
+        if constexpr (IS == 1) { line = u / BF; j = u % BF; }
+        else { line = u % NL; j = u / NL; }
+        line_of[p] = line;
+        j_of[p] = j;
+        if (TOTAL % NT == 0 || u < TOTAL) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[p][r] = lds[lds_pad(line * LS + (j + r * BF) * IS)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int u = tid + p * NT;
+        if (TOTAL % NT == 0 || u < TOTAL) {
+            const int j = j_of[p];
+            const int k = j & (Ns - 1);
+            if constexpr (Ns > 1) {
+                constexpr int shift = kMaxFftLog2 - clog2(Ns * R);
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    float2 w = tw[(k * r) << shift];
+                    if (INV) w.y = -w.y;
+                    v[p][r] = cmul(v[p][r], w);
+                }
+            }
+            dft<R, INV>(v[p]);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[lds_pad(line_of[p] * LS + (base + r * Ns) * IS)] = v[p][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int Ns, int NL, int NT, bool INV, int LS, int IS>
+__device__ __forceinline__ void fft_stages(float2* lds, const float2* __restrict__ tw, int tid) {
+    if constexpr (Ns < N) {
+        fft_stage<N, Ns, NL, NT, INV, LS, IS>(lds, tw, tid);
+        fft_stages<N, Ns * StageRadix<N, Ns>::value, NL, NT, INV, LS, IS>(lds, tw, tid);
+    }
+}
+
+// Tile geometry for a line length and layout.
+template <int N, bool ROWS> struct TileShape {
+    // rows: at least 1024 elements per tile; columns: 16 adjacent columns (128-byte segments)
+    // times enough line groups for 1024 elements.
+    static constexpr int NL = ROWS ? (N >= 1024 ? 1 : 1024 / N) : (16 * (N >= 64 ? 1 : 64 / N));
+    static constexpr int ELEMS = NL * N;
+    static constexpr int NT0 = ELEMS / 32;
+    static constexpr int NT = NT0 < 64 ? 64 : (NT0 > 512 ? 512 : NT0);
+    static constexpr int LDS_BYTES = (ELEMS + ELEMS / 32 + 1) * 8;
+};
+
+// Maps the hardware workgroup id to a logical tile so that consecutive logical tiles run on
+// the same XCD (workgroups are dealt round-robin over the 8 XCDs).
+__device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
+    if ((nblocks & 7) != 0) return bid;
+    return (bid & 7) * (nblocks >> 3) + (bid >> 3);
+}
+
+// The tile kernel. IO supplies:
+//   __device__ float2 load(int tile, int line, int i) const;
+//   __device__ void store(int tile, int line, int i, float2 v) const;
+template <int N, bool ROWS, bool INV, class IO>
+__global__ void __launch_bounds__((TileShape<N, ROWS>::NT))
+fft_tile_kernel(IO io, const float2* __restrict__ tw, int ntiles) {
+    using S = TileShape<N, ROWS>;
+    constexpr int NL = S::NL, NT = S::NT, E = S::ELEMS;
+    constexpr int LS = ROWS ? N : 1;
+    constexpr int IS = ROWS ? 1 : NL;
+    extern __shared__ float2 lds[];
+    const int tile = xcd_tile(blockIdx.x, ntiles);
+    const int tid = threadIdx.x;
+    for (int e = tid; e < E; e += NT) {
+        int line, i;
+        if constexpr (ROWS) { line = e / N; i = e % N; }
+        else { line = e % NL; i = e / NL; }
+        lds[lds_pad(line * LS + i * IS)] = io.load(tile, line, i);
+    }
+    __syncthreads();
+    fft_stages<N, 1, NL, NT, INV, LS, IS>(lds, tw, tid);
+    for (int e = tid; e < E; e += NT) {
+        int line, i;
+        if constexpr (ROWS) { line = e / N; i = e % N; }
+        else { line = e % NL; i = e / NL; }
+        io.store(tile, line, i, lds[lds_pad(line * LS + i * IS)]);
+    }
+}
+
+// Kernels whose dynamic LDS exceeds 64 KiB must opt in before launch (gfx950 has 160 KiB/CU).
+inline void allow_large_lds(const void* fn, int bytes) {
+    if (bytes <= 65536) return;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
+        fail(LDH_ERR_DEVICE, "lds-attribute-failed");
+}
+
+// Launch helper: dispatches the compile-time line length, instantiating only lengths in
+// [2^LMIN, 2^LMAX] (the lengths a given pass can receive).
+template <bool ROWS, bool INV, int LMIN, int LMAX, class IO>
+void launch_tiles(hipStream_t s, int n, const IO& io, const float2* tw, int ntiles) {
+    bool launched = false;
+#define LDH_TILE_CASE(LG)                                                                        \
+    if constexpr (LMIN <= LG && LG <= LMAX) {                                                    \
+        if (n == (1 << LG)) {                                                                    \
+            using S = TileShape<(1 << LG), ROWS>;                                                \
+            allow_large_lds(reinterpret_cast<const void*>(&fft_tile_kernel<(1 << LG), ROWS, INV, IO>), \
+                            S::LDS_BYTES);                                                       \
+            hipLaunchKernelGGL((fft_tile_kernel<(1 << LG), ROWS, INV, IO>), dim3(ntiles),        \
+                               dim3(S::NT), S::LDS_BYTES, s, io, tw, ntiles);                    \
+            launched = true;                                                                     \
+        }                                                                                        \
+    }
+    LDH_TILE_CASE(1) LDH_TILE_CASE(2) LDH_TILE_CASE(3) LDH_TILE_CASE(4) LDH_TILE_CASE(5)
+    LDH_TILE_CASE(6) LDH_TILE_CASE(7) LDH_TILE_CASE(8) LDH_TILE_CASE(9) LDH_TILE_CASE(10)
+    LDH_TILE_CASE(11) LDH_TILE_CASE(12) LDH_TILE_CASE(13) LDH_TILE_CASE(14)
+#undef LDH_TILE_CASE
+    if (!launched) fail(LDH_ERR_TOO_LARGE, "fft-size-unsupported");
+    LDH_CHECK_LAUNCH("fft-tile-launch");
+}
+
+template <int N, bool ROWS> constexpr int tile_lines() { return TileShape<N, ROWS>::NL; }
+inline int row_tile_lines(int n) { return n >= 1024 ? 1 : 1024 / n; }
+inline int col_tile_lines(int n) { return 16 * (n >= 64 ? 1 : 64 / n); }
+
+// Split of a column length N (> 64) into N1 x N2 for the two-pass column transform.
+__host__ __device__ inline void col_split(int n, int* n1, int* n2) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    *n1 = 1 << (l / 2);
+    *n2 = n / *n1;
+}
+
+// Storage row of spectrum row k after the forward column transform of length `height`.
+__host__ __device__ inline int col_storage_row(int height, int k) {
+    if (height <= 64) return k;
+    int n1, n2;
+    col_split(height, &n1, &n2);
+    return n2 * (k % n1) + k / n1;
+}
+
+}  // namespace ldh

@@ -1,0 +1,69 @@
+// Per-frame stages: selection, FFT convolution of the highlights with the bank, spectral map
+// and composite (RunLensDiffCpuReference, LensDiffCpuReference.cpp:2073-2261).
+#pragma once
+
+#include <vector>
+
+#include "bank.h"
+#include "ldh_internal.h"
+#include "seqsum.h"
+
+namespace ldh {
+
+// Spectral bin -> RGB matrices ([channel][bin]), natural and style (BuildLensDiffSpectrumConfig).
+struct SpectrumConfig {
+    int bins = 1;
+    std::vector<float> natural, style;
+};
+SpectrumConfig build_spectrum_config(const ldh_params& p, const std::vector<float>& wavelengths);
+
+// FFT grid of a frame convolved with kernels of support radius `support`: the reference's
+// nextPow2(W + n - 1) x nextPow2(H + n - 1), n = 2*support+1 (LensDiffCpuReference.cpp:1168-1169).
+struct Grid {
+    int w = 0, h = 0;     // frame
+    int pw = 0, ph = 0;   // FFT grid
+    size_t cells() const { return size_t(pw) * ph; }
+};
+Grid make_grid(int w, int h, int support);
+
+// Kernel spectra of a set of real K x K kernels, packed two per complex spectrum
+// (k_a + i k_b), in the split-column layout of the forward transform.
+struct KernelSpectra {
+    Grid grid;
+    int kernels = 0;      // real kernels
+    int pairs = 0;        // complex spectra
+    bool fp16 = false;
+    DeviceBuffer data;    // float2 or __half2 [pairs][ph][pw]
+    size_t bytes() const { return data.bytes(); }
+};
+// kernels: device pointers (each K x K), packed in order (0,1), (2,3), ...; `radius` bounds
+// the non-zero support of every kernel.
+void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<const float*>& kernels, int K,
+                          int radius, const Grid& g, bool fp16, KernelSpectra& out, DeviceBuffer& scratch);
+// Combined RGB kernels of a spectral bank: out[c] = sum_b M[c][b] k_b for the natural (c=0..2)
+// and, if with_style, style (c=3..5) matrices. out: device float[6][K*K].
+void combine_kernels(hipStream_t s, const float* kernels, int bins, int K, const SpectrumConfig& cfg,
+                     bool with_style, float* out, DeviceBuffer& scratch);
+
+// Scratch and outputs of one frame on one stream.
+struct FrameWork {
+    DeviceBuffer src, dst, mask, image_spec, out_spec, planes, effect, lum, totals, factors, matrices, kspec_tmp,
+        combined, max_buf;
+    KernelSpectra kspec;  // per-frame kernel spectra (plain path)
+    SeqSum seqsum;
+};
+
+// Effect-producing convolution modes.
+enum class EffectMode { PerBin, Precombined };
+
+// Renders one frame with the bank. `src`/`dst`: device RGBA float32, tightly packed, w x h.
+// `cached`: kernel spectra matching the bank and grid (full kernels, bins or combined), or null
+// to compute them in `fw.kspec` for this frame.
+void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
+                  const float4* src, float4* dst, int w, int h, EffectMode mode, const KernelSpectra* cached);
+
+// Which real kernels (device pointers) feed the effect, for building cached spectra.
+std::vector<const float*> effect_kernels(const ldh_params& p, const DeviceBank& bank, EffectMode mode,
+                                         const float* combined, int* which_out);
+
+}  // namespace ldh

@@ -1,0 +1,566 @@
+// Per-frame path on the device:
+//   selection mask (buildSelectionMask, LensDiffCpuReference.cpp:1219-1249)
+//   -> image spectra: the row pass reads the frame and forms the redistributed channels or the
+//      scalar driver on the fly (applyMask / applyMaskScalarDriver, :1251-1279), zero-padded to
+//      the reference's nextPow2 grid (:1168-1169)
+//   -> kernel spectra: the row pass wraps each kernel's centre to the origin
+//      (paddedSpectrumFromKernel, :1153-1165); two real kernels share one complex transform
+//   -> per (image spectrum, kernel spectrum): inverse transform whose first pass multiplies the
+//      spectra and whose last pass crops to the frame, clamps at zero and writes the planes
+//      (convolveScalar, :1167-1193)
+//   -> spectral map (MapLensDiffSpectralBins, LensDiffSpectrum.cpp:462-497), split-look
+//      shoulder/gains (:1340-1368), Preserve rescale (:2172-2178) and the composite (:2200-2220).
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "fft_io.h"
+#include "frame.h"
+#include "host_params.h"
+
+namespace ldh {
+
+namespace {
+
+__device__ __forceinline__ float decode_channel(float x, int tf) {
+    if (tf == LDH_TRANSFER_DAVINCI_INTERMEDIATE)
+        return x <= 0.02740668f ? x / 10.44426855f : exp2f(x / 0.07329248f - 7.0f) - 0.0075f;
+    return x;
+}
+__device__ __forceinline__ float encode_channel(float x, int tf) {
+    if (tf == LDH_TRANSFER_DAVINCI_INTERMEDIATE)
+        return x <= 0.00262409f ? (x * 10.44426855f) : ((log2f(fmaxf(x, 0.0f) + 0.0075f) + 7.0f) * 0.07329248f);
+    return x;
+}
+__device__ __forceinline__ float4 decoded(const float4* src, size_t i, int tf) {
+    const float4 q = src[i];
+    return make_float4(decode_channel(q.x, tf), decode_channel(q.y, tf), decode_channel(q.z, tf), q.w);
+}
+__device__ __forceinline__ float luma709(float r, float g, float b) { return 0.2126f * r + 0.7152f * g + 0.0722f * b; }
+__device__ __forceinline__ float max3(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
+
+struct SelectParams {
+    int tf, extraction;
+    float threshold_stops, softness, threshold_linear, point_emphasis;
+};
+
+__global__ void selection_kernel(const float4* __restrict__ src, int64_t npx, SelectParams sp, float* __restrict__ mask) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    const float4 q = decoded(src, size_t(i), sp.tf);
+    const float mx = max3(q.x, q.y, q.z);
+    const float sig = sp.extraction == LDH_EXTRACT_LUMA ? luma709(q.x, q.y, q.z) : mx;
+    float gate = 0.0f;
+    if (sig > 0.0f) {
+        const float stops = log2f(fmaxf(sig, 1e-6f) / 0.18f);
+        const float e0 = sp.threshold_stops - sp.softness * 0.5f;
+        const float e1 = sp.threshold_stops + sp.softness * 0.5f;
+        const float t = fminf(fmaxf((stops - e0) / fmaxf(e1 - e0, 1e-4f), 0.0f), 1.0f);
+        gate = t * t * (3.0f - 2.0f * t);
+    }
+    const float boost = 1.0f + sp.point_emphasis * fmaxf(0.0f, mx / fmaxf(sp.threshold_linear, 1e-4f) - 1.0f);
+    mask[i] = fminf(fmaxf(gate * boost, 0.0f), 1.0f);
+}
+
+// Row pass of an image spectrum: reads the frame, writes the grid.
+struct FrameRows {
+    const float4* src;
+    const float* mask;
+    float2* out;
+    int w, h, pw, tf, extraction;
+    int driver;     // 1: scalar driver; 0: channels ca (+ i cb)
+    int ca, cb;     // cb < 0: zero imaginary part
+    float scale;
+    __device__ float2 load(int row, int x) const {
+        if (row >= h || x >= w) return make_float2(0.0f, 0.0f);
+        const size_t i = size_t(row) * w + x;
+        const float4 q = decoded(src, i, tf);
+        const float m = mask[i];
+        if (driver) {
+            const float sig = extraction == LDH_EXTRACT_LUMA ? luma709(q.x, q.y, q.z) : max3(q.x, q.y, q.z);
+            return make_float2(sig * m * scale, 0.0f);
+        }
+        const float k = m * scale;
+        const float c[3] = {q.x, q.y, q.z};
+        return make_float2(c[ca] * k, cb >= 0 ? c[cb] * k : 0.0f);
+    }
+    __device__ void store(int row, int x, float2 v) const { out[size_t(row) * pw + x] = v; }
+};
+
+// Row pass of a kernel spectrum: kernel centre wrapped to the origin, two real kernels packed.
+struct KernelRows {
+    const float* ka;
+    const float* kb;
+    float2* out;
+    int K, c, r, pw, ph;
+    __device__ float2 load(int row, int x) const {
+        const int dy = row < ph / 2 ? row : row - ph;
+        const int dx = x < pw / 2 ? x : x - pw;
+        if (dx < -r || dx > r || dy < -r || dy > r) return make_float2(0.0f, 0.0f);
+        const size_t i = size_t(c + dy) * K + size_t(c + dx);
+        return make_float2(ka[i], kb ? kb[i] : 0.0f);
+    }
+    __device__ void store(int row, int x, float2 v) const { out[size_t(row) * pw + x] = v; }
+};
+
+// Last inverse pass: crop to the frame, scale by 1/(pw*ph) (exact), clamp at zero.
+struct ExtractRows {
+    float2* data;
+    float* oa;
+    float* ob;
+    int w, h, pw;
+    float scale;
+    __device__ float2 load(int row, int x) const { return data[size_t(row) * pw + x]; }
+    __device__ void store(int row, int x, float2 v) const {
+        if (row >= h || x >= w) return;
+        const size_t i = size_t(row) * w + x;
+        oa[i] = fmaxf(0.0f, v.x * scale);
+        if (ob) ob[i] = fmaxf(0.0f, v.y * scale);
+    }
+};
+
+__global__ void to_half_kernel(const float2* __restrict__ in, __half2* __restrict__ out, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = __float22half2_rn(in[i]);
+}
+
+// MapLensDiffSpectralBins per pixel over `n` bin planes -> 3 effect planes.
+__global__ void spectral_map_kernel(const float* __restrict__ planes, int n, int64_t npx,
+                                    const float* __restrict__ nat_m, const float* __restrict__ sty_m, float force,
+                                    float saturation, int affects_luma, float* __restrict__ eff) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    float nat[3], sty[3];
+    if (n <= 1) {
+        const float v = planes[i];
+        nat[0] = nat[1] = nat[2] = sty[0] = sty[1] = sty[2] = v;
+    } else {
+        for (int c = 0; c < 3; ++c) { nat[c] = 0.0f; sty[c] = 0.0f; }
+        for (int b = 0; b < n; ++b) {
+            const float v = planes[int64_t(b) * npx + i];
+            for (int c = 0; c < 3; ++c) {
+                nat[c] += nat_m[c * n + b] * v;
+                sty[c] += sty_m[c * n + b] * v;
+            }
+        }
+    }
+    float o[3];
+    for (int c = 0; c < 3; ++c) o[c] = fmaxf(0.0f, nat[c] * (1.0f - force) + sty[c] * force);
+    const float gray = 0.2126f * o[0] + 0.7152f * o[1] + 0.0722f * o[2];
+    for (int c = 0; c < 3; ++c) o[c] = gray + (o[c] - gray) * saturation;
+    if (!affects_luma) {
+        const float target = 0.2126f * nat[0] + 0.7152f * nat[1] + 0.0722f * nat[2];
+        const float cur = 0.2126f * o[0] + 0.7152f * o[1] + 0.0722f * o[2];
+        if (cur > 1e-6f) {
+            const float k = target / cur;
+            for (int c = 0; c < 3; ++c) o[c] *= k;
+        }
+    }
+    for (int c = 0; c < 3; ++c) eff[int64_t(c) * npx + i] = o[c];
+}
+
+// The same map when the bins were pre-combined: planes hold natural R,G,B (and style R,G,B).
+__global__ void precombined_map_kernel(const float* __restrict__ planes, int with_style, int64_t npx, float force,
+                                       float saturation, int affects_luma, float* __restrict__ eff) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    float nat[3], sty[3];
+    for (int c = 0; c < 3; ++c) {
+        nat[c] = planes[int64_t(c) * npx + i];
+        sty[c] = with_style ? planes[int64_t(3 + c) * npx + i] : nat[c];
+    }
+    float o[3];
+    for (int c = 0; c < 3; ++c) o[c] = fmaxf(0.0f, nat[c] * (1.0f - force) + sty[c] * force);
+    const float gray = 0.2126f * o[0] + 0.7152f * o[1] + 0.0722f * o[2];
+    for (int c = 0; c < 3; ++c) o[c] = gray + (o[c] - gray) * saturation;
+    if (!affects_luma) {
+        const float target = 0.2126f * nat[0] + 0.7152f * nat[1] + 0.0722f * nat[2];
+        const float cur = 0.2126f * o[0] + 0.7152f * o[1] + 0.0722f * o[2];
+        if (cur > 1e-6f) {
+            const float k = target / cur;
+            for (int c = 0; c < 3; ++c) o[c] *= k;
+        }
+    }
+    for (int c = 0; c < 3; ++c) eff[int64_t(c) * npx + i] = o[c];
+}
+
+__device__ __forceinline__ float soft_shoulder(float v, float sh) {
+    const float x = fmaxf(0.0f, v);
+    return sh * (1.0f - expf(-x / sh));
+}
+
+// Split look: effect = shoulder(core) * core_gain + shoulder(structure) * structure_gain.
+__global__ void split_combine_kernel(const float* __restrict__ core, const float* __restrict__ structure, int64_t npx,
+                                     float core_sh, float struct_sh, float core_gain, float struct_gain,
+                                     float* __restrict__ eff) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    for (int c = 0; c < 3; ++c) {
+        float a = core[int64_t(c) * npx + i];
+        float b = structure[int64_t(c) * npx + i];
+        if (core_sh > 0.0f) a = soft_shoulder(a, core_sh);
+        if (struct_sh > 0.0f) b = soft_shoulder(b, struct_sh);
+        eff[int64_t(c) * npx + i] = a * core_gain + b * struct_gain;
+    }
+}
+
+// Luma of the redistributed image (segment 0) and of the effect (segment 1), for Preserve.
+__global__ void preserve_luma_kernel(const float4* __restrict__ src, const float* __restrict__ mask, int tf,
+                                     float scale, const float* __restrict__ eff, int64_t npx, float* __restrict__ lum) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    const float4 q = decoded(src, size_t(i), tf);
+    const float m = mask[i] * scale;
+    lum[i] = luma709(q.x * m, q.y * m, q.z * m);
+    lum[npx + i] = luma709(eff[i], eff[npx + i], eff[2 * npx + i]);
+}
+
+__global__ void preserve_factor_kernel(const float* __restrict__ totals, float* __restrict__ factor) {
+    const float in_e = totals[0], eff_e = totals[1];
+    factor[0] = eff_e > 1e-6f ? in_e / eff_e : 1.0f;
+    factor[1] = eff_e > 1e-6f ? 1.0f : 0.0f;
+}
+
+__global__ void max_kernel(const float* __restrict__ x, int64_t n, unsigned* __restrict__ out) {
+    unsigned m = 0u;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        m = max(m, __float_as_uint(fmaxf(x[i], 0.0f)));
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+struct CompositeParams {
+    int tf, view;
+    float scale;        // redistribution scale 1 - corePreserve
+    float comp, gain;   // core compensation, effect gain
+    float max_sub;      // (1 - protect) / scale
+};
+
+__global__ void composite_kernel(const float4* __restrict__ src, const float* __restrict__ mask,
+                                 const float* __restrict__ eff, const float* __restrict__ factor,
+                                 const unsigned* __restrict__ mask_max, int64_t npx, CompositeParams cp,
+                                 float4* __restrict__ dst) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    const float4 s = decoded(src, size_t(i), cp.tf);
+    if (cp.view == LDH_VIEW_SELECTION) {
+        const float mx = __uint_as_float(*mask_max);
+        const float im = mx > 0.0f ? 1.0f / mx : 0.0f;
+        const float d = fminf(fmaxf(mask[i] * im, 0.0f), 1.0f);
+        dst[i] = make_float4(d, d, d, 1.0f);
+        return;
+    }
+    const bool scaled = factor != nullptr && factor[1] > 0.0f;
+    float e[3];
+    for (int c = 0; c < 3; ++c) {
+        e[c] = eff[int64_t(c) * npx + i];
+        if (scaled) e[c] *= factor[0];
+    }
+    if (cp.view != LDH_VIEW_FINAL) {
+        dst[i] = make_float4(e[0], e[1], e[2], 1.0f);
+        return;
+    }
+    const float m = mask[i] * cp.scale;
+    const float sv[3] = {s.x, s.y, s.z};
+    float o[3];
+    for (int c = 0; c < 3; ++c) {
+        const float r = sv[c] * m;
+        const float raw = fmaxf(0.0f, sv[c] - cp.comp * r + cp.gain * e[c]);
+        const float flo = fmaxf(0.0f, sv[c] - r * cp.max_sub);
+        o[c] = encode_channel(fmaxf(raw, flo), cp.tf);
+    }
+    dst[i] = make_float4(o[0], o[1], o[2], s.w);
+}
+
+__global__ void combine_kernel(const float* __restrict__ kernels, int bins, int64_t kk, const float* __restrict__ m,
+                               int rows, float* __restrict__ out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= kk) return;
+    for (int r = 0; r < rows; ++r) {
+        float acc = 0.0f;
+        for (int b = 0; b < bins; ++b) acc += m[r * bins + b] * kernels[int64_t(b) * kk + i];
+        out[int64_t(r) * kk + i] = acc;
+    }
+}
+
+inline dim3 grid1(int64_t n) { return dim3(unsigned(div_up(n, 256))); }
+
+}  // namespace
+
+Grid make_grid(int w, int h, int support) {
+    Grid g;
+    g.w = w;
+    g.h = h;
+    const int n = 2 * support + 1;
+    g.pw = std::max(16, pow2_at_least(w + n - 1));
+    g.ph = std::max(16, pow2_at_least(h + n - 1));
+    if (g.pw > kMaxFft || g.ph > kMaxFft) fail(LDH_ERR_TOO_LARGE, "frame-too-large-for-fft-grid");
+    return g;
+}
+
+void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<const float*>& kernels, int K,
+                          int radius, const Grid& g, bool fp16, KernelSpectra& out, DeviceBuffer& scratch) {
+    const int n = int(kernels.size());
+    const int pairs = (n + 1) / 2;
+    out.grid = g;
+    out.kernels = n;
+    out.pairs = pairs;
+    out.fp16 = fp16;
+    const size_t cells = g.cells();
+    out.data.reset((fp16 ? sizeof(__half2) : sizeof(float2)) * cells * pairs);
+    const float2* tw = t.twiddle.as<float2>();
+    if (fp16) scratch.reset(sizeof(float2) * cells);
+    for (int p = 0; p < pairs; ++p) {
+        float2* dst = fp16 ? scratch.as<float2>() : out.data.as<float2>() + cells * p;
+        const float* kb = 2 * p + 1 < n ? kernels[size_t(2 * p + 1)] : nullptr;
+        KernelRows op{kernels[size_t(2 * p)], kb, dst, K, K / 2, radius, g.pw, g.ph};
+        rows_pass(s, tw, g.pw, g.ph, op, false);
+        cols_forward(s, tw, dst, g.pw, g.ph, PlainLoad{});
+        if (fp16) {
+            hipLaunchKernelGGL(to_half_kernel, grid1(int64_t(cells)), dim3(256), 0, s, dst,
+                               reinterpret_cast<__half2*>(out.data.as<__half2>()) + cells * p, int64_t(cells));
+            LDH_CHECK_LAUNCH("spectra-to-half");
+        }
+    }
+}
+
+void combine_kernels(hipStream_t s, const float* kernels, int bins, int K, const SpectrumConfig& cfg, bool with_style,
+                     float* out, DeviceBuffer& scratch) {
+    const int rows = with_style ? 6 : 3;
+    std::vector<float> m(static_cast<size_t>(rows * bins));
+    for (int r = 0; r < 3; ++r)
+        for (int b = 0; b < bins; ++b) {
+            m[size_t(r * bins + b)] = cfg.natural[size_t(r * bins + b)];
+            if (with_style) m[size_t((3 + r) * bins + b)] = cfg.style[size_t(r * bins + b)];
+        }
+    scratch.reset(sizeof(float) * m.size());
+    LDH_HIP(hipMemcpyAsync(scratch.get(), m.data(), sizeof(float) * m.size(), hipMemcpyHostToDevice, s), "combine");
+    const int64_t kk = int64_t(K) * K;
+    hipLaunchKernelGGL(combine_kernel, grid1(kk), dim3(256), 0, s, kernels, bins, kk, scratch.as<float>(), rows, out);
+    LDH_CHECK_LAUNCH("combine-kernels");
+    LDH_HIP(hipStreamSynchronize(s), "combine");
+}
+
+namespace {
+
+// Which kernel set an effect needs.
+enum Set { kFull = LDH_KERNEL_FULL, kCore = LDH_KERNEL_CORE, kStructure = LDH_KERNEL_STRUCTURE };
+
+struct Plan {
+    bool spectral = false;
+    bool split = false;
+    std::vector<int> sets;  // kernel sets convolved, in order
+};
+
+Plan plan_of(const ldh_params& p) {
+    Plan pl;
+    pl.spectral = is_spectral(p);
+    pl.split = p.look_mode == LDH_LOOK_SPLIT;
+    if (pl.split) {
+        pl.sets = {kCore, kStructure};
+    } else if (p.debug_view == LDH_VIEW_CORE) {
+        pl.sets = {kCore};
+    } else if (p.debug_view == LDH_VIEW_STRUCTURE) {
+        pl.sets = {kStructure};
+    } else {
+        pl.sets = {kFull};
+    }
+    if (pl.split && p.debug_view == LDH_VIEW_CORE) pl.sets = {kCore, kStructure};
+    return pl;
+}
+
+}  // namespace
+
+std::vector<const float*> effect_kernels(const ldh_params& p, const DeviceBank& bank, EffectMode mode,
+                                         const float* combined, int* which_out) {
+    const Plan pl = plan_of(p);
+    std::vector<const float*> ks;
+    const int which = pl.sets.front();
+    if (which_out) *which_out = which;
+    if (!pl.spectral) {
+        ks.push_back(bank.kernel_ptr(which, 0));
+    } else if (mode == EffectMode::PerBin) {
+        for (int b = 0; b < bank.bins; ++b) ks.push_back(bank.kernel_ptr(which, b));
+    } else {
+        const int64_t kk = int64_t(bank.kernel) * bank.kernel;
+        for (int r = 0; r < 6; ++r) ks.push_back(combined + r * kk);
+    }
+    return ks;
+}
+
+void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
+                  const float4* src, float4* dst, int w, int h, EffectMode mode, const KernelSpectra* cached) {
+    const Plan pl = plan_of(p);
+    const int64_t npx = int64_t(w) * h;
+    const Grid g = make_grid(w, h, bank.support);
+    const size_t cells = g.cells();
+    const float2* tw = t.twiddle.as<float2>();
+    const int n = bank.bins;
+    const SpectrumConfig cfg = build_spectrum_config(p, bank.wavelengths);
+    const float force = std::max(0.0f, std::min(float(p.spectrum_force), 1.0f));
+    const bool with_style = p.spectrum_style != LDH_STYLE_NATURAL && force > 0.0f;
+    if (mode == EffectMode::Precombined && (!pl.spectral || pl.split || pl.sets.front() != kFull))
+        mode = EffectMode::PerBin;  // pre-combination applies to the spectral full-kernel effect
+
+    // 1. selection mask
+    fw.mask.reset(sizeof(float) * npx);
+    SelectParams sp;
+    sp.tf = p.input_transfer;
+    sp.extraction = p.extraction_mode;
+    sp.softness = float(std::max(0.01, p.softness_stops));
+    sp.threshold_stops = float(p.threshold);
+    sp.threshold_linear = 0.18f * std::pow(2.0f, sp.threshold_stops);
+    sp.point_emphasis = float(p.point_emphasis);
+    hipLaunchKernelGGL(selection_kernel, grid1(npx), dim3(256), 0, s, src, npx, sp, fw.mask.as<float>());
+    LDH_CHECK_LAUNCH("selection");
+    const float redist_scale = 1.0f - float(std::max(0.0, std::min(p.core_preserve, 1.0)));
+
+    // 2. image spectra: mono -> (R + iG), (B); spectral -> driver
+    const int nimg = pl.spectral ? 1 : 2;
+    fw.image_spec.reset(sizeof(float2) * cells * nimg);
+    for (int j = 0; j < nimg; ++j) {
+        float2* spec = fw.image_spec.as<float2>() + cells * j;
+        FrameRows op{src, fw.mask.as<float>(), spec, w, h, g.pw, p.input_transfer, p.extraction_mode,
+                     pl.spectral ? 1 : 0, j == 0 ? 0 : 2, j == 0 ? 1 : -1, redist_scale};
+        rows_pass(s, tw, g.pw, g.ph, op, false);
+        cols_forward(s, tw, spec, g.pw, g.ph, PlainLoad{});
+    }
+
+    // 3-4. per kernel set: spectra x image spectra -> planes -> 3 effect planes
+    fw.out_spec.reset(sizeof(float2) * cells);
+    const int nplanes = pl.spectral ? (mode == EffectMode::Precombined ? 6 : n) : 3;
+    fw.planes.reset(sizeof(float) * npx * std::max(nplanes, 3));
+    fw.effect.reset(sizeof(float) * npx * 9);
+    fw.matrices.reset(sizeof(float) * 6 * n);
+    LDH_HIP(hipMemcpyAsync(fw.matrices.as<float>(), cfg.natural.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, s),
+            "matrices");
+    LDH_HIP(hipMemcpyAsync(fw.matrices.as<float>() + 3 * n, cfg.style.data(), sizeof(float) * 3 * n,
+                           hipMemcpyHostToDevice, s), "matrices");
+    const float inv_cells = 1.0f / float(cells);
+    float* planes = fw.planes.as<float>();
+
+    for (size_t si = 0; si < pl.sets.size(); ++si) {
+        const int set = pl.sets[si];
+        // kernels of this set
+        std::vector<const float*> ks;
+        const int64_t kk = int64_t(bank.kernel) * bank.kernel;
+        if (!pl.spectral) {
+            ks.push_back(bank.kernel_ptr(set, 0));
+        } else if (mode == EffectMode::PerBin) {
+            for (int b = 0; b < n; ++b) ks.push_back(bank.kernel_ptr(set, b));
+        } else {
+            if (!cached) {
+                fw.combined.reset(sizeof(float) * kk * 6);
+                combine_kernels(s, bank.kernel_ptr(set, 0), n, bank.kernel, cfg, with_style, fw.combined.as<float>(),
+                                fw.kspec_tmp);
+            }
+            const int nk = with_style ? 6 : 3;
+            for (int r = 0; r < nk; ++r) ks.push_back(fw.combined.as<float>() + r * kk);
+        }
+        const int pairs = pl.spectral ? int((ks.size() + 1) / 2) : 1;
+        const bool use_cache = cached != nullptr && si == 0 && pl.sets.size() == 1;
+        for (int pi = 0; pi < pairs; ++pi) {
+            // kernel spectrum of this pair
+            const void* kspec;
+            bool half = false;
+            if (use_cache) {
+                half = cached->fp16;
+                kspec = half ? static_cast<const void*>(cached->data.as<__half2>() + cells * pi)
+                              : static_cast<const void*>(cached->data.as<float2>() + cells * pi);
+            } else {
+                std::vector<const float*> pair{ks[size_t(pl.spectral ? 2 * pi : 0)]};
+                if (pl.spectral && size_t(2 * pi + 1) < ks.size()) pair.push_back(ks[size_t(2 * pi + 1)]);
+                build_kernel_spectra(s, t, pair, bank.kernel, bank.support, g, false, fw.kspec, fw.kspec_tmp);
+                kspec = fw.kspec.data.get();
+            }
+            // convolutions against the image spectra
+            for (int j = 0; j < nimg; ++j) {
+                const float2* ispec = fw.image_spec.as<float2>() + cells * j;
+                float2* o = fw.out_spec.as<float2>();
+                if (half) cols_inverse(s, tw, ispec, o, g.pw, g.ph, MulSpectrumHalfLoad{static_cast<const __half2*>(kspec)});
+                else cols_inverse(s, tw, ispec, o, g.pw, g.ph, MulSpectrumLoad{static_cast<const float2*>(kspec)});
+                float* oa;
+                float* ob;
+                if (!pl.spectral) {
+                    oa = planes + npx * (j == 0 ? 0 : 2);
+                    ob = j == 0 ? planes + npx : nullptr;
+                } else {
+                    oa = planes + npx * (2 * pi);
+                    ob = size_t(2 * pi + 1) < ks.size() ? planes + npx * (2 * pi + 1) : nullptr;
+                }
+                ExtractRows ex{o, oa, ob, w, h, g.pw, inv_cells};
+                rows_pass(s, tw, g.pw, h, ex, true);
+            }
+        }
+        // effect planes of this set
+        float* eff = fw.effect.as<float>() + npx * 3 * (pl.split ? 1 + int(si) : 0);
+        if (!pl.spectral) {
+            LDH_HIP(hipMemcpyAsync(eff, planes, sizeof(float) * npx * 3, hipMemcpyDeviceToDevice, s), "effect-copy");
+        } else if (mode == EffectMode::PerBin) {
+            hipLaunchKernelGGL(spectral_map_kernel, grid1(npx), dim3(256), 0, s, planes, n, npx, fw.matrices.as<float>(),
+                               fw.matrices.as<float>() + 3 * n, force, float(std::max(0.0, p.spectrum_saturation)),
+                               p.chromatic_affects_luma, eff);
+        } else {
+            hipLaunchKernelGGL(precombined_map_kernel, grid1(npx), dim3(256), 0, s, planes, with_style ? 1 : 0, npx,
+                               force, float(std::max(0.0, p.spectrum_saturation)), p.chromatic_affects_luma, eff);
+        }
+        LDH_CHECK_LAUNCH("effect-map");
+    }
+    float* effect = fw.effect.as<float>();
+    if (pl.split) {
+        const float* core = effect + npx * 3;
+        const float* structure = effect + npx * 6;
+        hipLaunchKernelGGL(split_combine_kernel, grid1(npx), dim3(256), 0, s, core, structure, npx,
+                           float(p.core_shoulder), float(p.structure_shoulder), float(std::max(0.0, p.core_gain)),
+                           float(std::max(0.0, p.structure_gain)), effect);
+        LDH_CHECK_LAUNCH("split-combine");
+        if (p.debug_view == LDH_VIEW_CORE) effect = effect + npx * 3;
+        else if (p.debug_view == LDH_VIEW_STRUCTURE) effect = effect + npx * 6;
+    }
+
+    // 5. Preserve rescale
+    const bool preserve = p.energy_mode == LDH_ENERGY_PRESERVE;
+    const float gain = preserve ? float(std::max(0.0, std::min(p.effect_gain, 1.0))) : float(std::max(0.0, p.effect_gain));
+    const float comp = preserve ? gain : float(std::max(0.0, p.core_compensation));
+    const float* factor = nullptr;
+    if (preserve) {
+        fw.lum.reset(sizeof(float) * npx * 2);
+        fw.totals.reset(sizeof(float) * 2);
+        fw.factors.reset(sizeof(float) * 2);
+        hipLaunchKernelGGL(preserve_luma_kernel, grid1(npx), dim3(256), 0, s, src, fw.mask.as<float>(), p.input_transfer,
+                           redist_scale, fw.effect.as<float>(), npx, fw.lum.as<float>());
+        fw.seqsum.run(s, fw.lum.as<float>(), npx, npx, 2, fw.totals.as<float>());
+        hipLaunchKernelGGL(preserve_factor_kernel, dim3(1), dim3(1), 0, s, fw.totals.as<float>(), fw.factors.as<float>());
+        LDH_CHECK_LAUNCH("preserve");
+        factor = fw.factors.as<float>();
+    }
+
+    // 6. composite
+    const unsigned* mask_max = nullptr;
+    if (p.debug_view == LDH_VIEW_SELECTION) {
+        fw.max_buf.reset(sizeof(unsigned));
+        LDH_HIP(hipMemsetAsync(fw.max_buf.get(), 0, sizeof(unsigned), s), "mask-max");
+        hipLaunchKernelGGL(max_kernel, dim3(256), dim3(256), 0, s, fw.mask.as<float>(), npx, fw.max_buf.as<unsigned>());
+        mask_max = fw.max_buf.as<unsigned>();
+    }
+    const float protect = std::max(0.2f, float(std::max(0.0, std::min(p.core_preserve, 1.0))));
+    CompositeParams cp;
+    cp.tf = p.input_transfer;
+    cp.view = p.debug_view;
+    cp.scale = redist_scale;
+    cp.comp = comp;
+    cp.gain = gain;
+    cp.max_sub = redist_scale > 1e-6f ? (1.0f - protect) / redist_scale : 0.0f;
+    const float* eff_out = (p.debug_view == LDH_VIEW_FINAL || p.debug_view == LDH_VIEW_EFFECT || !pl.split)
+                               ? fw.effect.as<float>() : effect;
+    if (p.debug_view == LDH_VIEW_CORE || p.debug_view == LDH_VIEW_STRUCTURE) {
+        eff_out = effect;
+        factor = nullptr;  // the reference rescales only the effect image
+    }
+    hipLaunchKernelGGL(composite_kernel, grid1(npx), dim3(256), 0, s, src, fw.mask.as<float>(), eff_out, factor,
+                       mask_max, npx, cp, dst);
+    LDH_CHECK_LAUNCH("composite");
+}
+
+}  // namespace ldh

@@ -1,0 +1,119 @@
+// Internal declarations shared by the HIP translation units of liblensdiff_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "lensdiff_hip.h"
+
+namespace ldh {
+
+// ---- error handling ----------------------------------------------------------------------
+struct Error {
+    ldh_status status;
+    std::string tag;  // kebab-case, as the reference's backends report (LensDiffCuda.cu:711-729)
+};
+
+#define LDH_HIP(call, tag)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            throw ::ldh::Error{LDH_ERR_DEVICE, std::string(tag) + ":" + hipGetErrorName(e_)}; \
+        }                                                                                  \
+    } while (0)
+
+#define LDH_CHECK_LAUNCH(tag) LDH_HIP(hipGetLastError(), tag)
+
+inline void fail(ldh_status s, const std::string& tag) { throw Error{s, tag}; }
+
+// ---- small device helpers ------------------------------------------------------------------
+__host__ __device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+inline int ilog2(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+inline bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+inline int pow2_at_least(int v) {
+    int n = 1;
+    while (n < v) n <<= 1;
+    return n;
+}
+inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---- device memory ---------------------------------------------------------------------------
+// A device allocation owned by RAII. Allocation failures become LDH_ERR_OUT_OF_MEMORY.
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t bytes) { reset(bytes); }
+    ~DeviceBuffer() { release(); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) { o.ptr_ = nullptr; o.bytes_ = 0; }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
+        if (this != &o) { release(); ptr_ = o.ptr_; bytes_ = o.bytes_; o.ptr_ = nullptr; o.bytes_ = 0; }
+        return *this;
+    }
+    void reset(size_t bytes) {
+        if (bytes <= bytes_ && ptr_) return;
+        release();
+        if (bytes == 0) return;
+        if (hipMalloc(&ptr_, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            ptr_ = nullptr;
+            fail(LDH_ERR_OUT_OF_MEMORY, "device-allocation-failed");
+        }
+        bytes_ = bytes;
+    }
+    void release() {
+        if (ptr_) (void)hipFree(ptr_);
+        ptr_ = nullptr;
+        bytes_ = 0;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(ptr_); }
+    void* get() const { return ptr_; }
+    size_t bytes() const { return bytes_; }
+
+private:
+    void* ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// ---- timing of kernel families with HIP events (on the launch stream) -----------------------
+struct Profiler;
+
+// ---- FFT (fft.hip) ---------------------------------------------------------------------------
+// Twiddle tables live in device memory per context: tw[j] = exp(-2 pi i j / kMaxFft).
+constexpr int kMaxFftLog2 = 14;
+constexpr int kMaxFft = 1 << kMaxFftLog2;   // 16384 points
+
+struct FftTables {
+    DeviceBuffer twiddle;  // float2[kMaxFft]
+    void init(hipStream_t s);
+};
+
+// 2D complex FFT of a height x width array (row pitch = width), both powers of two in
+// [2, kMaxFft]. Forward leaves the spectrum with its column index in "digit-split" order when
+// split_columns is set (see fft.hip); the matching inverse takes that order back. Inverse is
+// scaled by 1/(width*height).
+void fft2d_forward(hipStream_t s, const FftTables& t, float2* data, int width, int height);
+void fft2d_inverse(hipStream_t s, const FftTables& t, float2* data, int width, int height);
+// Row or column passes only (used by fused pipelines).
+void fft_rows(hipStream_t s, const FftTables& t, float2* data, int width, int rows, bool inverse);
+void fft_cols_forward(hipStream_t s, const FftTables& t, float2* data, int width, int height);
+void fft_cols_inverse(hipStream_t s, const FftTables& t, float2* data, int width, int height);
+// Map a natural frequency row index to the storage row used by the split column transform.
+int fft_col_storage_row(int height, int k);
+
+}  // namespace ldh

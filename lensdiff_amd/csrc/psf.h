@@ -1,0 +1,34 @@
+// PSF synthesis plan and launchers.
+#pragma once
+
+#include "ldh_internal.h"
+#include "pupil.h"
+#include "seqsum.h"
+
+namespace ldh {
+
+constexpr int kMaxRawPsf = 8192;  // Bluestein length 16384 is the largest LDS-resident line
+
+// Per (P, R) state: chirp and Bluestein filter spectrum, plus stage buffers.
+class PsfPlan {
+public:
+    void prepare(hipStream_t s, int pupil, int raw);
+    // `count` raw PSFs from one amplitude (P*P) and `count` phase maps (count*P*P) into
+    // out (count*R*R), each normalised to unit sequential-FP32 sum; totals: device[count].
+    void raw_psfs(hipStream_t s, const FftTables& t, const float* amp, const float* phase, int count,
+                  float* out, float* totals, SeqSum& seqsum);
+    int pupil() const { return pupil_; }
+    int raw() const { return raw_; }
+    int bluestein_length() const { return m_; }
+    bool bluestein() const { return bluestein_; }
+
+private:
+    int pupil_ = 0, raw_ = 0, m_ = 0, offset_ = 0;
+    bool bluestein_ = false;
+    DeviceBuffer chirp_, bhat_, g_, gt_, it_;
+};
+
+void pupil_amplitude(hipStream_t s, const PupilGeom& g, float* out);
+void pupil_phase(hipStream_t s, const PhaseCoef* d_coefs, int n, int count, float* out);
+
+}  // namespace ldh

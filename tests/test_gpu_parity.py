@@ -1,0 +1,218 @@
+"""HIP path vs the oracle (FP32 restatement of the reference CPU path), stage by stage and end
+to end, on seeded inputs at sizes the oracle finishes in seconds. Tolerances: relative L2
+<= 1e-4 for PSFs, kernels and frames; bit-exact selection masks and identical integer radii."""
+import math
+
+import numpy as np
+import pytest
+
+from lensdiff_amd import _abi, workloads
+from oracle.oracle import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def params(**kw):
+    p = _abi.default_params()
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def crel(a, b):
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+@pytest.mark.parametrize("w,h", [(16, 8), (64, 64), (256, 128), (128, 1024), (2048, 512), (16384, 16), (32, 16384)])
+def test_fft2d_matches_numpy(ctx, w, h):
+    rng = np.random.default_rng(w * 7 + h)
+    z = rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))
+    f = ctx.fft2d(z)
+    assert crel(f, np.fft.fft2(z)) < 2e-6
+    back = ctx.fft2d(f, inverse=True)
+    assert crel(back, z) < 2e-6
+
+
+def test_sequential_sum_reproduces_fp32_accumulation(ctx, oracle):
+    rng = np.random.default_rng(3)
+    n = 2047
+    yy, xx = np.mgrid[0:n, 0:n]
+    r = np.hypot(xx - n // 2, yy - n // 2) + 1e-3
+    airy = (np.sinc(r / 20.0) ** 2 / (1 + r / 20.0)).astype(np.float32)
+    cases = [rng.random(1_000_003).astype(np.float32),
+             rng.exponential(1e-6, 3_000_000).astype(np.float32),
+             (airy / airy.sum()).astype(np.float32),
+             np.zeros(10, np.float32), np.ones(1, np.float32)]
+    for x in cases:
+        want = oracle.sequential_sum(x)
+        got = ctx.sequential_sum(x)
+        # equal up to an exact-tie rounding (one ulp of the running sum)
+        assert abs(got - want) <= 2 * np.spacing(np.float32(abs(want))), (got, want, x.size)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 5, 6, 7])
+def test_pupil_amplitude(ctx, oracle, mode):
+    p = params(aperture_mode=mode, pupil_resolution=200, blade_count=7, roundness=0.3, rotation_deg=17.0,
+               central_obstruction=0.2, vane_count=3, vane_thickness=0.02, apodization_mode=mode % 3)
+    a = ctx.pupil_amplitude(p)
+    b = oracle.pupil_amplitude(p)
+    differ = np.count_nonzero(np.abs(a - b) > 1e-6)
+    # device transcendental functions may flip a pixel lying exactly on an aperture edge
+    assert differ <= 2, differ
+    assert np.abs(a - b)[np.abs(a - b) <= 1e-6].max(initial=0) <= 1e-6
+
+
+def test_pupil_phase(ctx, oracle):
+    p = params(pupil_resolution=160, rotation_deg=11.0, pupil_decenter_x=0.03)
+    p.set_noll({2: 0.1, 3: -0.2, 4: 0.3, 5: 0.1, 6: -0.1, 7: 0.05, 8: 0.2, 9: -0.07, 10: 0.04, 11: -0.2,
+                12: 0.03, 13: -0.06, 14: 0.02, 15: 0.05, 16: -0.04, 17: 0.01, 18: 0.03, 19: -0.02, 20: 0.01,
+                21: 0.02})
+    a = ctx.pupil_phase(p)
+    b = oracle.pupil_phase(p)
+    assert np.abs(a - b).max() < 2e-6
+
+
+@pytest.mark.parametrize("pupil,raw", [(64, 256), (64, 300), (128, 512), (96, 1020)])
+def test_raw_psf(ctx, oracle, pupil, raw):
+    p = params(pupil_resolution=pupil, aperture_mode=1, blade_count=6, roundness=0.2)
+    p.set_noll({4: 0.3, 11: -0.2, 8: 0.1})
+    amp = oracle.pupil_amplitude(p)
+    ph = oracle.pupil_phase(p)
+    got = ctx.raw_psf(amp, ph, raw)
+    want = oracle.raw_psf(amp, ph, raw)
+    assert rel_l2(got, want) < TOL
+    assert abs(float(got.astype(np.float64).sum()) - 1.0) < 1e-3
+
+
+def test_first_minimum_radius(ctx, oracle):
+    p = params(pupil_resolution=128, aperture_mode=0, roundness=1.0)
+    amp = oracle.pupil_amplitude(p)
+    psf = oracle.raw_psf(amp, None, 512)
+    assert abs(ctx.first_minimum_radius(psf) - oracle.first_minimum_radius(psf)) < 1e-4
+
+
+@pytest.mark.parametrize("scale,support", [(0.37, 40), (2.5, 30), (1.0, 25)])
+def test_resample_kernel(ctx, oracle, scale, support):
+    p = params(pupil_resolution=64, aperture_mode=1, blade_count=5)
+    psf = oracle.raw_psf(oracle.pupil_amplitude(p), None, 300)
+    got = ctx.resample_kernel(psf, scale, support)
+    want = oracle.resample_kernel(psf, scale, support)
+    assert rel_l2(got, want) < 1e-5
+
+
+@pytest.mark.parametrize("aniso", [0.0, 0.3])
+def test_shape_kernels(ctx, oracle, aniso):
+    p = params(pupil_resolution=64, aperture_mode=1, blade_count=6, frame_short_side_px=400,
+               max_kernel_radius_px=10.0, anisotropy_emphasis=aniso)
+    psf = oracle.raw_psf(oracle.pupil_amplitude(p), None, 512)
+    base = oracle.resample_kernel(psf, 0.5, 40)
+    got = ctx.shape_kernels(p, base)
+    want = oracle.shape_kernels(p, base)
+    assert got[0].shape == want[0].shape
+    for g, w in zip(got, want[:3]):
+        assert rel_l2(g, w) < TOL
+
+
+BANK_CASES = {
+    "mono-polygon": dict(spectral_mode=0),
+    "tristimulus-phase": dict(spectral_mode=1, chromatic_focus=0.3, chromatic_spread=0.1),
+    "spectral5-star": dict(spectral_mode=2, aperture_mode=2, anisotropy_emphasis=0.2),
+    "spectral9": dict(spectral_mode=3),
+}
+
+
+@pytest.mark.parametrize("case", sorted(BANK_CASES))
+def test_bank(ctx, oracle, case):
+    p = workloads.base_params(200)
+    p.pupil_resolution = 64
+    p.blade_count = 7
+    p.max_kernel_radius_px = 16.0
+    p.set_noll({4: 0.2, 11: -0.1})
+    for k, v in BANK_CASES[case].items():
+        setattr(p, k, v)
+    got = ctx.bank(p)
+    want = oracle.bank(p)
+    assert got["support"] == want["support"]
+    assert np.allclose(got["wavelengths"], want["wavelengths"])
+    for name in ("full", "core", "structure"):
+        for g, w in zip(got[name], want[name]):
+            assert g.shape == w.shape
+            assert rel_l2(g, w) < TOL, (name, rel_l2(g, w))
+
+
+def small_frame(w, h, seed=0):
+    return workloads.frame(w, h, 0.004, seed=seed)
+
+
+RENDER_CASES = {
+    "mono-physical-augment": dict(),
+    "mono-split-preserve": dict(look_mode=1, energy_mode=0, core_preserve=0.1, core_shoulder=2.0,
+                                structure_gain=0.7),
+    "tristimulus-physical": dict(spectral_mode=1, chromatic_focus=0.3),
+    "spectral5-style-force": dict(spectral_mode=2, spectrum_style=1, spectrum_force=0.6,
+                                  spectrum_saturation=1.3),
+    "spectral9-split-luma": dict(spectral_mode=3, look_mode=1, extraction_mode=1),
+    "list16": dict(wavelengths=np.linspace(400, 700, 16)),
+    "davinci-transfer": dict(input_transfer=1, spectral_mode=1),
+}
+
+
+@pytest.mark.parametrize("case", sorted(RENDER_CASES))
+def test_render_matches_oracle(ctx, oracle, case):
+    p = workloads.base_params(64)
+    p.pupil_resolution = 64
+    p.blade_count = 6
+    p.max_kernel_radius_px = 20.0
+    p.set_noll({4: 0.25, 11: -0.15})
+    for k, v in RENDER_CASES[case].items():
+        if k == "wavelengths":
+            p.set_wavelengths(v)
+        else:
+            setattr(p, k, v)
+    img = small_frame(96, 64)
+    if p.input_transfer == _abi.TRANSFER_DAVINCI_INTERMEDIATE:
+        img = workloads.davinci_intermediate_encode(img)
+    got = ctx.render_image(p, img)
+    want = oracle.render_image(p, img)
+    assert rel_l2(got, want) < TOL, rel_l2(got, want)
+
+
+def test_selection_view_bit_exact(ctx, oracle):
+    p = workloads.base_params(64)
+    p.pupil_resolution = 64
+    p.max_kernel_radius_px = 10.0
+    p.debug_view = _abi.VIEW_SELECTION
+    img = small_frame(80, 64, seed=5)
+    got = ctx.render_image(p, img)
+    want = oracle.render_image(p, img)
+    assert np.array_equal(got, want.astype(np.float32))
+
+
+def test_precombined_matches_per_bin(ctx):
+    p = workloads.base_params(64)
+    p.pupil_resolution = 64
+    p.max_kernel_radius_px = 20.0
+    p.set_wavelengths(np.linspace(400, 700, 12))
+    p.spectrum_style = _abi.STYLE_WARM_COOL
+    p.spectrum_force = 0.5
+    img = small_frame(96, 64, seed=2)
+    a = ctx.render_image(p, img)
+    p.conv_mode = _abi.CONV_PRECOMBINED
+    b = ctx.render_image(p, img)
+    assert rel_l2(b, a) < 1e-5
+
+
+def test_c0_frame_matches_oracle(ctx, oracle):
+    """The reference's own CPU case at full size (512x512, P=256, R=1024, K=513)."""
+    p = workloads.c0_params()
+    img = workloads.c0_frame()
+    got = ctx.render_image(p, img)
+    want = oracle.render_image(p, img)
+    assert rel_l2(got, want) < TOL
+    want64 = oracle.render_image(p, img, fp64=True)
+    # stated, not asserted: both FP32 results against the FP64 oracle
+    print("c0 rel L2 vs FP64: gpu %.3g, reference-fp32 %.3g" % (rel_l2(got, want64), rel_l2(want, want64)))

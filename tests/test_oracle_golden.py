@@ -1,0 +1,114 @@
+"""The oracle (FP32 restatement) against the reference's own outputs.
+
+tests/golden/reference_vectors.npz holds outputs of the compiled reference CPU path (written by
+tests/golden/make_golden.py). The restatement reproduces them bit for bit (same float operation
+order, same libm); where the compiled reference is built (oracle/_ref), more cases are compared
+live in test_oracle_reference.py.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from lensdiff_amd import _abi
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz")
+
+
+@pytest.fixture(scope="module")
+def g():
+    return np.load(GOLDEN, allow_pickle=False)
+
+
+def unpack(a) -> _abi.Params:
+    p = _abi.Params()
+    raw = bytes(np.asarray(a, np.uint8))
+    assert len(raw) == ctypes.sizeof(p)
+    ctypes.memmove(ctypes.byref(p), raw, ctypes.sizeof(p))
+    return p
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 5, 6, 7])
+def test_pupil(g, oracle, mode):
+    p = unpack(g["pupil_%d_params" % mode])
+    assert np.array_equal(oracle.pupil_amplitude(p), g["pupil_%d" % mode])
+
+
+def test_phase(g, oracle):
+    assert np.array_equal(oracle.pupil_phase(unpack(g["phase_params"])), g["phase"])
+
+
+@pytest.mark.parametrize("raw", [256, 300])
+def test_raw_psf(g, oracle, raw):
+    amp = g["psf_amp"]
+    assert np.array_equal(oracle.raw_psf(amp, None, raw), g["psf_flat_%d" % raw])
+    assert np.array_equal(oracle.raw_psf(amp, g["psf_phase"], raw), g["psf_phase_%d" % raw])
+
+
+@pytest.mark.parametrize("n", [37, 64, 100])
+def test_fft1d(g, oracle, n):
+    z = g["fft_in_%d" % n]
+    assert np.array_equal(oracle.fft1d(z).astype(np.complex64), g["fft_fwd_%d" % n])
+    assert np.array_equal(oracle.fft1d(z, inverse=True).astype(np.complex64), g["fft_inv_%d" % n])
+
+
+def test_fft2d(g, oracle):
+    assert np.array_equal(oracle.fft2d(g["fft2_in"]).astype(np.complex64), g["fft2_fwd"])
+
+
+def test_first_minimum(g, oracle):
+    assert np.float32(oracle.first_minimum_radius(g["first_min_psf"])) == g["first_min"]
+
+
+@pytest.mark.parametrize("i", [0, 1, 2])
+def test_resample(g, oracle, i):
+    scale, support = g["resample_%d_args" % i]
+    got = oracle.resample_kernel(g["psf_phase_300"], np.float32(scale), int(support))
+    assert np.array_equal(got, g["resample_%d" % i])
+
+
+@pytest.mark.parametrize("key", ["shape_0", "shape_3"])
+def test_shape(g, oracle, key):
+    f, c, s, r = oracle.shape_kernels(unpack(g[key + "_params"]), g["resample_0"])
+    assert np.array_equal(f, g[key + "_full"])
+    assert np.array_equal(c, g[key + "_core"])
+    assert np.array_equal(s, g[key + "_structure"])
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_bank(g, oracle, mode):
+    key = "bank_%d" % mode
+    b = oracle.bank(unpack(g[key + "_params"]))
+    assert b["support"] == int(g[key + "_support"])
+    for i in range(len(b["full"])):
+        for name in ("full", "core", "structure"):
+            assert np.array_equal(b[name][i], g["%s_%d_%s" % (key, i, name)]), (i, name)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("style", [0, 1, 2])
+def test_spectrum(g, oracle, mode, style):
+    p = _abi.default_params()
+    p.spectral_mode = mode
+    p.spectrum_style = style
+    nat, sty = oracle.spectrum_config(p, g["spectrum_%d_%d_wl" % (mode, style)])
+    assert np.array_equal(nat, g["spectrum_%d_%d_nat" % (mode, style)])
+    assert np.array_equal(sty, g["spectrum_%d_%d_sty" % (mode, style)])
+
+
+def test_selection(g, oracle):
+    m, r, d = oracle.selection(unpack(g["selection_params"]), g["selection_img"])
+    assert np.array_equal(m, g["selection_mask"])
+    assert np.array_equal(r, g["selection_redist"])
+    assert np.array_equal(d, g["selection_driver"])
+
+
+def test_convolve(g, oracle):
+    assert np.array_equal(oracle.convolve(g["conv_plane"], g["conv_kernel"]), g["conv_out"])
+
+
+@pytest.mark.parametrize("name", ["mono_physical_augment", "mono_split_preserve", "tristimulus", "spectral9_split"])
+def test_frame(g, oracle, name):
+    got = oracle.render_image(unpack(g["frame_%s_params" % name]), g["frame_in"])
+    assert np.array_equal(got, g["frame_%s" % name])
