@@ -1,15 +1,19 @@
-"""Benchmark: UHD frames/s with a 31-wavelength PSF bank on one MI355X.
+"""Benchmark: UHD frames/s with 31-wavelength PSF banks on one MI355X, served vs plain.
 
-Workload (3840x2160 RGBA float32, 0.05% highlights log-uniform in [1, 5000], P=1024 7-blade
-pupil, Noll 2-21 N(0, 0.2) waves, 31 wavelengths 400-700 nm, K=2047 kernels, R=4092 raw PSFs,
-8192x8192 FFT grid). Inputs are resident in HBM before the timed region. A step is one frame.
+Workload: 3840x2160 RGBA float32 frames (0.05% highlights, log-uniform [1, 5000]); lenses shaped
+like the precision config (P=1024 7-blade pupil, Noll 2-21 N(0, 0.2) waves, 31 wavelengths
+400-700 nm, R=4092 raw PSFs, K=2047 kernels, 8192x8192 FFT grid). The request stream draws 70%
+of its lenses from 8 resident lens sets and 30% fresh ones (a new bank each). Inputs are resident
+in HBM before timing. A step is one frame (one request).
 
-  plain   per-frame path with the reference's semantics: single-entry bank cache (hit after
-          warm-up), kernel spectra recomputed per frame, one convolution per wavelength bin
-  served  (see --mode) frames through the serving loop
+  plain   the per-frame path with the reference's semantics: single-entry bank cache, kernel
+          spectra recomputed per frame, one convolution per wavelength bin (ldh_render_device)
+  served  the request loop: resident banks and pre-combined kernel spectra keyed by lens, 4 slots
+          on their own streams, admission under a 64 GB budget (ldh_submit / ldh_wait)
 
-Prints one JSON line (the driver's contract) with `roofline` (dominant kernel family, HIP-event
-timed on its launch stream) and `cpu_baseline` (the reference CPU path on a bounded sample).
+`value` is served frames/s with the whole stream submitted at once; p50/p99 latency of both paths
+is measured under Poisson arrivals at 0.8x the plain path's sustainable rate. `roofline` is the
+dominant kernel family of the served timed region (HIP events on the slot streams).
 """
 from __future__ import annotations
 
@@ -25,21 +29,33 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "UHD frames/s @31λ bank + p99 render latency, served vs plain; rel L2 ≤1e-4"
+
+
+def lens(seed):
+    from lensdiff_amd import workloads
+    return workloads.spectral_params(2160, 1024, 7, range(2, 22), 0.2, seed, 31, 47.36)
+
+
+def stream(n, rng, fresh_base):
+    """(lens seed, frame index) per request: 70% one of 8 resident lenses, 30% fresh."""
+    out = []
+    for i in range(n):
+        seed = int(rng.integers(10, 18)) if rng.random() < 0.7 else fresh_base + i
+        out.append((seed, i % 2))
+    return out
 
 
 def cpu_baseline(p, w, h, bins):
-    """One bin's convolution and one raw PSF of the workload, on one host core, scaled to a frame."""
+    """One bin's convolution and one raw PSF of the workload on one host core, scaled to a frame."""
     import ctypes as C
-    try:
-        from oracle.oracle import Reference, reference_available
-        if reference_available():
-            ref = Reference()
-            t_conv = ref.lib.ref_time_convolve(w, h, 2047)
-            t_psf = ref.lib.ref_time_raw_psf(C.byref(p))
-            kind = "reference"
-        else:
-            raise FileNotFoundError
-    except Exception:
+    from oracle.oracle import Reference, reference_available
+    if reference_available():
+        ref = Reference()
+        t_conv = ref.lib.ref_time_convolve(w, h, 2047)
+        t_psf = ref.lib.ref_time_raw_psf(C.byref(p))
+        kind = "reference"
+    else:
         from oracle.oracle import Oracle
         o = Oracle()
         plane = np.zeros((h, w), np.float32)
@@ -55,74 +71,132 @@ def cpu_baseline(p, w, h, bins):
         kind = "port"
     frame_s = bins * (t_conv + t_psf)
     return {"value": 1.0 / frame_s, "unit": "frames/s", "cores": 1, "kind": kind,
-            "sample": "1 of %d bin convolutions (8192x8192 grid, %.1f s) + 1 of %d raw PSFs (R=4092, %.1f s), "
-                      "scaled x%d to one frame (%.0f s); needed work only" % (bins, t_conv, bins, t_psf, bins,
-                                                                               frame_s)}
+            "sample": "1 of %d bin convolutions (8192x8192 grid, %.1f s) + 1 of %d raw PSFs (R=4092, %.1f s) "
+                      "of one UHD frame with a fresh 31-wavelength bank, scaled x%d (%.0f s/frame); needed work "
+                      "only" % (bins, t_conv, bins, t_psf, bins, frame_s)}
+
+
+def fifo_latency(arrivals, service):
+    done, out = 0.0, []
+    for a, s in zip(arrivals, service):
+        done = max(done, a) + s
+        out.append(done - a)
+    return np.array(out)
+
+
+def dominant(stats):
+    fams = {k: v for k, v in stats.items() if k.startswith("fft-") and v[0] > 0}
+    name, (launches, total_ms, nbytes) = max(fams.items(), key=lambda kv: kv[1][1])
+    avg_s = total_ms / 1e3 / launches
+    achieved = nbytes / launches / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "launches": launches, "avg_launch_ms": round(avg_s * 1e3, 4), "bytes_per_launch": nbytes / launches}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", choices=["plain", "precombined"], default="plain")
+    ap.add_argument("--steps", type=int, default=40, help="served requests in the timed stream")
+    ap.add_argument("--warmup", type=int, default=8, help="resident lens sets built before timing")
+    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
-              file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    from lensdiff_amd import _abi, workloads
+    from lensdiff_amd import workloads
     from lensdiff_amd.backend import Context
+    from oracle.oracle import rel_l2
 
-    p = workloads.c2_params()
-    if args.mode == "precombined":
-        p.conv_mode = _abi.CONV_PRECOMBINED
-    img = workloads.c2_frame()
-    h, w = img.shape[:2]
-    bins = p.wavelength_count
+    t_bench = time.perf_counter()
+    rng = np.random.default_rng(4)
+    frames = [workloads.c2_frame(), workloads.frame(3840, 2160, 0.0005, seed=1, log_uniform=(1.0, 5000.0))]
+    h, w = frames[0].shape[:2]
     ctx = Context(0)
-    d_src = ctx.alloc(img.nbytes)
-    d_dst = ctx.alloc(img.nbytes)
-    ctx.upload(d_src, img)
+    d_src = []
+    for f in frames:
+        d = ctx.alloc(f.nbytes)
+        ctx.upload(d, f)
+        d_src.append(d)
+    nbytes = frames[0].nbytes
+    lenses = {}
 
-    t0 = time.perf_counter()
-    ctx.render_device(p, d_src, d_dst, w, h)  # builds the bank (single-entry cache)
-    ctx.synchronize()
-    bank_s = time.perf_counter() - t0
-    for _ in range(args.warmup):
-        ctx.render_device(p, d_src, d_dst, w, h)
-    ctx.synchronize()
+    def L(seed):
+        if seed not in lenses:
+            lenses[seed] = lens(seed)
+        return lenses[seed]
 
+    def progress(msg):
+        print("[bench %.1fs] %s" % (time.perf_counter() - t_bench, msg), file=sys.stderr, flush=True)
+
+    progress("inputs resident")
+    # ---- plain: one request at a time, single-entry cache, per-bin convolution ----
+    plain_reqs = stream(args.plain_steps, rng, 1000)
+    d_plain = ctx.alloc(nbytes)
+    ctx.render_device(L(10), d_src[0], d_plain, w, h)  # warm: kernels loaded, one bank built
+    ctx.synchronize()
     ctx.profile_reset()
     ctx.profiling(True)
-    lat = []
-    ctx.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
+    plain_service = []
+    for seed, fi in plain_reqs:
         t0 = time.perf_counter()
-        ctx.render_device(p, d_src, d_dst, w, h)
+        ctx.render_device(L(seed), d_src[fi], d_plain, w, h)
         ctx.synchronize()
-        lat.append(time.perf_counter() - t0)
-    ctx.synchronize()
+        plain_service.append(time.perf_counter() - t0)
+    ctx.profiling(False)
+    plain_stats = ctx.profile_read()
+    plain_fps = len(plain_service) / sum(plain_service)
+
+    progress("plain: %d frames, %.2f frames/s" % (len(plain_service), plain_fps))
+    # ---- served ----
+    ctx.server_start(args.slots, 64 << 30)
+    d_dst = [ctx.alloc(nbytes) for _ in range(args.steps)]
+    for k, seed in enumerate(range(10, 10 + args.warmup)):  # resident lens sets
+        ctx.wait(ctx.submit(L(seed), d_src[0], d_dst[k % len(d_dst)], w, h))
+    served_reqs = stream(args.steps, rng, 2000)
+    ctx.profile_reset()
+    ctx.profiling(True)
+    t_start = time.perf_counter()
+    tickets = [ctx.submit(L(seed), d_src[fi], d_dst[i], w, h) for i, (seed, fi) in enumerate(served_reqs)]
+    sat = [ctx.wait(t) for t in tickets]
     elapsed = time.perf_counter() - t_start
     ctx.profiling(False)
-    stats = ctx.profile_read()
+    served_stats = ctx.profile_read()
+    served_fps = len(tickets) / elapsed
 
-    fams = {k: v for k, v in stats.items() if k.startswith("fft-")}
-    dom = max(fams.items(), key=lambda kv: kv[1][1])
-    launches, total_ms, nbytes = dom[1]
-    avg_s = total_ms / 1e3 / launches
-    achieved = nbytes / launches / avg_s / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom[0],
-                "avg_launch_ms": round(avg_s * 1e3, 4), "bytes_per_launch": nbytes / launches}
+    progress("served: %d frames, %.2f frames/s" % (len(tickets), served_fps))
+    # parity spot check: a served frame against the plain path for the same request
+    seed, fi = served_reqs[0]
+    ctx.render_device(L(seed), d_src[fi], d_plain, w, h)
+    a = np.empty_like(frames[0])
+    b = np.empty_like(frames[0])
+    ctx.download(d_plain, a)
+    ctx.download(d_dst[0], b)
+    served_vs_plain = rel_l2(b, a)
 
-    fps = args.steps / elapsed
+    # ---- latency under Poisson arrivals at 0.8x the plain sustainable rate ----
+    rate = 0.8 * plain_fps
+    lat_reqs = stream(args.steps, rng, 3000)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(lat_reqs)))
+    t0 = time.perf_counter()
+    tickets = []
+    for (seed, fi), at in zip(lat_reqs, arrivals):
+        delay = at - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append(ctx.submit(L(seed), d_src[fi], d_dst[len(tickets)], w, h))
+    served_lat = np.array([ctx.wait(t).latency_ms for t in tickets])
+    sstats = ctx.server_stats()
+    ctx.server_stop()
+    service = [plain_service[i % len(plain_service)] for i in range(len(arrivals))]
+    plain_lat = fifo_latency(arrivals, service) * 1e3
+
     out = {
-        "metric": "UHD frames/s @31-wavelength bank (%s path)" % args.mode,
-        "value": round(fps, 4),
+        "metric": METRIC,
+        "value": round(served_fps, 3),
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -133,20 +207,34 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "uhd-31wl: 3840x2160 RGBA f32, 31 wavelengths 400-700 nm, P=1024, R=4092, "
-                               "K=2047, 8192x8192 grid", "mode": args.mode, "global_batch": 1},
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
-                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
-        "bank_build_s": round(bank_s, 3),
-        "roofline": roofline,
+        "config": {"workload": "uhd-31wl served stream: 3840x2160 RGBA f32, 31 wavelengths 400-700 nm, P=1024, "
+                               "R=4092, K=2047, 8192^2 FFT grid; 70% of requests on 8 resident lens sets, 30% fresh",
+                   "slots": args.slots, "budget_gb": 64},
+        "plain": {"frames_per_s": round(plain_fps, 3), "requests": len(plain_service),
+                  "service_ms_p50": round(float(np.median(plain_service)) * 1e3, 2)},
+        "speedup_served_vs_plain": round(served_fps / plain_fps, 2),
+        "latency_ms": {"arrival_rate_per_s": round(rate, 3),
+                       "served_p50": round(float(np.percentile(served_lat, 50)), 2),
+                       "served_p99": round(float(np.percentile(served_lat, 99)), 2),
+                       "plain_p50": round(float(np.percentile(plain_lat, 50)), 2),
+                       "plain_p99": round(float(np.percentile(plain_lat, 99)), 2)},
+        "served_cache": {"bank_hits": sstats.bank_hits, "bank_misses": sstats.bank_misses,
+                         "spectra_hits": sstats.spectra_hits, "resident_gb": round(sstats.resident_bytes / 2 ** 30, 2),
+                         "evictions": sstats.evictions},
+        "served_vs_plain_rel_l2": served_vs_plain,
+        "roofline": dominant(served_stats),
+        "roofline_plain": dominant(plain_stats),
         "kernel_families": {k: {"launches": v[0], "ms": round(v[1], 3),
-                                "GB/s": round(v[2] / (v[1] / 1e3) / 1e9, 1) if v[1] > 0 else None}
-                            for k, v in stats.items()},
+                                "GB/s": round(v[2] / (v[1] / 1e3) / 1e9, 1) if v[1] > 0 and v[2] > 0 else None}
+                            for k, v in served_stats.items()},
     }
+    progress("latency pass done")
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(p, w, h, bins)
-    ctx.free(d_src)
-    ctx.free(d_dst)
+        progress("cpu baseline (reference, one core)")
+        out["cpu_baseline"] = cpu_baseline(L(10), w, h, 31)
+        progress("cpu baseline done")
+    for d in d_src + d_dst + [d_plain]:
+        ctx.free(d)
     ctx.close()
     print(json.dumps(out))
 

@@ -225,6 +225,37 @@ ldh_status ldh_render(ldh_context* ctx, const ldh_render_request* req, const ldh
 ldh_status ldh_render_device(ldh_context* ctx, const ldh_params* p, const float* d_src,
                              float* d_dst, int32_t width, int32_t height);
 
+/* ---- served form -------------------------------------------------------------------------
+ * No counterpart in the reference (its plugin renders one frame per call with a single-entry
+ * instance cache, LensDiff.cpp:2318-2319). A context can run a request loop: `slots` device
+ * slots (one HIP stream, frame scratch and bank workspace each) take queued render requests in
+ * arrival order as soon as a slot is free and the resident bank/spectra cache fits the memory
+ * budget; banks and their kernel spectra stay resident in HBM keyed by the lens parameters
+ * (LRU eviction of entries no request is using). Frames are device-resident RGBA float32. */
+typedef struct ldh_request_stats {
+    double latency_ms;   /* completion - submission */
+    double queue_ms;     /* start - submission */
+    double service_ms;   /* completion - start */
+    int32_t bank_hit, spectra_hit;
+    ldh_status status;
+    char error[48];
+} ldh_request_stats;
+
+typedef struct ldh_server_stats {
+    int64_t submitted, completed, bank_hits, bank_misses, spectra_hits, spectra_misses, evictions;
+    uint64_t resident_bytes, budget_bytes;
+    int32_t slots, cached_banks;
+} ldh_server_stats;
+
+ldh_status ldh_server_start(ldh_context* ctx, int32_t slots, uint64_t budget_bytes);
+ldh_status ldh_server_stop(ldh_context* ctx);
+ldh_status ldh_submit(ldh_context* ctx, const ldh_params* p, const float* d_src, float* d_dst, int32_t width,
+                      int32_t height, int64_t* ticket);
+/* LDH_ERR_NOT_READY while the request is queued or running. */
+ldh_status ldh_poll(ldh_context* ctx, int64_t ticket, ldh_request_stats* out);
+ldh_status ldh_wait(ldh_context* ctx, int64_t ticket, ldh_request_stats* out);
+ldh_status ldh_server_stats_get(ldh_context* ctx, ldh_server_stats* out);
+
 /* Device memory helpers for callers without a GPU framework. */
 ldh_status ldh_device_alloc(ldh_context* ctx, uint64_t bytes, void** out);
 ldh_status ldh_device_free(ldh_context* ctx, void* ptr);

@@ -164,6 +164,27 @@ class BankInfo(C.Structure):
     ]
 
 
+class RequestStats(C.Structure):
+    _fields_ = [
+        ("latency_ms", C.c_double),
+        ("queue_ms", C.c_double),
+        ("service_ms", C.c_double),
+        ("bank_hit", C.c_int32),
+        ("spectra_hit", C.c_int32),
+        ("status", C.c_int32),
+        ("error", C.c_char * 48),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64), ("completed", C.c_int64), ("bank_hits", C.c_int64),
+        ("bank_misses", C.c_int64), ("spectra_hits", C.c_int64), ("spectra_misses", C.c_int64),
+        ("evictions", C.c_int64), ("resident_bytes", C.c_uint64), ("budget_bytes", C.c_uint64),
+        ("slots", C.c_int32), ("cached_banks", C.c_int32),
+    ]
+
+
 class KernelStat(C.Structure):
     _fields_ = [
         ("name", C.c_char * 48),

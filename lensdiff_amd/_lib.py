@@ -39,6 +39,12 @@ SIGNATURES = {
     "ldh_bank_download": (C.c_int32, [_CTX, _BANK, C.c_int32, C.c_int32, _F, C.c_int64]),
     "ldh_render": (C.c_int32, [_CTX, C.POINTER(_abi.RenderRequest), _P]),
     "ldh_render_device": (C.c_int32, [_CTX, _P, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]),
+    "ldh_server_start": (C.c_int32, [_CTX, C.c_int32, C.c_uint64]),
+    "ldh_server_stop": (C.c_int32, [_CTX]),
+    "ldh_submit": (C.c_int32, [_CTX, _P, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int64)]),
+    "ldh_poll": (C.c_int32, [_CTX, C.c_int64, C.POINTER(_abi.RequestStats)]),
+    "ldh_wait": (C.c_int32, [_CTX, C.c_int64, C.POINTER(_abi.RequestStats)]),
+    "ldh_server_stats_get": (C.c_int32, [_CTX, C.POINTER(_abi.ServerStats)]),
     "ldh_device_alloc": (C.c_int32, [_CTX, C.c_uint64, C.POINTER(C.c_void_p)]),
     "ldh_device_free": (C.c_int32, [_CTX, C.c_void_p]),
     "ldh_memcpy_h2d": (C.c_int32, [_CTX, C.c_void_p, C.c_void_p, C.c_uint64]),

@@ -171,6 +171,41 @@ class Context:
     def render_device(self, p, d_src: int, d_dst: int, w: int, h: int):
         self._check(self.lib.ldh_render_device(self.h, C.byref(p), C.c_void_p(d_src), C.c_void_p(d_dst), w, h))
 
+    # ---- served form --------------------------------------------------------------------------
+    def server_start(self, slots: int = 4, budget: int = 64 << 30):
+        self._check(self.lib.ldh_server_start(self.h, slots, budget))
+
+    def server_stop(self):
+        self._check(self.lib.ldh_server_stop(self.h))
+
+    def submit(self, p, d_src: int, d_dst: int, w: int, h: int) -> int:
+        t = C.c_int64()
+        st = self.lib.ldh_submit(self.h, C.byref(p), C.c_void_p(d_src), C.c_void_p(d_dst), w, h, C.byref(t))
+        if st != 0:
+            raise LensDiffError(st, self.lib.ldh_status_string(st).decode())
+        return t.value
+
+    def wait(self, ticket: int) -> _abi.RequestStats:
+        s = _abi.RequestStats()
+        st = self.lib.ldh_wait(self.h, ticket, C.byref(s))
+        if st != 0:
+            raise LensDiffError(st, s.error.decode() or self.lib.ldh_status_string(st).decode())
+        return s
+
+    def poll(self, ticket: int):
+        s = _abi.RequestStats()
+        st = self.lib.ldh_poll(self.h, ticket, C.byref(s))
+        if st == _abi.LDH_ERR_NOT_READY:
+            return None
+        if st != 0:
+            raise LensDiffError(st, s.error.decode() or self.lib.ldh_status_string(st).decode())
+        return s
+
+    def server_stats(self) -> _abi.ServerStats:
+        s = _abi.ServerStats()
+        self._check(self.lib.ldh_server_stats_get(self.h, C.byref(s)))
+        return s
+
     def synchronize(self):
         self._check(self.lib.ldh_synchronize(self.h))
 

@@ -15,6 +15,7 @@
 #include "frame.h"
 #include "host_params.h"
 #include "profiler.h"
+#include "server.h"
 
 struct ldh_bank {
     std::shared_ptr<ldh::DeviceBank> bank;
@@ -40,6 +41,8 @@ struct ldh_context {
     float* pinned = nullptr;
     size_t pinned_bytes = 0;
     ldh::DeviceBuffer dsrc, ddst, scratch, scratch2;
+    // served request loop (created by ldh_server_start)
+    std::unique_ptr<ldh::Server> server;
 };
 
 namespace {
@@ -178,6 +181,7 @@ ldh_status ldh_context_destroy(ldh_context* ctx) {
     {
         std::lock_guard<std::mutex> lock(ctx->mu);
         (void)hipSetDevice(ctx->device);
+        ctx->server.reset();
         if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
         if (ctx->pinned) (void)hipHostFree(ctx->pinned);
         ctx->pinned = nullptr;
@@ -452,6 +456,70 @@ ldh_status ldh_render(ldh_context* ctx, const ldh_render_request* req, const ldh
     });
 }
 
+ldh_status ldh_server_start(ldh_context* ctx, int32_t slots, uint64_t budget) {
+    return guarded(ctx, [&] {
+        require(slots >= 1 && slots <= 16, LDH_ERR_INVALID_ARGUMENT, "invalid-slot-count");
+        require(!ctx->server, LDH_ERR_INVALID_ARGUMENT, "server-already-running");
+        ctx->server = std::make_unique<ldh::Server>(ctx->fft, ctx->device, slots, budget ? budget : ctx->budget);
+    });
+}
+
+ldh_status ldh_server_stop(ldh_context* ctx) {
+    return guarded(ctx, [&] { ctx->server.reset(); });
+}
+
+// The served calls take the server pointer under the context lock and then wait without it, so
+// submissions and polls from other threads are not blocked by a waiting caller.
+static ldh::Server* server_of(ldh_context* ctx) {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    return ctx->server.get();
+}
+
+ldh_status ldh_submit(ldh_context* ctx, const ldh_params* p, const float* d_src, float* d_dst, int32_t w, int32_t h,
+                      int64_t* ticket) {
+    if (!ctx || !p || !ticket) return LDH_ERR_INVALID_ARGUMENT;
+    if (!d_src || !d_dst) return LDH_ERR_MISSING_BUFFER;
+    if (w <= 0 || h <= 0) return LDH_ERR_INVALID_BOUNDS;
+    ldh::Server* s = server_of(ctx);
+    if (!s) return LDH_ERR_INVALID_ARGUMENT;
+    *ticket = s->submit(*p, reinterpret_cast<const float4*>(d_src), reinterpret_cast<float4*>(d_dst), w, h);
+    return LDH_OK;
+}
+
+static ldh_status poll_impl(ldh_context* ctx, int64_t ticket, ldh_request_stats* out, bool wait) {
+    if (!ctx) return LDH_ERR_INVALID_ARGUMENT;
+    ldh::Server* s = server_of(ctx);
+    if (!s) return LDH_ERR_INVALID_ARGUMENT;
+    ldh::ServedRequest r;
+    try {
+        if (!s->poll(ticket, &r, wait)) return LDH_ERR_NOT_READY;
+    } catch (const ldh::Error& e) {
+        return e.status;
+    }
+    if (out) {
+        std::memset(out, 0, sizeof(*out));
+        out->latency_ms = r.t_done - r.t_submit;
+        out->queue_ms = r.t_start - r.t_submit;
+        out->service_ms = r.t_done - r.t_start;
+        out->bank_hit = r.bank_hit ? 1 : 0;
+        out->spectra_hit = r.spectra_hit ? 1 : 0;
+        out->status = r.status;
+        std::strncpy(out->error, r.error.c_str(), sizeof(out->error) - 1);
+    }
+    return r.status;
+}
+
+ldh_status ldh_poll(ldh_context* ctx, int64_t ticket, ldh_request_stats* out) { return poll_impl(ctx, ticket, out, false); }
+ldh_status ldh_wait(ldh_context* ctx, int64_t ticket, ldh_request_stats* out) { return poll_impl(ctx, ticket, out, true); }
+
+ldh_status ldh_server_stats_get(ldh_context* ctx, ldh_server_stats* out) {
+    if (!ctx || !out) return LDH_ERR_INVALID_ARGUMENT;
+    ldh::Server* s = server_of(ctx);
+    if (!s) return LDH_ERR_INVALID_ARGUMENT;
+    *out = s->stats();
+    return LDH_OK;
+}
+
 ldh_status ldh_device_alloc(ldh_context* ctx, uint64_t bytes, void** out) {
     return guarded(ctx, [&] {
         require(out != nullptr, LDH_ERR_INVALID_ARGUMENT, "missing-argument");
@@ -475,12 +543,17 @@ ldh_status ldh_memcpy_d2h(ldh_context* ctx, void* dst, const void* src, uint64_t
 }
 
 ldh_status ldh_profiling_enable(ldh_context* ctx, int32_t enable) {
-    return guarded(ctx, [&] { ctx->prof.enabled = enable != 0; });
+    return guarded(ctx, [&] {
+        ctx->prof.enabled = enable != 0;
+        if (ctx->server) ctx->server->set_profiling(enable != 0);
+    });
 }
 int32_t ldh_profiling_read(ldh_context* ctx, ldh_kernel_stat* out, int32_t capacity) {
     int32_t n = 0;
     guarded(ctx, [&] {
         sync(ctx, "profiling-read");
+        ctx->prof.resolve();
+        if (ctx->server) ctx->server->merge_profiles(ctx->prof.stats);
         n = ctx->prof.read(out, capacity);
     });
     return n;

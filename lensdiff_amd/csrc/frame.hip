@@ -445,21 +445,24 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
         // kernels of this set
         std::vector<const float*> ks;
         const int64_t kk = int64_t(bank.kernel) * bank.kernel;
-        if (!pl.spectral) {
+        const bool use_cache = cached != nullptr && si == 0 && pl.sets.size() == 1;
+        if (use_cache) {
+            // the cached spectra stand for the kernels; only their count matters here
+            ks.assign(size_t(cached->kernels), nullptr);
+        } else if (!pl.spectral) {
             ks.push_back(bank.kernel_ptr(set, 0));
         } else if (mode == EffectMode::PerBin) {
             for (int b = 0; b < n; ++b) ks.push_back(bank.kernel_ptr(set, b));
         } else {
-            if (!cached) {
-                fw.combined.reset(sizeof(float) * kk * 6);
-                combine_kernels(s, bank.kernel_ptr(set, 0), n, bank.kernel, cfg, with_style, fw.combined.as<float>(),
-                                fw.kspec_tmp);
-            }
+            fw.combined.reset(sizeof(float) * kk * 6);
+            combine_kernels(s, bank.kernel_ptr(set, 0), n, bank.kernel, cfg, with_style, fw.combined.as<float>(),
+                            fw.kspec_tmp);
             const int nk = with_style ? 6 : 3;
             for (int r = 0; r < nk; ++r) ks.push_back(fw.combined.as<float>() + r * kk);
         }
         const int pairs = pl.spectral ? int((ks.size() + 1) / 2) : 1;
-        const bool use_cache = cached != nullptr && si == 0 && pl.sets.size() == 1;
+        if (use_cache && (cached->grid.pw != g.pw || cached->grid.ph != g.ph || cached->pairs != pairs))
+            fail(LDH_ERR_INVALID_ARGUMENT, "cached-spectra-mismatch");
         for (int pi = 0; pi < pairs; ++pi) {
             // kernel spectrum of this pair
             const void* kspec;

@@ -1,0 +1,98 @@
+// Served form: a request loop over several device slots (one HIP stream, frame scratch and bank
+// workspace each) that keeps PSF banks and their kernel spectra resident in HBM, keyed by the
+// lens parameters, admits requests under a memory budget and lets requests that arrive while
+// others run proceed on free slots, so their work overlaps on the device.
+#pragma once
+
+#include <condition_variable>
+#include <deque>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "bank.h"
+#include "frame.h"
+#include "host_params.h"
+#include "profiler.h"
+
+namespace ldh {
+
+struct ServedRequest {
+    int64_t ticket = 0;
+    ldh_params params{};
+    const float4* src = nullptr;
+    float4* dst = nullptr;
+    int w = 0, h = 0;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    bool done = false;
+    bool bank_hit = false, spectra_hit = false;
+    ldh_status status = LDH_OK;
+    std::string error;
+};
+
+struct SpectraKey {
+    int pw, ph, kernels, fp16;
+    bool operator==(const SpectraKey& o) const {
+        return pw == o.pw && ph == o.ph && kernels == o.kernels && fp16 == o.fp16;
+    }
+};
+
+struct CacheEntry {
+    BankKey key;
+    std::shared_ptr<DeviceBank> bank;
+    DeviceBuffer combined;  // precombined natural (+ style) kernels, float[6][K*K]
+    bool combined_style = false;
+    struct Spectra {
+        SpectraKey key;
+        std::shared_ptr<KernelSpectra> spec;
+        bool ready = false;
+    };
+    std::vector<Spectra> spectra;
+    bool ready = false;     // bank (and combined kernels) built
+    bool failed = false;
+    std::string error;
+    int pins = 0;
+    uint64_t last_use = 0;
+    size_t bytes() const;
+};
+
+class Server {
+public:
+    Server(const FftTables& fft, int device, int slots, uint64_t budget);
+    ~Server();
+    int64_t submit(const ldh_params& p, const float4* src, float4* dst, int w, int h);
+    // true when done (fills *out)
+    bool poll(int64_t ticket, ServedRequest* out, bool wait);
+    ldh_server_stats stats();
+
+private:
+    struct Slot;
+    void worker(Slot* s);
+    ServedRequest* admit(Slot* s, std::shared_ptr<CacheEntry>* entry, bool* builder);
+    size_t resident_bytes_locked() const;
+    bool evict_locked(size_t need);
+
+    const FftTables& fft_;
+    int device_;
+    uint64_t budget_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    bool stopping_ = false;
+    int64_t next_ticket_ = 1;
+    uint64_t clock_ = 0;
+    std::deque<ServedRequest*> queue_;
+    std::map<int64_t, std::unique_ptr<ServedRequest>> requests_;
+    std::list<std::shared_ptr<CacheEntry>> cache_;
+    std::vector<std::unique_ptr<Slot>> slots_;
+    ldh_server_stats stats_{};
+
+public:
+    // Per-slot kernel timing (HIP events on each slot's stream), merged on read.
+    void set_profiling(bool on);
+    void merge_profiles(std::map<std::string, Profiler::Stat>& into);
+};
+
+}  // namespace ldh

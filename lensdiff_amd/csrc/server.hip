@@ -1,0 +1,343 @@
+// The served request loop (see server.h). Each slot is a worker thread with its own HIP stream,
+// frame scratch and bank workspace; the FFT tables are shared read-only. A request's bank and
+// the kernel spectra for its FFT grid come from the resident cache; the first request needing
+// an entry builds it on its own stream while requests for other entries keep running on the
+// other slots, and later requests for the same entry wait for it to be ready.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "server.h"
+
+namespace ldh {
+
+namespace {
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+bool spectral_fullkernel(const ldh_params& p) {
+    return is_spectral(p) && p.look_mode == LDH_LOOK_PHYSICAL && p.debug_view != LDH_VIEW_CORE &&
+           p.debug_view != LDH_VIEW_STRUCTURE;
+}
+bool with_style(const ldh_params& p) {
+    return p.spectrum_style != LDH_STYLE_NATURAL && std::max(0.0, std::min(p.spectrum_force, 1.0)) > 0.0;
+}
+// Real kernels whose spectra serve this request: precombined RGB (3, or 6 with style) for a
+// spectral bank, the single full kernel for a mono bank; 0 = not cacheable (split look).
+int cached_kernel_count(const ldh_params& p) {
+    if (spectral_fullkernel(p)) return with_style(p) ? 6 : 3;
+    if (!is_spectral(p) && p.look_mode == LDH_LOOK_PHYSICAL && p.debug_view != LDH_VIEW_CORE &&
+        p.debug_view != LDH_VIEW_STRUCTURE)
+        return 1;
+    return 0;
+}
+
+size_t bank_bytes_estimate(const ldh_params& p) {
+    const Sizes s = resolve_sizes(p);
+    const size_t kk = size_t(2 * s.max_radius + 1) * size_t(2 * s.max_radius + 1);
+    return kk * 4 * (3 * wavelength_list(p).size() + 6);
+}
+size_t spectra_bytes(const Grid& g, int kernels, bool fp16) {
+    return g.cells() * size_t((kernels + 1) / 2) * (fp16 ? 4 : 8);
+}
+}  // namespace
+
+size_t CacheEntry::bytes() const {
+    size_t b = (bank ? bank->bytes() : 0) + combined.bytes();
+    for (const auto& s : spectra)
+        if (s.spec) b += s.spec->bytes();
+    return b;
+}
+
+struct Server::Slot {
+    int index = 0;
+    hipStream_t stream = nullptr;
+    FrameWork fw;
+    BankWorkspace bw;
+    DeviceBuffer tmp;
+    Profiler prof;
+    std::mutex prof_mu;
+    std::thread thread;
+};
+
+void Server::set_profiling(bool on) {
+    for (auto& s : slots_) {
+        std::lock_guard<std::mutex> lk(s->prof_mu);
+        s->prof.enabled = on;
+    }
+}
+
+void Server::merge_profiles(std::map<std::string, Profiler::Stat>& into) {
+    for (auto& s : slots_) {
+        std::lock_guard<std::mutex> lk(s->prof_mu);
+        (void)hipStreamSynchronize(s->stream);
+        s->prof.resolve();
+        for (const auto& kv : s->prof.stats) {
+            Profiler::Stat& d = into[kv.first];
+            d.launches += kv.second.launches;
+            d.ms += kv.second.ms;
+            d.bytes += kv.second.bytes;
+        }
+        s->prof.stats.clear();
+    }
+}
+
+Server::Server(const FftTables& fft, int device, int slots, uint64_t budget)
+    : fft_(fft), device_(device), budget_(budget) {
+    stats_.slots = slots;
+    stats_.budget_bytes = budget;
+    for (int i = 0; i < slots; ++i) {
+        auto s = std::make_unique<Slot>();
+        s->index = i;
+        LDH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "server-stream");
+        s->prof.stream = s->stream;
+        slots_.push_back(std::move(s));
+    }
+    for (auto& s : slots_) {
+        Slot* sp = s.get();
+        sp->thread = std::thread([this, sp] { worker(sp); });
+    }
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stopping_ = true;
+    }
+    cv_.notify_all();
+    for (auto& s : slots_) {
+        if (s->thread.joinable()) s->thread.join();
+        (void)hipStreamSynchronize(s->stream);
+    }
+    for (auto& s : slots_) {
+        s->fw = FrameWork();
+        (void)hipStreamDestroy(s->stream);
+    }
+}
+
+int64_t Server::submit(const ldh_params& p, const float4* src, float4* dst, int w, int h) {
+    auto r = std::make_unique<ServedRequest>();
+    r->params = p;
+    r->src = src;
+    r->dst = dst;
+    r->w = w;
+    r->h = h;
+    r->t_submit = now_ms();
+    std::lock_guard<std::mutex> lk(mu_);
+    r->ticket = next_ticket_++;
+    ServedRequest* raw = r.get();
+    requests_[raw->ticket] = std::move(r);
+    queue_.push_back(raw);
+    stats_.submitted += 1;
+    cv_.notify_all();
+    return raw->ticket;
+}
+
+bool Server::poll(int64_t ticket, ServedRequest* out, bool wait) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto it = requests_.find(ticket);
+    if (it == requests_.end()) throw Error{LDH_ERR_INVALID_ARGUMENT, "unknown-ticket"};
+    if (wait) cv_.wait(lk, [&] { return it->second->done; });
+    if (!it->second->done) return false;
+    *out = *it->second;
+    requests_.erase(it);
+    return true;
+}
+
+ldh_server_stats Server::stats() {
+    std::lock_guard<std::mutex> lk(mu_);
+    ldh_server_stats s = stats_;
+    s.resident_bytes = resident_bytes_locked();
+    s.cached_banks = int32_t(cache_.size());
+    return s;
+}
+
+size_t Server::resident_bytes_locked() const {
+    size_t b = 0;
+    for (const auto& e : cache_) b += e->bytes();
+    return b;
+}
+
+bool Server::evict_locked(size_t need) {
+    size_t resident = resident_bytes_locked();
+    for (auto it = cache_.end(); resident + need > budget_ && it != cache_.begin();) {
+        --it;
+        if ((*it)->pins == 0) {
+            resident -= (*it)->bytes();
+            it = cache_.erase(it);
+            stats_.evictions += 1;
+        }
+    }
+    return resident + need <= budget_;
+}
+
+// Takes the oldest queued request once its cache needs fit the budget. Pins its cache entry
+// (creating it, with the caller as builder, when absent).
+ServedRequest* Server::admit(Slot* s, std::shared_ptr<CacheEntry>* entry, bool* builder) {
+    std::unique_lock<std::mutex> lk(mu_);
+    while (true) {
+        if (stopping_) return nullptr;
+        if (!queue_.empty()) {
+            ServedRequest* r = queue_.front();
+            const BankKey key = make_bank_key(r->params);
+            std::shared_ptr<CacheEntry> e;
+            for (auto it = cache_.begin(); it != cache_.end(); ++it)
+                if ((*it)->key == key) {
+                    e = *it;
+                    cache_.splice(cache_.begin(), cache_, it);  // most recently used first
+                    break;
+                }
+            size_t need = e ? 0 : bank_bytes_estimate(r->params);
+            const int nk = cached_kernel_count(r->params);
+            if (nk > 0) {
+                const int support = e && e->ready ? e->bank->support : resolve_sizes(r->params).max_radius;
+                need += spectra_bytes(make_grid(r->w, r->h, support), nk, r->params.spectra_fp16 != 0);
+            }
+            bool in_flight = false;
+            for (const auto& c : cache_) in_flight = in_flight || c->pins > 0;
+            if (evict_locked(need) || !in_flight) {
+                queue_.pop_front();
+                *builder = !e;
+                if (!e) {
+                    e = std::make_shared<CacheEntry>();
+                    e->key = key;
+                    cache_.push_front(e);
+                    stats_.bank_misses += 1;
+                } else {
+                    stats_.bank_hits += 1;
+                    r->bank_hit = true;
+                }
+                e->pins += 1;
+                e->last_use = ++clock_;
+                *entry = e;
+                r->t_start = now_ms();
+                (void)s;
+                return r;
+            }
+        }
+        cv_.wait(lk);
+    }
+}
+
+void Server::worker(Slot* s) {
+    (void)hipSetDevice(device_);
+    while (true) {
+        std::shared_ptr<CacheEntry> e;
+        bool builder = false;
+        ServedRequest* r = admit(s, &e, &builder);
+        if (!r) return;
+        std::unique_lock<std::mutex> prof_lock(s->prof_mu);
+        ProfilerBinding bind(s->prof);
+        try {
+            const std::string why = unsupported_reason(r->params);
+            if (!why.empty()) throw Error{LDH_ERR_UNSUPPORTED, why};
+            if (builder) {
+                try {
+                    std::shared_ptr<DeviceBank> bank(build_bank(s->stream, fft_, s->bw, r->params).release());
+                    bank->base.release();  // only the shaped kernels are needed to render
+                    if (spectral_fullkernel(r->params)) {
+                        const bool sty = with_style(r->params);
+                        const SpectrumConfig cfg = build_spectrum_config(r->params, bank->wavelengths);
+                        e->combined.reset(sizeof(float) * size_t(bank->kernel) * bank->kernel * 6);
+                        combine_kernels(s->stream, bank->kernel_ptr(LDH_KERNEL_FULL, 0), bank->bins, bank->kernel, cfg,
+                                        sty, e->combined.as<float>(), s->tmp);
+                        e->combined_style = sty;
+                    }
+                    LDH_HIP(hipStreamSynchronize(s->stream), "served-bank");
+                    std::lock_guard<std::mutex> lk(mu_);
+                    e->bank = bank;
+                    e->ready = true;
+                } catch (const Error& err) {
+                    std::lock_guard<std::mutex> lk(mu_);
+                    e->failed = true;
+                    e->error = err.tag;
+                    cache_.remove(e);
+                    cv_.notify_all();
+                    throw;
+                }
+                cv_.notify_all();
+            } else {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return e->ready || e->failed || stopping_; });
+                if (!e->ready) throw Error{LDH_ERR_DEVICE, e->failed ? e->error : "server-stopping"};
+            }
+            const DeviceBank& bank = *e->bank;
+            const int nk = cached_kernel_count(r->params);
+            std::shared_ptr<KernelSpectra> spec;
+            if (nk > 0 && (nk == 1 || e->combined_style == with_style(r->params))) {
+                const Grid g = make_grid(r->w, r->h, bank.support);
+                const SpectraKey sk{g.pw, g.ph, nk, r->params.spectra_fp16 != 0 ? 1 : 0};
+                bool mine = false;
+                {
+                    std::unique_lock<std::mutex> lk(mu_);
+                    CacheEntry::Spectra* found = nullptr;
+                    for (auto& sp : e->spectra)
+                        if (sp.key == sk) found = &sp;
+                    if (!found) {
+                        e->spectra.push_back({sk, std::make_shared<KernelSpectra>(), false});
+                        found = &e->spectra.back();
+                        mine = true;
+                        stats_.spectra_misses += 1;
+                    } else {
+                        stats_.spectra_hits += 1;
+                        r->spectra_hit = true;
+                    }
+                    spec = found->spec;
+                    if (!mine) cv_.wait(lk, [&] {
+                        for (auto& sp : e->spectra)
+                            if (sp.key == sk) return sp.ready;
+                        return true;
+                    });
+                }
+                if (mine) {
+                    std::vector<const float*> ks;
+                    if (nk == 1) {
+                        ks.push_back(bank.kernel_ptr(LDH_KERNEL_FULL, 0));
+                    } else {
+                        const size_t kk = size_t(bank.kernel) * bank.kernel;
+                        for (int i = 0; i < nk; ++i) ks.push_back(e->combined.as<float>() + kk * i);
+                    }
+                    build_kernel_spectra(s->stream, fft_, ks, bank.kernel, bank.support, g, sk.fp16 != 0, *spec, s->tmp);
+                    LDH_HIP(hipStreamSynchronize(s->stream), "served-spectra");
+                    std::lock_guard<std::mutex> lk(mu_);
+                    for (auto& sp : e->spectra)
+                        if (sp.key == sk) sp.ready = true;
+                    cv_.notify_all();
+                }
+            }
+            ldh_params p = r->params;
+            if (spec && nk > 1) {
+                p.conv_mode = LDH_CONV_PRECOMBINED;
+                s->fw.combined.reset(0);  // cached spectra carry the combined kernels
+            }
+            render_frame(s->stream, fft_, s->fw, p, bank, r->src, r->dst, r->w, r->h,
+                         spec && nk > 1 ? EffectMode::Precombined : EffectMode::PerBin, spec.get());
+            LDH_HIP(hipStreamSynchronize(s->stream), "served-frame");
+            r->status = LDH_OK;
+        } catch (const Error& err) {
+            r->status = err.status;
+            r->error = err.tag;
+            (void)hipGetLastError();
+        } catch (...) {
+            r->status = LDH_ERR_DEVICE;
+            r->error = "internal-error";
+        }
+        prof_lock.unlock();
+        std::lock_guard<std::mutex> lk(mu_);
+        if (builder && !e->ready && !e->failed) {
+            // the entry this request was to build will never be ready: release its waiters
+            e->failed = true;
+            e->error = r->error;
+            cache_.remove(e);
+        }
+        r->t_done = now_ms();
+        r->done = true;
+        e->pins -= 1;
+        stats_.completed += 1;
+        cv_.notify_all();
+    }
+}
+
+}  // namespace ldh

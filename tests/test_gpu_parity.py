@@ -85,7 +85,9 @@ def test_raw_psf(ctx, oracle, pupil, raw):
     got = ctx.raw_psf(amp, ph, raw)
     want = oracle.raw_psf(amp, ph, raw)
     assert rel_l2(got, want) < TOL
-    assert abs(float(got.astype(np.float64).sum()) - 1.0) < 1e-3
+    # unit sum in the reference's sequential FP32 sense: the exact sum can sit 0.3% off 1
+    gs, ws = float(got.astype(np.float64).sum()), float(want.astype(np.float64).sum())
+    assert abs(gs - ws) / ws < 1e-5, (gs, ws)
 
 
 def test_first_minimum_radius(ctx, oracle):
