@@ -268,7 +268,7 @@ ldh_status ldh_resample_kernel(ldh_context* ctx, const float* psf, int32_t R, fl
             tot(sizeof(float));
         h2d(ctx, d.get(), psf, sizeof(float) * size_t(R) * R);
         h2d(ctx, sc.get(), &scale, sizeof(float));
-        ldh::resample_kernels(ctx->stream, d.as<float>(), R, sc.as<float>(), support, 1, k.as<float>());
+        ldh::resample_kernels(ctx->stream, d.as<float>(), R, sc.as<float>(), nullptr, support, 1, k.as<float>());
         ctx->bw.seqsum.normalize(ctx->stream, k.as<float>(), int64_t(K) * K, int64_t(K) * K, 1, tot.as<float>());
         d2h(ctx, out, k.get(), sizeof(float) * size_t(K) * K);
     });

@@ -35,7 +35,7 @@ struct DeviceBank {
 struct BankWorkspace {
     PsfPlan plan;
     SeqSum seqsum;
-    DeviceBuffer amp, phase, coefs, raw, totals, scales, mean, shaped, structure, ring_sum, ring_cnt, ring_peak,
+    DeviceBuffer amp, phase, coefs, raw, totals, base_totals, scales, mean, shaped, structure, ring_sum, ring_cnt, ring_peak,
         radius, ref_amp, ref_phase, ref_raw, ref_total;
     // first-zero radius of the clear-circle PSF per (P, R) (GetLensDiffReferenceRawPsfCached)
     std::vector<std::pair<std::pair<int, int>, float>> first_zero;
@@ -44,8 +44,9 @@ struct BankWorkspace {
 float first_minimum_from_profile(const std::vector<float>& profile, int size);
 
 // Steps of the build, exposed for the parity entry points.
-void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_scales, int support, int count,
-                      float* out);
+// d_raw_totals (nullable): per-PSF sums of raw PSFs that are not yet normalised.
+void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_scales, const float* d_raw_totals,
+                      int support, int count, float* out);
 // Shapes `count` base kernels (count x K*K) into full/core/structure (3 x count x K*K) and
 // radius[count] (device). `max_radius` as LensDiffCpuReference.cpp:1633.
 void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, int K, int count, double anisotropy,

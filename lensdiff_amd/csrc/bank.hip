@@ -17,21 +17,26 @@ namespace ldh {
 
 namespace {
 
-__device__ __forceinline__ float bilinear_sq(const float* __restrict__ img, int size, float x, float y) {
+// Bilinear sample of img * norm: each texel is scaled before interpolation, as when the image
+// was normalised in place first.
+__device__ __forceinline__ float bilinear_sq(const float* __restrict__ img, int size, float x, float y, float norm) {
     if (x < 0.0f || y < 0.0f || x > float(size - 1) || y > float(size - 1)) return 0.0f;
     const int x0 = int(floorf(x)), y0 = int(floorf(y));
     const int x1 = min(x0 + 1, size - 1), y1 = min(y0 + 1, size - 1);
     const float tx = x - float(x0), ty = y - float(y0);
-    const float v00 = img[size_t(y0) * size + x0], v10 = img[size_t(y0) * size + x1];
-    const float v01 = img[size_t(y1) * size + x0], v11 = img[size_t(y1) * size + x1];
+    const float v00 = img[size_t(y0) * size + x0] * norm, v10 = img[size_t(y0) * size + x1] * norm;
+    const float v01 = img[size_t(y1) * size + x0] * norm, v11 = img[size_t(y1) * size + x1] * norm;
     const float a = v00 + (v10 - v00) * tx;
     const float b = v01 + (v11 - v01) * tx;
     return a + (b - a) * ty;
 }
 
-__global__ void resample_kernel(const float* __restrict__ raw, int R, const float* __restrict__ scales, int support,
-                                float* __restrict__ out) {
+__global__ void resample_kernel(const float* __restrict__ raw, int R, const float* __restrict__ scales,
+                                const float* __restrict__ raw_totals, int support, float* __restrict__ out) {
     const int lam = blockIdx.y;
+    // raw_totals: the raw PSFs are not yet normalised; scale texels by 1/total (total > 0)
+    float norm = 1.0f;
+    if (raw_totals != nullptr && raw_totals[lam] > 0.0f) norm = 1.0f / raw_totals[lam];
     const int K = 2 * support + 1;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= K * K) return;
@@ -44,7 +49,7 @@ __global__ void resample_kernel(const float* __restrict__ raw, int R, const floa
     const float sy = rc + (float(y) - kc) * inv;
     float v;
     if (inv <= 1.0f) {
-        v = bilinear_sq(img, R, sx, sy);
+        v = bilinear_sq(img, R, sx, sy, norm);
     } else {
         const int taps = min(max(int(ceilf(inv)), 2), 6);
         const float step = inv / float(taps);
@@ -53,7 +58,7 @@ __global__ void resample_kernel(const float* __restrict__ raw, int R, const floa
             const float oy = (float(ty) + 0.5f) * step - inv * 0.5f;
             for (int tx = 0; tx < taps; ++tx) {
                 const float ox = (float(tx) + 0.5f) * step - inv * 0.5f;
-                sum += bilinear_sq(img, R, sx + ox, sy + oy);
+                sum += bilinear_sq(img, R, sx + ox, sy + oy, norm);
             }
         }
         v = sum / float(taps * taps);
@@ -224,10 +229,11 @@ void ring_stats(hipStream_t s, const float* img, int size, float center, int rma
 
 }  // namespace
 
-void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_scales, int support, int count,
-                      float* out) {
+void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_scales, const float* d_raw_totals,
+                      int support, int count, float* out) {
     const int K = 2 * support + 1;
-    hipLaunchKernelGGL(resample_kernel, grid1(int64_t(K) * K, count), dim3(256), 0, s, raw, R, d_scales, support, out);
+    hipLaunchKernelGGL(resample_kernel, grid1(int64_t(K) * K, count), dim3(256), 0, s, raw, R, d_scales, d_raw_totals,
+                       support, out);
     LDH_CHECK_LAUNCH("resample");
 }
 
@@ -409,12 +415,14 @@ std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWo
     w.plan.prepare(s, P, R);
     w.raw.reset(sizeof(float) * size_t(R) * R * L);
     w.totals.reset(sizeof(float) * L);
-    w.plan.raw_psfs(s, t, w.amp.as<float>(), w.phase.as<float>(), L, w.raw.as<float>(), w.totals.as<float>(), w.seqsum);
+    // the unit-sum scale of each raw PSF is applied inside the resample
+    w.plan.raw_psfs(s, t, w.amp.as<float>(), w.phase.as<float>(), L, w.raw.as<float>(), w.totals.as<float>(), w.seqsum,
+                    false);
     const int64_t kk = int64_t(K) * K;
     bank->base.reset(sizeof(float) * kk * L);
-    resample_kernels(s, w.raw.as<float>(), R, w.scales.as<float>(), r, L, bank->base.as<float>());
-    w.totals.reset(sizeof(float) * L);
-    w.seqsum.normalize(s, bank->base.as<float>(), kk, kk, L, w.totals.as<float>());
+    resample_kernels(s, w.raw.as<float>(), R, w.scales.as<float>(), w.totals.as<float>(), r, L, bank->base.as<float>());
+    w.base_totals.reset(sizeof(float) * L);
+    w.seqsum.normalize(s, bank->base.as<float>(), kk, kk, L, w.base_totals.as<float>());
 
     // shaping into full / core / structure
     bank->kernels.reset(sizeof(float) * kk * L * 3);

@@ -14,9 +14,10 @@ class PsfPlan {
 public:
     void prepare(hipStream_t s, int pupil, int raw);
     // `count` raw PSFs from one amplitude (P*P) and `count` phase maps (count*P*P) into
-    // out (count*R*R), each normalised to unit sequential-FP32 sum; totals: device[count].
+    // out (count*R*R); totals: device[count], their sequential-FP32 sums. With `normalize`
+    // each PSF is scaled to unit sum, else left for the consumer to scale by 1/total.
     void raw_psfs(hipStream_t s, const FftTables& t, const float* amp, const float* phase, int count,
-                  float* out, float* totals, SeqSum& seqsum);
+                  float* out, float* totals, SeqSum& seqsum, bool normalize = true);
     int pupil() const { return pupil_; }
     int raw() const { return raw_; }
     int bluestein_length() const { return m_; }

@@ -244,7 +244,7 @@ void pupil_phase(hipStream_t s, const PhaseCoef* d_coefs, int n, int count, floa
 }
 
 void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, const float* phase, int count,
-                       float* out, float* totals, SeqSum& seqsum) {
+                       float* out, float* totals, SeqSum& seqsum, bool normalize) {
     const int P = pupil_, R = raw_, L = count;
     g_.reset(sizeof(float2) * size_t(P) * R * L);
     gt_.reset(sizeof(float2) * size_t(P) * R * L);
@@ -281,7 +281,10 @@ void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, cons
     hipLaunchKernelGGL(shift_transpose_kernel, dim3(div_up(R, 32), div_up(R, 32), L), dim3(256), 0, s,
                        it_.as<float>(), out, R);
     LDH_CHECK_LAUNCH("psf-shift");
-    seqsum.normalize(s, out, int64_t(R) * R, int64_t(R) * R, L, totals);
+    if (normalize)
+        seqsum.normalize(s, out, int64_t(R) * R, int64_t(R) * R, L, totals);
+    else
+        seqsum.run(s, out, int64_t(R) * R, int64_t(R) * R, L, totals);
 }
 
 }  // namespace ldh

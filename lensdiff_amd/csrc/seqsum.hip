@@ -5,18 +5,24 @@
 // running sum rises monotonically through binades [2^e, 2^(e+1)); while it stays inside one
 // binade, adding x rounds x to a multiple of u = 2^(e-23), so
 //     s_after_block = s_before + sum_i rint(x_i / u) * u
-// whatever the order inside the block. The sum is therefore found by
-//   1. per-block FP64 sums, to fix the 32-binade window the running sum passes through;
-//   2. per-block, per-binade sums of the rounded contributions (exact in FP64);
-//   3. a one-wavefront walk over the blocks in order: blocks whose whole contribution keeps the
-//      running sum inside its binade are added at once; the few blocks where it crosses into
-//      the next binade are replayed element by element, taking the crossing addition as a
-//      true FP32 add.
-// The result equals the sequential FP32 sum except where an addition is an exact rounding tie
-// (ties are resolved on the addend alone), an error of one unit in the last place of the
-// running sum. Contributions added while the running sum is below the window are taken as
-// exact (their rounding is below 2^-54 of the total).
+// whatever the order inside the block. The sum is found in four steps:
+//   1. per-block FP64 sums;
+//   2. their exclusive prefix per segment, which places the running sum of each block to well
+//      within a binade; each block gets a base binade eb just below that estimate;
+//   3. per block, the sums of rounded contributions for the four binades eb .. eb+3 (exact in
+//      FP64);
+//   4. a one-wavefront walk over the blocks in order: a block whose binade sum keeps the
+//      running sum inside its binade is added at once; a block where the running sum crosses
+//      into the next binade, or whose binade lies outside the four computed, is replayed: its
+//      64-element chunks are added through a lane scan of their rounded contributions, and the
+//      chunk holding the crossing is added element by element in true FP32.
+// An addend that lands exactly halfway between two multiples of u rounds by the parity of the
+// running sum, which the block sums cannot know: binade sums and chunks holding such a tie are
+// marked and always go the element-by-element way. The estimate only decides which blocks are
+// replayed, never the value, so the result is the sequential FP32 sum, bit for bit, for any
+// non-negative finite input.
 #include <climits>
+#include <cstdint>
 
 #include "seqsum.h"
 
@@ -26,8 +32,10 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kPerThread = kSeqSumBlock / kThreads;
-constexpr int kBinades = 32;
-constexpr int kWindowAbove = 1;  // binades kept above the binade of the FP64 total
+constexpr int kChunk = kSeqSumBlock / 64;  // elements per lane in a replay
+constexpr int kBins = 4;                   // binades per block
+constexpr int kNoBase = INT_MIN;           // running sum estimated at zero: replay the block
+constexpr double kTied = -1.0;             // a binade sum holding a rounding tie
 
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -64,28 +72,44 @@ block_sums_kernel(const float* __restrict__ x, int64_t n, int64_t stride, int nb
     }
 }
 
+// Exclusive prefix of the block sums of one segment; base binade of each block placed 1/64
+// below the prefix so that the running sum (within a fraction of a percent of the exact prefix)
+// starts inside the four binades computed for the block.
 __global__ void __launch_bounds__(kThreads)
-window_kernel(const double* __restrict__ bsum, int nb, int* __restrict__ elo) {
-    const int seg = blockIdx.x;
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kThreads) acc += bsum[int64_t(seg) * nb + b];
-    __shared__ double red[kThreads / 64];
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+prefix_kernel(const double* __restrict__ bsum, int nb, int* __restrict__ ebase) {
+    const int seg = blockIdx.x, t = threadIdx.x;
+    const double* bs = bsum + int64_t(seg) * nb;
+    int* eb = ebase + int64_t(seg) * nb;
+    const int per = (nb + kThreads - 1) / kThreads;
+    const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+    double local = 0.0;
+    for (int b = b0; b < b1; ++b) local += bs[b];
+    __shared__ double part[kThreads];
+    part[t] = local;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int w = 0; w < kThreads / 64; ++w) t += red[w];
-        elo[seg] = t > 0.0 ? ilogb(t) + kWindowAbove - (kBinades - 1) : INT_MIN;
+    if (t == 0) {
+        double run = 0.0;
+        for (int i = 0; i < kThreads; ++i) {
+            const double v = part[i];
+            part[i] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    double run = part[t];
+    for (int b = b0; b < b1; ++b) {
+        const double lo = run * (1.0 - 1.0 / 64.0);
+        eb[b] = lo > 0.0 ? ilogb(lo) : kNoBase;
+        run += bs[b];
     }
 }
 
 __global__ void __launch_bounds__(kThreads)
-binade_sums_kernel(const float* __restrict__ x, int64_t n, int64_t stride, int nb, const int* __restrict__ elo,
+binade_sums_kernel(const float* __restrict__ x, int64_t n, int64_t stride, int nb, const int* __restrict__ ebase,
                    double* __restrict__ bins) {
     const int seg = blockIdx.y, b = blockIdx.x;
-    const int e_lo = elo[seg];
-    if (e_lo == INT_MIN) return;
+    const int eb = ebase[int64_t(seg) * nb + b];
+    if (eb == kNoBase) return;
     const float* base = x + seg * stride;
     const int64_t start = int64_t(b) * kSeqSumBlock;
     float v[kPerThread];
@@ -94,118 +118,188 @@ binade_sums_kernel(const float* __restrict__ x, int64_t n, int64_t stride, int n
         const int64_t i = start + int64_t(k) * kThreads + threadIdx.x;
         v[k] = i < n ? base[i] : 0.0f;
     }
-    __shared__ double red[kBinades][kThreads / 64];
-#pragma unroll 4
-    for (int q = 0; q < kBinades; ++q) {
-        // contributions counted in units of u = 2^(e-23): rint(x * 2^(23-e))
-        const int shift = 23 - (e_lo + q);
-        double acc = 0.0;
+    __shared__ double red[kBins][kThreads / 64];
+    __shared__ int tied[kBins][kThreads / 64];
 #pragma unroll
-        for (int k = 0; k < kPerThread; ++k) acc += double(rintf(ldexpf(v[k], shift)));
+    for (int q = 0; q < kBins; ++q) {
+        // contributions counted in units of u = 2^(e-23): rint(x * 2^(23-e))
+        const int shift = 23 - (eb + q);
+        double acc = 0.0;
+        bool tie = false;
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k) {
+            const float t = ldexpf(v[k], shift);
+            const float r = rintf(t);
+            tie |= fabsf(t - r) == 0.5f;
+            acc += double(r);
+        }
         acc = wave_sum(acc);
-        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = acc;
+        const bool any_tie = __ballot(tie) != 0ull;
+        if ((threadIdx.x & 63) == 0) {
+            red[q][threadIdx.x >> 6] = acc;
+            tied[q][threadIdx.x >> 6] = any_tie ? 1 : 0;
+        }
     }
     __syncthreads();
-    if (threadIdx.x < kBinades) {
+    if (threadIdx.x < kBins) {
         const int q = threadIdx.x;
         double t = 0.0;
-        for (int w = 0; w < kThreads / 64; ++w) t += red[q][w];
-        bins[(int64_t(seg) * kBinades + q) * nb + b] = ldexp(t, (e_lo + q) - 23);
-    }
-}
-
-// Contribution of x at binade slot q (q < 0: below the window, taken exactly).
-__device__ __forceinline__ double contribution(float x, int e_lo, int q) {
-    if (q < 0) return double(x);
-    return ldexp(double(rintf(ldexpf(x, 23 - (e_lo + q)))), (e_lo + q) - 23);
-}
-__device__ __forceinline__ double slot_limit(int e_lo, int q) {
-    return q < 0 ? ldexp(1.0, e_lo) : ldexp(1.0, e_lo + q + 1);
-}
-__device__ __forceinline__ int slot_of(double s, int e_lo) {
-    if (s < ldexp(1.0, e_lo)) return -1;
-    const int q = ilogb(s) - e_lo;
-    return q < kBinades - 1 ? q : kBinades - 1;
-}
-
-// Replays one block element by element from running sum *s (slot *q). Lane-parallel within
-// 64-element chunks; each crossing addition is a real FP32 add.
-__device__ void replay_block(const float* __restrict__ base, int64_t n, int64_t block, int e_lo, double* s,
-                             int* q, int lane) {
-    const int64_t start = block * kSeqSumBlock;
-    constexpr int kChunks = kSeqSumBlock / 64;
-    float v[kChunks];
-#pragma unroll
-    for (int c = 0; c < kChunks; ++c) {
-        const int64_t i = start + int64_t(c) * 64 + lane;
-        v[c] = i < n ? base[i] : 0.0f;
-    }
-#pragma unroll 1
-    for (int c = 0; c < kChunks; ++c) {
-        int first = 0;
-        while (true) {
-            const double contrib = lane >= first ? contribution(v[c], e_lo, *q) : 0.0;
-            const double incl = wave_inclusive_scan(contrib, lane);
-            const double limit = slot_limit(e_lo, *q);
-            const bool over = lane >= first && (*s + incl) >= limit;
-            const unsigned long long mask = __ballot(over);
-            if (mask == 0ull) {
-                *s += __shfl(incl, 63);
-                break;
-            }
-            const int l = __ffsll((long long)mask) - 1;
-            const double before = __shfl(incl - contrib, l);
-            const float xl = __shfl(v[c], l);
-            const float sf = float(*s + before);   // the running sum is an FP32 value
-            *s = double(sf + xl);                   // the crossing addition, in FP32
-            *q = slot_of(*s, e_lo);
-            first = l + 1;
-            if (first >= 64) break;
+        int tie = 0;
+        for (int w = 0; w < kThreads / 64; ++w) {
+            t += red[q][w];
+            tie |= tied[q][w];
         }
+        // a rounding tie depends on the running sum's last bit: marked for replay
+        bins[(int64_t(seg) * nb + b) * kBins + q] = tie ? kTied : ldexp(t, (eb + q) - 23);
+    }
+}
+
+// Running-sum state of the walk: s is always an FP32 value; e = ilogb(s), or kNoBase at s = 0.
+struct Run {
+    double s;
+    int e;
+};
+
+__device__ __forceinline__ double binade_limit(int e) { return ldexp(1.0, e + 1); }
+
+// Sum of the rounded contributions of a lane's chunk at binade e (exact values at s = 0), or
+// kTied when one of them is a rounding tie.
+__device__ __forceinline__ double chunk_contribution(const float* v, int e) {
+    double acc = 0.0;
+    if (e == kNoBase) {
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) acc += double(v[j]);
+        return acc;
+    }
+    const int shift = 23 - e;
+    bool tie = false;
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+        const float t = ldexpf(v[j], shift);
+        const float r = rintf(t);
+        tie |= fabsf(t - r) == 0.5f;
+        acc += double(r);
+    }
+    return tie ? kTied : ldexp(acc, e - 23);
+}
+
+// Adds one block element by element in effect: lane l holds chunk l (kChunk consecutive
+// elements). Chunks that keep the running sum inside its binade are added through a scan of
+// their rounded contributions; the chunk where it leaves the binade (or where s leaves 0) is
+// added by its own lane in true FP32, in order; so is a chunk holding a rounding tie.
+__device__ void replay_block(const float* __restrict__ base, int64_t n, int64_t block, Run& r, int lane) {
+    const int64_t start = block * kSeqSumBlock + int64_t(lane) * kChunk;
+    float v[kChunk];
+    if (start + kChunk <= n && (reinterpret_cast<uintptr_t>(base + start) & 15) == 0) {
+        const float4* p4 = reinterpret_cast<const float4*>(base + start);
+#pragma unroll
+        for (int j = 0; j < kChunk / 4; ++j) {
+            const float4 q = p4[j];
+            v[4 * j] = q.x; v[4 * j + 1] = q.y; v[4 * j + 2] = q.z; v[4 * j + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) v[j] = start + j < n ? base[start + j] : 0.0f;
+    }
+    int first = 0;
+    while (first < 64) {
+        double c = lane >= first ? chunk_contribution(v, r.e) : 0.0;
+        const bool tie = c == kTied;
+        if (tie) c = 0.0;
+        const double incl = wave_inclusive_scan(c, lane);
+        const bool over = tie || (lane >= first && (r.e == kNoBase ? incl > 0.0 : r.s + incl >= binade_limit(r.e)));
+        const unsigned long long mask = __ballot(over);
+        if (mask == 0ull) {
+            r.s += __shfl(incl, 63);
+            return;
+        }
+        const int l = __ffsll((long long)mask) - 1;
+        const double before = __shfl(incl - c, l);
+        float sf = float(r.s + before);  // exact: multiples of u below the binade limit
+        if (lane == l) {
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) sf = sf + v[j];
+        }
+        sf = __shfl(sf, l);
+        r.s = double(sf);
+        r.e = sf > 0.0f ? ilogb(sf) : kNoBase;
+        first = l + 1;
     }
 }
 
 __global__ void __launch_bounds__(64)
-walk_kernel(const float* __restrict__ x, int64_t n, int64_t stride, int nb, const int* __restrict__ elo,
-            const double* __restrict__ bsum, const double* __restrict__ bins, float* __restrict__ out) {
+walk_kernel(const float* __restrict__ x, int64_t n, int64_t stride, int nb, const double* __restrict__ bsum,
+            const int* __restrict__ ebase, const double* __restrict__ bins, float* __restrict__ out) {
     const int seg = blockIdx.x;
     const int lane = threadIdx.x;
-    const int e_lo = elo[seg];
-    if (e_lo == INT_MIN) {
-        if (lane == 0) out[seg] = 0.0f;
-        return;
-    }
     const float* base = x + seg * stride;
     const double* bs = bsum + int64_t(seg) * nb;
-    const double* bn = bins + int64_t(seg) * kBinades * nb;
-    double s = 0.0;
-    int q = -1;
+    const int* eb = ebase + int64_t(seg) * nb;
+    const double* bn = bins + int64_t(seg) * nb * kBins;
+    Run r{0.0, kNoBase};
+    // per-lane block data of the current group, loaded one group ahead
+    auto load = [&](int blk, double& sum, int& e0, double (&b)[kBins]) {
+        sum = 0.0;
+        e0 = kNoBase;
+        if (blk < nb) {
+            sum = bs[blk];
+            e0 = eb[blk];
+            if (e0 != kNoBase) {
+#pragma unroll
+                for (int q = 0; q < kBins; ++q) b[q] = bn[int64_t(blk) * kBins + q];
+            }
+        }
+    };
+    double cur_sum, nxt_sum;
+    int cur_e0, nxt_e0;
+    double cur_b[kBins] = {0.0, 0.0, 0.0, 0.0}, nxt_b[kBins] = {0.0, 0.0, 0.0, 0.0};
+    load(lane, cur_sum, cur_e0, cur_b);
 #pragma unroll 1
     for (int g0 = 0; g0 < nb; g0 += 64) {
+        load(g0 + 64 + lane, nxt_sum, nxt_e0, nxt_b);
         const int blk = g0 + lane;
         int first = 0;
         while (first < 64) {
+            // contribution of this lane's block at the current binade, or a forced replay
             double col = 0.0;
-            if (lane >= first && blk < nb) col = q < 0 ? bs[blk] : bn[int64_t(q) * nb + blk];
+            bool force = false;
+            if (lane >= first && blk < nb && cur_sum != 0.0) {
+                const int q = r.e - cur_e0;
+                if (r.e != kNoBase && cur_e0 != kNoBase && q >= 0 && q < kBins) {
+                    col = cur_b[0];
+                    if (q == 1) col = cur_b[1];
+                    if (q == 2) col = cur_b[2];
+                    if (q == 3) col = cur_b[3];
+                    if (col == kTied) {
+                        col = 0.0;
+                        force = true;
+                    }
+                } else {
+                    force = true;
+                }
+            }
             const double incl = wave_inclusive_scan(col, lane);
-            const bool over = lane >= first && blk < nb && (s + incl) >= slot_limit(e_lo, q);
+            const bool over = force || (lane >= first && blk < nb && r.e != kNoBase && r.s + incl >= binade_limit(r.e));
             const unsigned long long mask = __ballot(over);
             if (mask == 0ull) {
-                s += __shfl(incl, 63);
+                r.s += __shfl(incl, 63);
                 break;
             }
             const int l = __ffsll((long long)mask) - 1;
-            s += __shfl(incl - col, l);
-            if (q >= 0) s = double(float(s));
-            replay_block(base, n, g0 + l, e_lo, &s, &q, lane);
+            r.s += __shfl(incl - col, l);
+            replay_block(base, n, g0 + l, r, lane);
             first = l + 1;
         }
+        cur_sum = nxt_sum;
+        cur_e0 = nxt_e0;
+#pragma unroll
+        for (int q = 0; q < kBins; ++q) cur_b[q] = nxt_b[q];
     }
-    if (lane == 0) out[seg] = float(s);
+    if (lane == 0) out[seg] = float(r.s);
 }
 
 __global__ void scale_by_inverse_kernel(float* __restrict__ x, int64_t n, int64_t stride,
-                                        const float* __restrict__ totals, int only_positive) {
+                                        const float* __restrict__ totals) {
     const int seg = blockIdx.y;
     const float total = totals[seg];
     if (total <= 0.0f) return;  // the reference leaves a non-positive total untouched
@@ -213,7 +307,6 @@ __global__ void scale_by_inverse_kernel(float* __restrict__ x, int64_t n, int64_
     float* base = x + seg * stride;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
         base[i] *= inv;
-    (void)only_positive;
 }
 
 }  // namespace
@@ -222,23 +315,27 @@ void SeqSum::run(hipStream_t s, const float* x, int64_t n, int64_t stride, int s
     if (n <= 0 || segments <= 0) return;
     const int nb = int(div_up(n, kSeqSumBlock));
     bsum_.reset(sizeof(double) * size_t(nb) * segments);
-    bins_.reset(sizeof(double) * size_t(nb) * segments * kBinades);
-    elo_.reset(sizeof(int) * size_t(segments));
+    bins_.reset(sizeof(double) * size_t(nb) * segments * kBins);
+    ebase_.reset(sizeof(int) * size_t(nb) * segments);
     hipLaunchKernelGGL(block_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, x, n, stride, nb,
                        bsum_.as<double>());
-    hipLaunchKernelGGL(window_kernel, dim3(segments), dim3(kThreads), 0, s, bsum_.as<double>(), nb, elo_.as<int>());
+    hipLaunchKernelGGL(prefix_kernel, dim3(segments), dim3(kThreads), 0, s, bsum_.as<double>(), nb, ebase_.as<int>());
     hipLaunchKernelGGL(binade_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, x, n, stride, nb,
-                       elo_.as<int>(), bins_.as<double>());
-    hipLaunchKernelGGL(walk_kernel, dim3(segments), dim3(64), 0, s, x, n, stride, nb, elo_.as<int>(),
-                       bsum_.as<double>(), bins_.as<double>(), totals);
+                       ebase_.as<int>(), bins_.as<double>());
+    hipLaunchKernelGGL(walk_kernel, dim3(segments), dim3(64), 0, s, x, n, stride, nb, bsum_.as<double>(),
+                       ebase_.as<int>(), bins_.as<double>(), totals);
     LDH_CHECK_LAUNCH("seqsum-launch");
+}
+
+void SeqSum::scale(hipStream_t s, float* x, int64_t n, int64_t stride, int segments, const float* totals) {
+    const int grid = int(std::min<int64_t>(div_up(n, 256 * 4), 2048));
+    hipLaunchKernelGGL(scale_by_inverse_kernel, dim3(grid, segments), dim3(256), 0, s, x, n, stride, totals);
+    LDH_CHECK_LAUNCH("normalize-launch");
 }
 
 void SeqSum::normalize(hipStream_t s, float* x, int64_t n, int64_t stride, int segments, float* totals) {
     run(s, x, n, stride, segments, totals);
-    const int grid = int(std::min<int64_t>(div_up(n, 256 * 4), 2048));
-    hipLaunchKernelGGL(scale_by_inverse_kernel, dim3(grid, segments), dim3(256), 0, s, x, n, stride, totals, 0);
-    LDH_CHECK_LAUNCH("normalize-launch");
+    scale(s, x, n, stride, segments, totals);
 }
 
 }  // namespace ldh

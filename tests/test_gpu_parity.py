@@ -37,21 +37,40 @@ def test_fft2d_matches_numpy(ctx, w, h):
     assert crel(back, z) < 2e-6
 
 
-def test_sequential_sum_reproduces_fp32_accumulation(ctx, oracle):
+def seqsum_cases():
     rng = np.random.default_rng(3)
     n = 2047
     yy, xx = np.mgrid[0:n, 0:n]
     r = np.hypot(xx - n // 2, yy - n // 2) + 1e-3
     airy = (np.sinc(r / 20.0) ** 2 / (1 + r / 20.0)).astype(np.float32)
-    cases = [rng.random(1_000_003).astype(np.float32),
-             rng.exponential(1e-6, 3_000_000).astype(np.float32),
-             (airy / airy.sum()).astype(np.float32),
-             np.zeros(10, np.float32), np.ones(1, np.float32)]
-    for x in cases:
+    spikes = np.zeros(5_000_000, np.float32)
+    spikes[rng.integers(0, spikes.size, 50)] = np.ldexp(1.0, rng.integers(-20, 20, 50)).astype(np.float32)
+    big_first = rng.exponential(1.0, 3_000_000).astype(np.float32)
+    big_first[:100] = 1e6
+    big_last = (rng.exponential(1.0, 3_000_000) * 1e-3).astype(np.float32)
+    big_last[-1000:] = 1e6
+    return {
+        "uniform": rng.random(1_000_003).astype(np.float32),
+        "exponential-tiny": rng.exponential(1e-6, 3_000_000).astype(np.float32),
+        "airy": (airy / airy.sum()).astype(np.float32),
+        "zeros": np.zeros(10, np.float32),
+        "one": np.ones(1, np.float32),
+        # stalls at 2^24: every later addend is a rounding tie
+        "ones-20M": np.ones(20_000_000, np.float32),
+        "spikes": spikes,
+        "wide-range": np.ldexp(rng.random(4_000_000), -rng.integers(0, 60, 4_000_000)).astype(np.float32),
+        "eighths": (rng.integers(0, 8, 4096 * 70 + 13) * 0.125).astype(np.float32),
+        "big-first": big_first,
+        "big-last": big_last,
+    }
+
+
+def test_sequential_sum_reproduces_fp32_accumulation(ctx, oracle):
+    """The device sum equals `float s = 0; for (...) s += x[i];` bit for bit."""
+    for name, x in seqsum_cases().items():
         want = oracle.sequential_sum(x)
         got = ctx.sequential_sum(x)
-        # equal up to an exact-tie rounding (one ulp of the running sum)
-        assert abs(got - want) <= 2 * np.spacing(np.float32(abs(want))), (got, want, x.size)
+        assert np.float32(got) == np.float32(want), (name, got, want, x.size)
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 5, 6, 7])
