@@ -2,7 +2,8 @@
 
 Workload: 3840x2160 RGBA float32 frames (0.05% highlights, log-uniform [1, 5000]); lenses shaped
 like the precision config (P=1024 7-blade pupil, Noll 2-21 N(0, 0.2) waves, 31 wavelengths
-400-700 nm, R=4092 raw PSFs, K=2047 kernels, 8192x8192 FFT grid). The request stream draws 70%
+400-700 nm, R=4092 raw PSFs, K=2047 kernels, 6144x4608 FFT grid where the reference pads to
+8192x8192). The request stream draws 70%
 of its lenses from 8 resident lens sets and 30% fresh ones (a new bank each). Inputs are resident
 in HBM before timing. A step is one frame (one request).
 
@@ -219,7 +220,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "uhd-31wl served stream: 3840x2160 RGBA f32, 31 wavelengths 400-700 nm, P=1024, "
-                               "R=4092, K=2047, 8192^2 FFT grid; 70% of requests on 8 resident lens sets, 30% fresh",
+                               "R=4092, K=2047, 6144x4608 FFT grid; 70% of requests on 8 resident lens sets, 30% fresh",
                    "slots": args.slots, "budget_gb": 64},
         "plain": {"frames_per_s": round(plain_fps, 3), "requests": len(plain_service),
                   "service_ms_p50": round(float(np.median(plain_service)) * 1e3, 2)},

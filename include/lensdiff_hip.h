@@ -194,7 +194,9 @@ ldh_status ldh_shape_kernels(ldh_context* ctx, const ldh_params* p, const float*
  * x[0..n) (std::accumulate with a float accumulator), computed on the device. */
 ldh_status ldh_sequential_sum(ldh_context* ctx, const float* x, int64_t n, float* out);
 /* In-place 2D complex FFT of interleaved complex64 (host buffer), rows then columns,
- * forward sign -1, inverse scaled by 1/(w*h) (LensDiffCpuReference.cpp:194-301). */
+ * forward sign -1, inverse scaled by 1/(w*h) (LensDiffCpuReference.cpp:194-301). Sizes:
+ * powers of two (width >= 16, height >= 8), 3*2^a from 48 and 9*2^a from 144, up to 16384;
+ * the reference takes powers of two only. */
 ldh_status ldh_fft2d(ldh_context* ctx, float* data, int32_t width, int32_t height, int32_t inverse);
 
 /* ---- PSF banks (EnsureLensDiffPsfBank, LensDiffCpuReference.cpp:2068-2071) ------------- */

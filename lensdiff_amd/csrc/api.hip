@@ -313,8 +313,8 @@ ldh_status ldh_sequential_sum(ldh_context* ctx, const float* x, int64_t n, float
 
 ldh_status ldh_fft2d(ldh_context* ctx, float* data, int32_t w, int32_t h, int32_t inverse) {
     return guarded(ctx, [&] {
-        require(data && ldh::is_pow2(w) && ldh::is_pow2(h) && w >= 16 && h >= 8 && w <= ldh::kMaxFft &&
-                    h <= ldh::kMaxFft, LDH_ERR_INVALID_ARGUMENT, "fft-size-unsupported");
+        require(data && ldh::grid_length_ok(w) && w >= 16 && ldh::grid_length_ok(h), LDH_ERR_INVALID_ARGUMENT,
+                "fft-size-unsupported");
         const int64_t n = int64_t(w) * h;
         ldh::DeviceBuffer a(sizeof(float2) * n), b(sizeof(float2) * n);
         h2d(ctx, a.get(), data, sizeof(float2) * n);

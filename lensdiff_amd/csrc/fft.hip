@@ -7,22 +7,20 @@
 namespace ldh {
 
 void FftTables::init(hipStream_t s) {
-    std::vector<float2> h(kMaxFft);
-    for (int j = 0; j < kMaxFft; ++j) {
-        // exp(-2 pi i j / kMaxFft), computed in double and rounded once
-        const double a = -2.0 * M_PI * double(j) / double(kMaxFft);
+    std::vector<float2> h(kTwiddleN);
+    for (int j = 0; j < kTwiddleN; ++j) {
+        // exp(-2 pi i j / kTwiddleN), computed in double and rounded once
+        const double a = -2.0 * M_PI * double(j) / double(kTwiddleN);
         h[size_t(j)] = make_float2(float(std::cos(a)), float(std::sin(a)));
     }
-    twiddle.reset(sizeof(float2) * kMaxFft);
-    LDH_HIP(hipMemcpyAsync(twiddle.get(), h.data(), sizeof(float2) * kMaxFft, hipMemcpyHostToDevice, s),
+    twiddle.reset(sizeof(float2) * kTwiddleN);
+    LDH_HIP(hipMemcpyAsync(twiddle.get(), h.data(), sizeof(float2) * kTwiddleN, hipMemcpyHostToDevice, s),
             "fft-table-upload");
     LDH_HIP(hipStreamSynchronize(s), "fft-table-upload");
 }
 
 static void check_dims(int width, int height) {
-    if (!is_pow2(width) || !is_pow2(height) || width < 16 || height < 8 || width > kMaxFft ||
-        height > kMaxFft)
-        fail(LDH_ERR_TOO_LARGE, "fft-size-unsupported");
+    if (!grid_length_ok(width) || width < 16 || !grid_length_ok(height)) fail(LDH_ERR_TOO_LARGE, "fft-size-unsupported");
 }
 
 void fft_rows(hipStream_t s, const FftTables& t, float2* data, int width, int rows, bool inverse) {

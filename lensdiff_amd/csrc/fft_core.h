@@ -93,19 +93,30 @@ template <bool INV> __device__ __forceinline__ void dft16(float2* v) {
     }
 }
 
+template <bool INV> __device__ __forceinline__ void dft3(float2& a0, float2& a1, float2& a2) {
+    const float s = 0.86602540378443865f;  // sin(2 pi / 3)
+    const float2 t = make_float2(a1.x + a2.x, a1.y + a2.y);
+    const float2 m = make_float2(a0.x - 0.5f * t.x, a0.y - 0.5f * t.y);
+    const float2 d = rot90<INV>(make_float2(a1.x - a2.x, a1.y - a2.y));
+    a0 = make_float2(a0.x + t.x, a0.y + t.y);
+    a1 = make_float2(m.x + s * d.x, m.y + s * d.y);
+    a2 = make_float2(m.x - s * d.x, m.y - s * d.y);
+}
+
 template <int R, bool INV> __device__ __forceinline__ void dft(float2* v) {
     if constexpr (R == 2) dft2<INV>(v[0], v[1]);
+    else if constexpr (R == 3) dft3<INV>(v[0], v[1], v[2]);
     else if constexpr (R == 4) dft4<INV>(v[0], v[1], v[2], v[3]);
     else if constexpr (R == 8) dft8<INV>(v);
     else dft16<INV>(v);
 }
 
-constexpr int clog2(int n) { return n <= 1 ? 0 : 1 + clog2(n / 2); }
-
-// Radix of the stage that starts at sub-length Ns: 16 while at least 16 remains, then the rest.
+// Radix of the stage that starts at sub-length Ns: the largest power of two up to 16 that
+// divides what remains, then 3 for the odd factors (lengths 2^a * 3^b).
 template <int N, int Ns> struct StageRadix {
     static constexpr int rem = N / Ns;
-    static constexpr int value = rem >= 16 ? 16 : rem;
+    static constexpr int p2 = rem & -rem;
+    static constexpr int value = p2 >= 16 ? 16 : (p2 >= 2 ? p2 : 3);
 };
 
 // One Stockham stage over all NL lines of the tile, executed by NT threads.
@@ -136,12 +147,12 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
         const int u = tid + p * NT;
         if (TOTAL % NT == 0 || u < TOTAL) {
             const int j = j_of[p];
-            const int k = j & (Ns - 1);
+            const int k = j % Ns;
             if constexpr (Ns > 1) {
-                constexpr int shift = kMaxFftLog2 - clog2(Ns * R);
+                constexpr int stride = kTwiddleN / (Ns * R);
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    float2 w = tw[(k * r) << shift];
+                    float2 w = tw[k * r * stride];
                     if (INV) w.y = -w.y;
                     v[p][r] = cmul(v[p][r], w);
                 }
@@ -163,14 +174,18 @@ __device__ __forceinline__ void fft_stages(float2* lds, const float2* __restrict
     }
 }
 
+// Adjacent columns per column tile (kColWidth * 8 bytes contiguous per row).
+constexpr int kColWidth = 16;
+
 // Tile geometry for a line length and layout.
 template <int N, bool ROWS> struct TileShape {
-    // rows: at least 1024 elements per tile; columns: 16 adjacent columns (128-byte segments)
-    // times enough line groups for 1024 elements.
-    static constexpr int NL = ROWS ? (N >= 1024 ? 1 : 1024 / N) : (16 * (N >= 64 ? 1 : 64 / N));
+    // rows: at least 1024 elements per tile; columns: kColWidth adjacent columns times enough
+    // line groups for 1024 elements.
+    static constexpr int NL = ROWS ? (N >= 1024 ? 1 : 1024 / N) : (kColWidth * (N >= 64 ? 1 : 64 / N));
     static constexpr int ELEMS = NL * N;
     static constexpr int NT0 = ELEMS / 32;
-    static constexpr int NT = NT0 < 64 ? 64 : (NT0 > 512 ? 512 : NT0);
+    static constexpr int NT1 = NT0 < 64 ? 64 : (NT0 > 512 ? 512 : NT0);
+    static constexpr int NT = (NT1 + 63) / 64 * 64;
     static constexpr int LDS_BYTES = (ELEMS + ELEMS / 32 + 1) * 8;
 };
 
@@ -217,47 +232,79 @@ inline void allow_large_lds(const void* fn, int bytes) {
         fail(LDH_ERR_DEVICE, "lds-attribute-failed");
 }
 
-// Launch helper: dispatches the compile-time line length, instantiating only lengths in
-// [2^LMIN, 2^LMAX] (the lengths a given pass can receive).
-template <bool ROWS, bool INV, int LMIN, int LMAX, class IO>
+template <bool ROWS, bool INV, int N, class IO>
+bool launch_tile_if(hipStream_t s, int n, const IO& io, const float2* tw, int ntiles) {
+    if (n != N) return false;
+    using S = TileShape<N, ROWS>;
+    allow_large_lds(reinterpret_cast<const void*>(&fft_tile_kernel<N, ROWS, INV, IO>), S::LDS_BYTES);
+    hipLaunchKernelGGL((fft_tile_kernel<N, ROWS, INV, IO>), dim3(ntiles), dim3(S::NT), S::LDS_BYTES, s, io, tw,
+                       ntiles);
+    return true;
+}
+
+// Line lengths the tile kernels are built for. Rows: every grid length from 16. Columns: the
+// single-pass heights and split factors (col_split).
+template <bool ROWS, bool INV, class IO>
 void launch_tiles(hipStream_t s, int n, const IO& io, const float2* tw, int ntiles) {
-    bool launched = false;
-#define LDH_TILE_CASE(LG)                                                                        \
-    if constexpr (LMIN <= LG && LG <= LMAX) {                                                    \
-        if (n == (1 << LG)) {                                                                    \
-            using S = TileShape<(1 << LG), ROWS>;                                                \
-            allow_large_lds(reinterpret_cast<const void*>(&fft_tile_kernel<(1 << LG), ROWS, INV, IO>), \
-                            S::LDS_BYTES);                                                       \
-            hipLaunchKernelGGL((fft_tile_kernel<(1 << LG), ROWS, INV, IO>), dim3(ntiles),        \
-                               dim3(S::NT), S::LDS_BYTES, s, io, tw, ntiles);                    \
-            launched = true;                                                                     \
-        }                                                                                        \
+    bool ok;
+    if constexpr (ROWS) {
+        ok = launch_tile_if<ROWS, INV, 16>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 32>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 48>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 64>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 96>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 128>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 144>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 192>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 256>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 288>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 384>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 512>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 576>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 768>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 1024>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 1152>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 1536>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 2048>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 2304>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 3072>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 4096>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 4608>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 6144>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 8192>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 9216>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 12288>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 16384>(s, n, io, tw, ntiles);
+    } else {
+        ok = launch_tile_if<ROWS, INV, 8>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 12>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 16>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 24>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 32>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 36>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 48>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 64>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 72>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 96>(s, n, io, tw, ntiles) ||
+             launch_tile_if<ROWS, INV, 128>(s, n, io, tw, ntiles) || launch_tile_if<ROWS, INV, 144>(s, n, io, tw, ntiles);
     }
-    LDH_TILE_CASE(1) LDH_TILE_CASE(2) LDH_TILE_CASE(3) LDH_TILE_CASE(4) LDH_TILE_CASE(5)
-    LDH_TILE_CASE(6) LDH_TILE_CASE(7) LDH_TILE_CASE(8) LDH_TILE_CASE(9) LDH_TILE_CASE(10)
-    LDH_TILE_CASE(11) LDH_TILE_CASE(12) LDH_TILE_CASE(13) LDH_TILE_CASE(14)
-#undef LDH_TILE_CASE
-    if (!launched) fail(LDH_ERR_TOO_LARGE, "fft-size-unsupported");
+    if (!ok) fail(LDH_ERR_TOO_LARGE, "fft-size-unsupported");
     LDH_CHECK_LAUNCH("fft-tile-launch");
 }
 
-template <int N, bool ROWS> constexpr int tile_lines() { return TileShape<N, ROWS>::NL; }
 inline int row_tile_lines(int n) { return n >= 1024 ? 1 : 1024 / n; }
-inline int col_tile_lines(int n) { return 16 * (n >= 64 ? 1 : 64 / n); }
+inline int col_tile_lines(int n) { return kColWidth * (n >= 64 ? 1 : 64 / n); }
 
-// Split of a column length N (> 64) into N1 x N2 for the two-pass column transform.
+// Column tile lengths: one pass for heights up to 144 in this set, else two factors from it.
+__host__ __device__ constexpr bool col_tile_length_ok(int n) {
+    return n == 8 || n == 12 || n == 16 || n == 24 || n == 32 || n == 36 || n == 48 || n == 64 || n == 72 ||
+           n == 96 || n == 128 || n == 144;
+}
+
+// Split of a column length into N1 x N2 (both column tile lengths) for the two-pass column
+// transform, as balanced as possible; n1 = 0 when the height takes a single pass.
 __host__ __device__ inline void col_split(int n, int* n1, int* n2) {
-    int l = 0;
-    while ((1 << l) < n) ++l;
-    *n1 = 1 << (l / 2);
-    *n2 = n / *n1;
+    *n1 = 0;
+    *n2 = n;
+    if (col_tile_length_ok(n)) return;
+    int best = 0;
+    const int cand[12] = {8, 12, 16, 24, 32, 36, 48, 64, 72, 96, 128, 144};
+    for (int i = 0; i < 12; ++i) {
+        const int a = cand[i];
+        if (n % a != 0 || !col_tile_length_ok(n / a) || a > n / a) continue;
+        best = a;  // the largest a <= n / a: the most balanced split
+    }
+    *n1 = best;
+    *n2 = best ? n / best : n;
 }
 
 // Storage row of spectrum row k after the forward column transform of length `height`.
 __host__ __device__ inline int col_storage_row(int height, int k) {
-    if (height <= 64) return k;
     int n1, n2;
     col_split(height, &n1, &n2);
+    if (n1 == 0) return k;
     return n2 * (k % n1) + k / n1;
 }
 

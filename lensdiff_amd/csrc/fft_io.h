@@ -1,8 +1,8 @@
 // IO policies for the FFT tile kernel and the row / column pass launchers built on them.
 //
 // A 2D transform of a H x W grid (row pitch W) is one row pass plus a column transform. Column
-// lengths up to 64 take one pass; longer ones are split N = N1 * N2 into two passes over tiles
-// of 16 adjacent columns (128-byte row segments):
+// lengths that are column tile lengths take one pass; longer ones are split N = N1 * N2
+// (col_split) into two passes over tiles of kColWidth adjacent columns:
 //   forward  F1: length-N1 DFTs over rows n2 + N2*n1, then twiddle W_N^(n2*k1), in place;
 //            F2: length-N2 DFTs over the contiguous rows N2*k1 + n2, in place.
 //   The spectrum row of frequency k = k1 + N1*k2 is then stored at row N2*k1 + k2.
@@ -12,6 +12,8 @@
 #pragma once
 
 #include <hip/hip_fp16.h>
+
+#include <algorithm>
 
 #include "fft_core.h"
 #include "profiler.h"
@@ -37,6 +39,14 @@ struct MulSpectrumHalfLoad {
 };
 
 // ---- column passes ----------------------------------------------------------------------------
+// Rows a column pass may skip: loads of rows in [zero_lo, zero_hi) read as zero without touching
+// memory (rows the caller knows are zero), and rows >= store_rows are not stored (rows nobody
+// reads afterwards).
+struct ColRows {
+    int zero_lo = 0, zero_hi = 0;
+    int store_rows = 1 << 30;
+};
+
 // row(g, i) = g * g_row_mul + i * i_row_mul; the twiddle exponent of an output point is g * i.
 template <class Load, int TW>
 struct ColIO {
@@ -44,108 +54,131 @@ struct ColIO {
     float2* data;        // stores
     const float2* tw;
     int pitch;
-    int ncolgroups;   // width / 16
+    int ncolgroups;   // width / kColWidth
     int groups;       // line groups (g values) per tile
+    int gcount;       // line groups in the pass: tiles past it (a partial last tile) do nothing
     int g_row_mul, i_row_mul;
-    int tw_shift;
+    int tw_stride;    // kTwiddleN / N
+    ColRows rows;
     Load op;
-    __device__ size_t index(int tile, int line, int i, int* g_out) const {
+    __device__ int row_of(int tile, int line, int i, int* g_out, int* col) const {
         const int cg = tile % ncolgroups;
-        const int g = (tile / ncolgroups) * groups + (line >> 4);
+        const int g = (tile / ncolgroups) * groups + line / kColWidth;
         *g_out = g;
-        const int row = g * g_row_mul + i * i_row_mul;
-        return size_t(row) * size_t(pitch) + size_t(cg * 16 + (line & 15));
+        *col = cg * kColWidth + line % kColWidth;
+        return g * g_row_mul + i * i_row_mul;
     }
     __device__ float2 load(int tile, int line, int i) const {
-        int g;
-        const size_t idx = index(tile, line, i, &g);
-        return op(src, idx);
+        int g, col;
+        const int row = row_of(tile, line, i, &g, &col);
+        if (g >= gcount || (row >= rows.zero_lo && row < rows.zero_hi)) return make_float2(0.0f, 0.0f);
+        return op(src, size_t(row) * size_t(pitch) + size_t(col));
     }
     __device__ void store(int tile, int line, int i, float2 v) const {
-        int g;
-        const size_t idx = index(tile, line, i, &g);
+        int g, col;
+        const int row = row_of(tile, line, i, &g, &col);
+        if (g >= gcount || row >= rows.store_rows) return;
         if constexpr (TW != 0) {
-            float2 w = tw[(g * i) << tw_shift];
+            float2 w = tw[g * i * tw_stride];
             if (TW < 0) w.y = -w.y;
             v = cmul(v, w);
         }
-        data[idx] = v;
+        data[size_t(row) * size_t(pitch) + size_t(col)] = v;
     }
 };
 
 template <int TW, class Load>
 void col_pass(hipStream_t s, const float2* tw, const float2* src, float2* data, int width, int npts, int gcount,
-              int g_row_mul, int i_row_mul, int tw_log2n, const Load& op, bool inverse) {
+              int g_row_mul, int i_row_mul, int height, const Load& op, ColRows rows, bool inverse) {
     const int nl = col_tile_lines(npts);
-    const int groups = nl / 16;
-    ColIO<Load, TW> io{src, data, tw, width, width / 16, groups, g_row_mul, i_row_mul,
-                       kMaxFftLog2 - tw_log2n, op};
-    const int ntiles = (width / 16) * int(div_up(gcount, groups));
-    // algorithmic bytes of one column pass: every element read and written once
+    const int groups = nl / kColWidth;
+    ColIO<Load, TW> io{src, data, tw, width, width / kColWidth, groups, gcount, g_row_mul, i_row_mul,
+                       kTwiddleN / height, rows, op};
+    const int ntiles = (width / kColWidth) * int(div_up(gcount, groups));
+    // algorithmic bytes: elements read (outside the zero band) and written (below store_rows)
+    const double all = double(width) * double(height);
+    const double zero = double(std::max(0, std::min(rows.zero_hi, height) - std::max(rows.zero_lo, 0)));
+    const double stored = double(std::min(rows.store_rows, height));
     ProfileScope ps(std::string("fft-col-pass") + (inverse ? "-inv" : "-fwd"),
-                    16.0 * double(width) * double(npts) * double(gcount));
-    // Column tiles see lengths 8..128: single passes up to 64, split factors 8..128.
-    if (inverse) launch_tiles<false, true, 3, 7>(s, npts, io, tw, ntiles);
-    else launch_tiles<false, false, 3, 7>(s, npts, io, tw, ntiles);
+                    8.0 * (all - zero * double(width)) + 8.0 * stored * double(width));
+    if (inverse) launch_tiles<false, true>(s, npts, io, tw, ntiles);
+    else launch_tiles<false, false>(s, npts, io, tw, ntiles);
 }
 
-// Forward column transform (in place). `op` applies to the loads of the first pass.
+inline void check_cols(int width, int height) {
+    if (width < kColWidth || width % kColWidth) fail(LDH_ERR_INVALID_ARGUMENT, "fft-width-not-multiple-of-16");
+    if (!grid_length_ok(height)) fail(LDH_ERR_INVALID_ARGUMENT, "fft-height-unsupported");
+}
+
+// Forward column transform (in place). `op` applies to the loads of the first pass, which also
+// reads the rows [zero_lo, zero_hi) of `first` as zero.
 template <class Load>
-void cols_forward(hipStream_t s, const float2* tw, float2* data, int width, int height, const Load& op) {
-    if (width < 16 || (width & 15)) fail(LDH_ERR_INVALID_ARGUMENT, "fft-width-not-multiple-of-16");
-    if (height < 8) fail(LDH_ERR_INVALID_ARGUMENT, "fft-height-below-8");
-    if (height <= 64) {
-        col_pass<0>(s, tw, data, data, width, height, 1, 0, 1, ilog2(height), op, false);
-        return;
-    }
+void cols_forward(hipStream_t s, const float2* tw, float2* data, int width, int height, const Load& op,
+                  ColRows first = ColRows{}) {
+    check_cols(width, height);
+    first.store_rows = 1 << 30;
     int n1, n2;
     col_split(height, &n1, &n2);
-    col_pass<1>(s, tw, data, data, width, n1, n2, 1, n2, ilog2(height), op, false);
-    col_pass<0>(s, tw, data, data, width, n2, n1, n2, 1, ilog2(height), PlainLoad{}, false);
+    if (n1 == 0) {
+        col_pass<0>(s, tw, data, data, width, height, 1, 0, 1, height, op, first, false);
+        return;
+    }
+    col_pass<1>(s, tw, data, data, width, n1, n2, 1, n2, height, op, first, false);
+    col_pass<0>(s, tw, data, data, width, n2, n1, n2, 1, height, PlainLoad{}, ColRows{}, false);
 }
 
 // Inverse column transform (unscaled). The first pass loads from `src` through `op` and every
-// pass stores into `data` (src == data for in place).
+// pass stores into `data` (src == data for in place); the last pass stores only rows
+// < store_rows.
 template <class Load>
 void cols_inverse(hipStream_t s, const float2* tw, const float2* src, float2* data, int width, int height,
-                  const Load& op) {
-    if (width < 16 || (width & 15)) fail(LDH_ERR_INVALID_ARGUMENT, "fft-width-not-multiple-of-16");
-    if (height < 8) fail(LDH_ERR_INVALID_ARGUMENT, "fft-height-below-8");
-    if (height <= 64) {
-        col_pass<0>(s, tw, src, data, width, height, 1, 0, 1, ilog2(height), op, true);
-        return;
-    }
+                  const Load& op, int store_rows = 1 << 30) {
+    check_cols(width, height);
+    ColRows last;
+    last.store_rows = store_rows;
     int n1, n2;
     col_split(height, &n1, &n2);
-    col_pass<-1>(s, tw, src, data, width, n2, n1, n2, 1, ilog2(height), op, true);
-    col_pass<0>(s, tw, data, data, width, n1, n2, 1, n2, ilog2(height), PlainLoad{}, true);
+    if (n1 == 0) {
+        col_pass<0>(s, tw, src, data, width, height, 1, 0, 1, height, op, last, true);
+        return;
+    }
+    col_pass<-1>(s, tw, src, data, width, n2, n1, n2, 1, height, op, ColRows{}, true);
+    col_pass<0>(s, tw, data, data, width, n1, n2, 1, n2, height, PlainLoad{}, last, true);
 }
 
 // ---- row passes ---------------------------------------------------------------------------------
-// RowOp supplies load(row, x) and store(row, x, v); rows beyond `nrows` are skipped.
+// RowOp supplies load(row, x) and store(row, x, v). The pass covers `nrows` logical rows; logical
+// row l is grid row l below `split` and l + gap from there (so a band of zero rows in the middle
+// of the grid is skipped).
 template <class RowOp>
 struct RowIO {
     RowOp op;
     int lines_per_tile;
     int nrows;
+    int split, gap;
+    __device__ int row_of(int tile, int line) const {
+        const int l = tile * lines_per_tile + line;
+        return l < split ? l : l + gap;
+    }
     __device__ float2 load(int tile, int line, int i) const {
-        const int row = tile * lines_per_tile + line;
-        return row < nrows ? op.load(row, i) : make_float2(0.0f, 0.0f);
+        const int l = tile * lines_per_tile + line;
+        return l < nrows ? op.load(row_of(tile, line), i) : make_float2(0.0f, 0.0f);
     }
     __device__ void store(int tile, int line, int i, float2 v) const {
-        const int row = tile * lines_per_tile + line;
-        if (row < nrows) op.store(row, i, v);
+        const int l = tile * lines_per_tile + line;
+        if (l < nrows) op.store(row_of(tile, line), i, v);
     }
 };
 
 template <class RowOp>
-void rows_pass(hipStream_t s, const float2* tw, int width, int nrows, const RowOp& op, bool inverse) {
+void rows_pass(hipStream_t s, const float2* tw, int width, int nrows, const RowOp& op, bool inverse,
+               int split = 1 << 30, int gap = 0) {
     const int nl = row_tile_lines(width);
-    RowIO<RowOp> io{op, nl, nrows};
+    RowIO<RowOp> io{op, nl, nrows, split, gap};
     const int ntiles = int(div_up(nrows, nl));
     ProfileScope ps(std::string("fft-row-pass") + (inverse ? "-inv" : "-fwd"), 16.0 * double(width) * double(nrows));
-    if (inverse) launch_tiles<true, true, 4, kMaxFftLog2>(s, width, io, tw, ntiles);
-    else launch_tiles<true, false, 4, kMaxFftLog2>(s, width, io, tw, ntiles);
+    if (inverse) launch_tiles<true, true>(s, width, io, tw, ntiles);
+    else launch_tiles<true, false>(s, width, io, tw, ntiles);
 }
 
 struct PlainRows {

@@ -293,9 +293,12 @@ Grid make_grid(int w, int h, int support) {
     g.w = w;
     g.h = h;
     const int n = 2 * support + 1;
-    g.pw = std::max(16, pow2_at_least(w + n - 1));
-    g.ph = std::max(16, pow2_at_least(h + n - 1));
-    if (g.pw > kMaxFft || g.ph > kMaxFft) fail(LDH_ERR_TOO_LARGE, "frame-too-large-for-fft-grid");
+    // Any grid of at least (w + n - 1) x (h + n - 1) holds the linear convolution without wrap,
+    // so the cropped result is the reference's (which pads to powers of two, :1168-1169); the
+    // smallest 2^a * 3^b length keeps the transforms short.
+    g.pw = grid_length_at_least(std::max(16, w + n - 1));
+    g.ph = grid_length_at_least(std::max(16, h + n - 1));
+    if (g.pw == 0 || g.ph == 0) fail(LDH_ERR_TOO_LARGE, "frame-too-large-for-fft-grid");
     return g;
 }
 
@@ -315,8 +318,13 @@ void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<c
         float2* dst = fp16 ? scratch.as<float2>() : out.data.as<float2>() + cells * p;
         const float* kb = 2 * p + 1 < n ? kernels[size_t(2 * p + 1)] : nullptr;
         KernelRows op{kernels[size_t(2 * p)], kb, dst, K, K / 2, radius, g.pw, g.ph};
-        rows_pass(s, tw, g.pw, g.ph, op, false);
-        cols_forward(s, tw, dst, g.pw, g.ph, PlainLoad{});
+        // only the rows [0, r] and [ph - r, ph) of the wrapped kernel are non-zero
+        const int band = 2 * radius + 1;
+        rows_pass(s, tw, g.pw, band, op, false, radius + 1, g.ph - band);
+        ColRows zero;
+        zero.zero_lo = radius + 1;
+        zero.zero_hi = g.ph - radius;
+        cols_forward(s, tw, dst, g.pw, g.ph, PlainLoad{}, zero);
         if (fp16) {
             hipLaunchKernelGGL(to_half_kernel, grid1(int64_t(cells)), dim3(256), 0, s, dst,
                                reinterpret_cast<__half2*>(out.data.as<__half2>()) + cells * p, int64_t(cells));
@@ -423,8 +431,12 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
         float2* spec = fw.image_spec.as<float2>() + cells * j;
         FrameRows op{src, fw.mask.as<float>(), spec, w, h, g.pw, p.input_transfer, p.extraction_mode,
                      pl.spectral ? 1 : 0, j == 0 ? 0 : 2, j == 0 ? 1 : -1, redist_scale};
-        rows_pass(s, tw, g.pw, g.ph, op, false);
-        cols_forward(s, tw, spec, g.pw, g.ph, PlainLoad{});
+        // rows >= h of the padded image are zero: not transformed, read as zero by the columns
+        rows_pass(s, tw, g.pw, h, op, false);
+        ColRows zero;
+        zero.zero_lo = h;
+        zero.zero_hi = g.ph;
+        cols_forward(s, tw, spec, g.pw, g.ph, PlainLoad{}, zero);
     }
 
     // 3-4. per kernel set: spectra x image spectra -> planes -> 3 effect planes
@@ -481,8 +493,11 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
             for (int j = 0; j < nimg; ++j) {
                 const float2* ispec = fw.image_spec.as<float2>() + cells * j;
                 float2* o = fw.out_spec.as<float2>();
-                if (half) cols_inverse(s, tw, ispec, o, g.pw, g.ph, MulSpectrumHalfLoad{static_cast<const __half2*>(kspec)});
-                else cols_inverse(s, tw, ispec, o, g.pw, g.ph, MulSpectrumLoad{static_cast<const float2*>(kspec)});
+                // only rows < h reach the crop
+                if (half)
+                    cols_inverse(s, tw, ispec, o, g.pw, g.ph, MulSpectrumHalfLoad{static_cast<const __half2*>(kspec)}, h);
+                else
+                    cols_inverse(s, tw, ispec, o, g.pw, g.ph, MulSpectrumLoad{static_cast<const float2*>(kspec)}, h);
                 float* oa;
                 float* ob;
                 if (!pl.spectral) {

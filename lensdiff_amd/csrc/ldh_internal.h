@@ -94,19 +94,43 @@ private:
 struct Profiler;
 
 // ---- FFT (fft.hip) ---------------------------------------------------------------------------
-// Twiddle tables live in device memory per context: tw[j] = exp(-2 pi i j / kMaxFft).
+// Transform lengths are 2^a * 3^b with b <= 2, up to kMaxFft. One twiddle table serves them all:
+// tw[j] = exp(-2 pi i j / kTwiddleN), and every supported length divides kTwiddleN.
 constexpr int kMaxFftLog2 = 14;
 constexpr int kMaxFft = 1 << kMaxFftLog2;   // 16384 points
+constexpr int kTwiddleN = 9 * kMaxFft;      // 147456
+
+// 2^a * 3^b, b <= 2, in [2, kMaxFft]
+__host__ __device__ constexpr bool fft_length_ok(int n) {
+    if (n < 2 || n > kMaxFft) return false;
+    while ((n & 1) == 0) n >>= 1;
+    return n == 1 || n == 3 || n == 9;
+}
+// Grid lengths: powers of two from 8, 3 * 2^a from 48, 9 * 2^a from 144 (the lengths the
+// row and column kernels are built for; rows also need at least 16).
+__host__ __device__ constexpr bool grid_length_ok(int n) {
+    if (!fft_length_ok(n)) return false;
+    int odd = n;
+    while ((odd & 1) == 0) odd >>= 1;
+    return odd == 1 ? n >= 8 : (odd == 3 ? n >= 48 : n >= 144);
+}
+// Smallest grid length >= v (0 when none fits kMaxFft).
+inline int grid_length_at_least(int v) {
+    int best = 0;
+    for (int odd : {1, 3, 9})
+        for (int n = odd; n <= kMaxFft; n *= 2)
+            if (n >= v && grid_length_ok(n) && (best == 0 || n < best)) best = n;
+    return best;
+}
 
 struct FftTables {
-    DeviceBuffer twiddle;  // float2[kMaxFft]
+    DeviceBuffer twiddle;  // float2[kTwiddleN]
     void init(hipStream_t s);
 };
 
-// 2D complex FFT of a height x width array (row pitch = width), both powers of two in
-// [2, kMaxFft]. Forward leaves the spectrum with its column index in "digit-split" order when
-// split_columns is set (see fft.hip); the matching inverse takes that order back. Inverse is
-// scaled by 1/(width*height).
+// 2D complex FFT of a height x width array (row pitch = width), both grid lengths. Forward
+// leaves the spectrum rows in the column transform's digit-split order (fft_col_storage_row);
+// the matching inverse takes that order back. Inverse is unscaled.
 void fft2d_forward(hipStream_t s, const FftTables& t, float2* data, int width, int height);
 void fft2d_inverse(hipStream_t s, const FftTables& t, float2* data, int width, int height);
 // Row or column passes only (used by fused pipelines).

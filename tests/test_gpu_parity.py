@@ -27,7 +27,9 @@ def crel(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-@pytest.mark.parametrize("w,h", [(16, 8), (64, 64), (256, 128), (128, 1024), (2048, 512), (16384, 16), (32, 16384)])
+@pytest.mark.parametrize("w,h", [(16, 8), (64, 64), (256, 128), (128, 1024), (2048, 512), (16384, 16), (32, 16384),
+                                 (48, 48), (144, 96), (192, 288), (6144, 48), (96, 4608), (12288, 16), (16, 9216),
+                                 (576, 1152), (768, 144)])
 def test_fft2d_matches_numpy(ctx, w, h):
     rng = np.random.default_rng(w * 7 + h)
     z = rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))
@@ -237,3 +239,18 @@ def test_c0_frame_matches_oracle(ctx, oracle):
     want64 = oracle.render_image(p, img, fp64=True)
     # stated, not asserted: both FP32 results against the FP64 oracle
     print("c0 rel L2 vs FP64: gpu %.3g, reference-fp32 %.3g" % (rel_l2(got, want64), rel_l2(want, want64)))
+
+
+def test_render_on_mixed_radix_grid(ctx, oracle):
+    """400x180 frame, kernel radius up to 60: the device grid has 3*2^a / 9*2^a sides (384 rows)
+    where the reference pads to powers of two; the cropped linear convolution is the same."""
+    p = workloads.base_params(180)
+    p.pupil_resolution = 96
+    p.spectral_mode = 1
+    p.max_kernel_radius_px = 33.3
+    p.set_noll({4: 0.2, 8: 0.1})
+    img = workloads.frame(400, 180, 0.003, seed=9)
+    assert oracle.sizes(p).max_kernel_radius_px == 60
+    got = ctx.render_image(p, img)
+    want = oracle.render_image(p, img)
+    assert rel_l2(got, want) < TOL, rel_l2(got, want)
