@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times (HBM counter calibration) and exit")
+    ap.add_argument("--bank-probe", action="store_true",
+                    help="only build two fresh 31-wavelength UHD banks (profiling the PSF path) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -121,6 +123,12 @@ def main():
 
     from lensdiff_amd import workloads
     from lensdiff_amd.backend import Context
+    if args.bank_probe:
+        ctx = Context(0)
+        ms = [round(ctx.bank_build_seconds(lens(5000 + i)) * 1e3, 2) for i in range(2)]
+        print(json.dumps({"bank_build_ms": ms, "autotune": ctx.tuning_report()}))
+        ctx.close()
+        return
     from oracle.oracle import rel_l2
 
     t_bench = time.perf_counter()
@@ -163,6 +171,17 @@ def main():
     plain_fps = len(plain_service) / sum(plain_service)
 
     progress("plain: %d frames, %.2f frames/s" % (len(plain_service), plain_fps))
+    # ---- breakdown: fresh 31-wavelength banks alone, and a plain frame whose bank is cached ----
+    build_s = sorted(ctx.bank_build_seconds(L(5000 + i)) for i in range(3))
+    ctx.render_device(L(10), d_src[0], d_plain, w, h)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.render_device(L(10), d_src[0], d_plain, w, h)
+    ctx.synchronize()
+    plain_cached_ms = (time.perf_counter() - t0) * 1e3
+    breakdown = {"bank_build_ms": round(build_s[1] * 1e3, 2), "psfs_per_s": round(31 / build_s[1], 1),
+                 "plain_frame_cached_bank_ms": round(plain_cached_ms, 2)}
+    progress("breakdown: %s" % breakdown)
     # ---- served ----
     ctx.server_start(args.slots, 64 << 30)
     d_dst = [ctx.alloc(nbytes) for _ in range(args.steps)]
@@ -234,6 +253,8 @@ def main():
                          "spectra_hits": sstats.spectra_hits, "resident_gb": round(sstats.resident_bytes / 2 ** 30, 2),
                          "evictions": sstats.evictions},
         "served_vs_plain_rel_l2": served_vs_plain,
+        "breakdown": breakdown,
+        "autotune": ctx.tuning_report(),
         "roofline": dominant(served_stats),
         "roofline_plain": dominant(plain_stats),
         "kernel_families": {k: {"launches": v[0], "ms": round(v[1], 3),

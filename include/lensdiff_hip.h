@@ -7,7 +7,7 @@
  * `std::string* error` convention (source/src/core/LensDiffCpuReference.cpp:2077-2084).
  *
  * The plugin-facing C++ seam, RunLensDiffHip(request, params, cache, error), is a thin layer
- * above these calls (lensdiff_amd/csrc/LensDiffHip.h); it mirrors
+ * above these calls (lensdiff_amd/plugin/LensDiffHip.h); it mirrors
  *   bool RunLensDiffCpuReference(const LensDiffRenderRequest&, const LensDiffParams&,
  *                                LensDiffPsfBankCache&, std::string*)
  *   (source/src/core/LensDiffCpuReference.h:64-67) and
@@ -136,8 +136,9 @@ typedef struct ldh_params {
     /* Zernike terms the reference basis lacks (LensDiffPhase.cpp:182-197), in waves:
      * Noll 2 (tilt x), 3 (tilt y), 18, 19, 20, 21. */
     double zernike_extra[6];
-    /* Side of the square FFT tile used by overlap-save convolution; 0 = the reference's
-     * nextPow2(W+k-1) x nextPow2(H+k-1) padding (LensDiffCpuReference.cpp:1168-1169). */
+    /* Reserved, must be 0. The convolution grid is the smallest 2^a*3^b (b <= 2) size
+     * >= (W+k-1) x (H+k-1); the reference pads to powers of two
+     * (LensDiffCpuReference.cpp:1168-1169), the cropped result is the same convolution. */
     int32_t fft_tile;
     /* 1 = store cached kernel spectra as FP16 (FP32 accumulation). */
     int32_t spectra_fp16;
@@ -276,6 +277,10 @@ ldh_status ldh_profiling_enable(ldh_context* ctx, int32_t enable);
 /* Returns the number of families written (<= capacity). */
 int32_t ldh_profiling_read(ldh_context* ctx, ldh_kernel_stat* out, int32_t capacity);
 ldh_status ldh_profiling_reset(ldh_context* ctx);
+/* Kernel-variant choices of the process-wide autotuner (no counterpart in the reference): one
+ * line per tuned key, "key@devN: vK  t0 t1 ... ms". Writes at most capacity - 1 bytes plus a NUL
+ * into buf (buf may be NULL when capacity is 0); returns the full length. */
+int32_t ldh_tuning_report(char* buf, int32_t capacity);
 
 #ifdef __cplusplus
 }

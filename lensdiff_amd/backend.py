@@ -9,6 +9,7 @@ used by the parity tests.
 from __future__ import annotations
 
 import ctypes as C
+import time
 
 import numpy as np
 
@@ -139,6 +140,17 @@ class Context:
         finally:
             self.lib.ldh_bank_release(self.h, b)
 
+    def bank_build_seconds(self, p) -> float:
+        """Wall time of building the bank for `p` on the device (synchronised), bank released."""
+        b = C.c_void_p()
+        self.synchronize()
+        t0 = time.perf_counter()
+        self._check(self.lib.ldh_bank_acquire(self.h, C.byref(p), C.byref(b)))
+        self.synchronize()
+        dt = time.perf_counter() - t0
+        self.lib.ldh_bank_release(self.h, b)
+        return dt
+
     # ---- render ------------------------------------------------------------------------------
     def render_image(self, p, rgba: np.ndarray) -> np.ndarray:
         """Whole-frame render of a packed RGBA float32 image through the host-view entry."""
@@ -220,6 +232,18 @@ class Context:
 
     def profile_reset(self):
         self._check(self.lib.ldh_profiling_reset(self.h))
+
+    def tuning_report(self) -> dict:
+        """The autotuner's choices: {key: {"variant": k, "ms": [per-variant times]}}."""
+        n = self.lib.ldh_tuning_report(None, 0)
+        buf = C.create_string_buffer(n + 1)
+        self.lib.ldh_tuning_report(buf, n + 1)
+        out = {}
+        for line in buf.value.decode().splitlines():
+            key, rest = line.split(": ", 1)
+            parts = rest.split()
+            out[key] = {"variant": int(parts[0][1:]), "ms": [float(x) for x in parts[1:-1]]}
+        return out
 
 
 def make_request(src: np.ndarray, dst: np.ndarray, render_window=None, src_origin=(0, 0),

@@ -10,6 +10,7 @@
 #include <memory>
 #include <mutex>
 
+#include "autotune.h"
 #include "bank.h"
 #include "fft_io.h"
 #include "frame.h"
@@ -563,6 +564,16 @@ ldh_status ldh_profiling_reset(ldh_context* ctx) {
         sync(ctx, "profiling-reset");
         ctx->prof.reset();
     });
+}
+
+int32_t ldh_tuning_report(char* buf, int32_t capacity) {
+    const std::string r = ldh::Autotuner::instance().report();
+    if (buf != nullptr && capacity > 0) {
+        const size_t n = std::min(r.size(), size_t(capacity - 1));
+        std::memcpy(buf, r.data(), n);
+        buf[n] = '\0';
+    }
+    return int32_t(r.size());
 }
 
 }  // extern "C"

@@ -7,14 +7,17 @@
 namespace ldh {
 
 void FftTables::init(hipStream_t s) {
-    std::vector<float2> h(kTwiddleN);
-    for (int j = 0; j < kTwiddleN; ++j) {
-        // exp(-2 pi i j / kTwiddleN), computed in double and rounded once
-        const double a = -2.0 * M_PI * double(j) / double(kTwiddleN);
-        h[size_t(j)] = make_float2(float(std::cos(a)), float(std::sin(a)));
+    std::vector<float2> h(kTwiddleEntries);
+    for (const int t : {kTw2, kTw3, kTw9}) {
+        const int off = tw_offset(t);
+        for (int j = 0; j < t; ++j) {
+            // exp(-2 pi i j / t), computed in double and rounded once
+            const double a = -2.0 * M_PI * double(j) / double(t);
+            h[size_t(off + j)] = make_float2(float(std::cos(a)), float(std::sin(a)));
+        }
     }
-    twiddle.reset(sizeof(float2) * kTwiddleN);
-    LDH_HIP(hipMemcpyAsync(twiddle.get(), h.data(), sizeof(float2) * kTwiddleN, hipMemcpyHostToDevice, s),
+    twiddle.reset(sizeof(float2) * kTwiddleEntries);
+    LDH_HIP(hipMemcpyAsync(twiddle.get(), h.data(), sizeof(float2) * kTwiddleEntries, hipMemcpyHostToDevice, s),
             "fft-table-upload");
     LDH_HIP(hipStreamSynchronize(s), "fft-table-upload");
 }

@@ -14,6 +14,14 @@ namespace ldh {
 
 __device__ __forceinline__ int lds_pad(int x) { return x + (x >> 5); }
 
+// Padded position of element base + r * STEP, given pbase = lds_pad(base). When STEP is a
+// multiple of 32 the padding is linear in r, so the offsets fold into the LDS instructions'
+// immediate field instead of costing address arithmetic per access.
+template <int STEP> __device__ __forceinline__ int lds_pad_step(int base, int pbase, int r) {
+    if constexpr (STEP % 32 == 0) return pbase + r * (STEP + STEP / 32);
+    else return lds_pad(base + r * STEP);
+}
+
 // multiply by -i (forward) or +i (inverse)
 template <bool INV> __device__ __forceinline__ float2 rot90(float2 a) {
     return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
@@ -94,6 +102,7 @@ template <bool INV> __device__ __forceinline__ void dft16(float2* v) {
 }
 
 template <bool INV> __device__ __forceinline__ void dft3(float2& a0, float2& a1, float2& a2) {
+#pragma clang fp contract(fast)
     const float s = 0.86602540378443865f;  // sin(2 pi / 3)
     const float2 t = make_float2(a1.x + a2.x, a1.y + a2.y);
     const float2 m = make_float2(a0.x - 0.5f * t.x, a0.y - 0.5f * t.y);
@@ -137,8 +146,10 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
         line_of[p] = line;
         j_of[p] = j;
         if (TOTAL % NT == 0 || u < TOTAL) {
+            const int b = line * LS + j * IS;
+            const int pb = lds_pad(b);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[p][r] = lds[lds_pad(line * LS + (j + r * BF) * IS)];
+            for (int r = 0; r < R; ++r) v[p][r] = lds[lds_pad_step<BF * IS>(b, pb, r)];
         }
     }
     __syncthreads();
@@ -149,18 +160,20 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
             const int j = j_of[p];
             const int k = j % Ns;
             if constexpr (Ns > 1) {
-                constexpr int stride = kTwiddleN / (Ns * R);
+                constexpr int stride = tw_table(N) / (Ns * R);
+                const float2* __restrict__ t = tw + tw_offset(N);
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    float2 w = tw[k * r * stride];
+                    float2 w = t[k * r * stride];
                     if (INV) w.y = -w.y;
                     v[p][r] = cmul(v[p][r], w);
                 }
             }
             dft<R, INV>(v[p]);
-            const int base = (j - k) * R + k;
+            const int b = line_of[p] * LS + ((j - k) * R + k) * IS;
+            const int pb = lds_pad(b);
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[lds_pad(line_of[p] * LS + (base + r * Ns) * IS)] = v[p][r];
+            for (int r = 0; r < R; ++r) lds[lds_pad_step<Ns * IS>(b, pb, r)] = v[p][r];
         }
     }
     __syncthreads();

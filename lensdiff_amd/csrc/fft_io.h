@@ -58,7 +58,7 @@ struct ColIO {
     int groups;       // line groups (g values) per tile
     int gcount;       // line groups in the pass: tiles past it (a partial last tile) do nothing
     int g_row_mul, i_row_mul;
-    int tw_stride;    // kTwiddleN / N
+    int tw_stride;    // table length / N for the twiddles W_N^(g i) of the split
     ColRows rows;
     Load op;
     __device__ int row_of(int tile, int line, int i, int* g_out, int* col) const {
@@ -92,8 +92,8 @@ void col_pass(hipStream_t s, const float2* tw, const float2* src, float2* data, 
               int g_row_mul, int i_row_mul, int height, const Load& op, ColRows rows, bool inverse) {
     const int nl = col_tile_lines(npts);
     const int groups = nl / kColWidth;
-    ColIO<Load, TW> io{src, data, tw, width, width / kColWidth, groups, gcount, g_row_mul, i_row_mul,
-                       kTwiddleN / height, rows, op};
+    ColIO<Load, TW> io{src, data, tw + tw_offset(height), width, width / kColWidth, groups, gcount, g_row_mul,
+                       i_row_mul, tw_table(height) / height, rows, op};
     const int ntiles = (width / kColWidth) * int(div_up(gcount, groups));
     // algorithmic bytes: elements read (outside the zero band) and written (below store_rows)
     const double all = double(width) * double(height);

@@ -1,0 +1,214 @@
+// Single-line FFTs held in LDS for the whole transform (the PSF line kernels), planned as one
+// composite first stage and radix-16 stages: M = R0 * 16^S with R0 in {2..48} (6144 = 24*16*16,
+// 8192 = 32*16*16, 3072 = 12*16*16). Stage inputs and outputs go through policies, so the first
+// stage can read its (mostly zero) input straight from global memory and the last stage can
+// write straight to global memory, and pointwise work is fused into whichever stage touches
+// the data.
+#pragma once
+
+#include "fft_core.h"
+
+namespace ldh {
+
+// exp(-2 pi i m / R) for the composite radices: cos and sin of 2 pi m / R rounded to float.
+template <int R> struct Roots;
+template <> struct Roots<6> {
+    static constexpr float c[6] = {1.000000000e+00f, 5.000000000e-01f, -5.000000000e-01f, -1.000000000e+00f, -5.000000000e-01f, 5.000000000e-01f};
+    static constexpr float s[6] = {0.000000000e+00f, 8.660254038e-01f, 8.660254038e-01f, 1.224646799e-16f, -8.660254038e-01f, -8.660254038e-01f};
+};
+template <> struct Roots<9> {
+    static constexpr float c[9] = {1.000000000e+00f, 7.660444431e-01f, 1.736481777e-01f, -5.000000000e-01f, -9.396926208e-01f, -9.396926208e-01f, -5.000000000e-01f, 1.736481777e-01f, 7.660444431e-01f};
+    static constexpr float s[9] = {0.000000000e+00f, 6.427876097e-01f, 9.848077530e-01f, 8.660254038e-01f, 3.420201433e-01f, -3.420201433e-01f, -8.660254038e-01f, -9.848077530e-01f, -6.427876097e-01f};
+};
+template <> struct Roots<12> {
+    static constexpr float c[12] = {1.000000000e+00f, 8.660254038e-01f, 5.000000000e-01f, 6.123233996e-17f, -5.000000000e-01f, -8.660254038e-01f, -1.000000000e+00f, -8.660254038e-01f, -5.000000000e-01f, -1.836970199e-16f, 5.000000000e-01f, 8.660254038e-01f};
+    static constexpr float s[12] = {0.000000000e+00f, 5.000000000e-01f, 8.660254038e-01f, 1.000000000e+00f, 8.660254038e-01f, 5.000000000e-01f, 1.224646799e-16f, -5.000000000e-01f, -8.660254038e-01f, -1.000000000e+00f, -8.660254038e-01f, -5.000000000e-01f};
+};
+template <> struct Roots<18> {
+    static constexpr float c[18] = {1.000000000e+00f, 9.396926208e-01f, 7.660444431e-01f, 5.000000000e-01f, 1.736481777e-01f, -1.736481777e-01f, -5.000000000e-01f, -7.660444431e-01f, -9.396926208e-01f, -1.000000000e+00f, -9.396926208e-01f, -7.660444431e-01f, -5.000000000e-01f, -1.736481777e-01f, 1.736481777e-01f, 5.000000000e-01f, 7.660444431e-01f, 9.396926208e-01f};
+    static constexpr float s[18] = {0.000000000e+00f, 3.420201433e-01f, 6.427876097e-01f, 8.660254038e-01f, 9.848077530e-01f, 9.848077530e-01f, 8.660254038e-01f, 6.427876097e-01f, 3.420201433e-01f, 1.224646799e-16f, -3.420201433e-01f, -6.427876097e-01f, -8.660254038e-01f, -9.848077530e-01f, -9.848077530e-01f, -8.660254038e-01f, -6.427876097e-01f, -3.420201433e-01f};
+};
+template <> struct Roots<24> {
+    static constexpr float c[24] = {1.000000000e+00f, 9.659258263e-01f, 8.660254038e-01f, 7.071067812e-01f, 5.000000000e-01f, 2.588190451e-01f, 6.123233996e-17f, -2.588190451e-01f, -5.000000000e-01f, -7.071067812e-01f, -8.660254038e-01f, -9.659258263e-01f, -1.000000000e+00f, -9.659258263e-01f, -8.660254038e-01f, -7.071067812e-01f, -5.000000000e-01f, -2.588190451e-01f, -1.836970199e-16f, 2.588190451e-01f, 5.000000000e-01f, 7.071067812e-01f, 8.660254038e-01f, 9.659258263e-01f};
+    static constexpr float s[24] = {0.000000000e+00f, 2.588190451e-01f, 5.000000000e-01f, 7.071067812e-01f, 8.660254038e-01f, 9.659258263e-01f, 1.000000000e+00f, 9.659258263e-01f, 8.660254038e-01f, 7.071067812e-01f, 5.000000000e-01f, 2.588190451e-01f, 1.224646799e-16f, -2.588190451e-01f, -5.000000000e-01f, -7.071067812e-01f, -8.660254038e-01f, -9.659258263e-01f, -1.000000000e+00f, -9.659258263e-01f, -8.660254038e-01f, -7.071067812e-01f, -5.000000000e-01f, -2.588190451e-01f};
+};
+template <> struct Roots<32> {
+    static constexpr float c[32] = {1.000000000e+00f, 9.807852804e-01f, 9.238795325e-01f, 8.314696123e-01f, 7.071067812e-01f, 5.555702330e-01f, 3.826834324e-01f, 1.950903220e-01f, 6.123233996e-17f, -1.950903220e-01f, -3.826834324e-01f, -5.555702330e-01f, -7.071067812e-01f, -8.314696123e-01f, -9.238795325e-01f, -9.807852804e-01f, -1.000000000e+00f, -9.807852804e-01f, -9.238795325e-01f, -8.314696123e-01f, -7.071067812e-01f, -5.555702330e-01f, -3.826834324e-01f, -1.950903220e-01f, -1.836970199e-16f, 1.950903220e-01f, 3.826834324e-01f, 5.555702330e-01f, 7.071067812e-01f, 8.314696123e-01f, 9.238795325e-01f, 9.807852804e-01f};
+    static constexpr float s[32] = {0.000000000e+00f, 1.950903220e-01f, 3.826834324e-01f, 5.555702330e-01f, 7.071067812e-01f, 8.314696123e-01f, 9.238795325e-01f, 9.807852804e-01f, 1.000000000e+00f, 9.807852804e-01f, 9.238795325e-01f, 8.314696123e-01f, 7.071067812e-01f, 5.555702330e-01f, 3.826834324e-01f, 1.950903220e-01f, 1.224646799e-16f, -1.950903220e-01f, -3.826834324e-01f, -5.555702330e-01f, -7.071067812e-01f, -8.314696123e-01f, -9.238795325e-01f, -9.807852804e-01f, -1.000000000e+00f, -9.807852804e-01f, -9.238795325e-01f, -8.314696123e-01f, -7.071067812e-01f, -5.555702330e-01f, -3.826834324e-01f, -1.950903220e-01f};
+};
+template <> struct Roots<36> {
+    static constexpr float c[36] = {1.000000000e+00f, 9.848077530e-01f, 9.396926208e-01f, 8.660254038e-01f, 7.660444431e-01f, 6.427876097e-01f, 5.000000000e-01f, 3.420201433e-01f, 1.736481777e-01f, 6.123233996e-17f, -1.736481777e-01f, -3.420201433e-01f, -5.000000000e-01f, -6.427876097e-01f, -7.660444431e-01f, -8.660254038e-01f, -9.396926208e-01f, -9.848077530e-01f, -1.000000000e+00f, -9.848077530e-01f, -9.396926208e-01f, -8.660254038e-01f, -7.660444431e-01f, -6.427876097e-01f, -5.000000000e-01f, -3.420201433e-01f, -1.736481777e-01f, -1.836970199e-16f, 1.736481777e-01f, 3.420201433e-01f, 5.000000000e-01f, 6.427876097e-01f, 7.660444431e-01f, 8.660254038e-01f, 9.396926208e-01f, 9.848077530e-01f};
+    static constexpr float s[36] = {0.000000000e+00f, 1.736481777e-01f, 3.420201433e-01f, 5.000000000e-01f, 6.427876097e-01f, 7.660444431e-01f, 8.660254038e-01f, 9.396926208e-01f, 9.848077530e-01f, 1.000000000e+00f, 9.848077530e-01f, 9.396926208e-01f, 8.660254038e-01f, 7.660444431e-01f, 6.427876097e-01f, 5.000000000e-01f, 3.420201433e-01f, 1.736481777e-01f, 1.224646799e-16f, -1.736481777e-01f, -3.420201433e-01f, -5.000000000e-01f, -6.427876097e-01f, -7.660444431e-01f, -8.660254038e-01f, -9.396926208e-01f, -9.848077530e-01f, -1.000000000e+00f, -9.848077530e-01f, -9.396926208e-01f, -8.660254038e-01f, -7.660444431e-01f, -6.427876097e-01f, -5.000000000e-01f, -3.420201433e-01f, -1.736481777e-01f};
+};
+template <> struct Roots<48> {
+    static constexpr float c[48] = {1.000000000e+00f, 9.914448614e-01f, 9.659258263e-01f, 9.238795325e-01f, 8.660254038e-01f, 7.933533403e-01f, 7.071067812e-01f, 6.087614290e-01f, 5.000000000e-01f, 3.826834324e-01f, 2.588190451e-01f, 1.305261922e-01f, 6.123233996e-17f, -1.305261922e-01f, -2.588190451e-01f, -3.826834324e-01f, -5.000000000e-01f, -6.087614290e-01f, -7.071067812e-01f, -7.933533403e-01f, -8.660254038e-01f, -9.238795325e-01f, -9.659258263e-01f, -9.914448614e-01f, -1.000000000e+00f, -9.914448614e-01f, -9.659258263e-01f, -9.238795325e-01f, -8.660254038e-01f, -7.933533403e-01f, -7.071067812e-01f, -6.087614290e-01f, -5.000000000e-01f, -3.826834324e-01f, -2.588190451e-01f, -1.305261922e-01f, -1.836970199e-16f, 1.305261922e-01f, 2.588190451e-01f, 3.826834324e-01f, 5.000000000e-01f, 6.087614290e-01f, 7.071067812e-01f, 7.933533403e-01f, 8.660254038e-01f, 9.238795325e-01f, 9.659258263e-01f, 9.914448614e-01f};
+    static constexpr float s[48] = {0.000000000e+00f, 1.305261922e-01f, 2.588190451e-01f, 3.826834324e-01f, 5.000000000e-01f, 6.087614290e-01f, 7.071067812e-01f, 7.933533403e-01f, 8.660254038e-01f, 9.238795325e-01f, 9.659258263e-01f, 9.914448614e-01f, 1.000000000e+00f, 9.914448614e-01f, 9.659258263e-01f, 9.238795325e-01f, 8.660254038e-01f, 7.933533403e-01f, 7.071067812e-01f, 6.087614290e-01f, 5.000000000e-01f, 3.826834324e-01f, 2.588190451e-01f, 1.305261922e-01f, 1.224646799e-16f, -1.305261922e-01f, -2.588190451e-01f, -3.826834324e-01f, -5.000000000e-01f, -6.087614290e-01f, -7.071067812e-01f, -7.933533403e-01f, -8.660254038e-01f, -9.238795325e-01f, -9.659258263e-01f, -9.914448614e-01f, -1.000000000e+00f, -9.914448614e-01f, -9.659258263e-01f, -9.238795325e-01f, -8.660254038e-01f, -7.933533403e-01f, -7.071067812e-01f, -6.087614290e-01f, -5.000000000e-01f, -3.826834324e-01f, -2.588190451e-01f, -1.305261922e-01f};
+};
+
+template <int R, bool INV> __device__ __forceinline__ float2 root(int m) {
+    const int i = m % R;
+    return make_float2(Roots<R>::c[i], INV ? Roots<R>::s[i] : -Roots<R>::s[i]);
+}
+
+template <int R, bool INV> __device__ __forceinline__ void dft_any(float2* v);
+
+// R = A * B in registers: n = B n1 + n2, k = k1 + A k2:
+//   X[k1 + A k2] = sum_n2 W_B^(n2 k2) W_R^(n2 k1) sum_n1 v[B n1 + n2] W_A^(n1 k1)
+template <int A, int B, bool INV> __device__ __forceinline__ void dft_ab(float2* v) {
+#pragma clang fp contract(fast)
+    constexpr int R = A * B;
+    float2 y[R];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+        float2 a[A];
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) a[n1] = v[B * n1 + n2];
+        dft_any<A, INV>(a);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) y[n2 * A + k1] = (n2 == 0 || k1 == 0) ? a[k1] : cmul(a[k1], root<R, INV>(n2 * k1));
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+        float2 b[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) b[n2] = y[n2 * A + k1];
+        dft_any<B, INV>(b);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = b[k2];
+    }
+}
+
+template <int R, bool INV> __device__ __forceinline__ void dft_any(float2* v) {
+    if constexpr (R == 2 || R == 3 || R == 4 || R == 8 || R == 16) dft<R, INV>(v);
+    else if constexpr (R == 6) dft_ab<2, 3, INV>(v);
+    else if constexpr (R == 9) dft_ab<3, 3, INV>(v);
+    else if constexpr (R == 12) dft_ab<4, 3, INV>(v);
+    else if constexpr (R == 18) dft_ab<2, 9, INV>(v);
+    else if constexpr (R == 24) dft_ab<8, 3, INV>(v);
+    else if constexpr (R == 32) dft_ab<2, 16, INV>(v);
+    else if constexpr (R == 36) dft_ab<4, 9, INV>(v);
+    else dft_ab<16, 3, INV>(v);  // 48
+}
+
+// M = R0 * 16^S with the fewest stages and R0 <= 48 (plannable: M is 2^a * 3^b, b <= 2,
+// except lengths like 1152 = 72 * 16).
+template <int M> struct LinePlan {
+    static constexpr int S = (M % 4096 == 0 && M / 4096 >= 2 && M / 256 > 48) ? 3
+                             : (M % 256 == 0 && M / 256 >= 2 && M / 16 > 48) ? 2
+                             : (M % 16 == 0 && M / 16 >= 2) ? 1 : 0;
+    static constexpr int R0 = S == 3 ? M / 4096 : S == 2 ? M / 256 : S == 1 ? M / 16 : M;
+    static constexpr bool ok = R0 <= 48 && (R0 == 2 || R0 == 3 || R0 == 4 || R0 == 6 || R0 == 8 || R0 == 9 ||
+                                            R0 == 12 || R0 == 16 || R0 == 18 || R0 == 24 || R0 == 32 || R0 == 36 ||
+                                            R0 == 48);
+    // threads: one butterfly each in the widest stage, at most 512
+    static constexpr int bf = (S > 0 && M / 16 > M / R0) ? M / 16 : M / R0;
+    static constexpr int NT = bf < 64 ? 64 : (bf > 512 ? 512 : (bf + 63) / 64 * 64);
+};
+
+// Stage policies. A stage reads elements base + r * STEP through in.get<STEP>(base, pbase, r)
+// and writes them through out.put<STEP>(base, pbase, r, v), with pbase = lds_pad(base); `lds`
+// tells whether a policy touches the line in LDS. Global policies only need operator().
+template <class F> struct GlobalIn {
+    static constexpr bool lds = false;
+    F f;
+    template <int STEP> __device__ float2 get(int base, int, int r) const { return f(base + r * STEP); }
+};
+template <class F> struct GlobalOut {
+    static constexpr bool lds = false;
+    F f;
+    template <int STEP> __device__ void put(int base, int, int r, float2 v) const { f(base + r * STEP, v); }
+};
+struct LdsIn {
+    static constexpr bool lds = true;
+    float2* line;
+    template <int STEP> __device__ float2 get(int base, int pbase, int r) const {
+        return line[lds_pad_step<STEP>(base, pbase, r)];
+    }
+};
+struct LdsOut {
+    static constexpr bool lds = true;
+    float2* line;
+    template <int STEP> __device__ void put(int base, int pbase, int r, float2 v) const {
+        line[lds_pad_step<STEP>(base, pbase, r)] = v;
+    }
+};
+// Output multiplied by a spectrum (the Bluestein filter).
+struct LdsMulOut {
+    static constexpr bool lds = true;
+    float2* line;
+    const float2* __restrict__ f;
+    template <int STEP> __device__ void put(int base, int pbase, int r, float2 v) const {
+        line[lds_pad_step<STEP>(base, pbase, r)] = cmul(v, f[base + r * STEP]);
+    }
+};
+
+// One Stockham stage over a single line of N points: sub-length Ns, radix R.
+template <int N, int Ns, int R, int NT, bool INV, class In, class Out>
+__device__ __forceinline__ void line_stage(const float2* __restrict__ tw, int tid, const In& in, const Out& out) {
+    constexpr int BF = N / R;
+    constexpr int PER = (BF + NT - 1) / NT;
+    float2 v[PER][R];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int u = tid + p * NT;
+        if (BF % NT == 0 || u < BF) {
+            const int pu = lds_pad(u);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[p][r] = in.template get<BF>(u, pu, r);
+        }
+    }
+    if constexpr (In::lds && Out::lds) __syncthreads();  // every read before the in-place writes
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int u = tid + p * NT;
+        if (BF % NT == 0 || u < BF) {
+            const int k = u % Ns;
+            if constexpr (Ns > 1) {
+                constexpr int stride = tw_table(N) / (Ns * R);
+                const float2* __restrict__ t = tw + tw_offset(N);
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    float2 w = t[k * r * stride];
+                    if (INV) w.y = -w.y;
+                    v[p][r] = cmul(v[p][r], w);
+                }
+            }
+            dft_any<R, INV>(v[p]);
+            const int base = (u - k) * R + k;
+            const int pbase = lds_pad(base);
+#pragma unroll
+            for (int r = 0; r < R; ++r) out.template put<Ns>(base, pbase, r, v[p][r]);
+        }
+    }
+    if constexpr (Out::lds) __syncthreads();
+}
+
+// The whole line transform: stage 0 reads through `in`, the last stage writes through `out`,
+// the stages between keep the line in LDS (`line`, padded).
+template <int M, bool INV, class In, class Out>
+__device__ __forceinline__ void line_fft(float2* line, const float2* __restrict__ tw, int tid, const In& in,
+                                         const Out& out) {
+    using P = LinePlan<M>;
+    static_assert(P::ok, "line length has no composite-first plan");
+    constexpr int NT = P::NT, R0 = P::R0, S = P::S;
+    const LdsIn li{line};
+    const LdsOut lo{line};
+    if constexpr (S == 0) {
+        line_stage<M, 1, R0, NT, INV>(tw, tid, in, out);
+    } else {
+        line_stage<M, 1, R0, NT, INV>(tw, tid, in, lo);
+        if constexpr (S == 1) {
+            line_stage<M, R0, 16, NT, INV>(tw, tid, li, out);
+        } else {
+            line_stage<M, R0, 16, NT, INV>(tw, tid, li, lo);
+            if constexpr (S == 2) {
+                line_stage<M, R0 * 16, 16, NT, INV>(tw, tid, li, out);
+            } else {
+                line_stage<M, R0 * 16, 16, NT, INV>(tw, tid, li, lo);
+                line_stage<M, R0 * 256, 16, NT, INV>(tw, tid, li, out);
+            }
+        }
+    }
+}
+
+}  // namespace ldh

@@ -31,10 +31,14 @@ struct Error {
 inline void fail(ldh_status s, const std::string& tag) { throw Error{s, tag}; }
 
 // ---- small device helpers ------------------------------------------------------------------
+// Complex products may contract to FMA (the library is built with -ffp-contract=off so that the
+// pupil, selection and ring arithmetic keep their float semantics; transforms gain from FMA).
 __host__ __device__ inline float2 cmul(float2 a, float2 b) {
+#pragma clang fp contract(fast)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __host__ __device__ inline float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+#pragma clang fp contract(fast)
     return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
@@ -94,11 +98,23 @@ private:
 struct Profiler;
 
 // ---- FFT (fft.hip) ---------------------------------------------------------------------------
-// Transform lengths are 2^a * 3^b with b <= 2, up to kMaxFft. One twiddle table serves them all:
-// tw[j] = exp(-2 pi i j / kTwiddleN), and every supported length divides kTwiddleN.
+// Transform lengths are 2^a * 3^b with b <= 2, up to kMaxFft. Twiddles come from one buffer
+// holding three tables, one per length family: exp(-2 pi i j / T) for T = 16384 (powers of two),
+// 12288 (3 * 2^a) and 9216 (9 * 2^a). A length N reads its family's table at stride T / N, so
+// the last stage of every transform reads consecutive entries.
 constexpr int kMaxFftLog2 = 14;
 constexpr int kMaxFft = 1 << kMaxFftLog2;   // 16384 points
-constexpr int kTwiddleN = 9 * kMaxFft;      // 147456
+constexpr int kTw2 = 16384, kTw3 = 12288, kTw9 = 9216;
+constexpr int kTwOff2 = 0, kTwOff3 = kTw2, kTwOff9 = kTw2 + kTw3;
+constexpr int kTwiddleEntries = kTw2 + kTw3 + kTw9;
+
+// odd part of n (1, 3 or 9 for supported lengths)
+__host__ __device__ constexpr int odd_part(int n) { return (n & 1) ? n : odd_part(n >> 1); }
+// Table length and offset of the family of length n.
+__host__ __device__ constexpr int tw_table(int n) { return odd_part(n) == 1 ? kTw2 : odd_part(n) == 3 ? kTw3 : kTw9; }
+__host__ __device__ constexpr int tw_offset(int n) {
+    return odd_part(n) == 1 ? kTwOff2 : odd_part(n) == 3 ? kTwOff3 : kTwOff9;
+}
 
 // 2^a * 3^b, b <= 2, in [2, kMaxFft]
 __host__ __device__ constexpr bool fft_length_ok(int n) {
@@ -114,6 +130,14 @@ __host__ __device__ constexpr bool grid_length_ok(int n) {
     while ((odd & 1) == 0) odd >>= 1;
     return odd == 1 ? n >= 8 : (odd == 3 ? n >= 48 : n >= 144);
 }
+// Smallest transform length >= v (at least 128; 0 when none fits kMaxFft).
+inline int fft_length_at_least(int v) {
+    int best = 0;
+    for (int odd : {1, 3, 9})
+        for (int n = odd; n <= kMaxFft; n *= 2)
+            if (n >= v && n >= 128 && fft_length_ok(n) && (best == 0 || n < best)) best = n;
+    return best;
+}
 // Smallest grid length >= v (0 when none fits kMaxFft).
 inline int grid_length_at_least(int v) {
     int best = 0;
@@ -124,7 +148,7 @@ inline int grid_length_at_least(int v) {
 }
 
 struct FftTables {
-    DeviceBuffer twiddle;  // float2[kTwiddleN]
+    DeviceBuffer twiddle;  // float2[kTwiddleEntries]: the three family tables
     void init(hipStream_t s);
 };
 

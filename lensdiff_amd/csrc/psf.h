@@ -7,7 +7,7 @@
 
 namespace ldh {
 
-constexpr int kMaxRawPsf = 8192;  // Bluestein length 16384 is the largest LDS-resident line
+constexpr int kMaxRawPsf = 8192;  // its Bluestein length (R + P - 1 <= 9215) fits one LDS-resident line
 
 // Per (P, R) state: chirp and Bluestein filter spectrum, plus stage buffers.
 class PsfPlan {

@@ -6,14 +6,20 @@
 //   rows:    for each pupil row y, the R-point DFT of its P embedded values  -> G[y][kx]
 //   columns: for each kx, the R-point DFT over the P pupil rows of G        -> F[ky][kx]
 // Each R-point DFT is done in LDS. For power-of-two R it is a plain Stockham FFT of length R.
-// Otherwise it is Bluestein's algorithm with a power-of-two M = nextPow2(2R-1), using the
-// reference's chirp exp(-i pi j^2 / R) evaluated in double with pi rounded to float
-// (LensDiffCpuReference.cpp:244-274) so the sampled spectrum matches the reference's.
+// Otherwise it is Bluestein's algorithm with the reference's chirp exp(-i pi j^2 / R) evaluated
+// in double with pi rounded to float (LensDiffCpuReference.cpp:244-274), which makes the
+// transform sum_n x[n] exp(-2 i pi_f k n / R): the reference's sampled spectrum. Moving the
+// pupil from offset floor((R-P)/2) to 0 multiplies each row transform by exp(-2 i pi_f kx off / R)
+// and each column transform by exp(-2 i pi_f ky off / R): unit-modulus factors that |F|^2
+// drops. With the input at 0 the circular convolution only has to hold lags -(P-1) .. R-1, so
+// the Bluestein length is the smallest FFT length M >= R + P - 1 (6144 for R = 4092, P = 1024,
+// where the reference uses 8192).
 // The column stage writes |F|^2 transposed; a shift-transpose pass then produces the centred
 // raw PSF, which the sequential-sum normalisation scales to unit energy.
 #include <cmath>
 
-#include "fft_core.h"
+#include "autotune.h"
+#include "fft_line.h"
 #include "psf.h"
 #include "pupil.h"
 
@@ -21,84 +27,109 @@ namespace ldh {
 
 namespace {
 
-template <int M>
-__device__ __forceinline__ void lds_zero(float2* lds, int tid) {
-    constexpr int NT = TileShape<M, true>::NT;
-    for (int j = tid; j < M; j += NT) lds[lds_pad(j)] = make_float2(0.0f, 0.0f);
-}
-
-// After the line holds its (chirped) input: forward DFT of length M, and for Bluestein the
-// multiply by B^ and the inverse DFT. Leaves X[k] * (1/M) (Bluestein) in LDS, unchirped.
-template <int M>
-__device__ __forceinline__ void line_transform(float2* lds, const float2* __restrict__ tw,
-                                               const float2* __restrict__ bhat, bool bluestein, int tid) {
-    constexpr int NT = TileShape<M, true>::NT;
-    fft_stages<M, 1, 1, NT, false, M, 1>(lds, tw, tid);
-    if (bluestein) {
-        for (int k = tid; k < M; k += NT) lds[lds_pad(k)] = cmul(lds[lds_pad(k)], bhat[k]);
-        __syncthreads();
-        fft_stages<M, 1, 1, NT, true, M, 1>(lds, tw, tid);
-    }
-}
-
-template <int M>
-__global__ void __launch_bounds__((TileShape<M, true>::NT))
-psf_rows_kernel(const float* __restrict__ amp, const float* __restrict__ phase, int P, int R, int off,
-                const float2* __restrict__ chirp, const float2* __restrict__ bhat, int bluestein,
-                float2* __restrict__ G, const float2* __restrict__ tw) {
-    constexpr int NT = TileShape<M, true>::NT;
-    extern __shared__ float2 lds[];
-    const int y = blockIdx.x, lam = blockIdx.y, tid = threadIdx.x;
-    lds_zero<M>(lds, tid);
-    __syncthreads();
-    const float* arow = amp + size_t(y) * P;
-    const float* prow = phase + (size_t(lam) * P + y) * P;
-    for (int x = tid; x < P; x += NT) {
+// Stage-0 input of the row transform of pupil row y: A * (cos 2 pi phi, sin 2 pi phi) where
+// A > 0, at line position x + off, times the chirp for Bluestein.
+struct RowInput {
+    const float* __restrict__ arow;
+    const float* __restrict__ prow;
+    const float2* __restrict__ chirp;
+    int P, off, bluestein;
+    __device__ float2 operator()(int j) const {
+        const int x = j - off;
+        if (x < 0 || x >= P) return make_float2(0.0f, 0.0f);
         const float a = arow[x];
         float rad = 0.0f;
         if (a > 0.0f) rad = prow[x] * 2.0f * kPiF;
         float2 v = make_float2(a * cosf(rad), a * sinf(rad));
-        const int j = x + off;
-        if (bluestein) v = cmul(v, chirp[j]);
-        lds[lds_pad(j)] = v;
+        if (bluestein) v = cmul(v, chirp[x]);
+        return v;
     }
-    __syncthreads();
-    line_transform<M>(lds, tw, bhat, bluestein != 0, tid);
-    float2* out = G + (size_t(lam) * P + y) * R;
-    const float inv_m = 1.0f / float(M);
-    for (int k = tid; k < R; k += NT) {
-        float2 v = lds[lds_pad(k)];
+};
+
+// Stage-0 input of the column transform of frequency column kx: the P row-transform values.
+struct ColInput {
+    const float2* __restrict__ col;
+    const float2* __restrict__ chirp;
+    int P, off, bluestein;
+    __device__ float2 operator()(int j) const {
+        const int y = j - off;
+        if (y < 0 || y >= P) return make_float2(0.0f, 0.0f);
+        float2 v = col[y];
+        if (bluestein) v = cmul(v, chirp[y]);
+        return v;
+    }
+};
+
+// Output k < R of a line: Bluestein unchirps and scales by 1/M.
+struct LineValue {
+    const float2* __restrict__ chirp;
+    float inv_m;
+    int bluestein;
+    __device__ float2 operator()(int k, float2 v) const {
         if (bluestein) v = cmul(make_float2(v.x * inv_m, v.y * inv_m), chirp[k]);
-        out[k] = v;
+        return v;
+    }
+};
+struct RowOutput {
+    float2* __restrict__ out;
+    LineValue val;
+    int R;
+    __device__ void operator()(int k, float2 v) const {
+        if (k < R) out[k] = val(k, v);
+    }
+};
+struct ColOutput {
+    float* __restrict__ out;
+    LineValue val;
+    int R;
+    __device__ void operator()(int k, float2 v) const {
+        if (k < R) {
+            const float2 f = val(k, v);
+            out[k] = f.x * f.x + f.y * f.y;
+        }
+    }
+};
+
+// The R-point transform of one line: a length-M FFT for power-of-two R, else Bluestein's
+// forward FFT, filter multiply (fused into its last stage) and inverse FFT.
+template <int M, class In, class Out>
+__device__ __forceinline__ void line_transform(float2* lds, const float2* __restrict__ tw,
+                                               const float2* __restrict__ bhat, bool bluestein, int tid,
+                                               const In& in, const Out& out) {
+    const GlobalIn<In> gin{in};
+    const GlobalOut<Out> gout{out};
+    if (bluestein) {
+        line_fft<M, false>(lds, tw, tid, gin, LdsMulOut{lds, bhat});
+        line_fft<M, true>(lds, tw, tid, LdsIn{lds}, gout);
+    } else {
+        line_fft<M, false>(lds, tw, tid, gin, gout);
     }
 }
 
-template <int M>
-__global__ void __launch_bounds__((TileShape<M, true>::NT))
+// W: minimum waves per SIMD the register allocation must allow (1: no constraint); the
+// autotuner picks among W = 1, 3, 4 for long lines.
+template <int M, int W>
+__global__ void __launch_bounds__((LinePlan<M>::NT), W)
+psf_rows_kernel(const float* __restrict__ amp, const float* __restrict__ phase, int P, int R, int off,
+                const float2* __restrict__ chirp, const float2* __restrict__ bhat, int bluestein,
+                float2* __restrict__ G, const float2* __restrict__ tw) {
+    extern __shared__ float2 lds[];
+    const int y = blockIdx.x, lam = blockIdx.y, tid = threadIdx.x;
+    const RowInput in{amp + size_t(y) * P, phase + (size_t(lam) * P + y) * P, chirp, P, off, bluestein};
+    const RowOutput out{G + (size_t(lam) * P + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R};
+    line_transform<M>(lds, tw, bhat, bluestein != 0, tid, in, out);
+}
+
+template <int M, int W>
+__global__ void __launch_bounds__((LinePlan<M>::NT), W)
 psf_cols_kernel(const float2* __restrict__ GT, int P, int R, int off, const float2* __restrict__ chirp,
                 const float2* __restrict__ bhat, int bluestein, float* __restrict__ IT,
                 const float2* __restrict__ tw) {
-    constexpr int NT = TileShape<M, true>::NT;
     extern __shared__ float2 lds[];
     const int kx = blockIdx.x, lam = blockIdx.y, tid = threadIdx.x;
-    lds_zero<M>(lds, tid);
-    __syncthreads();
-    const float2* in = GT + (size_t(lam) * R + kx) * P;
-    for (int yy = tid; yy < P; yy += NT) {
-        float2 v = in[yy];
-        const int j = yy + off;
-        if (bluestein) v = cmul(v, chirp[j]);
-        lds[lds_pad(j)] = v;
-    }
-    __syncthreads();
-    line_transform<M>(lds, tw, bhat, bluestein != 0, tid);
-    float* out = IT + (size_t(lam) * R + kx) * R;
-    const float inv_m = 1.0f / float(M);
-    for (int k = tid; k < R; k += NT) {
-        float2 v = lds[lds_pad(k)];
-        if (bluestein) v = cmul(make_float2(v.x * inv_m, v.y * inv_m), chirp[k]);
-        out[k] = v.x * v.x + v.y * v.y;
-    }
+    const ColInput in{GT + (size_t(lam) * R + kx) * P, chirp, P, off, bluestein};
+    const ColOutput out{IT + (size_t(lam) * R + kx) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R};
+    line_transform<M>(lds, tw, bhat, bluestein != 0, tid, in, out);
 }
 
 // Batched transpose of complex [rows][cols] -> [cols][rows] through a 32x33 LDS tile.
@@ -159,7 +190,8 @@ __global__ void pupil_phase_kernel(const PhaseCoef* __restrict__ coefs, int n, f
     out[size_t(lam) * n * n + i] = phase_waves_at(c, i % n, i / n, n);
 }
 
-// Double-precision iterative radix-2 FFT on the host (tables only).
+// Double-precision transforms on the host (Bluestein filter tables only): radix-2 for the
+// power-of-two part, then direct DFTs over the factor 3 or 9 (M = 2^a * 3^b).
 void host_fft_pow2(std::vector<double>& re, std::vector<double>& im) {
     const int n = int(re.size());
     for (int i = 1, j = 0; i < n; ++i) {
@@ -182,21 +214,95 @@ void host_fft_pow2(std::vector<double>& re, std::vector<double>& im) {
     }
 }
 
+// Forward DFT of length n = 2^a * q (q = 1, 3 or 9) in double: q interleaved radix-2 FFTs of
+// length 2^a, then twiddles and length-q DFTs.
+void host_dft(std::vector<double>& re, std::vector<double>& im) {
+    const int n = int(re.size());
+    int q = n;
+    while ((q & 1) == 0) q >>= 1;
+    const int m = n / q;
+    const std::vector<double> line(static_cast<size_t>(m), 0.0);
+    std::vector<std::vector<double>> sr(static_cast<size_t>(q), line), si(static_cast<size_t>(q), line);
+    for (int r = 0; r < q; ++r)
+        for (int j = 0; j < m; ++j) { sr[size_t(r)][size_t(j)] = re[size_t(j * q + r)]; si[size_t(r)][size_t(j)] = im[size_t(j * q + r)]; }
+    for (int r = 0; r < q; ++r) host_fft_pow2(sr[size_t(r)], si[size_t(r)]);
+    // X[k1 + m k2] = sum_r W_n^(r (k1 + m k2)) S_r[k1]
+    for (int k = 0; k < n; ++k) {
+        double xr = 0.0, xi = 0.0;
+        const int k1 = k % m;
+        for (int r = 0; r < q; ++r) {
+            const double a = -2.0 * M_PI * double((int64_t(r) * k) % n) / double(n);
+            const double c = std::cos(a), sn = std::sin(a);
+            const double vr = sr[size_t(r)][size_t(k1)], vi = si[size_t(r)][size_t(k1)];
+            xr += vr * c - vi * sn;
+            xi += vr * sn + vi * c;
+        }
+        re[size_t(k)] = xr;
+        im[size_t(k)] = xi;
+    }
+}
+
+constexpr int kPsfVariantWaves[3] = {1, 3, 4};
+// Tuning keys carry the line length and the batch (wavelength count rounded up to a power of
+// two): a one-wavelength launch does not fill the device the way a 31-wavelength bank does.
+// Long lines are register-bound: they get all three occupancy variants.
+template <int M> constexpr int psf_variants() { return M >= 2048 ? 3 : 1; }
+
+template <int M, int W>
+void launch_rows_w(hipStream_t s, const float* amp, const float* phase, int P, int R, int off, const float2* chirp,
+                   const float2* bhat, int bluestein, float2* G, const float2* tw, int L) {
+    allow_large_lds(reinterpret_cast<const void*>(&psf_rows_kernel<M, W>), (M + M / 32 + 1) * 8);
+    hipLaunchKernelGGL((psf_rows_kernel<M, W>), dim3(P, L), dim3(LinePlan<M>::NT), (M + M / 32 + 1) * 8, s, amp,
+                       phase, P, R, off, chirp, bhat, bluestein, G, tw);
+}
+template <int M, int W>
+void launch_cols_w(hipStream_t s, const float2* GT, int P, int R, int off, const float2* chirp, const float2* bhat,
+                   int bluestein, float* IT, const float2* tw, int L) {
+    allow_large_lds(reinterpret_cast<const void*>(&psf_cols_kernel<M, W>), (M + M / 32 + 1) * 8);
+    hipLaunchKernelGGL((psf_cols_kernel<M, W>), dim3(R, L), dim3(LinePlan<M>::NT), (M + M / 32 + 1) * 8, s, GT, P,
+                       R, off, chirp, bhat, bluestein, IT, tw);
+}
+
 template <int M>
-void launch_rows(hipStream_t s, const float* amp, const float* phase, int P, int R, int off,
+bool launch_rows(int m, hipStream_t s, const float* amp, const float* phase, int P, int R, int off,
                  const float2* chirp, const float2* bhat, int bluestein, float2* G, const float2* tw, int L) {
-    using S = TileShape<M, true>;
-    allow_large_lds(reinterpret_cast<const void*>(&psf_rows_kernel<M>), (M + M / 32 + 1) * 8);
-    hipLaunchKernelGGL((psf_rows_kernel<M>), dim3(P, L), dim3(S::NT), (M + M / 32 + 1) * 8, s, amp, phase, P, R,
-                       off, chirp, bhat, bluestein, G, tw);
+    if (m != M) return false;
+    Autotuner::instance().run("psf-rows/" + std::to_string(M) + "/L" + std::to_string(pow2_at_least(L)), psf_variants<M>(), s, [&](int v) {
+        if constexpr (psf_variants<M>() > 1) {
+            if (v == 1) return launch_rows_w<M, kPsfVariantWaves[1]>(s, amp, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
+            if (v == 2) return launch_rows_w<M, kPsfVariantWaves[2]>(s, amp, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
+        }
+        launch_rows_w<M, 1>(s, amp, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
+    });
+    return true;
 }
 template <int M>
-void launch_cols(hipStream_t s, const float2* GT, int P, int R, int off, const float2* chirp, const float2* bhat,
-                 int bluestein, float* IT, const float2* tw, int L) {
-    using S = TileShape<M, true>;
-    allow_large_lds(reinterpret_cast<const void*>(&psf_cols_kernel<M>), (M + M / 32 + 1) * 8);
-    hipLaunchKernelGGL((psf_cols_kernel<M>), dim3(R, L), dim3(S::NT), (M + M / 32 + 1) * 8, s, GT, P, R, off,
-                       chirp, bhat, bluestein, IT, tw);
+bool launch_cols(int m, hipStream_t s, const float2* GT, int P, int R, int off, const float2* chirp,
+                 const float2* bhat, int bluestein, float* IT, const float2* tw, int L) {
+    if (m != M) return false;
+    Autotuner::instance().run("psf-cols/" + std::to_string(M) + "/L" + std::to_string(pow2_at_least(L)), psf_variants<M>(), s, [&](int v) {
+        if constexpr (psf_variants<M>() > 1) {
+            if (v == 1) return launch_cols_w<M, kPsfVariantWaves[1]>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
+            if (v == 2) return launch_cols_w<M, kPsfVariantWaves[2]>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
+        }
+        launch_cols_w<M, 1>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
+    });
+    return true;
+}
+
+// Line lengths of the PSF kernels: powers of two (plain transforms of R, and Bluestein) and
+// 3 * 2^a, 9 * 2^a Bluestein lengths with a composite-first plan.
+#define LDH_PSF_LENGTHS(X)                                                                                 \
+    X(128) || X(144) || X(192) || X(256) || X(288) || X(384) || X(512) || X(576) || X(768) || X(1024) ||   \
+        X(1536) || X(2048) || X(2304) || X(3072) || X(4096) || X(4608) || X(6144) || X(8192) || X(9216) || \
+        X(12288) || X(16384)
+constexpr int kPsfLengths[] = {128,  144,  192,  256,  288,  384,  512,  576,  768,  1024, 1536,
+                               2048, 2304, 3072, 4096, 4608, 6144, 8192, 9216, 12288, 16384};
+
+int psf_length_at_least(int v) {
+    for (int n : kPsfLengths)
+        if (n >= v) return n;
+    return 0;
 }
 
 }  // namespace
@@ -207,12 +313,15 @@ void PsfPlan::prepare(hipStream_t s, int pupil, int raw) {
     pupil_ = pupil;
     raw_ = raw;
     bluestein_ = !is_pow2(raw);
-    m_ = bluestein_ ? pow2_at_least(2 * raw - 1) : raw;
-    offset_ = std::max(0, (raw - pupil) / 2);
+    // Bluestein: input at 0 (see above), lags -(P-1) .. R-1 must not wrap
+    m_ = bluestein_ ? psf_length_at_least(raw + std::min(pupil, raw) - 1) : raw;
+    if (m_ == 0) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
+    offset_ = bluestein_ ? 0 : std::max(0, (raw - pupil) / 2);
     if (!bluestein_) return;
     // chirp[j] = exp(-i * pi_f * j^2 / R), the reference's forward chirp (sign -1), with the
     // phase formed in double from pi rounded to float.
     const double pi_f = double(kPiF);
+    const int lags_neg = std::min(pupil, raw);  // b[M - j] for j in [1, P)
     std::vector<float2> chirp(static_cast<size_t>(raw));
     std::vector<double> br(static_cast<size_t>(m_), 0.0), bi(static_cast<size_t>(m_), 0.0);
     for (int j = 0; j < raw; ++j) {
@@ -220,9 +329,9 @@ void PsfPlan::prepare(hipStream_t s, int pupil, int raw) {
         chirp[size_t(j)] = make_float2(float(std::cos(ph)), float(std::sin(ph)));
         const float cr = float(std::cos(-ph)), ci = float(std::sin(-ph));
         br[size_t(j)] = cr; bi[size_t(j)] = ci;
-        if (j != 0) { br[size_t(m_ - j)] = cr; bi[size_t(m_ - j)] = ci; }
+        if (j != 0 && j < lags_neg) { br[size_t(m_ - j)] = cr; bi[size_t(m_ - j)] = ci; }
     }
-    host_fft_pow2(br, bi);
+    host_dft(br, bi);
     std::vector<float2> bhat(static_cast<size_t>(m_));
     for (int k = 0; k < m_; ++k) bhat[size_t(k)] = make_float2(float(br[size_t(k)]), float(bi[size_t(k)]));
     chirp_.reset(sizeof(float2) * raw);
@@ -253,30 +362,16 @@ void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, cons
     const float2* chirp = chirp_.as<float2>();
     const float2* bhat = bhat_.as<float2>();
     const int blu = bluestein_ ? 1 : 0;
-    switch (m_) {
-#define LDH_PSF_M(MM)                                                                                \
-    case MM:                                                                                         \
-        launch_rows<MM>(s, amp, phase, P, R, offset_, chirp, bhat, blu, g_.as<float2>(), tw, L);   \
-        break;
-        LDH_PSF_M(128) LDH_PSF_M(256) LDH_PSF_M(512) LDH_PSF_M(1024) LDH_PSF_M(2048) LDH_PSF_M(4096)
-        LDH_PSF_M(8192) LDH_PSF_M(16384)
-#undef LDH_PSF_M
-        default: fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
-    }
+#define LDH_PSF_ROWS(MM) launch_rows<MM>(m_, s, amp, phase, P, R, offset_, chirp, bhat, blu, g_.as<float2>(), tw, L)
+    if (!(LDH_PSF_LENGTHS(LDH_PSF_ROWS))) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
+#undef LDH_PSF_ROWS
     LDH_CHECK_LAUNCH("psf-rows");
     hipLaunchKernelGGL(transpose_c64_kernel, dim3(div_up(R, 32), div_up(P, 32), L), dim3(256), 0, s,
                        g_.as<float2>(), gt_.as<float2>(), P, R);
     LDH_CHECK_LAUNCH("psf-transpose");
-    switch (m_) {
-#define LDH_PSF_M(MM)                                                                                  \
-    case MM:                                                                                           \
-        launch_cols<MM>(s, gt_.as<float2>(), P, R, offset_, chirp, bhat, blu, it_.as<float>(), tw, L); \
-        break;
-        LDH_PSF_M(128) LDH_PSF_M(256) LDH_PSF_M(512) LDH_PSF_M(1024) LDH_PSF_M(2048) LDH_PSF_M(4096)
-        LDH_PSF_M(8192) LDH_PSF_M(16384)
-#undef LDH_PSF_M
-        default: fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
-    }
+#define LDH_PSF_COLS(MM) launch_cols<MM>(m_, s, gt_.as<float2>(), P, R, offset_, chirp, bhat, blu, it_.as<float>(), tw, L)
+    if (!(LDH_PSF_LENGTHS(LDH_PSF_COLS))) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
+#undef LDH_PSF_COLS
     LDH_CHECK_LAUNCH("psf-cols");
     hipLaunchKernelGGL(shift_transpose_kernel, dim3(div_up(R, 32), div_up(R, 32), L), dim3(256), 0, s,
                        it_.as<float>(), out, R);
