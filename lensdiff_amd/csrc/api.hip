@@ -284,7 +284,7 @@ ldh_status ldh_shape_kernels(ldh_context* ctx, const ldh_params* p, const float*
         const int64_t kk = int64_t(K) * K;
         ldh::DeviceBuffer db(sizeof(float) * kk), d3(sizeof(float) * kk * 3), dr(sizeof(int));
         h2d(ctx, db.get(), base, sizeof(float) * kk);
-        ldh::shape_kernels(ctx->stream, ctx->bw, db.as<float>(), K, 1, p->anisotropy_emphasis, max_r, d3.as<float>(),
+        ldh::shape_kernels(ctx->stream, ctx->bw, db.as<float>(), nullptr, K, 1, p->anisotropy_emphasis, max_r, d3.as<float>(),
                            dr.as<int>());
         std::vector<float> h(size_t(kk) * 3);
         int r = 0;

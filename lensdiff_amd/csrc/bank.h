@@ -24,8 +24,7 @@ struct DeviceBank {
     std::vector<float> wavelengths;
     std::vector<int> radius;       // cropped radius per bin
     DeviceBuffer kernels;          // float[3][bins][K*K]
-    DeviceBuffer base;             // float[bins][K*K]  (resampled, before shaping)
-    size_t bytes() const { return kernels.bytes() + base.bytes(); }
+    size_t bytes() const { return kernels.bytes(); }
     const float* kernel_ptr(int which, int bin) const {
         return kernels.as<float>() + (size_t(which) * bins + bin) * size_t(kernel) * kernel;
     }
@@ -35,7 +34,7 @@ struct DeviceBank {
 struct BankWorkspace {
     PsfPlan plan;
     SeqSum seqsum;
-    DeviceBuffer amp, phase, coefs, raw, totals, base_totals, scales, mean, shaped, structure, ring_sum, ring_cnt, ring_peak,
+    DeviceBuffer amp, phase, coefs, raw, totals, base, base_totals, shape_totals, scales, mean, shaped, structure, ring_sum, ring_cnt, ring_peak,
         radius, ref_amp, ref_phase, ref_raw, ref_total;
     // first-zero radius of the clear-circle PSF per (P, R) (GetLensDiffReferenceRawPsfCached)
     std::vector<std::pair<std::pair<int, int>, float>> first_zero;
@@ -48,9 +47,10 @@ float first_minimum_from_profile(const std::vector<float>& profile, int size);
 void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_scales, const float* d_raw_totals,
                       int support, int count, float* out);
 // Shapes `count` base kernels (count x K*K) into full/core/structure (3 x count x K*K) and
-// radius[count] (device). `max_radius` as LensDiffCpuReference.cpp:1633.
-void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, int K, int count, double anisotropy,
-                   int max_radius, float* out3, int* d_radius);
+// radius[count] (device). `max_radius` as LensDiffCpuReference.cpp:1633. base_totals (nullable):
+// per-kernel sums of a base that is not yet normalised (read as base / total).
+void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const float* base_totals, int K, int count,
+                   double anisotropy, int max_radius, float* out3, int* d_radius);
 // Radial mean profile of an R x R PSF about its centre (radialMeanProfile, :865-887), computed
 // on the device and returned to the host.
 std::vector<float> radial_profile(hipStream_t s, BankWorkspace& w, const float* psf, int R);

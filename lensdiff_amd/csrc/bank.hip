@@ -17,6 +17,14 @@ namespace ldh {
 
 namespace {
 
+// 1 / totals[seg] when totals is given and the total is positive, else 1 (a non-positive sum
+// leaves an array unnormalised, LensDiffCpuReference.cpp:440-449).
+__device__ __forceinline__ float inverse_total(const float* __restrict__ totals, int seg) {
+    if (totals == nullptr) return 1.0f;
+    const float t = totals[seg];
+    return t > 0.0f ? 1.0f / t : 1.0f;
+}
+
 // Bilinear sample of img * norm: each texel is scaled before interpolation, as when the image
 // was normalised in place first.
 __device__ __forceinline__ float bilinear_sq(const float* __restrict__ img, int size, float x, float y, float norm) {
@@ -34,9 +42,8 @@ __device__ __forceinline__ float bilinear_sq(const float* __restrict__ img, int 
 __global__ void resample_kernel(const float* __restrict__ raw, int R, const float* __restrict__ scales,
                                 const float* __restrict__ raw_totals, int support, float* __restrict__ out) {
     const int lam = blockIdx.y;
-    // raw_totals: the raw PSFs are not yet normalised; scale texels by 1/total (total > 0)
-    float norm = 1.0f;
-    if (raw_totals != nullptr && raw_totals[lam] > 0.0f) norm = 1.0f / raw_totals[lam];
+    // raw_totals: the raw PSFs are not yet normalised; texels are scaled by 1/total
+    const float norm = inverse_total(raw_totals, lam);
     const int K = 2 * support + 1;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= K * K) return;
@@ -81,10 +88,12 @@ constexpr int kRingThreads = 256;
 constexpr int kRingPixelsPerBlock = 16384;
 
 // sums[r] += v, counts[r] += 1, peaks[r] = max(v) over the pixels of ring r, per segment.
+// prescale (nullable): per-segment totals; pixels are read as v * (1 / total), the values an
+// in-place normalisation would have left.
 template <bool CEIL>
 __global__ void __launch_bounds__(kRingThreads)
-ring_stats_kernel(const float* __restrict__ img, int size, float center, int rmax, double* __restrict__ sums,
-                  int* __restrict__ counts, unsigned* __restrict__ peaks) {
+ring_stats_kernel(const float* __restrict__ img, int size, float center, int rmax, const float* __restrict__ prescale,
+                  double* __restrict__ sums, int* __restrict__ counts, unsigned* __restrict__ peaks) {
     extern __shared__ unsigned char smem[];
     const int nr = rmax + 1;
     double* s_sum = reinterpret_cast<double*>(smem);
@@ -95,12 +104,13 @@ ring_stats_kernel(const float* __restrict__ img, int size, float center, int rma
     const int seg = blockIdx.y;
     const int64_t npx = int64_t(size) * size;
     const float* base = img + seg * npx;
+    const float sc = inverse_total(prescale, seg);
     const int64_t p0 = int64_t(blockIdx.x) * kRingPixelsPerBlock;
     const int64_t p1 = min(npx, p0 + kRingPixelsPerBlock);
     for (int64_t p = p0 + threadIdx.x; p < p1; p += kRingThreads) {
         const int x = int(p % size), y = int(p / size);
         const int r = ring_of<CEIL>(x, y, center, rmax);
-        const float v = base[p];
+        const float v = base[p] * sc;
         atomicAdd(&s_sum[r], double(v));
         if (counts) atomicAdd(&s_cnt[r], 1);
         if (peaks) atomicMax(&s_pk[r], __float_as_uint(v));  // v >= 0: bit order is value order
@@ -127,31 +137,48 @@ __global__ void ring_mean_kernel(const double* __restrict__ sums, const int* __r
     out[int64_t(seg) * K * K + i] = s / float(n);
 }
 
-__global__ void reshape_residual_kernel(const float* __restrict__ base, const float* __restrict__ mean, float gain,
-                                        int64_t n, float* __restrict__ shaped) {
+// reshapeAnisotropy on the normalised base and mean (each read through its totals).
+__global__ void reshape_kernel(const float* __restrict__ base, const float* __restrict__ base_tot,
+                               const float* __restrict__ mean, const float* __restrict__ mean_tot, float gain,
+                               int64_t kk, float* __restrict__ shaped) {
+    const int seg = blockIdx.y;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float m = mean[i];
-    shaped[i] = fmaxf(0.0f, m + (base[i] - m) * gain);
+    if (i >= kk) return;
+    const int64_t o = int64_t(seg) * kk + i;
+    const float m = mean[o] * inverse_total(mean_tot, seg);
+    const float b = base[o] * inverse_total(base_tot, seg);
+    shaped[o] = fmaxf(0.0f, m + (b - m) * gain);
 }
 
-__global__ void residual_kernel(const float* __restrict__ shaped, const float* __restrict__ mean, int64_t n,
+// positiveResidualKernel of the normalised shaped and mean kernels.
+__global__ void residual_kernel(const float* __restrict__ shaped, const float* __restrict__ shaped_tot,
+                                const float* __restrict__ mean, const float* __restrict__ mean_tot, int64_t kk,
                                 float* __restrict__ out) {
+    const int seg = blockIdx.y;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    out[i] = fmaxf(0.0f, shaped[i] - mean[i]);
+    if (i >= kk) return;
+    const int64_t o = int64_t(seg) * kk + i;
+    out[o] = fmaxf(0.0f, shaped[o] * inverse_total(shaped_tot, seg) - mean[o] * inverse_total(mean_tot, seg));
 }
 
 // estimateAdaptiveSupportRadius + paddedAdaptiveSupportRadius (LensDiffCpuReference.cpp:984-1036),
-// one thread per bin. `peaks` is turned in place into the outside-peak suffix maxima.
-__global__ void support_radius_kernel(const double* __restrict__ energy, unsigned* __restrict__ peaks,
-                                      const float* __restrict__ totals, int rmax, int max_radius, int count,
-                                      int* __restrict__ radius) {
-    const int seg = blockIdx.x * blockDim.x + threadIdx.x;
-    if (seg >= count) return;
+// one workgroup per bin: the ring energies and peaks are staged in LDS by the workgroup, then
+// one thread runs the reference's sequential scan (its float running sum) over LDS.
+constexpr int kSupportThreads = 256;
+__global__ void __launch_bounds__(kSupportThreads)
+support_radius_kernel(const double* __restrict__ energy, const unsigned* __restrict__ peaks,
+                      const float* __restrict__ totals, int rmax, int max_radius, int* __restrict__ radius) {
+    extern __shared__ unsigned char smem[];
+    const int seg = blockIdx.x;
     const int nr = rmax + 1;
-    const double* e = energy + int64_t(seg) * nr;
-    unsigned* pk = peaks + int64_t(seg) * nr;
+    double* e = reinterpret_cast<double*>(smem);
+    unsigned* pk = reinterpret_cast<unsigned*>(e + nr);
+    for (int r = threadIdx.x; r < nr; r += kSupportThreads) {
+        e[r] = energy[int64_t(seg) * nr + r];
+        pk[r] = peaks[int64_t(seg) * nr + r];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     for (int r = rmax - 1; r >= 0; --r) pk[r] = max(pk[r], pk[r + 1]);  // outsidePeak[r]
     const float total = totals[seg];
     const float gpeak = __uint_as_float(pk[0]);
@@ -173,47 +200,64 @@ __global__ void support_radius_kernel(const double* __restrict__ energy, unsigne
     radius[seg] = min(max(est + pad, 4), hi);
 }
 
-// cropKernelToRadius + applySupportBoundaryTaper on the K x K grid: keeps the centred
-// (2r+1)^2 square, multiplies by the cosine taper, zeroes everything else.
-__global__ void crop_taper_kernel(const float* __restrict__ src, const int* __restrict__ radius, int K, int count,
+// cropKernelToRadius + applySupportBoundaryTaper on the K x K grid, for the full (shaped), core
+// (mean) and structure kernels of a bin at once (they share the radius and so the taper weight):
+// keeps the centred (2r+1)^2 square, multiplies by the cosine taper, zeroes everything else.
+// Sources are read through their totals (normalised values); dst is [3][count][K*K].
+struct TaperSources {
+    const float* src[3];
+    const float* tot[3];
+};
+__global__ void crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, int count,
                                   float* __restrict__ dst) {
-    const int seg = blockIdx.y;  // which * count + bin
-    const int bin = seg % count;
+    const int bin = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= K * K) return;
     const int r = min(max(radius[bin], 1), K / 2);
     const int c = K / 2;
     const int x = i % K, y = i / K;
-    const int64_t off = int64_t(seg) * K * K + i;
-    if (abs(x - c) > r || abs(y - c) > r) {
-        dst[off] = 0.0f;
-        return;
+    const int64_t kk = int64_t(K) * K;
+    const bool inside = abs(x - c) <= r && abs(y - c) <= r;
+    float w = 0.0f;
+    if (inside) {
+        const float dx = float(x - c), dy = float(y - c);
+        const float rr = sqrtf(dx * dx + dy * dy);
+        const float extent = fmaxf(1.0f, float(r));
+        const float fade = fmaxf(6.0f, fminf(24.0f, extent * 0.04f));
+        const float start = fmaxf(0.0f, extent - fade);
+        if (rr < extent) {
+            const float t = fminf(fmaxf((rr - start) / fmaxf(fade, 1e-6f), 0.0f), 1.0f);
+            const float sm = t * t * (3.0f - 2.0f * t);
+            w = fmaxf(0.0f, cosf(sm * 1.57079637f));
+        }
     }
-    const float dx = float(x - c), dy = float(y - c);
-    const float rr = sqrtf(dx * dx + dy * dy);
-    const float extent = fmaxf(1.0f, float(r));
-    const float fade = fmaxf(6.0f, fminf(24.0f, extent * 0.04f));
-    const float start = fmaxf(0.0f, extent - fade);
-    float w;
-    if (rr >= extent) {
-        w = 0.0f;
-    } else {
-        const float t = fminf(fmaxf((rr - start) / fmaxf(fade, 1e-6f), 0.0f), 1.0f);
-        const float sm = t * t * (3.0f - 2.0f * t);
-        w = fmaxf(0.0f, cosf(sm * 1.57079637f));
+    const int64_t o = int64_t(bin) * kk + i;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float v = inside ? ts.src[k][o] * inverse_total(ts.tot[k], bin) * w : 0.0f;
+        dst[(int64_t(k) * count + bin) * kk + i] = v;
     }
-    dst[off] = src[off] * w;
+}
+
+__global__ void fill_zero_kernel(uint32_t* __restrict__ p, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        p[i] = 0u;
 }
 
 inline dim3 grid1(int64_t n, int segs = 1) { return dim3(unsigned(div_up(n, 256)), unsigned(segs)); }
 
 template <bool CEIL>
-void ring_stats(hipStream_t s, const float* img, int size, float center, int rmax, int segs, double* sums,
-                int* counts, unsigned* peaks) {
+void ring_stats(hipStream_t s, const float* img, int size, float center, int rmax, int segs, const float* prescale,
+                double* sums, int* counts, unsigned* peaks) {
     const int nr = rmax + 1;
-    LDH_HIP(hipMemsetAsync(sums, 0, sizeof(double) * nr * segs, s), "ring-stats");
-    if (counts) LDH_HIP(hipMemsetAsync(counts, 0, sizeof(int) * nr * segs, s), "ring-stats");
-    if (peaks) LDH_HIP(hipMemsetAsync(peaks, 0, sizeof(unsigned) * nr * segs, s), "ring-stats");
+    // cleared by a kernel, not hipMemsetAsync, so the sequence can be captured into a graph
+    auto clear = [&](void* p, int64_t words) {
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(unsigned(std::min<int64_t>(div_up(words, 256), 1024))), dim3(256),
+                           0, s, static_cast<uint32_t*>(p), words);
+    };
+    clear(sums, int64_t(nr) * segs * 2);
+    if (counts) clear(counts, int64_t(nr) * segs);
+    if (peaks) clear(peaks, int64_t(nr) * segs);
     const size_t lds = size_t(nr) * (sizeof(double) + sizeof(int) + sizeof(unsigned));
     const int64_t npx = int64_t(size) * size;
     static bool attr_set = false;
@@ -223,7 +267,7 @@ void ring_stats(hipStream_t s, const float* img, int size, float center, int rma
         attr_set = true;
     }
     hipLaunchKernelGGL((ring_stats_kernel<CEIL>), dim3(unsigned(div_up(npx, kRingPixelsPerBlock)), unsigned(segs)),
-                       dim3(kRingThreads), lds, s, img, size, center, rmax, sums, counts, peaks);
+                       dim3(kRingThreads), lds, s, img, size, center, rmax, prescale, sums, counts, peaks);
     LDH_CHECK_LAUNCH("ring-stats");
 }
 
@@ -237,59 +281,66 @@ void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_sca
     LDH_CHECK_LAUNCH("resample");
 }
 
-void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, int K, int count, double anisotropy,
-                   int max_radius, float* out3, int* d_radius) {
+void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const float* base_totals, int K, int count,
+                   double anisotropy, int max_radius, float* out3, int* d_radius) {
     const int64_t kk = int64_t(K) * K;
     const int64_t n = kk * count;
     const int rmax_mean = K / 2;
     w.mean.reset(sizeof(float) * n);
     w.shaped.reset(sizeof(float) * n);
     w.structure.reset(sizeof(float) * n);
-    w.totals.reset(sizeof(float) * 3 * count);
+    // sums: mean, shaped, structure, normalised-shaped run (count each), final (3 * count)
+    w.shape_totals.reset(sizeof(float) * 7 * count);
+    float* t_mean = w.shape_totals.as<float>();
+    float* t_shaped = t_mean + count;
+    float* t_struct = t_shaped + count;
+    float* t_run = t_struct + count;
+    float* t_final = t_run + count;
     const int nr = std::max(rmax_mean, max_radius) + 1;
     w.ring_sum.reset(sizeof(double) * nr * count);
     w.ring_cnt.reset(sizeof(int) * nr * count);
     w.ring_peak.reset(sizeof(unsigned) * nr * count);
-    float* tot = w.totals.as<float>();
-    // azimuthal mean, normalised
-    ring_stats<false>(s, base, K, float(K - 1) * 0.5f, rmax_mean, count, w.ring_sum.as<double>(),
+    // Each normalisation below only sums its array; readers apply 1 / total as they load, which
+    // gives the values an in-place normalisation would have left.
+    // azimuthal mean of the normalised base
+    ring_stats<false>(s, base, K, float(K - 1) * 0.5f, rmax_mean, count, base_totals, w.ring_sum.as<double>(),
                       w.ring_cnt.as<int>(), nullptr);
     hipLaunchKernelGGL(ring_mean_kernel, grid1(kk, count), dim3(256), 0, s, w.ring_sum.as<double>(),
                        w.ring_cnt.as<int>(), K, rmax_mean, w.mean.as<float>());
     LDH_CHECK_LAUNCH("ring-mean");
-    w.seqsum.normalize(s, w.mean.as<float>(), kk, kk, count, tot);
-    // anisotropy reshape, normalised
+    w.seqsum.run(s, w.mean.as<float>(), kk, kk, count, t_mean);
+    // anisotropy reshape
     const float gain = 1.0f + std::max(0.0f, float(anisotropy)) * 4.0f;
-    hipLaunchKernelGGL(reshape_residual_kernel, grid1(n), dim3(256), 0, s, base, w.mean.as<float>(), gain, n,
-                       w.shaped.as<float>());
+    hipLaunchKernelGGL(reshape_kernel, grid1(kk, count), dim3(256), 0, s, base, base_totals, w.mean.as<float>(),
+                       t_mean, gain, kk, w.shaped.as<float>());
     LDH_CHECK_LAUNCH("reshape");
-    w.seqsum.normalize(s, w.shaped.as<float>(), kk, kk, count, tot);
-    // positive residual, normalised when its sum is positive
-    hipLaunchKernelGGL(residual_kernel, grid1(n), dim3(256), 0, s, w.shaped.as<float>(), w.mean.as<float>(), n,
-                       w.structure.as<float>());
+    w.seqsum.run(s, w.shaped.as<float>(), kk, kk, count, t_shaped);
+    // positive residual
+    hipLaunchKernelGGL(residual_kernel, grid1(kk, count), dim3(256), 0, s, w.shaped.as<float>(), t_shaped,
+                       w.mean.as<float>(), t_mean, kk, w.structure.as<float>());
     LDH_CHECK_LAUNCH("residual");
-    w.seqsum.normalize(s, w.structure.as<float>(), kk, kk, count, tot);
-    // adaptive support radius of the shaped kernel
+    // one launch sums the structure kernels and the normalised shaped kernels (the support
+    // estimate's total)
+    SeqSegments g;
+    g.array[0] = w.structure.as<float>();
+    g.array[1] = w.shaped.as<float>();
+    g.prescale[1] = t_shaped;
+    g.per_array = count;
+    g.stride = kk;
+    w.seqsum.run(s, g, kk, 2 * count, t_struct);  // t_struct, then t_run
+    // adaptive support radius of the normalised shaped kernel
     const int rmax_sup = std::min(max_radius, K / 2);
-    ring_stats<true>(s, w.shaped.as<float>(), K, float(K - 1) * 0.5f, rmax_sup, count, w.ring_sum.as<double>(),
-                     nullptr, w.ring_peak.as<unsigned>());
-    w.seqsum.run(s, w.shaped.as<float>(), kk, kk, count, tot);
-    hipLaunchKernelGGL(support_radius_kernel, dim3(unsigned(div_up(count, 64))), dim3(64), 0, s,
-                       w.ring_sum.as<double>(), w.ring_peak.as<unsigned>(), tot, rmax_sup, max_radius, count, d_radius);
+    ring_stats<true>(s, w.shaped.as<float>(), K, float(K - 1) * 0.5f, rmax_sup, count, t_shaped,
+                     w.ring_sum.as<double>(), nullptr, w.ring_peak.as<unsigned>());
+    const size_t sup_lds = size_t(rmax_sup + 1) * (sizeof(double) + sizeof(unsigned));
+    hipLaunchKernelGGL(support_radius_kernel, dim3(unsigned(count)), dim3(kSupportThreads), sup_lds, s,
+                       w.ring_sum.as<double>(), w.ring_peak.as<unsigned>(), t_run, rmax_sup, max_radius, d_radius);
     LDH_CHECK_LAUNCH("support-radius");
     // crop + taper of full (shaped), core (mean), structure, then normalise all three
-    float* full = out3;
-    float* core = out3 + n;
-    float* structure = out3 + 2 * n;
-    hipLaunchKernelGGL(crop_taper_kernel, grid1(kk, count), dim3(256), 0, s, w.shaped.as<float>(), d_radius, K, count,
-                       full);
-    hipLaunchKernelGGL(crop_taper_kernel, grid1(kk, count), dim3(256), 0, s, w.mean.as<float>(), d_radius, K, count,
-                       core);
-    hipLaunchKernelGGL(crop_taper_kernel, grid1(kk, count), dim3(256), 0, s, w.structure.as<float>(), d_radius, K,
-                       count, structure);
+    TaperSources ts{{w.shaped.as<float>(), w.mean.as<float>(), w.structure.as<float>()}, {t_shaped, t_mean, t_struct}};
+    hipLaunchKernelGGL(crop_taper_kernel, grid1(kk, count), dim3(256), 0, s, ts, d_radius, K, count, out3);
     LDH_CHECK_LAUNCH("crop-taper");
-    w.totals.reset(sizeof(float) * 3 * count);
-    w.seqsum.normalize(s, out3, kk, kk, 3 * count, w.totals.as<float>());
+    w.seqsum.normalize(s, out3, kk, kk, 3 * count, t_final);
 }
 
 std::vector<float> radial_profile(hipStream_t s, BankWorkspace& w, const float* psf, int R) {
@@ -298,7 +349,7 @@ std::vector<float> radial_profile(hipStream_t s, BankWorkspace& w, const float* 
     w.ring_sum.reset(sizeof(double) * nr);
     w.ring_cnt.reset(sizeof(int) * nr);
     const float center = (R & 1) == 0 ? float(R) * 0.5f : float(R - 1) * 0.5f;
-    ring_stats<false>(s, psf, R, center, rmax, 1, w.ring_sum.as<double>(), w.ring_cnt.as<int>(), nullptr);
+    ring_stats<false>(s, psf, R, center, rmax, 1, nullptr, w.ring_sum.as<double>(), w.ring_cnt.as<int>(), nullptr);
     std::vector<double> sums(static_cast<size_t>(nr));
     std::vector<int> counts(static_cast<size_t>(nr));
     LDH_HIP(hipMemcpyAsync(sums.data(), w.ring_sum.get(), sizeof(double) * nr, hipMemcpyDeviceToHost, s), "profile");
@@ -419,16 +470,16 @@ std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWo
     w.plan.raw_psfs(s, t, w.amp.as<float>(), w.phase.as<float>(), L, w.raw.as<float>(), w.totals.as<float>(), w.seqsum,
                     false);
     const int64_t kk = int64_t(K) * K;
-    bank->base.reset(sizeof(float) * kk * L);
-    resample_kernels(s, w.raw.as<float>(), R, w.scales.as<float>(), w.totals.as<float>(), r, L, bank->base.as<float>());
+    w.base.reset(sizeof(float) * kk * L);
+    resample_kernels(s, w.raw.as<float>(), R, w.scales.as<float>(), w.totals.as<float>(), r, L, w.base.as<float>());
     w.base_totals.reset(sizeof(float) * L);
-    w.seqsum.normalize(s, bank->base.as<float>(), kk, kk, L, w.base_totals.as<float>());
+    w.seqsum.run(s, w.base.as<float>(), kk, kk, L, w.base_totals.as<float>());
 
-    // shaping into full / core / structure
+    // shaping into full / core / structure (the base is read normalised through its totals)
     bank->kernels.reset(sizeof(float) * kk * L * 3);
     w.radius.reset(sizeof(int) * L);
-    shape_kernels(s, w, bank->base.as<float>(), K, L, p.anisotropy_emphasis, r, bank->kernels.as<float>(),
-                  w.radius.as<int>());
+    shape_kernels(s, w, w.base.as<float>(), w.base_totals.as<float>(), K, L, p.anisotropy_emphasis, r,
+                  bank->kernels.as<float>(), w.radius.as<int>());
     bank->radius.resize(size_t(L));
     LDH_HIP(hipMemcpyAsync(bank->radius.data(), w.radius.get(), sizeof(int) * L, hipMemcpyDeviceToHost, s), "bank-radius");
     LDH_HIP(hipStreamSynchronize(s), "bank-radius");

@@ -51,15 +51,16 @@ __device__ __forceinline__ double wave_inclusive_scan(double v, int lane) {
 }
 
 __global__ void __launch_bounds__(kThreads)
-block_sums_kernel(const float* __restrict__ x, int64_t n, int64_t stride, int nb, double* __restrict__ bsum) {
+block_sums_kernel(SeqSegments g, int64_t n, int nb, double* __restrict__ bsum) {
     const int seg = blockIdx.y, b = blockIdx.x;
-    const float* base = x + seg * stride;
+    const float* base = g.base(seg);
+    const float sc = g.scale(seg);
     const int64_t start = int64_t(b) * kSeqSumBlock;
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < kPerThread; ++k) {
         const int64_t i = start + int64_t(k) * kThreads + threadIdx.x;
-        if (i < n) acc += double(base[i]);
+        if (i < n) acc += double(base[i] * sc);
     }
     __shared__ double red[kThreads / 64];
     acc = wave_sum(acc);
@@ -105,18 +106,23 @@ prefix_kernel(const double* __restrict__ bsum, int nb, int* __restrict__ ebase) 
 }
 
 __global__ void __launch_bounds__(kThreads)
-binade_sums_kernel(const float* __restrict__ x, int64_t n, int64_t stride, int nb, const int* __restrict__ ebase,
-                   double* __restrict__ bins) {
+binade_sums_kernel(SeqSegments g, int64_t n, int nb, const int* __restrict__ ebase, double* __restrict__ bins) {
     const int seg = blockIdx.y, b = blockIdx.x;
     const int eb = ebase[int64_t(seg) * nb + b];
     if (eb == kNoBase) return;
-    const float* base = x + seg * stride;
+    const float* base = g.base(seg);
+    const float sc = g.scale(seg);
     const int64_t start = int64_t(b) * kSeqSumBlock;
     float v[kPerThread];
 #pragma unroll
     for (int k = 0; k < kPerThread; ++k) {
         const int64_t i = start + int64_t(k) * kThreads + threadIdx.x;
         v[k] = i < n ? base[i] : 0.0f;
+    }
+    // scaled after every load is issued (scaling at each load serialises the loads)
+    if (sc != 1.0f) {
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k) v[k] *= sc;
     }
     __shared__ double red[kBins][kThreads / 64];
     __shared__ int tied[kBins][kThreads / 64];
@@ -187,7 +193,7 @@ __device__ __forceinline__ double chunk_contribution(const float* v, int e) {
 // elements). Chunks that keep the running sum inside its binade are added through a scan of
 // their rounded contributions; the chunk where it leaves the binade (or where s leaves 0) is
 // added by its own lane in true FP32, in order; so is a chunk holding a rounding tie.
-__device__ void replay_block(const float* __restrict__ base, int64_t n, int64_t block, Run& r, int lane) {
+__device__ void replay_block(const float* __restrict__ base, float sc, int64_t n, int64_t block, Run& r, int lane) {
     const int64_t start = block * kSeqSumBlock + int64_t(lane) * kChunk;
     float v[kChunk];
     if (start + kChunk <= n && (reinterpret_cast<uintptr_t>(base + start) & 15) == 0) {
@@ -200,6 +206,10 @@ __device__ void replay_block(const float* __restrict__ base, int64_t n, int64_t 
     } else {
 #pragma unroll
         for (int j = 0; j < kChunk; ++j) v[j] = start + j < n ? base[start + j] : 0.0f;
+    }
+    if (sc != 1.0f) {
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) v[j] *= sc;
     }
     int first = 0;
     while (first < 64) {
@@ -228,11 +238,12 @@ __device__ void replay_block(const float* __restrict__ base, int64_t n, int64_t 
 }
 
 __global__ void __launch_bounds__(64)
-walk_kernel(const float* __restrict__ x, int64_t n, int64_t stride, int nb, const double* __restrict__ bsum,
-            const int* __restrict__ ebase, const double* __restrict__ bins, float* __restrict__ out) {
+walk_kernel(SeqSegments g, int64_t n, int nb, const double* __restrict__ bsum, const int* __restrict__ ebase,
+            const double* __restrict__ bins, float* __restrict__ out) {
     const int seg = blockIdx.x;
     const int lane = threadIdx.x;
-    const float* base = x + seg * stride;
+    const float* base = g.base(seg);
+    const float sc = g.scale(seg);
     const double* bs = bsum + int64_t(seg) * nb;
     const int* eb = ebase + int64_t(seg) * nb;
     const double* bn = bins + int64_t(seg) * nb * kBins;
@@ -287,7 +298,7 @@ walk_kernel(const float* __restrict__ x, int64_t n, int64_t stride, int nb, cons
             }
             const int l = __ffsll((long long)mask) - 1;
             r.s += __shfl(incl - col, l);
-            replay_block(base, n, g0 + l, r, lane);
+            replay_block(base, sc, n, g0 + l, r, lane);
             first = l + 1;
         }
         cur_sum = nxt_sum;
@@ -312,18 +323,26 @@ __global__ void scale_by_inverse_kernel(float* __restrict__ x, int64_t n, int64_
 }  // namespace
 
 void SeqSum::run(hipStream_t s, const float* x, int64_t n, int64_t stride, int segments, float* totals) {
+    SeqSegments g;
+    g.array[0] = x;
+    g.per_array = segments;
+    g.stride = stride;
+    run(s, g, n, segments, totals);
+}
+
+void SeqSum::run(hipStream_t s, const SeqSegments& g, int64_t n, int segments, float* totals) {
     if (n <= 0 || segments <= 0) return;
+    if (segments > g.per_array * SeqSegments::kMaxArrays) fail(LDH_ERR_INVALID_ARGUMENT, "seqsum-segments");
     const int nb = int(div_up(n, kSeqSumBlock));
     bsum_.reset(sizeof(double) * size_t(nb) * segments);
     bins_.reset(sizeof(double) * size_t(nb) * segments * kBins);
     ebase_.reset(sizeof(int) * size_t(nb) * segments);
-    hipLaunchKernelGGL(block_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, x, n, stride, nb,
-                       bsum_.as<double>());
+    hipLaunchKernelGGL(block_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, g, n, nb, bsum_.as<double>());
     hipLaunchKernelGGL(prefix_kernel, dim3(segments), dim3(kThreads), 0, s, bsum_.as<double>(), nb, ebase_.as<int>());
-    hipLaunchKernelGGL(binade_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, x, n, stride, nb,
-                       ebase_.as<int>(), bins_.as<double>());
-    hipLaunchKernelGGL(walk_kernel, dim3(segments), dim3(64), 0, s, x, n, stride, nb, bsum_.as<double>(),
-                       ebase_.as<int>(), bins_.as<double>(), totals);
+    hipLaunchKernelGGL(binade_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, g, n, nb, ebase_.as<int>(),
+                       bins_.as<double>());
+    hipLaunchKernelGGL(walk_kernel, dim3(segments), dim3(64), 0, s, g, n, nb, bsum_.as<double>(), ebase_.as<int>(),
+                       bins_.as<double>(), totals);
     LDH_CHECK_LAUNCH("seqsum-launch");
 }
 

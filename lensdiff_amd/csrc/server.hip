@@ -236,7 +236,6 @@ void Server::worker(Slot* s) {
             if (builder) {
                 try {
                     std::shared_ptr<DeviceBank> bank(build_bank(s->stream, fft_, s->bw, r->params).release());
-                    bank->base.release();  // only the shaped kernels are needed to render
                     if (spectral_fullkernel(r->params)) {
                         const bool sty = with_style(r->params);
                         const SpectrumConfig cfg = build_spectrum_config(r->params, bank->wavelengths);
