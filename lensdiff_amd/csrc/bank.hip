@@ -12,6 +12,7 @@
 
 #include "bank.h"
 #include "host_params.h"
+#include "profiler.h"
 
 namespace ldh {
 
@@ -266,6 +267,7 @@ void ring_stats(hipStream_t s, const float* img, int size, float center, int rma
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
+    ProfileScope ps("shape-ring-stats", 4.0 * double(npx) * segs);
     hipLaunchKernelGGL((ring_stats_kernel<CEIL>), dim3(unsigned(div_up(npx, kRingPixelsPerBlock)), unsigned(segs)),
                        dim3(kRingThreads), lds, s, img, size, center, rmax, prescale, sums, counts, peaks);
     LDH_CHECK_LAUNCH("ring-stats");
@@ -276,9 +278,12 @@ void ring_stats(hipStream_t s, const float* img, int size, float center, int rma
 void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_scales, const float* d_raw_totals,
                       int support, int count, float* out) {
     const int K = 2 * support + 1;
-    hipLaunchKernelGGL(resample_kernel, grid1(int64_t(K) * K, count), dim3(256), 0, s, raw, R, d_scales, d_raw_totals,
+{
+    ProfileScope ps_("resample", 8.0 * double(K) * K * count);
+        hipLaunchKernelGGL(resample_kernel, grid1(int64_t(K) * K, count), dim3(256), 0, s, raw, R, d_scales, d_raw_totals,
                        support, out);
     LDH_CHECK_LAUNCH("resample");
+    }
 }
 
 void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const float* base_totals, int K, int count,
@@ -305,20 +310,29 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
     // azimuthal mean of the normalised base
     ring_stats<false>(s, base, K, float(K - 1) * 0.5f, rmax_mean, count, base_totals, w.ring_sum.as<double>(),
                       w.ring_cnt.as<int>(), nullptr);
-    hipLaunchKernelGGL(ring_mean_kernel, grid1(kk, count), dim3(256), 0, s, w.ring_sum.as<double>(),
+{
+    ProfileScope ps_("shape-ring-mean", 4.0 * double(kk) * count);
+        hipLaunchKernelGGL(ring_mean_kernel, grid1(kk, count), dim3(256), 0, s, w.ring_sum.as<double>(),
                        w.ring_cnt.as<int>(), K, rmax_mean, w.mean.as<float>());
     LDH_CHECK_LAUNCH("ring-mean");
+    }
     w.seqsum.run(s, w.mean.as<float>(), kk, kk, count, t_mean);
     // anisotropy reshape
     const float gain = 1.0f + std::max(0.0f, float(anisotropy)) * 4.0f;
-    hipLaunchKernelGGL(reshape_kernel, grid1(kk, count), dim3(256), 0, s, base, base_totals, w.mean.as<float>(),
+{
+    ProfileScope ps_("shape-reshape", 12.0 * double(kk) * count);
+        hipLaunchKernelGGL(reshape_kernel, grid1(kk, count), dim3(256), 0, s, base, base_totals, w.mean.as<float>(),
                        t_mean, gain, kk, w.shaped.as<float>());
     LDH_CHECK_LAUNCH("reshape");
+    }
     w.seqsum.run(s, w.shaped.as<float>(), kk, kk, count, t_shaped);
     // positive residual
-    hipLaunchKernelGGL(residual_kernel, grid1(kk, count), dim3(256), 0, s, w.shaped.as<float>(), t_shaped,
+{
+    ProfileScope ps_("shape-residual", 12.0 * double(kk) * count);
+        hipLaunchKernelGGL(residual_kernel, grid1(kk, count), dim3(256), 0, s, w.shaped.as<float>(), t_shaped,
                        w.mean.as<float>(), t_mean, kk, w.structure.as<float>());
     LDH_CHECK_LAUNCH("residual");
+    }
     // one launch sums the structure kernels and the normalised shaped kernels (the support
     // estimate's total)
     SeqSegments g;
@@ -333,13 +347,19 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
     ring_stats<true>(s, w.shaped.as<float>(), K, float(K - 1) * 0.5f, rmax_sup, count, t_shaped,
                      w.ring_sum.as<double>(), nullptr, w.ring_peak.as<unsigned>());
     const size_t sup_lds = size_t(rmax_sup + 1) * (sizeof(double) + sizeof(unsigned));
-    hipLaunchKernelGGL(support_radius_kernel, dim3(unsigned(count)), dim3(kSupportThreads), sup_lds, s,
+{
+    ProfileScope ps_("shape-support-radius", 0.0);
+        hipLaunchKernelGGL(support_radius_kernel, dim3(unsigned(count)), dim3(kSupportThreads), sup_lds, s,
                        w.ring_sum.as<double>(), w.ring_peak.as<unsigned>(), t_run, rmax_sup, max_radius, d_radius);
     LDH_CHECK_LAUNCH("support-radius");
+    }
     // crop + taper of full (shaped), core (mean), structure, then normalise all three
     TaperSources ts{{w.shaped.as<float>(), w.mean.as<float>(), w.structure.as<float>()}, {t_shaped, t_mean, t_struct}};
-    hipLaunchKernelGGL(crop_taper_kernel, grid1(kk, count), dim3(256), 0, s, ts, d_radius, K, count, out3);
+{
+    ProfileScope ps_("shape-crop-taper", 24.0 * double(kk) * count);
+        hipLaunchKernelGGL(crop_taper_kernel, grid1(kk, count), dim3(256), 0, s, ts, d_radius, K, count, out3);
     LDH_CHECK_LAUNCH("crop-taper");
+    }
     w.seqsum.normalize(s, out3, kk, kk, 3 * count, t_final);
 }
 

@@ -21,16 +21,20 @@
 namespace ldh {
 
 // ---- element ops ---------------------------------------------------------------------------
+// extra_bytes: bytes an op reads per element besides the element itself (byte model).
 struct PlainLoad {
+    static constexpr double extra_bytes = 0.0;
     __device__ float2 operator()(const float2* d, size_t idx) const { return d[idx]; }
 };
 // Spectrum multiply fused into the first inverse pass: v = d[idx] * k[idx].
 struct MulSpectrumLoad {
+    static constexpr double extra_bytes = 8.0;
     const float2* k;
     __device__ float2 operator()(const float2* d, size_t idx) const { return cmul(d[idx], k[idx]); }
 };
 // Same with the kernel spectrum stored as FP16 pairs (FP32 arithmetic).
 struct MulSpectrumHalfLoad {
+    static constexpr double extra_bytes = 4.0;
     const __half2* k;
     __device__ float2 operator()(const float2* d, size_t idx) const {
         const float2 kk = __half22float2(k[idx]);
@@ -99,8 +103,9 @@ void col_pass(hipStream_t s, const float2* tw, const float2* src, float2* data, 
     const double all = double(width) * double(height);
     const double zero = double(std::max(0, std::min(rows.zero_hi, height) - std::max(rows.zero_lo, 0)));
     const double stored = double(std::min(rows.store_rows, height));
+    const double loaded = all - zero * double(width);
     ProfileScope ps(std::string("fft-col-pass") + (inverse ? "-inv" : "-fwd"),
-                    8.0 * (all - zero * double(width)) + 8.0 * stored * double(width));
+                    (8.0 + Load::extra_bytes) * loaded + 8.0 * stored * double(width));
     if (inverse) launch_tiles<false, true>(s, npts, io, tw, ntiles);
     else launch_tiles<false, false>(s, npts, io, tw, ntiles);
 }

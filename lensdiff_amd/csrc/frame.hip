@@ -345,8 +345,11 @@ void combine_kernels(hipStream_t s, const float* kernels, int bins, int K, const
     scratch.reset(sizeof(float) * m.size());
     LDH_HIP(hipMemcpyAsync(scratch.get(), m.data(), sizeof(float) * m.size(), hipMemcpyHostToDevice, s), "combine");
     const int64_t kk = int64_t(K) * K;
-    hipLaunchKernelGGL(combine_kernel, grid1(kk), dim3(256), 0, s, kernels, bins, kk, scratch.as<float>(), rows, out);
+{
+    ProfileScope ps_("spectra-combine", 4.0 * double(kk) * (bins + rows));
+        hipLaunchKernelGGL(combine_kernel, grid1(kk), dim3(256), 0, s, kernels, bins, kk, scratch.as<float>(), rows, out);
     LDH_CHECK_LAUNCH("combine-kernels");
+    }
     LDH_HIP(hipStreamSynchronize(s), "combine");
 }
 
@@ -420,8 +423,11 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
     sp.threshold_stops = float(p.threshold);
     sp.threshold_linear = 0.18f * std::pow(2.0f, sp.threshold_stops);
     sp.point_emphasis = float(p.point_emphasis);
-    hipLaunchKernelGGL(selection_kernel, grid1(npx), dim3(256), 0, s, src, npx, sp, fw.mask.as<float>());
+{
+    ProfileScope ps_("frame-select", 20.0 * double(npx));
+        hipLaunchKernelGGL(selection_kernel, grid1(npx), dim3(256), 0, s, src, npx, sp, fw.mask.as<float>());
     LDH_CHECK_LAUNCH("selection");
+    }
     const float redist_scale = 1.0f - float(std::max(0.0, std::min(p.core_preserve, 1.0)));
 
     // 2. image spectra: mono -> (R + iG), (B); spectral -> driver
@@ -516,10 +522,12 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
         if (!pl.spectral) {
             LDH_HIP(hipMemcpyAsync(eff, planes, sizeof(float) * npx * 3, hipMemcpyDeviceToDevice, s), "effect-copy");
         } else if (mode == EffectMode::PerBin) {
+            ProfileScope ps_map("frame-spectral-map", (4.0 * n + 12.0) * double(npx));
             hipLaunchKernelGGL(spectral_map_kernel, grid1(npx), dim3(256), 0, s, planes, n, npx, fw.matrices.as<float>(),
                                fw.matrices.as<float>() + 3 * n, force, float(std::max(0.0, p.spectrum_saturation)),
                                p.chromatic_affects_luma, eff);
         } else {
+            ProfileScope ps_map("frame-spectral-map", (4.0 * 6 + 12.0) * double(npx));
             hipLaunchKernelGGL(precombined_map_kernel, grid1(npx), dim3(256), 0, s, planes, with_style ? 1 : 0, npx,
                                force, float(std::max(0.0, p.spectrum_saturation)), p.chromatic_affects_luma, eff);
         }
@@ -576,9 +584,12 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
         eff_out = effect;
         factor = nullptr;  // the reference rescales only the effect image
     }
-    hipLaunchKernelGGL(composite_kernel, grid1(npx), dim3(256), 0, s, src, fw.mask.as<float>(), eff_out, factor,
+{
+    ProfileScope ps_("frame-composite", 56.0 * double(npx));
+        hipLaunchKernelGGL(composite_kernel, grid1(npx), dim3(256), 0, s, src, fw.mask.as<float>(), eff_out, factor,
                        mask_max, npx, cp, dst);
     LDH_CHECK_LAUNCH("composite");
+    }
 }
 
 }  // namespace ldh

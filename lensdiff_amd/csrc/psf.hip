@@ -20,6 +20,7 @@
 
 #include "autotune.h"
 #include "fft_line.h"
+#include "profiler.h"
 #include "psf.h"
 #include "pupil.h"
 
@@ -362,20 +363,34 @@ void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, cons
     const float2* chirp = chirp_.as<float2>();
     const float2* bhat = bhat_.as<float2>();
     const int blu = bluestein_ ? 1 : 0;
+    {
+        // algorithmic bytes: amplitude and phase in, row transforms out
+        ProfileScope ps("psf-rows", (8.0 * P * P + 8.0 * P * R) * L);
 #define LDH_PSF_ROWS(MM) launch_rows<MM>(m_, s, amp, phase, P, R, offset_, chirp, bhat, blu, g_.as<float2>(), tw, L)
-    if (!(LDH_PSF_LENGTHS(LDH_PSF_ROWS))) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
+        if (!(LDH_PSF_LENGTHS(LDH_PSF_ROWS))) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
 #undef LDH_PSF_ROWS
-    LDH_CHECK_LAUNCH("psf-rows");
+        LDH_CHECK_LAUNCH("psf-rows");
+    }
+    {
+    ProfileScope ps_t("psf-transpose", 16.0 * P * R * L);
     hipLaunchKernelGGL(transpose_c64_kernel, dim3(div_up(R, 32), div_up(P, 32), L), dim3(256), 0, s,
                        g_.as<float2>(), gt_.as<float2>(), P, R);
     LDH_CHECK_LAUNCH("psf-transpose");
+    }
+    {
+        // algorithmic bytes: the P row-transform values of each column in, |F|^2 out
+        ProfileScope ps("psf-cols", (8.0 * P * R + 4.0 * R * R) * L);
 #define LDH_PSF_COLS(MM) launch_cols<MM>(m_, s, gt_.as<float2>(), P, R, offset_, chirp, bhat, blu, it_.as<float>(), tw, L)
-    if (!(LDH_PSF_LENGTHS(LDH_PSF_COLS))) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
+        if (!(LDH_PSF_LENGTHS(LDH_PSF_COLS))) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
 #undef LDH_PSF_COLS
-    LDH_CHECK_LAUNCH("psf-cols");
+        LDH_CHECK_LAUNCH("psf-cols");
+    }
+    {
+    ProfileScope ps_s("psf-shift", 8.0 * R * R * L);
     hipLaunchKernelGGL(shift_transpose_kernel, dim3(div_up(R, 32), div_up(R, 32), L), dim3(256), 0, s,
                        it_.as<float>(), out, R);
     LDH_CHECK_LAUNCH("psf-shift");
+    }
     if (normalize)
         seqsum.normalize(s, out, int64_t(R) * R, int64_t(R) * R, L, totals);
     else

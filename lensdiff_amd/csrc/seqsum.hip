@@ -24,6 +24,7 @@
 #include <climits>
 #include <cstdint>
 
+#include "profiler.h"
 #include "seqsum.h"
 
 namespace ldh {
@@ -337,17 +338,30 @@ void SeqSum::run(hipStream_t s, const SeqSegments& g, int64_t n, int segments, f
     bsum_.reset(sizeof(double) * size_t(nb) * segments);
     bins_.reset(sizeof(double) * size_t(nb) * segments * kBins);
     ebase_.reset(sizeof(int) * size_t(nb) * segments);
-    hipLaunchKernelGGL(block_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, g, n, nb, bsum_.as<double>());
-    hipLaunchKernelGGL(prefix_kernel, dim3(segments), dim3(kThreads), 0, s, bsum_.as<double>(), nb, ebase_.as<int>());
-    hipLaunchKernelGGL(binade_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, g, n, nb, ebase_.as<int>(),
-                       bins_.as<double>());
-    hipLaunchKernelGGL(walk_kernel, dim3(segments), dim3(64), 0, s, g, n, nb, bsum_.as<double>(), ebase_.as<int>(),
-                       bins_.as<double>(), totals);
+    const double data_bytes = 4.0 * double(n) * segments;  // one read of the summed data
+    {
+        ProfileScope ps("seqsum-block-sums", data_bytes);
+        hipLaunchKernelGGL(block_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, g, n, nb, bsum_.as<double>());
+        hipLaunchKernelGGL(prefix_kernel, dim3(segments), dim3(kThreads), 0, s, bsum_.as<double>(), nb,
+                           ebase_.as<int>());
+    }
+    {
+        ProfileScope ps("seqsum-binade-sums", data_bytes);
+        hipLaunchKernelGGL(binade_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, g, n, nb, ebase_.as<int>(),
+                           bins_.as<double>());
+    }
+    {
+        // reads the block tables and replays a few blocks: latency-bound, no byte model
+        ProfileScope ps("seqsum-walk", 0.0);
+        hipLaunchKernelGGL(walk_kernel, dim3(segments), dim3(64), 0, s, g, n, nb, bsum_.as<double>(), ebase_.as<int>(),
+                           bins_.as<double>(), totals);
+    }
     LDH_CHECK_LAUNCH("seqsum-launch");
 }
 
 void SeqSum::scale(hipStream_t s, float* x, int64_t n, int64_t stride, int segments, const float* totals) {
     const int grid = int(std::min<int64_t>(div_up(n, 256 * 4), 2048));
+    ProfileScope ps("seqsum-scale", 8.0 * double(n) * segments);
     hipLaunchKernelGGL(scale_by_inverse_kernel, dim3(grid, segments), dim3(256), 0, s, x, n, stride, totals);
     LDH_CHECK_LAUNCH("normalize-launch");
 }
