@@ -114,6 +114,29 @@ def families(stats):
             for k, v in sorted(stats.items(), key=lambda kv: -kv[1][1])}
 
 
+def batched_form(ctx, progress):
+    """64 PSF-synthesis requests of uneven pupils (128-1024) and 8-31 wavelengths: one bank build after
+    another, shared launches per size group, and the groups captured into one HIP graph. A first pass
+    of each mode warms caches and the autotuner; the second is reported."""
+    from lensdiff_amd import _abi, workloads
+    reqs = workloads.batch_requests()
+    res = {"requests": len(reqs)}
+    for mode, name in ((_abi.LDH_BATCH_PER_REQUEST, "per_request"), (_abi.LDH_BATCH_GROUPED, "grouped"),
+                       (_abi.LDH_BATCH_GRAPH, "graph")):
+        for _ in range(2):
+            handles, st = ctx.build_banks(reqs, mode)
+            for h in handles:
+                ctx.bank_release(h)
+        res[name] = {"ms": round(st.exec_ms, 2), "psfs_per_s": round(st.psfs / (st.exec_ms / 1e3), 1),
+                     "groups": st.groups}
+        if mode == _abi.LDH_BATCH_GRAPH:
+            res[name].update({"capture_ms": round(st.capture_ms, 2), "graph_nodes": st.graph_nodes})
+        res["psfs"] = st.psfs
+        progress("batched %s: %.1f ms, %.0f PSFs/s" % (name, st.exec_ms, res[name]["psfs_per_s"]))
+    res["graph_vs_per_request"] = round(res["per_request"]["ms"] / res["graph"]["ms"], 2)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +149,7 @@ def main():
                     help="only copy 1 GiB device-to-device 3 times (HBM counter calibration) and exit")
     ap.add_argument("--bank-probe", action="store_true",
                     help="only build two fresh 31-wavelength UHD banks (profiling the PSF path) and exit")
+    ap.add_argument("--batch-probe", action="store_true", help="only run the batched form and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -142,6 +166,11 @@ def main():
 
     from lensdiff_amd import workloads
     from lensdiff_amd.backend import Context
+    if args.batch_probe:
+        ctx = Context(0)
+        print(json.dumps(batched_form(ctx, lambda m: print(m, file=sys.stderr, flush=True))))
+        ctx.close()
+        return
     if args.bank_probe:
         ctx = Context(0)
         ms = [round(ctx.bank_build_seconds(lens(5000 + i)) * 1e3, 2) for i in range(2)]
@@ -282,6 +311,7 @@ def main():
         "kernel_families_served": families(served_stats),
     }
     progress("latency pass done")
+    out["batched"] = batched_form(ctx, progress)
     if not args.no_cpu_baseline:
         progress("cpu baseline (reference, one core)")
         out["cpu_baseline"] = cpu_baseline(L(10), w, h, 31)

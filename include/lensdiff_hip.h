@@ -201,6 +201,20 @@ ldh_status ldh_sequential_sum(ldh_context* ctx, const float* x, int64_t n, float
 ldh_status ldh_fft2d(ldh_context* ctx, float* data, int32_t width, int32_t height, int32_t inverse);
 
 /* ---- PSF banks (EnsureLensDiffPsfBank, LensDiffCpuReference.cpp:2068-2071) ------------- */
+/* Batched bank synthesis (no counterpart in the reference, which builds one bank per render). */
+enum {
+    LDH_BATCH_PER_REQUEST = 0,  /* one bank build after another (the single-bank path) */
+    LDH_BATCH_GROUPED = 1,      /* requests with equal pupil/raw/kernel sizes share every launch */
+    LDH_BATCH_GRAPH = 2         /* LDH_BATCH_GROUPED captured into one HIP graph, launched once */
+};
+typedef struct ldh_batch_stats {
+    double exec_ms;        /* device time of the build (HIP events around its launches) */
+    double capture_ms;     /* host time to capture and instantiate the graph (0 without one) */
+    int32_t groups;        /* launch groups (requests in LDH_BATCH_PER_REQUEST) */
+    int32_t psfs;          /* (request, wavelength) PSFs built */
+    int32_t graph_nodes;   /* nodes of the captured graph (0 without one) */
+    int32_t mode;
+} ldh_batch_stats;
 typedef struct ldh_bank_info {
     int32_t bin_count;
     int32_t support_radius_px;     /* LensDiffPsfBankCache::supportRadiusPx */
@@ -213,6 +227,10 @@ typedef struct ldh_bank_info {
 /* Build or fetch the bank for `p` from the context's cache (refcounted). */
 ldh_status ldh_bank_acquire(ldh_context* ctx, const ldh_params* p, ldh_bank** out);
 ldh_status ldh_bank_release(ldh_context* ctx, ldh_bank* bank);
+/* Builds `count` banks (params[i] -> banks[i], each released with ldh_bank_release) in the
+ * given LDH_BATCH_* mode; all modes give the same banks. stats may be NULL. */
+ldh_status ldh_bank_build_batch(ldh_context* ctx, const ldh_params* params, int32_t count, int32_t mode,
+                                ldh_bank** banks, ldh_batch_stats* stats);
 ldh_status ldh_bank_info_get(ldh_context* ctx, const ldh_bank* bank, ldh_bank_info* out);
 /* Copy one kernel of one bin to host (capacity in floats). */
 ldh_status ldh_bank_download(ldh_context* ctx, const ldh_bank* bank, int32_t bin, int32_t which,

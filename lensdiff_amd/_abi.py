@@ -164,6 +164,20 @@ class BankInfo(C.Structure):
     ]
 
 
+LDH_BATCH_PER_REQUEST, LDH_BATCH_GROUPED, LDH_BATCH_GRAPH = 0, 1, 2
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("exec_ms", C.c_double),
+        ("capture_ms", C.c_double),
+        ("groups", C.c_int32),
+        ("psfs", C.c_int32),
+        ("graph_nodes", C.c_int32),
+        ("mode", C.c_int32),
+    ]
+
+
 class RequestStats(C.Structure):
     _fields_ = [
         ("latency_ms", C.c_double),

@@ -35,6 +35,8 @@ SIGNATURES = {
     "ldh_fft2d": (C.c_int32, [_CTX, _F, C.c_int32, C.c_int32, C.c_int32]),
     "ldh_bank_acquire": (C.c_int32, [_CTX, _P, C.POINTER(C.c_void_p)]),
     "ldh_bank_release": (C.c_int32, [_CTX, _BANK]),
+    "ldh_bank_build_batch": (C.c_int32, [_CTX, _P, C.c_int32, C.c_int32, C.POINTER(C.c_void_p),
+                                         C.POINTER(_abi.BatchStats)]),
     "ldh_bank_info_get": (C.c_int32, [_CTX, _BANK, C.POINTER(_abi.BankInfo)]),
     "ldh_bank_download": (C.c_int32, [_CTX, _BANK, C.c_int32, C.c_int32, _F, C.c_int64]),
     "ldh_render": (C.c_int32, [_CTX, C.POINTER(_abi.RenderRequest), _P]),

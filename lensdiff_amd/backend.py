@@ -125,20 +125,36 @@ class Context:
         b = C.c_void_p()
         self._check(self.lib.ldh_bank_acquire(self.h, C.byref(p), C.byref(b)))
         try:
-            info = _abi.BankInfo()
-            self._check(self.lib.ldh_bank_info_get(self.h, b, C.byref(info)))
-            out = {"wavelengths": [], "support": info.support_radius_px, "raw_size": info.raw_psf_size,
-                   "full": [], "core": [], "structure": []}
-            for i in range(info.bin_count):
-                k = info.kernel_size[i]
-                out["wavelengths"].append(info.wavelengths_nm[i])
-                for which, name in ((0, "full"), (1, "core"), (2, "structure")):
-                    a = np.zeros((k, k), np.float32)
-                    self._check(self.lib.ldh_bank_download(self.h, b, i, which, _fptr(a), a.size))
-                    out[name].append(a)
-            return out
+            return self.bank_arrays(b)
         finally:
             self.lib.ldh_bank_release(self.h, b)
+
+    def build_banks(self, params: list, mode: int):
+        """ldh_bank_build_batch: (bank handles, BatchStats). Release each with bank_release."""
+        n = len(params)
+        arr = (_abi.Params * n)(*params)
+        handles = (C.c_void_p * n)()
+        st = _abi.BatchStats()
+        self._check(self.lib.ldh_bank_build_batch(self.h, arr, n, mode, handles, C.byref(st)))
+        return [C.c_void_p(h) for h in handles], st
+
+    def bank_release(self, b):
+        self._check(self.lib.ldh_bank_release(self.h, b))
+
+    def bank_arrays(self, b) -> dict:
+        """Host copy of a bank handle's kernels."""
+        info = _abi.BankInfo()
+        self._check(self.lib.ldh_bank_info_get(self.h, b, C.byref(info)))
+        out = {"wavelengths": [], "support": info.support_radius_px, "raw_size": info.raw_psf_size,
+               "full": [], "core": [], "structure": []}
+        for i in range(info.bin_count):
+            k = info.kernel_size[i]
+            out["wavelengths"].append(info.wavelengths_nm[i])
+            for which, name in ((0, "full"), (1, "core"), (2, "structure")):
+                a = np.zeros((k, k), np.float32)
+                self._check(self.lib.ldh_bank_download(self.h, b, i, which, _fptr(a), a.size))
+                out[name].append(a)
+        return out
 
     def bank_build_seconds(self, p) -> float:
         """Wall time of building the bank for `p` on the device (synchronised), bank released."""

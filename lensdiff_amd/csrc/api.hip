@@ -247,7 +247,7 @@ ldh_status ldh_raw_psf(ldh_context* ctx, const float* amp, const float* phase, i
         else LDH_HIP(hipMemsetAsync(dph.get(), 0, sizeof(float) * P * P, ctx->stream), "raw-psf");
         ctx->bw.plan.prepare(ctx->stream, P, R);
         ctx->bw.plan.raw_psfs(ctx->stream, ctx->fft, damp.as<float>(), dph.as<float>(), 1, dout.as<float>(),
-                              dtot.as<float>(), ctx->bw.seqsum);
+                              dtot.as<float>(), ctx->bw.seqsum, ctx->bw.psf);
         d2h(ctx, out, dout.get(), sizeof(float) * size_t(R) * R);
     });
 }
@@ -341,6 +341,34 @@ ldh_status ldh_bank_acquire(ldh_context* ctx, const ldh_params* p, ldh_bank** ou
         b->bank = ensure_bank(ctx, *p);
         b->key = ctx->cached_key;
         *out = b;
+    });
+}
+
+ldh_status ldh_bank_build_batch(ldh_context* ctx, const ldh_params* params, int32_t count, int32_t mode,
+                                ldh_bank** banks, ldh_batch_stats* stats) {
+    return guarded(ctx, [&] {
+        require(params && banks && count > 0 && mode >= LDH_BATCH_PER_REQUEST && mode <= LDH_BATCH_GRAPH,
+                LDH_ERR_INVALID_ARGUMENT, "invalid-argument");
+        for (int32_t i = 0; i < count; ++i) banks[i] = nullptr;
+        std::vector<ldh_params> reqs(params, params + count);
+        ldh::BatchTiming tm;
+        // a capture must not record timing events: bind a disabled profiler
+        ldh::Profiler none;
+        ldh::ProfilerBinding unbind(none);
+        auto built = ldh::build_banks_batched(ctx->stream, ctx->fft, ctx->bw, reqs, mode, &tm);
+        for (int32_t i = 0; i < count; ++i) {
+            auto* b = new ldh_bank();
+            b->bank = built[size_t(i)];
+            banks[i] = b;
+        }
+        if (stats) {
+            stats->exec_ms = tm.exec_ms;
+            stats->capture_ms = tm.capture_ms;
+            stats->groups = tm.groups;
+            stats->psfs = tm.psfs;
+            stats->graph_nodes = tm.graph_nodes;
+            stats->mode = tm.mode;
+        }
     });
 }
 

@@ -14,10 +14,13 @@ int Autotuner::run(const std::string& key, int variants, hipStream_t s, const st
     int dev = 0;
     (void)hipGetDevice(&dev);
     const std::string k = key + "@dev" + std::to_string(dev);
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(s, &cap);
     {
         std::lock_guard<std::mutex> g(mu_);
         auto it = table_.find(k);
-        if (it != table_.end() || variants <= 1) {
+        // inside a graph capture nothing may synchronise: an untuned key takes variant 0
+        if (it != table_.end() || variants <= 1 || cap != hipStreamCaptureStatusNone) {
             const int v = it != table_.end() ? it->second.choice : 0;
             launch(v);
             return v;

@@ -5,6 +5,7 @@
 #include <memory>
 #include <vector>
 
+#include "host_params.h"
 #include "ldh_internal.h"
 #include "psf.h"
 #include "seqsum.h"
@@ -14,7 +15,9 @@ namespace ldh {
 constexpr int kMaxKernelRadius = 2048;
 
 // Device-resident bank. Kernels are stored K x K (K = 2*max_radius+1) with zeros outside each
-// bin's cropped support, in one array [which][bin][K*K] (which: full, core, structure).
+// bin's cropped support: kernel (which, bin) at store + which * which_stride + bin * K*K (which:
+// full, core, structure). A bank built alone owns `kernels` ([which][bin][K*K]); the banks of a
+// batched build share one store ([which][unit][K*K], each bank's bins consecutive units).
 struct DeviceBank {
     int bins = 0;
     int pupil = 0, raw = 0;
@@ -23,21 +26,27 @@ struct DeviceBank {
     int support = 4;      // LensDiffPsfBankCache::supportRadiusPx
     std::vector<float> wavelengths;
     std::vector<int> radius;       // cropped radius per bin
-    DeviceBuffer kernels;          // float[3][bins][K*K]
-    size_t bytes() const { return kernels.bytes(); }
+    DeviceBuffer kernels;                   // own storage, float[3][bins][K*K]
+    std::shared_ptr<DeviceBuffer> shared;   // or the batch's store
+    float* store = nullptr;
+    int64_t which_stride = 0;
+    size_t bytes() const { return size_t(3) * bins * size_t(kernel) * kernel * sizeof(float); }
     const float* kernel_ptr(int which, int bin) const {
-        return kernels.as<float>() + (size_t(which) * bins + bin) * size_t(kernel) * kernel;
+        return store + which * which_stride + int64_t(bin) * kernel * kernel;
     }
 };
 
 // Scratch shared by bank builds on one stream.
 struct BankWorkspace {
     PsfPlan plan;
+    PsfScratch psf;
     SeqSum seqsum;
     DeviceBuffer amp, phase, coefs, raw, totals, base, base_totals, shape_totals, scales, mean, shaped, structure, ring_sum, ring_cnt, ring_peak,
         radius, ref_amp, ref_phase, ref_raw, ref_total;
     // first-zero radius of the clear-circle PSF per (P, R) (GetLensDiffReferenceRawPsfCached)
     std::vector<std::pair<std::pair<int, int>, float>> first_zero;
+    // Grows the shaping buffers for `count` kernels of side K ahead of a graph capture.
+    void reserve_shaping(int K, int count, int max_radius);
 };
 
 float first_minimum_from_profile(const std::vector<float>& profile, int size);
@@ -49,12 +58,32 @@ void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_sca
 // Shapes `count` base kernels (count x K*K) into full/core/structure (3 x count x K*K) and
 // radius[count] (device). `max_radius` as LensDiffCpuReference.cpp:1633. base_totals (nullable):
 // per-kernel sums of a base that is not yet normalised (read as base / total).
+// d_gains (nullable): per-kernel anisotropy gain 1 + 4 * max(0, emphasis), else from `anisotropy`.
 void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const float* base_totals, int K, int count,
-                   double anisotropy, int max_radius, float* out3, int* d_radius);
+                   double anisotropy, int max_radius, float* out3, int* d_radius, const float* d_gains = nullptr);
 // Radial mean profile of an R x R PSF about its centre (radialMeanProfile, :865-887), computed
 // on the device and returned to the host.
 std::vector<float> radial_profile(hipStream_t s, BankWorkspace& w, const float* psf, int R);
 
 std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWorkspace& w, const ldh_params& p);
+
+// First minimum of the clear-circle PSF for (P, R) (GetLensDiffReferenceRawPsfCached +
+// estimateFirstMinimumRadius, LensDiffCpuReference.cpp:2018-2038): built once per workspace and
+// (P, R), with a host synchronisation.
+float clear_circle_first_zero(hipStream_t s, const FftTables& t, BankWorkspace& w, int P, int R);
+
+// Per-wavelength phase coefficients and resample scales of a bank (the wavelength loop of
+// buildPsfBankGlobalOnly, LensDiffCpuReference.cpp:1702-1734).
+void bank_wavelength_params(const ldh_params& p, const Sizes& sz, const std::vector<float>& wl, float first_zero,
+                            std::vector<PhaseCoef>& coefs, std::vector<float>& scales);
+
+// Batched build of many banks (batch.hip). mode: LDH_BATCH_* (include/lensdiff_hip.h).
+struct BatchTiming {
+    double exec_ms = 0.0, capture_ms = 0.0;
+    int groups = 0, psfs = 0, graph_nodes = 0, mode = 0;
+};
+std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, const FftTables& t, BankWorkspace& w,
+                                                             const std::vector<ldh_params>& requests, int mode,
+                                                             BatchTiming* timing);
 
 }  // namespace ldh

@@ -140,9 +140,10 @@ __global__ void ring_mean_kernel(const double* __restrict__ sums, const int* __r
 
 // reshapeAnisotropy on the normalised base and mean (each read through its totals).
 __global__ void reshape_kernel(const float* __restrict__ base, const float* __restrict__ base_tot,
-                               const float* __restrict__ mean, const float* __restrict__ mean_tot, float gain,
-                               int64_t kk, float* __restrict__ shaped) {
+                               const float* __restrict__ mean, const float* __restrict__ mean_tot, float gain0,
+                               const float* __restrict__ gains, int64_t kk, float* __restrict__ shaped) {
     const int seg = blockIdx.y;
+    const float gain = gains != nullptr ? gains[seg] : gain0;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= kk) return;
     const int64_t o = int64_t(seg) * kk + i;
@@ -286,25 +287,31 @@ void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_sca
     }
 }
 
-void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const float* base_totals, int K, int count,
-                   double anisotropy, int max_radius, float* out3, int* d_radius) {
+void BankWorkspace::reserve_shaping(int K, int count, int max_radius) {
     const int64_t kk = int64_t(K) * K;
     const int64_t n = kk * count;
-    const int rmax_mean = K / 2;
-    w.mean.reset(sizeof(float) * n);
-    w.shaped.reset(sizeof(float) * n);
-    w.structure.reset(sizeof(float) * n);
+    mean.reset(sizeof(float) * n);
+    shaped.reset(sizeof(float) * n);
+    structure.reset(sizeof(float) * n);
     // sums: mean, shaped, structure, normalised-shaped run (count each), final (3 * count)
-    w.shape_totals.reset(sizeof(float) * 7 * count);
+    shape_totals.reset(sizeof(float) * 7 * count);
+    const int nr = std::max(K / 2, max_radius) + 1;
+    ring_sum.reset(sizeof(double) * nr * count);
+    ring_cnt.reset(sizeof(int) * nr * count);
+    ring_peak.reset(sizeof(unsigned) * nr * count);
+    seqsum.reserve(kk, 3 * count);
+}
+
+void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const float* base_totals, int K, int count,
+                   double anisotropy, int max_radius, float* out3, int* d_radius, const float* d_gains) {
+    const int64_t kk = int64_t(K) * K;
+    const int rmax_mean = K / 2;
+    w.reserve_shaping(K, count, max_radius);
     float* t_mean = w.shape_totals.as<float>();
     float* t_shaped = t_mean + count;
     float* t_struct = t_shaped + count;
     float* t_run = t_struct + count;
     float* t_final = t_run + count;
-    const int nr = std::max(rmax_mean, max_radius) + 1;
-    w.ring_sum.reset(sizeof(double) * nr * count);
-    w.ring_cnt.reset(sizeof(int) * nr * count);
-    w.ring_peak.reset(sizeof(unsigned) * nr * count);
     // Each normalisation below only sums its array; readers apply 1 / total as they load, which
     // gives the values an in-place normalisation would have left.
     // azimuthal mean of the normalised base
@@ -322,7 +329,7 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
 {
     ProfileScope ps_("shape-reshape", 12.0 * double(kk) * count);
         hipLaunchKernelGGL(reshape_kernel, grid1(kk, count), dim3(256), 0, s, base, base_totals, w.mean.as<float>(),
-                       t_mean, gain, kk, w.shaped.as<float>());
+                       t_mean, gain, d_gains, kk, w.shaped.as<float>());
     LDH_CHECK_LAUNCH("reshape");
     }
     w.seqsum.run(s, w.shaped.as<float>(), kk, kk, count, t_shaped);
@@ -420,7 +427,7 @@ float first_minimum_from_profile(const std::vector<float>& prof, int size) {
     return float(best);
 }
 
-static float clear_circle_first_zero(hipStream_t s, const FftTables& t, BankWorkspace& w, int P, int R) {
+float clear_circle_first_zero(hipStream_t s, const FftTables& t, BankWorkspace& w, int P, int R) {
     for (const auto& e : w.first_zero)
         if (e.first.first == P && e.first.second == R) return e.second;
     ldh_params q;
@@ -437,10 +444,27 @@ static float clear_circle_first_zero(hipStream_t s, const FftTables& t, BankWork
     LDH_HIP(hipMemsetAsync(w.ref_phase.get(), 0, sizeof(float) * P * P, s), "first-zero");
     w.plan.prepare(s, P, R);
     w.plan.raw_psfs(s, t, w.ref_amp.as<float>(), w.ref_phase.as<float>(), 1, w.ref_raw.as<float>(),
-                    w.ref_total.as<float>(), w.seqsum);
+                    w.ref_total.as<float>(), w.seqsum, w.psf);
     const float fz = first_minimum_from_profile(radial_profile(s, w, w.ref_raw.as<float>(), R), R);
     w.first_zero.push_back({{P, R}, fz});
     return fz;
+}
+
+void bank_wavelength_params(const ldh_params& p, const Sizes& sz, const std::vector<float>& wl, float first_zero,
+                            std::vector<PhaseCoef>& coefs, std::vector<float>& scales) {
+    const int L = int(wl.size());
+    const float scale_base = float(std::max(1.0, sz.diffraction_scale_px)) / std::max(1.0f, first_zero);
+    coefs.resize(size_t(L));
+    scales.resize(size_t(L));
+    for (int i = 0; i < L; ++i) {
+        ldh_params q = p;
+        const float off = (wl[size_t(i)] - 550.0f) / 110.0f;
+        q.phase_defocus += p.chromatic_focus * off;
+        if (std::abs(q.phase_defocus) > 1e-6) q.phase_mode = LDH_PHASE_ENABLED;
+        coefs[size_t(i)] = make_phase_coef(q);
+        const float chroma = std::max(0.25f, 1.0f + float(p.chromatic_spread) * off);
+        scales[size_t(i)] = scale_base * (wl[size_t(i)] / 550.0f) * chroma;
+    }
 }
 
 std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWorkspace& w, const ldh_params& p) {
@@ -458,23 +482,14 @@ std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWo
     bank->kernel = K;
     bank->wavelengths = wl;
 
-    const float first_zero = std::max(1.0f, clear_circle_first_zero(s, t, w, P, R));
-    const float scale_base = float(std::max(1.0, sz.diffraction_scale_px)) / first_zero;
+    const float first_zero = clear_circle_first_zero(s, t, w, P, R);
 
     // amplitude once, phase per wavelength (chromatic defocus offset, :1720-1727)
     w.amp.reset(sizeof(float) * P * P);
     pupil_amplitude(s, make_pupil_geom(p), w.amp.as<float>());
-    std::vector<PhaseCoef> coefs(static_cast<size_t>(L));
-    std::vector<float> scales(static_cast<size_t>(L));
-    for (int i = 0; i < L; ++i) {
-        ldh_params q = p;
-        const float off = (wl[size_t(i)] - 550.0f) / 110.0f;
-        q.phase_defocus += p.chromatic_focus * off;
-        if (std::abs(q.phase_defocus) > 1e-6) q.phase_mode = LDH_PHASE_ENABLED;
-        coefs[size_t(i)] = make_phase_coef(q);
-        const float chroma = std::max(0.25f, 1.0f + float(p.chromatic_spread) * off);
-        scales[size_t(i)] = scale_base * (wl[size_t(i)] / 550.0f) * chroma;
-    }
+    std::vector<PhaseCoef> coefs;
+    std::vector<float> scales;
+    bank_wavelength_params(p, sz, wl, first_zero, coefs, scales);
     w.coefs.reset(sizeof(PhaseCoef) * L);
     w.scales.reset(sizeof(float) * L);
     LDH_HIP(hipMemcpyAsync(w.coefs.get(), coefs.data(), sizeof(PhaseCoef) * L, hipMemcpyHostToDevice, s), "bank-upload");
@@ -488,7 +503,7 @@ std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWo
     w.totals.reset(sizeof(float) * L);
     // the unit-sum scale of each raw PSF is applied inside the resample
     w.plan.raw_psfs(s, t, w.amp.as<float>(), w.phase.as<float>(), L, w.raw.as<float>(), w.totals.as<float>(), w.seqsum,
-                    false);
+                    w.psf, false);
     const int64_t kk = int64_t(K) * K;
     w.base.reset(sizeof(float) * kk * L);
     resample_kernels(s, w.raw.as<float>(), R, w.scales.as<float>(), w.totals.as<float>(), r, L, w.base.as<float>());
@@ -497,6 +512,8 @@ std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWo
 
     // shaping into full / core / structure (the base is read normalised through its totals)
     bank->kernels.reset(sizeof(float) * kk * L * 3);
+    bank->store = bank->kernels.as<float>();
+    bank->which_stride = kk * L;
     w.radius.reset(sizeof(int) * L);
     shape_kernels(s, w, w.base.as<float>(), w.base_totals.as<float>(), K, L, p.anisotropy_emphasis, r,
                   bank->kernels.as<float>(), w.radius.as<int>());

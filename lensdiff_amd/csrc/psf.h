@@ -9,15 +9,27 @@ namespace ldh {
 
 constexpr int kMaxRawPsf = 8192;  // its Bluestein length (R + P - 1 <= 9215) fits one LDS-resident line
 
-// Per (P, R) state: chirp and Bluestein filter spectrum, plus stage buffers.
+// Stage buffers of the row and column transforms (shareable by plans of different sizes).
+struct PsfScratch {
+    DeviceBuffer g, gt, it;
+    void reserve(int pupil, int raw, int count) {
+        g.reset(sizeof(float2) * size_t(pupil) * raw * count);
+        gt.reset(sizeof(float2) * size_t(pupil) * raw * count);
+        it.reset(sizeof(float) * size_t(raw) * raw * count);
+    }
+};
+
+// Per (P, R) state: chirp and Bluestein filter spectrum.
 class PsfPlan {
 public:
     void prepare(hipStream_t s, int pupil, int raw);
     // `count` raw PSFs from one amplitude (P*P) and `count` phase maps (count*P*P) into
     // out (count*R*R); totals: device[count], their sequential-FP32 sums. With `normalize`
     // each PSF is scaled to unit sum, else left for the consumer to scale by 1/total.
+    // amp_index (device, nullable): PSF l uses amplitude amp + amp_index[l] * P * P.
     void raw_psfs(hipStream_t s, const FftTables& t, const float* amp, const float* phase, int count,
-                  float* out, float* totals, SeqSum& seqsum, bool normalize = true);
+                  float* out, float* totals, SeqSum& seqsum, PsfScratch& scratch, bool normalize = true,
+                  const int* amp_index = nullptr);
     int pupil() const { return pupil_; }
     int raw() const { return raw_; }
     int bluestein_length() const { return m_; }
@@ -26,10 +38,12 @@ public:
 private:
     int pupil_ = 0, raw_ = 0, m_ = 0, offset_ = 0;
     bool bluestein_ = false;
-    DeviceBuffer chirp_, bhat_, g_, gt_, it_;
+    DeviceBuffer chirp_, bhat_;
 };
 
 void pupil_amplitude(hipStream_t s, const PupilGeom& g, float* out);
+// One amplitude per geometry (device array of `count`), all of size n: out[count][n*n].
+void pupil_amplitudes(hipStream_t s, const PupilGeom* d_geoms, int n, int count, float* out);
 void pupil_phase(hipStream_t s, const PhaseCoef* d_coefs, int n, int count, float* out);
 
 }  // namespace ldh

@@ -111,12 +111,13 @@ __device__ __forceinline__ void line_transform(float2* lds, const float2* __rest
 // autotuner picks among W = 1, 3, 4 for long lines.
 template <int M, int W>
 __global__ void __launch_bounds__((LinePlan<M>::NT), W)
-psf_rows_kernel(const float* __restrict__ amp, const float* __restrict__ phase, int P, int R, int off,
-                const float2* __restrict__ chirp, const float2* __restrict__ bhat, int bluestein,
-                float2* __restrict__ G, const float2* __restrict__ tw) {
+psf_rows_kernel(const float* __restrict__ amp, const int* __restrict__ amp_index, const float* __restrict__ phase,
+                int P, int R, int off, const float2* __restrict__ chirp, const float2* __restrict__ bhat,
+                int bluestein, float2* __restrict__ G, const float2* __restrict__ tw) {
     extern __shared__ float2 lds[];
     const int y = blockIdx.x, lam = blockIdx.y, tid = threadIdx.x;
-    const RowInput in{amp + size_t(y) * P, phase + (size_t(lam) * P + y) * P, chirp, P, off, bluestein};
+    const size_t a0 = amp_index != nullptr ? size_t(amp_index[lam]) * P * P : 0;
+    const RowInput in{amp + a0 + size_t(y) * P, phase + (size_t(lam) * P + y) * P, chirp, P, off, bluestein};
     const RowOutput out{G + (size_t(lam) * P + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R};
     line_transform<M>(lds, tw, bhat, bluestein != 0, tid, in, out);
 }
@@ -183,6 +184,14 @@ __global__ void pupil_amplitude_kernel(PupilGeom g, float* __restrict__ out) {
     out[i] = pupil_amplitude_at(g, i % n, i / n);
 }
 
+__global__ void pupil_amplitude_batch_kernel(const PupilGeom* __restrict__ geoms, int n, float* __restrict__ out) {
+    const int k = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * n) return;
+    const PupilGeom g = geoms[k];
+    out[size_t(k) * n * n + i] = pupil_amplitude_at(g, i % n, i / n);
+}
+
 __global__ void pupil_phase_kernel(const PhaseCoef* __restrict__ coefs, int n, float* __restrict__ out) {
     const int lam = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -245,16 +254,17 @@ void host_dft(std::vector<double>& re, std::vector<double>& im) {
 
 constexpr int kPsfVariantWaves[3] = {1, 3, 4};
 // Tuning keys carry the line length and the batch (wavelength count rounded up to a power of
-// two): a one-wavelength launch does not fill the device the way a 31-wavelength bank does.
+// two, 32 and more alike): a one-wavelength launch does not fill the device the way a
+// 31-wavelength bank does.
 // Long lines are register-bound: they get all three occupancy variants.
 template <int M> constexpr int psf_variants() { return M >= 2048 ? 3 : 1; }
 
 template <int M, int W>
-void launch_rows_w(hipStream_t s, const float* amp, const float* phase, int P, int R, int off, const float2* chirp,
-                   const float2* bhat, int bluestein, float2* G, const float2* tw, int L) {
+void launch_rows_w(hipStream_t s, const float* amp, const int* amp_index, const float* phase, int P, int R, int off,
+                   const float2* chirp, const float2* bhat, int bluestein, float2* G, const float2* tw, int L) {
     allow_large_lds(reinterpret_cast<const void*>(&psf_rows_kernel<M, W>), (M + M / 32 + 1) * 8);
     hipLaunchKernelGGL((psf_rows_kernel<M, W>), dim3(P, L), dim3(LinePlan<M>::NT), (M + M / 32 + 1) * 8, s, amp,
-                       phase, P, R, off, chirp, bhat, bluestein, G, tw);
+                       amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw);
 }
 template <int M, int W>
 void launch_cols_w(hipStream_t s, const float2* GT, int P, int R, int off, const float2* chirp, const float2* bhat,
@@ -265,15 +275,15 @@ void launch_cols_w(hipStream_t s, const float2* GT, int P, int R, int off, const
 }
 
 template <int M>
-bool launch_rows(int m, hipStream_t s, const float* amp, const float* phase, int P, int R, int off,
-                 const float2* chirp, const float2* bhat, int bluestein, float2* G, const float2* tw, int L) {
+bool launch_rows(int m, hipStream_t s, const float* amp, const int* amp_index, const float* phase, int P, int R,
+                 int off, const float2* chirp, const float2* bhat, int bluestein, float2* G, const float2* tw, int L) {
     if (m != M) return false;
-    Autotuner::instance().run("psf-rows/" + std::to_string(M) + "/L" + std::to_string(pow2_at_least(L)), psf_variants<M>(), s, [&](int v) {
+    Autotuner::instance().run("psf-rows/" + std::to_string(M) + "/L" + std::to_string(std::min(32, pow2_at_least(L))), psf_variants<M>(), s, [&](int v) {
         if constexpr (psf_variants<M>() > 1) {
-            if (v == 1) return launch_rows_w<M, kPsfVariantWaves[1]>(s, amp, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
-            if (v == 2) return launch_rows_w<M, kPsfVariantWaves[2]>(s, amp, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
+            if (v == 1) return launch_rows_w<M, kPsfVariantWaves[1]>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
+            if (v == 2) return launch_rows_w<M, kPsfVariantWaves[2]>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
         }
-        launch_rows_w<M, 1>(s, amp, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
+        launch_rows_w<M, 1>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
     });
     return true;
 }
@@ -281,7 +291,7 @@ template <int M>
 bool launch_cols(int m, hipStream_t s, const float2* GT, int P, int R, int off, const float2* chirp,
                  const float2* bhat, int bluestein, float* IT, const float2* tw, int L) {
     if (m != M) return false;
-    Autotuner::instance().run("psf-cols/" + std::to_string(M) + "/L" + std::to_string(pow2_at_least(L)), psf_variants<M>(), s, [&](int v) {
+    Autotuner::instance().run("psf-cols/" + std::to_string(M) + "/L" + std::to_string(std::min(32, pow2_at_least(L))), psf_variants<M>(), s, [&](int v) {
         if constexpr (psf_variants<M>() > 1) {
             if (v == 1) return launch_cols_w<M, kPsfVariantWaves[1]>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
             if (v == 2) return launch_cols_w<M, kPsfVariantWaves[2]>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
@@ -348,17 +358,25 @@ void pupil_amplitude(hipStream_t s, const PupilGeom& g, float* out) {
     LDH_CHECK_LAUNCH("pupil-amplitude");
 }
 
+void pupil_amplitudes(hipStream_t s, const PupilGeom* d_geoms, int n, int count, float* out) {
+    hipLaunchKernelGGL(pupil_amplitude_batch_kernel, dim3(div_up(int64_t(n) * n, 256), count), dim3(256), 0, s, d_geoms,
+                       n, out);
+    LDH_CHECK_LAUNCH("pupil-amplitude");
+}
+
 void pupil_phase(hipStream_t s, const PhaseCoef* d_coefs, int n, int count, float* out) {
     hipLaunchKernelGGL(pupil_phase_kernel, dim3(div_up(int64_t(n) * n, 256), count), dim3(256), 0, s, d_coefs, n, out);
     LDH_CHECK_LAUNCH("pupil-phase");
 }
 
 void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, const float* phase, int count,
-                       float* out, float* totals, SeqSum& seqsum, bool normalize) {
+                       float* out, float* totals, SeqSum& seqsum, PsfScratch& scratch, bool normalize,
+                       const int* amp_index) {
     const int P = pupil_, R = raw_, L = count;
-    g_.reset(sizeof(float2) * size_t(P) * R * L);
-    gt_.reset(sizeof(float2) * size_t(P) * R * L);
-    it_.reset(sizeof(float) * size_t(R) * R * L);
+    scratch.reserve(P, R, L);
+    DeviceBuffer& g_ = scratch.g;
+    DeviceBuffer& gt_ = scratch.gt;
+    DeviceBuffer& it_ = scratch.it;
     const float2* tw = t.twiddle.as<float2>();
     const float2* chirp = chirp_.as<float2>();
     const float2* bhat = bhat_.as<float2>();
@@ -366,7 +384,7 @@ void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, cons
     {
         // algorithmic bytes: amplitude and phase in, row transforms out
         ProfileScope ps("psf-rows", (8.0 * P * P + 8.0 * P * R) * L);
-#define LDH_PSF_ROWS(MM) launch_rows<MM>(m_, s, amp, phase, P, R, offset_, chirp, bhat, blu, g_.as<float2>(), tw, L)
+#define LDH_PSF_ROWS(MM) launch_rows<MM>(m_, s, amp, amp_index, phase, P, R, offset_, chirp, bhat, blu, g_.as<float2>(), tw, L)
         if (!(LDH_PSF_LENGTHS(LDH_PSF_ROWS))) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
 #undef LDH_PSF_ROWS
         LDH_CHECK_LAUNCH("psf-rows");

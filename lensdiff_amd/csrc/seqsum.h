@@ -37,6 +37,8 @@ public:
     void scale(hipStream_t s, float* x, int64_t n, int64_t stride, int segments, const float* totals);
     // run(), then scale().
     void normalize(hipStream_t s, float* x, int64_t n, int64_t stride, int segments, float* totals);
+    // Grows the block tables for sums of `segments` arrays of n ahead of a graph capture.
+    void reserve(int64_t n, int segments);
 
 private:
     DeviceBuffer bsum_, bins_, ebase_;

@@ -331,13 +331,18 @@ void SeqSum::run(hipStream_t s, const float* x, int64_t n, int64_t stride, int s
     run(s, g, n, segments, totals);
 }
 
-void SeqSum::run(hipStream_t s, const SeqSegments& g, int64_t n, int segments, float* totals) {
-    if (n <= 0 || segments <= 0) return;
-    if (segments > g.per_array * SeqSegments::kMaxArrays) fail(LDH_ERR_INVALID_ARGUMENT, "seqsum-segments");
+void SeqSum::reserve(int64_t n, int segments) {
     const int nb = int(div_up(n, kSeqSumBlock));
     bsum_.reset(sizeof(double) * size_t(nb) * segments);
     bins_.reset(sizeof(double) * size_t(nb) * segments * kBins);
     ebase_.reset(sizeof(int) * size_t(nb) * segments);
+}
+
+void SeqSum::run(hipStream_t s, const SeqSegments& g, int64_t n, int segments, float* totals) {
+    if (n <= 0 || segments <= 0) return;
+    if (segments > g.per_array * SeqSegments::kMaxArrays) fail(LDH_ERR_INVALID_ARGUMENT, "seqsum-segments");
+    const int nb = int(div_up(n, kSeqSumBlock));
+    reserve(n, segments);
     const double data_bytes = 4.0 * double(n) * segments;  // one read of the summed data
     {
         ProfileScope ps("seqsum-block-sums", data_bytes);

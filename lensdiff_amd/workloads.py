@@ -128,3 +128,24 @@ def scaled_spectral_params(short_side, pupil, n_wavelengths, radius_px, seed=5, 
     """A spectral workload shaped like c1/c2 at a smaller size (for parity tests)."""
     percent = 100.0 * (radius_px - 0.5) / short_side
     return spectral_params(short_side, pupil, blades, noll, sigma, seed, n_wavelengths, percent)
+
+
+def batch_requests(n=64, seed=3, pupils=(128, 256, 384, 512, 768, 1024), wl_range=(8, 31), noll_sigma=0.15):
+    """PSF-synthesis requests of the batched form: pupil sizes drawn from `pupils`, 5-9 blades,
+    rotation uniform over the circle, Noll 2-15 N(0, noll_sigma) waves, 8-31 wavelengths evenly
+    over 400-700 nm. Each lens is set on a frame whose short side equals its pupil size with a
+    50% kernel radius, so r = P/2 and R = 2P (Bluestein for P = 384 and 768)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        P = int(rng.choice(pupils))
+        p = base_params(P)
+        p.pupil_resolution = P
+        p.blade_count = int(rng.integers(5, 10))
+        p.rotation_deg = float(rng.uniform(0.0, 360.0))
+        p.max_kernel_radius_px = 50.0
+        p.set_noll(noll_coefficients(range(2, 16), noll_sigma, seed * 1000 + i))
+        p.set_wavelengths(np.linspace(400.0, 700.0, int(rng.integers(wl_range[0], wl_range[1] + 1))))
+        p.chromatic_focus = 0.3
+        out.append(p)
+    return out

@@ -63,6 +63,9 @@ int main(void) {
          offsetof(ldh_params, phase_defocus), offsetof(ldh_params, wavelength_count),
          offsetof(ldh_params, zernike_extra), offsetof(ldh_params, spectra_fp16));
   printf("%zu %zu\n", offsetof(ldh_render_request, src), offsetof(ldh_render_request, frame_short_side_px));
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(ldh_batch_stats), offsetof(ldh_batch_stats, groups),
+         sizeof(ldh_request_stats), sizeof(ldh_server_stats), offsetof(ldh_batch_stats, mode),
+         offsetof(ldh_batch_stats, capture_ms));
   return 0;
 }
 """
@@ -84,6 +87,9 @@ def test_ctypes_layout_matches_c():
                     P.wavelength_count.offset, P.zernike_extra.offset, P.spectra_fp16.offset]
     offs = [int(v) for v in lines[2].split()]
     assert offs == [_abi.RenderRequest.src.offset, _abi.RenderRequest.frame_short_side_px.offset]
+    B = _abi.BatchStats
+    assert [int(v) for v in lines[3].split()] == [ctypes.sizeof(B), B.groups.offset, ctypes.sizeof(_abi.RequestStats),
+                                                  ctypes.sizeof(_abi.ServerStats), B.mode.offset, B.capture_ms.offset]
 
 
 def test_context_create_fails_cleanly_without_device():
