@@ -217,7 +217,7 @@ ldh_status ldh_pupil_amplitude(ldh_context* ctx, const ldh_params* p, float* out
         const ldh::PupilGeom g = ldh::make_pupil_geom(*p);
         const size_t bytes = sizeof(float) * g.size * g.size;
         ctx->scratch.reset(bytes);
-        ldh::pupil_amplitude(ctx->stream, g, ctx->scratch.as<float>());
+        ldh::pupil_amplitude(ctx->stream, g, ctx->scratch.as<float>(), false);
         d2h(ctx, out, ctx->scratch.get(), bytes);
     });
 }
@@ -231,7 +231,7 @@ ldh_status ldh_pupil_phase(ldh_context* ctx, const ldh_params* p, float* out) {
         h2d(ctx, ctx->scratch2.get(), &c, sizeof(c));
         const size_t bytes = sizeof(float) * n * n;
         ctx->scratch.reset(bytes);
-        ldh::pupil_phase(ctx->stream, ctx->scratch2.as<ldh::PhaseCoef>(), n, 1, ctx->scratch.as<float>());
+        ldh::pupil_phase(ctx->stream, ctx->scratch2.as<ldh::PhaseCoef>(), n, 1, ctx->scratch.as<float>(), false);
         d2h(ctx, out, ctx->scratch.get(), bytes);
     });
 }
@@ -242,9 +242,22 @@ ldh_status ldh_raw_psf(ldh_context* ctx, const float* amp, const float* phase, i
         require(R <= ldh::kMaxRawPsf, LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
         ldh::DeviceBuffer damp(sizeof(float) * P * P), dph(sizeof(float) * P * P), dout(sizeof(float) * size_t(R) * R),
             dtot(sizeof(float));
-        h2d(ctx, damp.get(), amp, sizeof(float) * P * P);
-        if (phase) h2d(ctx, dph.get(), phase, sizeof(float) * P * P);
-        else LDH_HIP(hipMemsetAsync(dph.get(), 0, sizeof(float) * P * P, ctx->stream), "raw-psf");
+        // the transform takes the pupil transposed ([x][y])
+        auto transposed = [P](const float* m) {
+            std::vector<float> t(size_t(P) * P);
+            for (int y = 0; y < P; ++y)
+                for (int x = 0; x < P; ++x) t[size_t(x) * P + y] = m[size_t(y) * P + x];
+            return t;
+        };
+        // host copies stay alive until the final synchronising download
+        const std::vector<float> amp_t = transposed(amp);
+        const std::vector<float> phase_t = phase ? transposed(phase) : std::vector<float>();
+        h2d(ctx, damp.get(), amp_t.data(), sizeof(float) * P * P);
+        if (phase) {
+            h2d(ctx, dph.get(), phase_t.data(), sizeof(float) * P * P);
+        } else {
+            LDH_HIP(hipMemsetAsync(dph.get(), 0, sizeof(float) * P * P, ctx->stream), "raw-psf");
+        }
         ctx->bw.plan.prepare(ctx->stream, P, R);
         ctx->bw.plan.raw_psfs(ctx->stream, ctx->fft, damp.as<float>(), dph.as<float>(), 1, dout.as<float>(),
                               dtot.as<float>(), ctx->bw.seqsum, ctx->bw.psf);

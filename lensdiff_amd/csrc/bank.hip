@@ -440,7 +440,7 @@ float clear_circle_first_zero(hipStream_t s, const FftTables& t, BankWorkspace& 
     w.ref_phase.reset(sizeof(float) * P * P);
     w.ref_raw.reset(sizeof(float) * size_t(R) * R);
     w.ref_total.reset(sizeof(float));
-    pupil_amplitude(s, make_pupil_geom(q), w.ref_amp.as<float>());
+    pupil_amplitude(s, make_pupil_geom(q), w.ref_amp.as<float>(), true);
     LDH_HIP(hipMemsetAsync(w.ref_phase.get(), 0, sizeof(float) * P * P, s), "first-zero");
     w.plan.prepare(s, P, R);
     w.plan.raw_psfs(s, t, w.ref_amp.as<float>(), w.ref_phase.as<float>(), 1, w.ref_raw.as<float>(),
@@ -484,9 +484,9 @@ std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWo
 
     const float first_zero = clear_circle_first_zero(s, t, w, P, R);
 
-    // amplitude once, phase per wavelength (chromatic defocus offset, :1720-1727)
+    // amplitude once, phase per wavelength (chromatic defocus offset, :1720-1727), both transposed
     w.amp.reset(sizeof(float) * P * P);
-    pupil_amplitude(s, make_pupil_geom(p), w.amp.as<float>());
+    pupil_amplitude(s, make_pupil_geom(p), w.amp.as<float>(), true);
     std::vector<PhaseCoef> coefs;
     std::vector<float> scales;
     bank_wavelength_params(p, sz, wl, first_zero, coefs, scales);
@@ -495,7 +495,7 @@ std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWo
     LDH_HIP(hipMemcpyAsync(w.coefs.get(), coefs.data(), sizeof(PhaseCoef) * L, hipMemcpyHostToDevice, s), "bank-upload");
     LDH_HIP(hipMemcpyAsync(w.scales.get(), scales.data(), sizeof(float) * L, hipMemcpyHostToDevice, s), "bank-upload");
     w.phase.reset(sizeof(float) * size_t(P) * P * L);
-    pupil_phase(s, w.coefs.as<PhaseCoef>(), P, L, w.phase.as<float>());
+    pupil_phase(s, w.coefs.as<PhaseCoef>(), P, L, w.phase.as<float>(), true);
 
     // raw PSFs, then per-wavelength resampled base kernels
     w.plan.prepare(s, P, R);

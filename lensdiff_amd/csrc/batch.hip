@@ -44,8 +44,8 @@ void upload(DeviceBuffer& d, const std::vector<T>& h, hipStream_t s) {
 void enqueue_group(hipStream_t s, const FftTables& t, BankWorkspace& w, Group& g) {
     const int P = g.P, R = g.R, U = g.units, K = g.K;
     const int64_t kk = int64_t(K) * K;
-    pupil_amplitudes(s, g.d_geoms.as<PupilGeom>(), P, int(g.requests.size()), g.amp.as<float>());
-    pupil_phase(s, g.d_coefs.as<PhaseCoef>(), P, U, g.phase.as<float>());
+    pupil_amplitudes(s, g.d_geoms.as<PupilGeom>(), P, int(g.requests.size()), g.amp.as<float>(), true);
+    pupil_phase(s, g.d_coefs.as<PhaseCoef>(), P, U, g.phase.as<float>(), true);
     // raw PSFs (unit sums applied inside the resample), base kernels, shaping
     g.plan.raw_psfs(s, t, g.amp.as<float>(), g.phase.as<float>(), U, w.raw.as<float>(), w.totals.as<float>(), w.seqsum,
                     w.psf, false, g.d_amp_index.as<int>());

@@ -9,13 +9,12 @@ namespace ldh {
 
 constexpr int kMaxRawPsf = 8192;  // its Bluestein length (R + P - 1 <= 9215) fits one LDS-resident line
 
-// Stage buffers of the row and column transforms (shareable by plans of different sizes).
+// Stage buffers of the two line passes (shareable by plans of different sizes).
 struct PsfScratch {
-    DeviceBuffer g, gt, it;
+    DeviceBuffer g, gt;
     void reserve(int pupil, int raw, int count) {
         g.reset(sizeof(float2) * size_t(pupil) * raw * count);
         gt.reset(sizeof(float2) * size_t(pupil) * raw * count);
-        it.reset(sizeof(float) * size_t(raw) * raw * count);
     }
 };
 
@@ -23,9 +22,10 @@ struct PsfScratch {
 class PsfPlan {
 public:
     void prepare(hipStream_t s, int pupil, int raw);
-    // `count` raw PSFs from one amplitude (P*P) and `count` phase maps (count*P*P) into
-    // out (count*R*R); totals: device[count], their sequential-FP32 sums. With `normalize`
-    // each PSF is scaled to unit sum, else left for the consumer to scale by 1/total.
+    // `count` raw PSFs from one amplitude (P*P) and `count` phase maps (count*P*P), both
+    // transposed ([x][y]), into out (count*R*R, centred); totals: device[count], their
+    // sequential-FP32 sums. With `normalize` each PSF is scaled to unit sum, else left for the
+    // consumer to scale by 1/total.
     // amp_index (device, nullable): PSF l uses amplitude amp + amp_index[l] * P * P.
     void raw_psfs(hipStream_t s, const FftTables& t, const float* amp, const float* phase, int count,
                   float* out, float* totals, SeqSum& seqsum, PsfScratch& scratch, bool normalize = true,
@@ -41,9 +41,10 @@ private:
     DeviceBuffer chirp_, bhat_;
 };
 
-void pupil_amplitude(hipStream_t s, const PupilGeom& g, float* out);
+// Pupil maps, row-major ([y][x]) or transposed ([x][y], the layout raw_psfs takes).
+void pupil_amplitude(hipStream_t s, const PupilGeom& g, float* out, bool transposed);
 // One amplitude per geometry (device array of `count`), all of size n: out[count][n*n].
-void pupil_amplitudes(hipStream_t s, const PupilGeom* d_geoms, int n, int count, float* out);
-void pupil_phase(hipStream_t s, const PhaseCoef* d_coefs, int n, int count, float* out);
+void pupil_amplitudes(hipStream_t s, const PupilGeom* d_geoms, int n, int count, float* out, bool transposed);
+void pupil_phase(hipStream_t s, const PhaseCoef* d_coefs, int n, int count, float* out, bool transposed);
 
 }  // namespace ldh

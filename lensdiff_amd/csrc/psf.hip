@@ -2,9 +2,12 @@
 // driven per wavelength by buildPsfBankGlobalOnly :1702-1734).
 //
 // The R x R transform only has a P x P non-zero block (the pupil, embedded at offset
-// floor((R-P)/2)), so it is computed as two separable line stages:
-//   rows:    for each pupil row y, the R-point DFT of its P embedded values  -> G[y][kx]
-//   columns: for each kx, the R-point DFT over the P pupil rows of G        -> F[ky][kx]
+// floor((R-P)/2)), so it is computed as two separable line stages, columns first:
+//   pass 1: for each pupil column x, the R-point DFT of its P embedded values -> H[x][ky]
+//   pass 2: for each ky, the R-point DFT over the P columns of H             -> F[ky][kx]
+// The pupil amplitude and phase are generated transposed ([x][y]) so pass 1 reads contiguous
+// lines, and pass 2's line is one row of F, whose |F|^2 lands straight in its row of the
+// centred raw PSF.
 // Each R-point DFT is done in LDS. For power-of-two R it is a plain Stockham FFT of length R.
 // Otherwise it is Bluestein's algorithm with the reference's chirp exp(-i pi j^2 / R) evaluated
 // in double with pi rounded to float (LensDiffCpuReference.cpp:244-274), which makes the
@@ -14,8 +17,7 @@
 // drops. With the input at 0 the circular convolution only has to hold lags -(P-1) .. R-1, so
 // the Bluestein length is the smallest FFT length M >= R + P - 1 (6144 for R = 4092, P = 1024,
 // where the reference uses 8192).
-// The column stage writes |F|^2 transposed; a shift-transpose pass then produces the centred
-// raw PSF, which the sequential-sum normalisation scales to unit energy.
+// The sequential-sum normalisation then scales each raw PSF to unit energy.
 #include <cmath>
 
 #include "autotune.h"
@@ -79,14 +81,17 @@ struct RowOutput {
         if (k < R) out[k] = val(k, v);
     }
 };
+// |F|^2 of output k into the centred row: column (k - R/2) mod R (fftShiftSquare, :303-314).
 struct ColOutput {
-    float* __restrict__ out;
+    float* __restrict__ row;
     LineValue val;
-    int R;
+    int R, shift;  // R - R/2
     __device__ void operator()(int k, float2 v) const {
         if (k < R) {
             const float2 f = val(k, v);
-            out[k] = f.x * f.x + f.y * f.y;
+            int x = k + shift;
+            if (x >= R) x -= R;
+            row[x] = f.x * f.x + f.y * f.y;
         }
     }
 };
@@ -128,9 +133,12 @@ psf_cols_kernel(const float2* __restrict__ GT, int P, int R, int off, const floa
                 const float2* __restrict__ bhat, int bluestein, float* __restrict__ IT,
                 const float2* __restrict__ tw) {
     extern __shared__ float2 lds[];
-    const int kx = blockIdx.x, lam = blockIdx.y, tid = threadIdx.x;
-    const ColInput in{GT + (size_t(lam) * R + kx) * P, chirp, P, off, bluestein};
-    const ColOutput out{IT + (size_t(lam) * R + kx) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R};
+    const int ky = blockIdx.x, lam = blockIdx.y, tid = threadIdx.x;
+    const int shift = R - R / 2;
+    int y = ky + shift;
+    if (y >= R) y -= R;
+    const ColInput in{GT + (size_t(lam) * R + ky) * P, chirp, P, off, bluestein};
+    const ColOutput out{IT + (size_t(lam) * R + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R, shift};
     line_transform<M>(lds, tw, bhat, bluestein != 0, tid, in, out);
 }
 
@@ -154,50 +162,40 @@ transpose_c64_kernel(const float2* __restrict__ in, float2* __restrict__ out, in
     }
 }
 
-// S[y][x] = IT[(x + h) % R][(y + h) % R]: undoes the column stage's transpose and applies the
-// centring shift by h = R/2 (fftShiftSquare, LensDiffCpuReference.cpp:303-314).
-__global__ void __launch_bounds__(256)
-shift_transpose_kernel(const float* __restrict__ IT, float* __restrict__ S, int R) {
-    __shared__ float tile[32][33];
-    const int lam = blockIdx.z;
-    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
-    const int h = R / 2;
-    const float* src = IT + size_t(lam) * R * R;
-    float* dst = S + size_t(lam) * R * R;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    // load: tile[k][tx] = IT[a = (x0+k+h)%R][b = (y0+tx+h)%R]
-    for (int k = ty; k < 32; k += 8) {
-        const int x = x0 + k, y = y0 + tx;
-        if (x < R && y < R) tile[k][tx] = src[size_t((x + h) % R) * R + (y + h) % R];
-    }
-    __syncthreads();
-    for (int k = ty; k < 32; k += 8) {
-        const int y = y0 + k, x = x0 + tx;
-        if (x < R && y < R) dst[size_t(y) * R + x] = tile[tx][k];
-    }
+// Pixel (x, y) of output element i: row-major out[y][x], or transposed out[x][y].
+__device__ __forceinline__ void pixel_of(int i, int n, int transposed, int* x, int* y) {
+    if (transposed) { *x = i / n; *y = i % n; }
+    else { *x = i % n; *y = i / n; }
 }
 
-__global__ void pupil_amplitude_kernel(PupilGeom g, float* __restrict__ out) {
+__global__ void pupil_amplitude_kernel(PupilGeom g, int transposed, float* __restrict__ out) {
     const int n = g.size;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * n) return;
-    out[i] = pupil_amplitude_at(g, i % n, i / n);
+    int x, y;
+    pixel_of(i, n, transposed, &x, &y);
+    out[i] = pupil_amplitude_at(g, x, y);
 }
 
-__global__ void pupil_amplitude_batch_kernel(const PupilGeom* __restrict__ geoms, int n, float* __restrict__ out) {
+__global__ void pupil_amplitude_batch_kernel(const PupilGeom* __restrict__ geoms, int n, int transposed,
+                                             float* __restrict__ out) {
     const int k = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * n) return;
     const PupilGeom g = geoms[k];
-    out[size_t(k) * n * n + i] = pupil_amplitude_at(g, i % n, i / n);
+    int x, y;
+    pixel_of(i, n, transposed, &x, &y);
+    out[size_t(k) * n * n + i] = pupil_amplitude_at(g, x, y);
 }
 
-__global__ void pupil_phase_kernel(const PhaseCoef* __restrict__ coefs, int n, float* __restrict__ out) {
+__global__ void pupil_phase_kernel(const PhaseCoef* __restrict__ coefs, int n, int transposed, float* __restrict__ out) {
     const int lam = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * n) return;
     const PhaseCoef c = coefs[lam];
-    out[size_t(lam) * n * n + i] = phase_waves_at(c, i % n, i / n, n);
+    int x, y;
+    pixel_of(i, n, transposed, &x, &y);
+    out[size_t(lam) * n * n + i] = phase_waves_at(c, x, y, n);
 }
 
 // Double-precision transforms on the host (Bluestein filter tables only): radix-2 for the
@@ -352,20 +350,22 @@ void PsfPlan::prepare(hipStream_t s, int pupil, int raw) {
     LDH_HIP(hipStreamSynchronize(s), "psf-chirp");
 }
 
-void pupil_amplitude(hipStream_t s, const PupilGeom& g, float* out) {
+void pupil_amplitude(hipStream_t s, const PupilGeom& g, float* out, bool transposed) {
     const int n = g.size;
-    hipLaunchKernelGGL(pupil_amplitude_kernel, dim3(div_up(int64_t(n) * n, 256)), dim3(256), 0, s, g, out);
+    hipLaunchKernelGGL(pupil_amplitude_kernel, dim3(div_up(int64_t(n) * n, 256)), dim3(256), 0, s, g,
+                       transposed ? 1 : 0, out);
     LDH_CHECK_LAUNCH("pupil-amplitude");
 }
 
-void pupil_amplitudes(hipStream_t s, const PupilGeom* d_geoms, int n, int count, float* out) {
+void pupil_amplitudes(hipStream_t s, const PupilGeom* d_geoms, int n, int count, float* out, bool transposed) {
     hipLaunchKernelGGL(pupil_amplitude_batch_kernel, dim3(div_up(int64_t(n) * n, 256), count), dim3(256), 0, s, d_geoms,
-                       n, out);
+                       n, transposed ? 1 : 0, out);
     LDH_CHECK_LAUNCH("pupil-amplitude");
 }
 
-void pupil_phase(hipStream_t s, const PhaseCoef* d_coefs, int n, int count, float* out) {
-    hipLaunchKernelGGL(pupil_phase_kernel, dim3(div_up(int64_t(n) * n, 256), count), dim3(256), 0, s, d_coefs, n, out);
+void pupil_phase(hipStream_t s, const PhaseCoef* d_coefs, int n, int count, float* out, bool transposed) {
+    hipLaunchKernelGGL(pupil_phase_kernel, dim3(div_up(int64_t(n) * n, 256), count), dim3(256), 0, s, d_coefs, n,
+                       transposed ? 1 : 0, out);
     LDH_CHECK_LAUNCH("pupil-phase");
 }
 
@@ -376,7 +376,6 @@ void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, cons
     scratch.reserve(P, R, L);
     DeviceBuffer& g_ = scratch.g;
     DeviceBuffer& gt_ = scratch.gt;
-    DeviceBuffer& it_ = scratch.it;
     const float2* tw = t.twiddle.as<float2>();
     const float2* chirp = chirp_.as<float2>();
     const float2* bhat = bhat_.as<float2>();
@@ -396,18 +395,12 @@ void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, cons
     LDH_CHECK_LAUNCH("psf-transpose");
     }
     {
-        // algorithmic bytes: the P row-transform values of each column in, |F|^2 out
+        // algorithmic bytes: the P pass-1 values of each output row in, |F|^2 out
         ProfileScope ps("psf-cols", (8.0 * P * R + 4.0 * R * R) * L);
-#define LDH_PSF_COLS(MM) launch_cols<MM>(m_, s, gt_.as<float2>(), P, R, offset_, chirp, bhat, blu, it_.as<float>(), tw, L)
+#define LDH_PSF_COLS(MM) launch_cols<MM>(m_, s, gt_.as<float2>(), P, R, offset_, chirp, bhat, blu, out, tw, L)
         if (!(LDH_PSF_LENGTHS(LDH_PSF_COLS))) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
 #undef LDH_PSF_COLS
         LDH_CHECK_LAUNCH("psf-cols");
-    }
-    {
-    ProfileScope ps_s("psf-shift", 8.0 * R * R * L);
-    hipLaunchKernelGGL(shift_transpose_kernel, dim3(div_up(R, 32), div_up(R, 32), L), dim3(256), 0, s,
-                       it_.as<float>(), out, R);
-    LDH_CHECK_LAUNCH("psf-shift");
     }
     if (normalize)
         seqsum.normalize(s, out, int64_t(R) * R, int64_t(R) * R, L, totals);

@@ -1,0 +1,30 @@
+"""Comparison of cropped bank kernels.
+
+The adaptive support radius (estimateAdaptiveSupportRadius, LensDiffCpuReference.cpp:984-1036) is
+a hard threshold on the captured-energy fraction and the outside peak at each ring. When a bin's
+fraction sits on the threshold, rounding differences far below the parity tolerance move the
+decision by one ring; the kernels then differ only in their outermost taper ring. Kernels are
+therefore compared after centring both in the larger square, with the radii allowed to differ by
+one ring."""
+import numpy as np
+
+from oracle.oracle import rel_l2
+
+
+def centred(a, n):
+    k = a.shape[0]
+    if k == n:
+        return a
+    out = np.zeros((n, n), a.dtype)
+    o = (n - k) // 2
+    out[o:o + k, o:o + k] = a
+    return out
+
+
+def kernel_rel_l2(got, want):
+    """rel L2 of two odd square kernels centred in the larger one; asserts the sides differ by at
+    most two (one ring of radius)."""
+    assert got.ndim == 2 and want.ndim == 2 and got.shape[0] == got.shape[1] and want.shape[0] == want.shape[1]
+    assert abs(got.shape[0] - want.shape[0]) <= 2, (got.shape, want.shape)
+    n = max(got.shape[0], want.shape[0])
+    return rel_l2(centred(got, n), centred(want, n))
