@@ -1,6 +1,8 @@
 // Single-line FFTs held in LDS for the whole transform (the PSF line kernels), planned as one
-// composite first stage and radix-16 stages: M = R0 * 16^S with R0 in {2..48} (6144 = 24*16*16,
-// 8192 = 32*16*16, 3072 = 12*16*16). Stage inputs and outputs go through policies, so the first
+// composite first stage and S stages of radix B = 16 or 8: M = R0 * B^S with R0 <= 48
+// (6144 = 24*16*16 = 12*8*8*8, 8192 = 32*16*16 = 16*8*8*8). Radix 16 means fewer passes over
+// LDS; radix 8 means fewer registers per thread and more threads per line; which is faster is
+// left to the autotuner. Stage inputs and outputs go through policies, so the first
 // stage can read its (mostly zero) input straight from global memory and the last stage can
 // write straight to global memory, and pointwise work is fused into whichever stage touches
 // the data.
@@ -90,19 +92,28 @@ template <int R, bool INV> __device__ __forceinline__ void dft_any(float2* v) {
     else dft_ab<16, 3, INV>(v);  // 48
 }
 
-// M = R0 * 16^S with the fewest stages and R0 <= 48 (plannable: M is 2^a * 3^b, b <= 2,
-// except lengths like 1152 = 72 * 16).
-template <int M> struct LinePlan {
-    static constexpr int S = (M % 4096 == 0 && M / 4096 >= 2 && M / 256 > 48) ? 3
-                             : (M % 256 == 0 && M / 256 >= 2 && M / 16 > 48) ? 2
-                             : (M % 16 == 0 && M / 16 >= 2) ? 1 : 0;
-    static constexpr int R0 = S == 3 ? M / 4096 : S == 2 ? M / 256 : S == 1 ? M / 16 : M;
-    static constexpr bool ok = R0 <= 48 && (R0 == 2 || R0 == 3 || R0 == 4 || R0 == 6 || R0 == 8 || R0 == 9 ||
-                                            R0 == 12 || R0 == 16 || R0 == 18 || R0 == 24 || R0 == 32 || R0 == 36 ||
-                                            R0 == 48);
-    // threads: one butterfly each in the widest stage, at most 512
-    static constexpr int bf = (S > 0 && M / 16 > M / R0) ? M / 16 : M / R0;
-    static constexpr int NT = bf < 64 ? 64 : (bf > 512 ? 512 : (bf + 63) / 64 * 64);
+// M = R0 * B^S with the fewest stages S >= 1 and 2 <= R0 <= 48 (S = 0 when M <= 48).
+constexpr int line_stages(int M, int B) {
+    int bs = 1;
+    for (int s = 0; s <= 6; ++s) {
+        if (M % bs == 0 && M / bs <= 48 && M / bs >= 2 && (s > 0 || M <= 48)) return s;
+        bs *= B;
+    }
+    return -1;
+}
+constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+constexpr bool radix_ok(int r) {
+    return r == 2 || r == 3 || r == 4 || r == 6 || r == 8 || r == 9 || r == 12 || r == 16 || r == 18 || r == 24 ||
+           r == 32 || r == 36 || r == 48;
+}
+template <int M, int B = 16> struct LinePlan {
+    static constexpr int S = line_stages(M, B);
+    static constexpr int R0 = S >= 0 ? M / ipow(B, S) : 0;
+    static constexpr bool ok = S >= 0 && radix_ok(R0);
+    // threads: one butterfly each in the widest stage, at most 512 (radix 16) or 1024 (radix 8)
+    static constexpr int bf = (S > 0 && M / B > M / R0) ? M / B : M / R0;
+    static constexpr int cap = B == 16 ? 512 : 1024;
+    static constexpr int NT = bf < 64 ? 64 : (bf > cap ? cap : (bf + 63) / 64 * 64);
 };
 
 // Stage policies. A stage reads elements base + r * STEP through in.get<STEP>(base, pbase, r)
@@ -183,12 +194,24 @@ __device__ __forceinline__ void line_stage(const float2* __restrict__ tw, int ti
     if constexpr (Out::lds) __syncthreads();
 }
 
+// Radix-B stages from sub-length Ns to M; the last writes through `out`.
+template <int M, int B, int Ns, int NT, bool INV, class Out>
+__device__ __forceinline__ void line_tail(const float2* __restrict__ tw, int tid, const LdsIn& li, const LdsOut& lo,
+                                          const Out& out) {
+    if constexpr (Ns * B == M) {
+        line_stage<M, Ns, B, NT, INV>(tw, tid, li, out);
+    } else {
+        line_stage<M, Ns, B, NT, INV>(tw, tid, li, lo);
+        line_tail<M, B, Ns * B, NT, INV>(tw, tid, li, lo, out);
+    }
+}
+
 // The whole line transform: stage 0 reads through `in`, the last stage writes through `out`,
 // the stages between keep the line in LDS (`line`, padded).
-template <int M, bool INV, class In, class Out>
+template <int M, int B, bool INV, class In, class Out>
 __device__ __forceinline__ void line_fft(float2* line, const float2* __restrict__ tw, int tid, const In& in,
                                          const Out& out) {
-    using P = LinePlan<M>;
+    using P = LinePlan<M, B>;
     static_assert(P::ok, "line length has no composite-first plan");
     constexpr int NT = P::NT, R0 = P::R0, S = P::S;
     const LdsIn li{line};
@@ -197,17 +220,7 @@ __device__ __forceinline__ void line_fft(float2* line, const float2* __restrict_
         line_stage<M, 1, R0, NT, INV>(tw, tid, in, out);
     } else {
         line_stage<M, 1, R0, NT, INV>(tw, tid, in, lo);
-        if constexpr (S == 1) {
-            line_stage<M, R0, 16, NT, INV>(tw, tid, li, out);
-        } else {
-            line_stage<M, R0, 16, NT, INV>(tw, tid, li, lo);
-            if constexpr (S == 2) {
-                line_stage<M, R0 * 16, 16, NT, INV>(tw, tid, li, out);
-            } else {
-                line_stage<M, R0 * 16, 16, NT, INV>(tw, tid, li, lo);
-                line_stage<M, R0 * 256, 16, NT, INV>(tw, tid, li, out);
-            }
-        }
+        line_tail<M, B, R0, NT, INV>(tw, tid, li, lo, out);
     }
 }
 

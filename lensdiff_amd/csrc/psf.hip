@@ -98,24 +98,24 @@ struct ColOutput {
 
 // The R-point transform of one line: a length-M FFT for power-of-two R, else Bluestein's
 // forward FFT, filter multiply (fused into its last stage) and inverse FFT.
-template <int M, class In, class Out>
+template <int M, int B, class In, class Out>
 __device__ __forceinline__ void line_transform(float2* lds, const float2* __restrict__ tw,
                                                const float2* __restrict__ bhat, bool bluestein, int tid,
                                                const In& in, const Out& out) {
     const GlobalIn<In> gin{in};
     const GlobalOut<Out> gout{out};
     if (bluestein) {
-        line_fft<M, false>(lds, tw, tid, gin, LdsMulOut{lds, bhat});
-        line_fft<M, true>(lds, tw, tid, LdsIn{lds}, gout);
+        line_fft<M, B, false>(lds, tw, tid, gin, LdsMulOut{lds, bhat});
+        line_fft<M, B, true>(lds, tw, tid, LdsIn{lds}, gout);
     } else {
-        line_fft<M, false>(lds, tw, tid, gin, gout);
+        line_fft<M, B, false>(lds, tw, tid, gin, gout);
     }
 }
 
-// W: minimum waves per SIMD the register allocation must allow (1: no constraint); the
-// autotuner picks among W = 1, 3, 4 for long lines.
-template <int M, int W>
-__global__ void __launch_bounds__((LinePlan<M>::NT), W)
+// B: radix of the stages after the first (16 or 8); W: minimum waves per SIMD the register
+// allocation must allow (1: no constraint). The autotuner picks (B, W) for long lines.
+template <int M, int B, int W>
+__global__ void __launch_bounds__((LinePlan<M, B>::NT), W)
 psf_rows_kernel(const float* __restrict__ amp, const int* __restrict__ amp_index, const float* __restrict__ phase,
                 int P, int R, int off, const float2* __restrict__ chirp, const float2* __restrict__ bhat,
                 int bluestein, float2* __restrict__ G, const float2* __restrict__ tw) {
@@ -124,11 +124,11 @@ psf_rows_kernel(const float* __restrict__ amp, const int* __restrict__ amp_index
     const size_t a0 = amp_index != nullptr ? size_t(amp_index[lam]) * P * P : 0;
     const RowInput in{amp + a0 + size_t(y) * P, phase + (size_t(lam) * P + y) * P, chirp, P, off, bluestein};
     const RowOutput out{G + (size_t(lam) * P + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R};
-    line_transform<M>(lds, tw, bhat, bluestein != 0, tid, in, out);
+    line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out);
 }
 
-template <int M, int W>
-__global__ void __launch_bounds__((LinePlan<M>::NT), W)
+template <int M, int B, int W>
+__global__ void __launch_bounds__((LinePlan<M, B>::NT), W)
 psf_cols_kernel(const float2* __restrict__ GT, int P, int R, int off, const float2* __restrict__ chirp,
                 const float2* __restrict__ bhat, int bluestein, float* __restrict__ IT,
                 const float2* __restrict__ tw) {
@@ -139,7 +139,7 @@ psf_cols_kernel(const float2* __restrict__ GT, int P, int R, int off, const floa
     if (y >= R) y -= R;
     const ColInput in{GT + (size_t(lam) * R + ky) * P, chirp, P, off, bluestein};
     const ColOutput out{IT + (size_t(lam) * R + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R, shift};
-    line_transform<M>(lds, tw, bhat, bluestein != 0, tid, in, out);
+    line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out);
 }
 
 // Batched transpose of complex [rows][cols] -> [cols][rows] through a 32x33 LDS tile.
@@ -250,38 +250,45 @@ void host_dft(std::vector<double>& re, std::vector<double>& im) {
     }
 }
 
-constexpr int kPsfVariantWaves[3] = {1, 3, 4};
+// Variants of the line kernels: (B, W) = (16, 1), (16, 3), (16, 4), (8, 1), (8, 3).
 // Tuning keys carry the line length and the batch (wavelength count rounded up to a power of
 // two, 32 and more alike): a one-wavelength launch does not fill the device the way a
 // 31-wavelength bank does.
-// Long lines are register-bound: they get all three occupancy variants.
-template <int M> constexpr int psf_variants() { return M >= 2048 ? 3 : 1; }
+constexpr int kPsfVariants = 5;
+// Long lines are register-bound: they get every variant.
+template <int M> constexpr int psf_variants() { return M >= 2048 ? kPsfVariants : 1; }
 
-template <int M, int W>
-void launch_rows_w(hipStream_t s, const float* amp, const int* amp_index, const float* phase, int P, int R, int off,
+template <int M, int B, int W>
+void launch_rows_v(hipStream_t s, const float* amp, const int* amp_index, const float* phase, int P, int R, int off,
                    const float2* chirp, const float2* bhat, int bluestein, float2* G, const float2* tw, int L) {
-    allow_large_lds(reinterpret_cast<const void*>(&psf_rows_kernel<M, W>), (M + M / 32 + 1) * 8);
-    hipLaunchKernelGGL((psf_rows_kernel<M, W>), dim3(P, L), dim3(LinePlan<M>::NT), (M + M / 32 + 1) * 8, s, amp,
+    allow_large_lds(reinterpret_cast<const void*>(&psf_rows_kernel<M, B, W>), (M + M / 32 + 1) * 8);
+    hipLaunchKernelGGL((psf_rows_kernel<M, B, W>), dim3(P, L), dim3(LinePlan<M, B>::NT), (M + M / 32 + 1) * 8, s, amp,
                        amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw);
 }
-template <int M, int W>
-void launch_cols_w(hipStream_t s, const float2* GT, int P, int R, int off, const float2* chirp, const float2* bhat,
+template <int M, int B, int W>
+void launch_cols_v(hipStream_t s, const float2* GT, int P, int R, int off, const float2* chirp, const float2* bhat,
                    int bluestein, float* IT, const float2* tw, int L) {
-    allow_large_lds(reinterpret_cast<const void*>(&psf_cols_kernel<M, W>), (M + M / 32 + 1) * 8);
-    hipLaunchKernelGGL((psf_cols_kernel<M, W>), dim3(R, L), dim3(LinePlan<M>::NT), (M + M / 32 + 1) * 8, s, GT, P,
-                       R, off, chirp, bhat, bluestein, IT, tw);
+    allow_large_lds(reinterpret_cast<const void*>(&psf_cols_kernel<M, B, W>), (M + M / 32 + 1) * 8);
+    hipLaunchKernelGGL((psf_cols_kernel<M, B, W>), dim3(R, L), dim3(LinePlan<M, B>::NT), (M + M / 32 + 1) * 8, s, GT,
+                       P, R, off, chirp, bhat, bluestein, IT, tw);
+}
+
+std::string tune_key(const char* kind, int M, int L) {
+    return std::string(kind) + "/" + std::to_string(M) + "/L" + std::to_string(std::min(32, pow2_at_least(L)));
 }
 
 template <int M>
 bool launch_rows(int m, hipStream_t s, const float* amp, const int* amp_index, const float* phase, int P, int R,
                  int off, const float2* chirp, const float2* bhat, int bluestein, float2* G, const float2* tw, int L) {
     if (m != M) return false;
-    Autotuner::instance().run("psf-rows/" + std::to_string(M) + "/L" + std::to_string(std::min(32, pow2_at_least(L))), psf_variants<M>(), s, [&](int v) {
+    Autotuner::instance().run(tune_key("psf-rows", M, L), psf_variants<M>(), s, [&](int v) {
         if constexpr (psf_variants<M>() > 1) {
-            if (v == 1) return launch_rows_w<M, kPsfVariantWaves[1]>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
-            if (v == 2) return launch_rows_w<M, kPsfVariantWaves[2]>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
+            if (v == 1) return launch_rows_v<M, 16, 3>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
+            if (v == 2) return launch_rows_v<M, 16, 4>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
+            if (v == 3) return launch_rows_v<M, 8, 1>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
+            if (v == 4) return launch_rows_v<M, 8, 3>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
         }
-        launch_rows_w<M, 1>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
+        launch_rows_v<M, 16, 1>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
     });
     return true;
 }
@@ -289,12 +296,14 @@ template <int M>
 bool launch_cols(int m, hipStream_t s, const float2* GT, int P, int R, int off, const float2* chirp,
                  const float2* bhat, int bluestein, float* IT, const float2* tw, int L) {
     if (m != M) return false;
-    Autotuner::instance().run("psf-cols/" + std::to_string(M) + "/L" + std::to_string(std::min(32, pow2_at_least(L))), psf_variants<M>(), s, [&](int v) {
+    Autotuner::instance().run(tune_key("psf-cols", M, L), psf_variants<M>(), s, [&](int v) {
         if constexpr (psf_variants<M>() > 1) {
-            if (v == 1) return launch_cols_w<M, kPsfVariantWaves[1]>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
-            if (v == 2) return launch_cols_w<M, kPsfVariantWaves[2]>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
+            if (v == 1) return launch_cols_v<M, 16, 3>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
+            if (v == 2) return launch_cols_v<M, 16, 4>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
+            if (v == 3) return launch_cols_v<M, 8, 1>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
+            if (v == 4) return launch_cols_v<M, 8, 3>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
         }
-        launch_cols_w<M, 1>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
+        launch_cols_v<M, 16, 1>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
     });
     return true;
 }
