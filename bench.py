@@ -174,7 +174,12 @@ def main():
     if args.bank_probe:
         ctx = Context(0)
         ms = [round(ctx.bank_build_seconds(lens(5000 + i)) * 1e3, 2) for i in range(2)]
-        print(json.dumps({"bank_build_ms": ms, "autotune": ctx.tuning_report()}))
+        ctx.profile_reset()
+        ctx.profiling(True)
+        ms.append(round(ctx.bank_build_seconds(lens(5002)) * 1e3, 2))
+        ctx.profiling(False)
+        print(json.dumps({"bank_build_ms": ms, "autotune": ctx.tuning_report(),
+                          "kernel_families": families(ctx.profile_read())}))
         ctx.close()
         return
     from oracle.oracle import rel_l2

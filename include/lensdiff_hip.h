@@ -194,6 +194,12 @@ ldh_status ldh_shape_kernels(ldh_context* ctx, const ldh_params* p, const float*
 /* sumKernel (LensDiffCpuReference.cpp:436-438): the value of a sequential FP32 accumulation of
  * x[0..n) (std::accumulate with a float accumulator), computed on the device. */
 ldh_status ldh_sequential_sum(ldh_context* ctx, const float* x, int64_t n, float* out);
+/* The same for `count` arrays of n floats stored back to back: out[i] = the sequential FP32
+ * sum of x[i*n .. (i+1)*n). block: the device's block length (0 for the default, else 512 to
+ * 4096); the value does not depend on it. The PSF synthesis sums each raw PSF over blocks of one
+ * row (sumKernel per wavelength, LensDiffCpuReference.cpp:440-449). */
+ldh_status ldh_sequential_sums(ldh_context* ctx, const float* x, int64_t n, int32_t count, int32_t block,
+                               float* out);
 /* In-place 2D complex FFT of interleaved complex64 (host buffer), rows then columns,
  * forward sign -1, inverse scaled by 1/(w*h) (LensDiffCpuReference.cpp:194-301). Sizes:
  * powers of two (width >= 16, height >= 8), 3*2^a from 48 and 9*2^a from 144, up to 16384;

@@ -32,6 +32,7 @@ SIGNATURES = {
     "ldh_resample_kernel": (C.c_int32, [_CTX, _F, C.c_int32, C.c_float, C.c_int32, _F]),
     "ldh_shape_kernels": (C.c_int32, [_CTX, _P, _F, C.c_int32, _F, _F, _F, C.POINTER(C.c_int32)]),
     "ldh_sequential_sum": (C.c_int32, [_CTX, _F, C.c_int64, _F]),
+    "ldh_sequential_sums": (C.c_int32, [_CTX, _F, C.c_int64, C.c_int32, C.c_int32, _F]),
     "ldh_fft2d": (C.c_int32, [_CTX, _F, C.c_int32, C.c_int32, C.c_int32]),
     "ldh_bank_acquire": (C.c_int32, [_CTX, _P, C.POINTER(C.c_void_p)]),
     "ldh_bank_release": (C.c_int32, [_CTX, _BANK]),

@@ -113,6 +113,13 @@ class Context:
         self._check(self.lib.ldh_sequential_sum(self.h, _fptr(x), x.size, C.byref(out)))
         return float(out.value)
 
+    def sequential_sums(self, x: np.ndarray, block: int = 0) -> np.ndarray:
+        """Sequential FP32 sums of the rows of a 2-D array, over device blocks of `block`."""
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros(x.shape[0], np.float32)
+        self._check(self.lib.ldh_sequential_sums(self.h, _fptr(x), x.shape[1], x.shape[0], block, _fptr(out)))
+        return out
+
     def fft2d(self, z: np.ndarray, inverse=False) -> np.ndarray:
         h, w = z.shape
         a = np.ascontiguousarray(np.stack([z.real, z.imag], -1), np.float32)

@@ -325,6 +325,27 @@ ldh_status ldh_sequential_sum(ldh_context* ctx, const float* x, int64_t n, float
     });
 }
 
+ldh_status ldh_sequential_sums(ldh_context* ctx, const float* x, int64_t n, int32_t count, int32_t block,
+                               float* out) {
+    return guarded(ctx, [&] {
+        require(x && out && n >= 0 && count > 0, LDH_ERR_INVALID_ARGUMENT, "invalid-argument");
+        require(block == 0 || (block >= ldh::kSeqSumMinBlock && block <= ldh::kSeqSumBlock), LDH_ERR_INVALID_ARGUMENT,
+                "seqsum-block");
+        if (n == 0) {
+            for (int i = 0; i < count; ++i) out[i] = 0.0f;
+            return;
+        }
+        ldh::DeviceBuffer d(sizeof(float) * size_t(n) * count), t(sizeof(float) * count);
+        h2d(ctx, d.get(), x, sizeof(float) * size_t(n) * count);
+        ldh::SeqSegments g;
+        g.array[0] = d.as<float>();
+        g.per_array = count;
+        g.stride = n;
+        ctx->bw.seqsum.run(ctx->stream, g, n, count, t.as<float>(), nullptr, block == 0 ? ldh::kSeqSumBlock : block);
+        d2h(ctx, out, t.get(), sizeof(float) * count);
+    });
+}
+
 ldh_status ldh_fft2d(ldh_context* ctx, float* data, int32_t w, int32_t h, int32_t inverse) {
     return guarded(ctx, [&] {
         require(data && ldh::grid_length_ok(w) && w >= 16 && ldh::grid_length_ok(h), LDH_ERR_INVALID_ARGUMENT,

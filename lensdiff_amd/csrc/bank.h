@@ -41,8 +41,9 @@ struct BankWorkspace {
     PsfPlan plan;
     PsfScratch psf;
     SeqSum seqsum;
-    DeviceBuffer amp, phase, coefs, raw, totals, base, base_totals, shape_totals, scales, mean, shaped, structure, ring_sum, ring_cnt, ring_peak,
-        radius, ref_amp, ref_phase, ref_raw, ref_total;
+    DeviceBuffer amp, phase, coefs, raw, totals, base, base_totals, shape_totals, scales, mean, shaped, structure,
+        ring_sum, ring_cnt, ring_peak, radius, ref_amp, ref_phase, ref_raw, ref_total;
+    DeviceBuffer block_sums;  // producers' block sums handed to the sequential sums, [3][count][blocks]
     // first-zero radius of the clear-circle PSF per (P, R) (GetLensDiffReferenceRawPsfCached)
     std::vector<std::pair<std::pair<int, int>, float>> first_zero;
     // Grows the shaping buffers for `count` kernels of side K ahead of a graph capture.
@@ -53,8 +54,12 @@ float first_minimum_from_profile(const std::vector<float>& profile, int size);
 
 // Steps of the build, exposed for the parity entry points.
 // d_raw_totals (nullable): per-PSF sums of raw PSFs that are not yet normalised.
+// block_sums (nullable): FP64 sums of the output's kSeqSumBlock-texel blocks, [count][blocks].
 void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_scales, const float* d_raw_totals,
-                      int support, int count, float* out);
+                      int support, int count, float* out, double* block_sums = nullptr);
+// resample_kernels, then the sequential sums of the base kernels into totals[count].
+void base_kernels(hipStream_t s, BankWorkspace& w, const float* raw, int R, const float* d_scales,
+                  const float* d_raw_totals, int support, int count, float* out, float* totals);
 // Shapes `count` base kernels (count x K*K) into full/core/structure (3 x count x K*K) and
 // radius[count] (device). `max_radius` as LensDiffCpuReference.cpp:1633. base_totals (nullable):
 // per-kernel sums of a base that is not yet normalised (read as base / total).

@@ -40,38 +40,49 @@ __device__ __forceinline__ float bilinear_sq(const float* __restrict__ img, int 
     return a + (b - a) * ty;
 }
 
-__global__ void resample_kernel(const float* __restrict__ raw, int R, const float* __restrict__ scales,
-                                const float* __restrict__ raw_totals, int support, float* __restrict__ out) {
+// Workgroups of kSumThreads threads over blocks of kSeqSumBlock texels (seqsum_blocks(K*K) per
+// kernel); block_sums (nullable): their FP64 sums, [lam][block].
+__global__ void __launch_bounds__(kSumThreads)
+resample_kernel(const float* __restrict__ raw, int R, const float* __restrict__ scales,
+                const float* __restrict__ raw_totals, int support, float* __restrict__ out,
+                double* __restrict__ block_sums) {
     const int lam = blockIdx.y;
     // raw_totals: the raw PSFs are not yet normalised; texels are scaled by 1/total
     const float norm = inverse_total(raw_totals, lam);
     const int K = 2 * support + 1;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= K * K) return;
-    const int x = i % K, y = i / K;
+    const int kk = K * K;
     const float* img = raw + size_t(lam) * R * R;
     const float rc = (R & 1) == 0 ? float(R) * 0.5f : float(R - 1) * 0.5f;
     const float kc = float(K - 1) * 0.5f;
     const float inv = 1.0f / fmaxf(scales[lam], 0.05f);
-    const float sx = rc + (float(x) - kc) * inv;
-    const float sy = rc + (float(y) - kc) * inv;
-    float v;
-    if (inv <= 1.0f) {
-        v = bilinear_sq(img, R, sx, sy, norm);
-    } else {
-        const int taps = min(max(int(ceilf(inv)), 2), 6);
-        const float step = inv / float(taps);
-        float sum = 0.0f;
-        for (int ty = 0; ty < taps; ++ty) {
-            const float oy = (float(ty) + 0.5f) * step - inv * 0.5f;
-            for (int tx = 0; tx < taps; ++tx) {
-                const float ox = (float(tx) + 0.5f) * step - inv * 0.5f;
-                sum += bilinear_sq(img, R, sx + ox, sy + oy, norm);
+    const int taps = min(max(int(ceilf(inv)), 2), 6);
+    const float step = inv / float(taps);
+    const int start = blockIdx.x * kSeqSumBlock, end = min(kk, start + kSeqSumBlock);
+    double share[1] = {0.0};
+    for (int i = start + threadIdx.x; i < end; i += kSumThreads) {
+        const int x = i % K, y = i / K;
+        const float sx = rc + (float(x) - kc) * inv;
+        const float sy = rc + (float(y) - kc) * inv;
+        float v;
+        if (inv <= 1.0f) {
+            v = bilinear_sq(img, R, sx, sy, norm);
+        } else {
+            float sum = 0.0f;
+            for (int ty = 0; ty < taps; ++ty) {
+                const float oy = (float(ty) + 0.5f) * step - inv * 0.5f;
+                for (int tx = 0; tx < taps; ++tx) {
+                    const float ox = (float(tx) + 0.5f) * step - inv * 0.5f;
+                    sum += bilinear_sq(img, R, sx + ox, sy + oy, norm);
+                }
             }
+            v = sum / float(taps * taps);
         }
-        v = sum / float(taps * taps);
+        out[size_t(lam) * kk + i] = v;
+        share[0] += double(v);
     }
-    out[size_t(lam) * K * K + i] = v;
+    if (block_sums == nullptr) return;
+    double* const dst[1] = {block_sums + size_t(lam) * gridDim.x + blockIdx.x};
+    store_block_sums<1>(share, dst);
 }
 
 // Ring index of pixel (x, y) about centre c: round(|d|) (azimuthal mean, radial profile) or
@@ -125,42 +136,77 @@ ring_stats_kernel(const float* __restrict__ img, int size, float center, int rma
     }
 }
 
-__global__ void ring_mean_kernel(const double* __restrict__ sums, const int* __restrict__ counts, int K, int rmax,
-                                 float* __restrict__ out) {
+// The elementwise shaping kernels below run workgroups of kSumThreads threads over blocks of
+// kSeqSumBlock elements of one kernel (grid: seqsum_blocks(K*K) x kernels) and write the FP64
+// sums of the blocks of each array they produce, [array][kernel][block].
+__device__ __forceinline__ int64_t block_start() { return int64_t(blockIdx.x) * kSeqSumBlock; }
+__device__ __forceinline__ double* block_sum_slot(double* sums, int array, int kernels) {
+    return sums + (size_t(array) * kernels + blockIdx.y) * gridDim.x + blockIdx.x;
+}
+
+__global__ void __launch_bounds__(kSumThreads)
+ring_mean_kernel(const double* __restrict__ sums, const int* __restrict__ counts, int K, int rmax,
+                 float* __restrict__ out, double* __restrict__ block_sums) {
     const int seg = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= K * K) return;
     const float c = float(K - 1) * 0.5f;
-    const int r = ring_of<false>(i % K, i / K, c, rmax);
     const int nr = rmax + 1;
-    const float s = float(sums[int64_t(seg) * nr + r]);
-    const int n = max(1, counts[int64_t(seg) * nr + r]);
-    out[int64_t(seg) * K * K + i] = s / float(n);
+    const int64_t kk = int64_t(K) * K, start = block_start(), end = min(kk, start + kSeqSumBlock);
+    double share[1] = {0.0};
+    for (int64_t i = start + threadIdx.x; i < end; i += kSumThreads) {
+        const int r = ring_of<false>(int(i % K), int(i / K), c, rmax);
+        const float s = float(sums[int64_t(seg) * nr + r]);
+        const int n = max(1, counts[int64_t(seg) * nr + r]);
+        const float v = s / float(n);
+        out[int64_t(seg) * kk + i] = v;
+        share[0] += double(v);
+    }
+    double* const dst[1] = {block_sum_slot(block_sums, 0, gridDim.y)};
+    store_block_sums<1>(share, dst);
 }
 
 // reshapeAnisotropy on the normalised base and mean (each read through its totals).
-__global__ void reshape_kernel(const float* __restrict__ base, const float* __restrict__ base_tot,
-                               const float* __restrict__ mean, const float* __restrict__ mean_tot, float gain0,
-                               const float* __restrict__ gains, int64_t kk, float* __restrict__ shaped) {
+__global__ void __launch_bounds__(kSumThreads)
+reshape_kernel(const float* __restrict__ base, const float* __restrict__ base_tot, const float* __restrict__ mean,
+               const float* __restrict__ mean_tot, float gain0, const float* __restrict__ gains, int64_t kk,
+               float* __restrict__ shaped, double* __restrict__ block_sums) {
     const int seg = blockIdx.y;
     const float gain = gains != nullptr ? gains[seg] : gain0;
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= kk) return;
-    const int64_t o = int64_t(seg) * kk + i;
-    const float m = mean[o] * inverse_total(mean_tot, seg);
-    const float b = base[o] * inverse_total(base_tot, seg);
-    shaped[o] = fmaxf(0.0f, m + (b - m) * gain);
+    const float inv_m = inverse_total(mean_tot, seg), inv_b = inverse_total(base_tot, seg);
+    const int64_t start = block_start(), end = min(kk, start + kSeqSumBlock);
+    double share[1] = {0.0};
+    for (int64_t i = start + threadIdx.x; i < end; i += kSumThreads) {
+        const int64_t o = int64_t(seg) * kk + i;
+        const float m = mean[o] * inv_m;
+        const float b = base[o] * inv_b;
+        const float v = fmaxf(0.0f, m + (b - m) * gain);
+        shaped[o] = v;
+        share[0] += double(v);
+    }
+    double* const dst[1] = {block_sum_slot(block_sums, 0, gridDim.y)};
+    store_block_sums<1>(share, dst);
 }
 
 // positiveResidualKernel of the normalised shaped and mean kernels.
-__global__ void residual_kernel(const float* __restrict__ shaped, const float* __restrict__ shaped_tot,
-                                const float* __restrict__ mean, const float* __restrict__ mean_tot, int64_t kk,
-                                float* __restrict__ out) {
+// Also sums the normalised shaped kernel (array 1 of the block sums), the support estimate's
+// total.
+__global__ void __launch_bounds__(kSumThreads)
+residual_kernel(const float* __restrict__ shaped, const float* __restrict__ shaped_tot, const float* __restrict__ mean,
+                const float* __restrict__ mean_tot, int64_t kk, float* __restrict__ out,
+                double* __restrict__ block_sums) {
     const int seg = blockIdx.y;
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= kk) return;
-    const int64_t o = int64_t(seg) * kk + i;
-    out[o] = fmaxf(0.0f, shaped[o] * inverse_total(shaped_tot, seg) - mean[o] * inverse_total(mean_tot, seg));
+    const float inv_s = inverse_total(shaped_tot, seg), inv_m = inverse_total(mean_tot, seg);
+    const int64_t start = block_start(), end = min(kk, start + kSeqSumBlock);
+    double share[2] = {0.0, 0.0};
+    for (int64_t i = start + threadIdx.x; i < end; i += kSumThreads) {
+        const int64_t o = int64_t(seg) * kk + i;
+        const float sn = shaped[o] * inv_s;
+        const float v = fmaxf(0.0f, sn - mean[o] * inv_m);
+        out[o] = v;
+        share[0] += double(v);
+        share[1] += double(sn);
+    }
+    double* const dst[2] = {block_sum_slot(block_sums, 0, gridDim.y), block_sum_slot(block_sums, 1, gridDim.y)};
+    store_block_sums<2>(share, dst);
 }
 
 // estimateAdaptiveSupportRadius + paddedAdaptiveSupportRadius (LensDiffCpuReference.cpp:984-1036),
@@ -210,35 +256,44 @@ struct TaperSources {
     const float* src[3];
     const float* tot[3];
 };
-__global__ void crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, int count,
-                                  float* __restrict__ dst) {
-    const int bin = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= K * K) return;
+__global__ void __launch_bounds__(kSumThreads)
+crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, float* __restrict__ dst,
+                  double* __restrict__ block_sums) {
+    const int bin = blockIdx.y, count = gridDim.y;
     const int r = min(max(radius[bin], 1), K / 2);
     const int c = K / 2;
-    const int x = i % K, y = i / K;
-    const int64_t kk = int64_t(K) * K;
-    const bool inside = abs(x - c) <= r && abs(y - c) <= r;
-    float w = 0.0f;
-    if (inside) {
-        const float dx = float(x - c), dy = float(y - c);
-        const float rr = sqrtf(dx * dx + dy * dy);
-        const float extent = fmaxf(1.0f, float(r));
-        const float fade = fmaxf(6.0f, fminf(24.0f, extent * 0.04f));
-        const float start = fmaxf(0.0f, extent - fade);
-        if (rr < extent) {
-            const float t = fminf(fmaxf((rr - start) / fmaxf(fade, 1e-6f), 0.0f), 1.0f);
-            const float sm = t * t * (3.0f - 2.0f * t);
-            w = fmaxf(0.0f, cosf(sm * 1.57079637f));
+    const int64_t kk = int64_t(K) * K, start = block_start(), end = min(kk, start + kSeqSumBlock);
+    float inv[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) inv[k] = inverse_total(ts.tot[k], bin);
+    const float extent = fmaxf(1.0f, float(r));
+    const float fade = fmaxf(6.0f, fminf(24.0f, extent * 0.04f));
+    const float ramp = fmaxf(0.0f, extent - fade);
+    double share[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = start + threadIdx.x; i < end; i += kSumThreads) {
+        const int x = int(i % K), y = int(i / K);
+        const bool inside = abs(x - c) <= r && abs(y - c) <= r;
+        float w = 0.0f;
+        if (inside) {
+            const float dx = float(x - c), dy = float(y - c);
+            const float rr = sqrtf(dx * dx + dy * dy);
+            if (rr < extent) {
+                const float t = fminf(fmaxf((rr - ramp) / fmaxf(fade, 1e-6f), 0.0f), 1.0f);
+                const float sm = t * t * (3.0f - 2.0f * t);
+                w = fmaxf(0.0f, cosf(sm * 1.57079637f));
+            }
+        }
+        const int64_t o = int64_t(bin) * kk + i;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float v = inside ? ts.src[k][o] * inv[k] * w : 0.0f;
+            dst[(int64_t(k) * count + bin) * kk + i] = v;
+            share[k] += double(v);
         }
     }
-    const int64_t o = int64_t(bin) * kk + i;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float v = inside ? ts.src[k][o] * inverse_total(ts.tot[k], bin) * w : 0.0f;
-        dst[(int64_t(k) * count + bin) * kk + i] = v;
-    }
+    double* const slots[3] = {block_sum_slot(block_sums, 0, count), block_sum_slot(block_sums, 1, count),
+                              block_sum_slot(block_sums, 2, count)};
+    store_block_sums<3>(share, slots);
 }
 
 __global__ void fill_zero_kernel(uint32_t* __restrict__ p, int64_t n) {
@@ -246,7 +301,7 @@ __global__ void fill_zero_kernel(uint32_t* __restrict__ p, int64_t n) {
         p[i] = 0u;
 }
 
-inline dim3 grid1(int64_t n, int segs = 1) { return dim3(unsigned(div_up(n, 256)), unsigned(segs)); }
+inline dim3 blocks_grid(int64_t n, int segs) { return dim3(unsigned(seqsum_blocks(n)), unsigned(segs)); }
 
 template <bool CEIL>
 void ring_stats(hipStream_t s, const float* img, int size, float center, int rmax, int segs, const float* prescale,
@@ -277,14 +332,21 @@ void ring_stats(hipStream_t s, const float* img, int size, float center, int rma
 }  // namespace
 
 void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_scales, const float* d_raw_totals,
-                      int support, int count, float* out) {
+                      int support, int count, float* out, double* block_sums) {
     const int K = 2 * support + 1;
-{
     ProfileScope ps_("resample", 8.0 * double(K) * K * count);
-        hipLaunchKernelGGL(resample_kernel, grid1(int64_t(K) * K, count), dim3(256), 0, s, raw, R, d_scales, d_raw_totals,
-                       support, out);
+    hipLaunchKernelGGL(resample_kernel, blocks_grid(int64_t(K) * K, count), dim3(kSumThreads), 0, s, raw, R, d_scales,
+                       d_raw_totals, support, out, block_sums);
     LDH_CHECK_LAUNCH("resample");
-    }
+}
+
+void base_kernels(hipStream_t s, BankWorkspace& w, const float* raw, int R, const float* d_scales,
+                  const float* d_raw_totals, int support, int count, float* out, float* totals) {
+    const int K = 2 * support + 1;
+    const int64_t kk = int64_t(K) * K;
+    w.reserve_shaping(K, count, support);
+    resample_kernels(s, raw, R, d_scales, d_raw_totals, support, count, out, w.block_sums.as<double>());
+    w.seqsum.run(s, out, kk, kk, count, totals, w.block_sums.as<double>());
 }
 
 void BankWorkspace::reserve_shaping(int K, int count, int max_radius) {
@@ -297,6 +359,7 @@ void BankWorkspace::reserve_shaping(int K, int count, int max_radius) {
     shape_totals.reset(sizeof(float) * 7 * count);
     const int nr = std::max(K / 2, max_radius) + 1;
     ring_sum.reset(sizeof(double) * nr * count);
+    block_sums.reset(sizeof(double) * size_t(seqsum_blocks(kk)) * 3 * count);
     ring_cnt.reset(sizeof(int) * nr * count);
     ring_peak.reset(sizeof(unsigned) * nr * count);
     seqsum.reserve(kk, 3 * count);
@@ -317,28 +380,30 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
     // azimuthal mean of the normalised base
     ring_stats<false>(s, base, K, float(K - 1) * 0.5f, rmax_mean, count, base_totals, w.ring_sum.as<double>(),
                       w.ring_cnt.as<int>(), nullptr);
-{
-    ProfileScope ps_("shape-ring-mean", 4.0 * double(kk) * count);
-        hipLaunchKernelGGL(ring_mean_kernel, grid1(kk, count), dim3(256), 0, s, w.ring_sum.as<double>(),
-                       w.ring_cnt.as<int>(), K, rmax_mean, w.mean.as<float>());
-    LDH_CHECK_LAUNCH("ring-mean");
+    double* bsum = w.block_sums.as<double>();
+    const dim3 grid = blocks_grid(kk, count);
+    {
+        ProfileScope ps_("shape-ring-mean", 4.0 * double(kk) * count);
+        hipLaunchKernelGGL(ring_mean_kernel, grid, dim3(kSumThreads), 0, s, w.ring_sum.as<double>(),
+                           w.ring_cnt.as<int>(), K, rmax_mean, w.mean.as<float>(), bsum);
+        LDH_CHECK_LAUNCH("ring-mean");
     }
-    w.seqsum.run(s, w.mean.as<float>(), kk, kk, count, t_mean);
+    w.seqsum.run(s, w.mean.as<float>(), kk, kk, count, t_mean, bsum);
     // anisotropy reshape
     const float gain = 1.0f + std::max(0.0f, float(anisotropy)) * 4.0f;
-{
-    ProfileScope ps_("shape-reshape", 12.0 * double(kk) * count);
-        hipLaunchKernelGGL(reshape_kernel, grid1(kk, count), dim3(256), 0, s, base, base_totals, w.mean.as<float>(),
-                       t_mean, gain, d_gains, kk, w.shaped.as<float>());
-    LDH_CHECK_LAUNCH("reshape");
+    {
+        ProfileScope ps_("shape-reshape", 12.0 * double(kk) * count);
+        hipLaunchKernelGGL(reshape_kernel, grid, dim3(kSumThreads), 0, s, base, base_totals, w.mean.as<float>(),
+                           t_mean, gain, d_gains, kk, w.shaped.as<float>(), bsum);
+        LDH_CHECK_LAUNCH("reshape");
     }
-    w.seqsum.run(s, w.shaped.as<float>(), kk, kk, count, t_shaped);
+    w.seqsum.run(s, w.shaped.as<float>(), kk, kk, count, t_shaped, bsum);
     // positive residual
-{
-    ProfileScope ps_("shape-residual", 12.0 * double(kk) * count);
-        hipLaunchKernelGGL(residual_kernel, grid1(kk, count), dim3(256), 0, s, w.shaped.as<float>(), t_shaped,
-                       w.mean.as<float>(), t_mean, kk, w.structure.as<float>());
-    LDH_CHECK_LAUNCH("residual");
+    {
+        ProfileScope ps_("shape-residual", 12.0 * double(kk) * count);
+        hipLaunchKernelGGL(residual_kernel, grid, dim3(kSumThreads), 0, s, w.shaped.as<float>(), t_shaped,
+                           w.mean.as<float>(), t_mean, kk, w.structure.as<float>(), bsum);
+        LDH_CHECK_LAUNCH("residual");
     }
     // one launch sums the structure kernels and the normalised shaped kernels (the support
     // estimate's total)
@@ -348,7 +413,7 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
     g.prescale[1] = t_shaped;
     g.per_array = count;
     g.stride = kk;
-    w.seqsum.run(s, g, kk, 2 * count, t_struct);  // t_struct, then t_run
+    w.seqsum.run(s, g, kk, 2 * count, t_struct, bsum);  // t_struct, then t_run
     // adaptive support radius of the normalised shaped kernel
     const int rmax_sup = std::min(max_radius, K / 2);
     ring_stats<true>(s, w.shaped.as<float>(), K, float(K - 1) * 0.5f, rmax_sup, count, t_shaped,
@@ -362,12 +427,13 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
     }
     // crop + taper of full (shaped), core (mean), structure, then normalise all three
     TaperSources ts{{w.shaped.as<float>(), w.mean.as<float>(), w.structure.as<float>()}, {t_shaped, t_mean, t_struct}};
-{
-    ProfileScope ps_("shape-crop-taper", 24.0 * double(kk) * count);
-        hipLaunchKernelGGL(crop_taper_kernel, grid1(kk, count), dim3(256), 0, s, ts, d_radius, K, count, out3);
-    LDH_CHECK_LAUNCH("crop-taper");
+    {
+        ProfileScope ps_("shape-crop-taper", 24.0 * double(kk) * count);
+        hipLaunchKernelGGL(crop_taper_kernel, grid, dim3(kSumThreads), 0, s, ts, d_radius, K, out3, bsum);
+        LDH_CHECK_LAUNCH("crop-taper");
     }
-    w.seqsum.normalize(s, out3, kk, kk, 3 * count, t_final);
+    w.seqsum.run(s, out3, kk, kk, 3 * count, t_final, bsum);
+    w.seqsum.scale(s, out3, kk, kk, 3 * count, t_final);
 }
 
 std::vector<float> radial_profile(hipStream_t s, BankWorkspace& w, const float* psf, int R) {
@@ -506,9 +572,9 @@ std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWo
                     w.psf, false);
     const int64_t kk = int64_t(K) * K;
     w.base.reset(sizeof(float) * kk * L);
-    resample_kernels(s, w.raw.as<float>(), R, w.scales.as<float>(), w.totals.as<float>(), r, L, w.base.as<float>());
     w.base_totals.reset(sizeof(float) * L);
-    w.seqsum.run(s, w.base.as<float>(), kk, kk, L, w.base_totals.as<float>());
+    base_kernels(s, w, w.raw.as<float>(), R, w.scales.as<float>(), w.totals.as<float>(), r, L, w.base.as<float>(),
+                 w.base_totals.as<float>());
 
     // shaping into full / core / structure (the base is read normalised through its totals)
     bank->kernels.reset(sizeof(float) * kk * L * 3);

@@ -43,14 +43,13 @@ void upload(DeviceBuffer& d, const std::vector<T>& h, hipStream_t s) {
 // the batch's largest group).
 void enqueue_group(hipStream_t s, const FftTables& t, BankWorkspace& w, Group& g) {
     const int P = g.P, R = g.R, U = g.units, K = g.K;
-    const int64_t kk = int64_t(K) * K;
     pupil_amplitudes(s, g.d_geoms.as<PupilGeom>(), P, int(g.requests.size()), g.amp.as<float>(), true);
     pupil_phase(s, g.d_coefs.as<PhaseCoef>(), P, U, g.phase.as<float>(), true);
     // raw PSFs (unit sums applied inside the resample), base kernels, shaping
     g.plan.raw_psfs(s, t, g.amp.as<float>(), g.phase.as<float>(), U, w.raw.as<float>(), w.totals.as<float>(), w.seqsum,
                     w.psf, false, g.d_amp_index.as<int>());
-    resample_kernels(s, w.raw.as<float>(), R, g.d_scales.as<float>(), w.totals.as<float>(), g.r, U, w.base.as<float>());
-    w.seqsum.run(s, w.base.as<float>(), kk, kk, U, w.base_totals.as<float>());
+    base_kernels(s, w, w.raw.as<float>(), R, g.d_scales.as<float>(), w.totals.as<float>(), g.r, U, w.base.as<float>(),
+                 w.base_totals.as<float>());
     shape_kernels(s, w, w.base.as<float>(), w.base_totals.as<float>(), K, U, 0.0, g.r, g.store->as<float>(),
                   g.radius.as<int>(), g.d_gains.as<float>());
 }

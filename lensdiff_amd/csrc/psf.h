@@ -11,10 +11,11 @@ constexpr int kMaxRawPsf = 8192;  // its Bluestein length (R + P - 1 <= 9215) fi
 
 // Stage buffers of the two line passes (shareable by plans of different sizes).
 struct PsfScratch {
-    DeviceBuffer g, gt;
+    DeviceBuffer g, gt, row_sums;
     void reserve(int pupil, int raw, int count) {
         g.reset(sizeof(float2) * size_t(pupil) * raw * count);
         gt.reset(sizeof(float2) * size_t(pupil) * raw * count);
+        row_sums.reset(sizeof(double) * size_t(raw) * count);
     }
 };
 

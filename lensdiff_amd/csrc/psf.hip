@@ -81,17 +81,21 @@ struct RowOutput {
         if (k < R) out[k] = val(k, v);
     }
 };
-// |F|^2 of output k into the centred row: column (k - R/2) mod R (fftShiftSquare, :303-314).
+// |F|^2 of output k into the centred row: column (k - R/2) mod R (fftShiftSquare, :303-314),
+// also added in FP64 to the thread's share of the row sum.
 struct ColOutput {
     float* __restrict__ row;
     LineValue val;
     int R, shift;  // R - R/2
+    double* acc;
     __device__ void operator()(int k, float2 v) const {
         if (k < R) {
             const float2 f = val(k, v);
             int x = k + shift;
             if (x >= R) x -= R;
-            row[x] = f.x * f.x + f.y * f.y;
+            const float p = f.x * f.x + f.y * f.y;
+            row[x] = p;
+            *acc += double(p);
         }
     }
 };
@@ -127,19 +131,35 @@ psf_rows_kernel(const float* __restrict__ amp, const int* __restrict__ amp_index
     line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out);
 }
 
+// Also writes the FP64 sum of each output row to row_sums[lam * R + y] (nullable): the block
+// sums of a sequential sum whose blocks are the rows.
 template <int M, int B, int W>
 __global__ void __launch_bounds__((LinePlan<M, B>::NT), W)
 psf_cols_kernel(const float2* __restrict__ GT, int P, int R, int off, const float2* __restrict__ chirp,
                 const float2* __restrict__ bhat, int bluestein, float* __restrict__ IT,
-                const float2* __restrict__ tw) {
+                const float2* __restrict__ tw, double* __restrict__ row_sums) {
     extern __shared__ float2 lds[];
+    constexpr int kWaves = LinePlan<M, B>::NT / 64;
+    __shared__ double part[kWaves];
     const int ky = blockIdx.x, lam = blockIdx.y, tid = threadIdx.x;
     const int shift = R - R / 2;
     int y = ky + shift;
     if (y >= R) y -= R;
+    double acc = 0.0;
     const ColInput in{GT + (size_t(lam) * R + ky) * P, chirp, P, off, bluestein};
-    const ColOutput out{IT + (size_t(lam) * R + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R, shift};
+    const ColOutput out{IT + (size_t(lam) * R + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R, shift,
+                        &acc};
     line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out);
+    if (row_sums == nullptr) return;
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((tid & 63) == 0) part[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < kWaves; ++i) t += part[i];
+        row_sums[size_t(lam) * R + y] = t;
+    }
 }
 
 // Batched transpose of complex [rows][cols] -> [cols][rows] through a 32x33 LDS tile.
@@ -267,10 +287,10 @@ void launch_rows_v(hipStream_t s, const float* amp, const int* amp_index, const 
 }
 template <int M, int B, int W>
 void launch_cols_v(hipStream_t s, const float2* GT, int P, int R, int off, const float2* chirp, const float2* bhat,
-                   int bluestein, float* IT, const float2* tw, int L) {
+                   int bluestein, float* IT, const float2* tw, double* row_sums, int L) {
     allow_large_lds(reinterpret_cast<const void*>(&psf_cols_kernel<M, B, W>), (M + M / 32 + 1) * 8);
     hipLaunchKernelGGL((psf_cols_kernel<M, B, W>), dim3(R, L), dim3(LinePlan<M, B>::NT), (M + M / 32 + 1) * 8, s, GT,
-                       P, R, off, chirp, bhat, bluestein, IT, tw);
+                       P, R, off, chirp, bhat, bluestein, IT, tw, row_sums);
 }
 
 std::string tune_key(const char* kind, int M, int L) {
@@ -294,16 +314,16 @@ bool launch_rows(int m, hipStream_t s, const float* amp, const int* amp_index, c
 }
 template <int M>
 bool launch_cols(int m, hipStream_t s, const float2* GT, int P, int R, int off, const float2* chirp,
-                 const float2* bhat, int bluestein, float* IT, const float2* tw, int L) {
+                 const float2* bhat, int bluestein, float* IT, const float2* tw, double* row_sums, int L) {
     if (m != M) return false;
     Autotuner::instance().run(tune_key("psf-cols", M, L), psf_variants<M>(), s, [&](int v) {
         if constexpr (psf_variants<M>() > 1) {
-            if (v == 1) return launch_cols_v<M, 16, 3>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
-            if (v == 2) return launch_cols_v<M, 16, 4>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
-            if (v == 3) return launch_cols_v<M, 8, 1>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
-            if (v == 4) return launch_cols_v<M, 8, 3>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
+            if (v == 1) return launch_cols_v<M, 16, 3>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
+            if (v == 2) return launch_cols_v<M, 16, 4>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
+            if (v == 3) return launch_cols_v<M, 8, 1>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
+            if (v == 4) return launch_cols_v<M, 8, 3>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
         }
-        launch_cols_v<M, 16, 1>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, L);
+        launch_cols_v<M, 16, 1>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
     });
     return true;
 }
@@ -389,6 +409,10 @@ void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, cons
     const float2* chirp = chirp_.as<float2>();
     const float2* bhat = bhat_.as<float2>();
     const int blu = bluestein_ ? 1 : 0;
+    // rows of at least kSeqSumMinBlock elements serve as the blocks of the sequential sum, with
+    // their sums formed by the column pass
+    const bool row_blocks = R >= kSeqSumMinBlock && R <= kSeqSumBlock;
+    double* row_sums = row_blocks ? scratch.row_sums.as<double>() : nullptr;
     {
         // algorithmic bytes: amplitude and phase in, row transforms out
         ProfileScope ps("psf-rows", (8.0 * P * P + 8.0 * P * R) * L);
@@ -406,15 +430,17 @@ void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, cons
     {
         // algorithmic bytes: the P pass-1 values of each output row in, |F|^2 out
         ProfileScope ps("psf-cols", (8.0 * P * R + 4.0 * R * R) * L);
-#define LDH_PSF_COLS(MM) launch_cols<MM>(m_, s, gt_.as<float2>(), P, R, offset_, chirp, bhat, blu, out, tw, L)
+#define LDH_PSF_COLS(MM) launch_cols<MM>(m_, s, gt_.as<float2>(), P, R, offset_, chirp, bhat, blu, out, tw, row_sums, L)
         if (!(LDH_PSF_LENGTHS(LDH_PSF_COLS))) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
 #undef LDH_PSF_COLS
         LDH_CHECK_LAUNCH("psf-cols");
     }
-    if (normalize)
-        seqsum.normalize(s, out, int64_t(R) * R, int64_t(R) * R, L, totals);
-    else
-        seqsum.run(s, out, int64_t(R) * R, int64_t(R) * R, L, totals);
+    SeqSegments g;
+    g.array[0] = out;
+    g.per_array = L;
+    g.stride = int64_t(R) * R;
+    seqsum.run(s, g, int64_t(R) * R, L, totals, row_sums, row_blocks ? R : kSeqSumBlock);
+    if (normalize) seqsum.scale(s, out, int64_t(R) * R, int64_t(R) * R, L, totals);
 }
 
 }  // namespace ldh

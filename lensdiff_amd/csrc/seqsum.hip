@@ -51,17 +51,20 @@ __device__ __forceinline__ double wave_inclusive_scan(double v, int lane) {
     return v;
 }
 
+// Blocks are `bl` consecutive elements (bl <= kSeqSumBlock); block b covers
+// [b * bl, min(n, (b + 1) * bl)).
 __global__ void __launch_bounds__(kThreads)
-block_sums_kernel(SeqSegments g, int64_t n, int nb, double* __restrict__ bsum) {
+block_sums_kernel(SeqSegments g, int64_t n, int bl, int nb, double* __restrict__ bsum) {
     const int seg = blockIdx.y, b = blockIdx.x;
     const float* base = g.base(seg);
     const float sc = g.scale(seg);
-    const int64_t start = int64_t(b) * kSeqSumBlock;
+    const int64_t start = int64_t(b) * bl;
+    const int64_t end = min(n, start + bl);
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < kPerThread; ++k) {
         const int64_t i = start + int64_t(k) * kThreads + threadIdx.x;
-        if (i < n) acc += double(base[i] * sc);
+        if (i < end) acc += double(base[i] * sc);
     }
     __shared__ double red[kThreads / 64];
     acc = wave_sum(acc);
@@ -107,18 +110,21 @@ prefix_kernel(const double* __restrict__ bsum, int nb, int* __restrict__ ebase) 
 }
 
 __global__ void __launch_bounds__(kThreads)
-binade_sums_kernel(SeqSegments g, int64_t n, int nb, const int* __restrict__ ebase, double* __restrict__ bins) {
+binade_sums_kernel(SeqSegments g, int64_t n, int bl, int nb, const double* __restrict__ bsum,
+                   const int* __restrict__ ebase, double* __restrict__ bins) {
     const int seg = blockIdx.y, b = blockIdx.x;
     const int eb = ebase[int64_t(seg) * nb + b];
-    if (eb == kNoBase) return;
+    // the walk adds nothing for an all-zero block (block sum 0) and replays blocks without a base
+    if (eb == kNoBase || bsum[int64_t(seg) * nb + b] == 0.0) return;
     const float* base = g.base(seg);
     const float sc = g.scale(seg);
-    const int64_t start = int64_t(b) * kSeqSumBlock;
+    const int64_t start = int64_t(b) * bl;
+    const int64_t end = min(n, start + bl);
     float v[kPerThread];
 #pragma unroll
     for (int k = 0; k < kPerThread; ++k) {
         const int64_t i = start + int64_t(k) * kThreads + threadIdx.x;
-        v[k] = i < n ? base[i] : 0.0f;
+        v[k] = i < end ? base[i] : 0.0f;
     }
     // scaled after every load is issued (scaling at each load serialises the loads)
     if (sc != 1.0f) {
@@ -194,10 +200,12 @@ __device__ __forceinline__ double chunk_contribution(const float* v, int e) {
 // elements). Chunks that keep the running sum inside its binade are added through a scan of
 // their rounded contributions; the chunk where it leaves the binade (or where s leaves 0) is
 // added by its own lane in true FP32, in order; so is a chunk holding a rounding tie.
-__device__ void replay_block(const float* __restrict__ base, float sc, int64_t n, int64_t block, Run& r, int lane) {
-    const int64_t start = block * kSeqSumBlock + int64_t(lane) * kChunk;
+__device__ void replay_block(const float* __restrict__ base, float sc, int64_t n, int bl, int64_t block, Run& r,
+                             int lane) {
+    const int64_t start = block * bl + int64_t(lane) * kChunk;
+    const int64_t end = min(n, block * bl + bl);
     float v[kChunk];
-    if (start + kChunk <= n && (reinterpret_cast<uintptr_t>(base + start) & 15) == 0) {
+    if (start + kChunk <= end && (reinterpret_cast<uintptr_t>(base + start) & 15) == 0) {
         const float4* p4 = reinterpret_cast<const float4*>(base + start);
 #pragma unroll
         for (int j = 0; j < kChunk / 4; ++j) {
@@ -206,7 +214,7 @@ __device__ void replay_block(const float* __restrict__ base, float sc, int64_t n
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < kChunk; ++j) v[j] = start + j < n ? base[start + j] : 0.0f;
+        for (int j = 0; j < kChunk; ++j) v[j] = start + j < end ? base[start + j] : 0.0f;
     }
     if (sc != 1.0f) {
 #pragma unroll
@@ -239,7 +247,7 @@ __device__ void replay_block(const float* __restrict__ base, float sc, int64_t n
 }
 
 __global__ void __launch_bounds__(64)
-walk_kernel(SeqSegments g, int64_t n, int nb, const double* __restrict__ bsum, const int* __restrict__ ebase,
+walk_kernel(SeqSegments g, int64_t n, int bl, int nb, const double* __restrict__ bsum, const int* __restrict__ ebase,
             const double* __restrict__ bins, float* __restrict__ out) {
     const int seg = blockIdx.x;
     const int lane = threadIdx.x;
@@ -299,7 +307,7 @@ walk_kernel(SeqSegments g, int64_t n, int nb, const double* __restrict__ bsum, c
             }
             const int l = __ffsll((long long)mask) - 1;
             r.s += __shfl(incl - col, l);
-            replay_block(base, sc, n, g0 + l, r, lane);
+            replay_block(base, sc, n, bl, g0 + l, r, lane);
             first = l + 1;
         }
         cur_sum = nxt_sum;
@@ -331,34 +339,47 @@ void SeqSum::run(hipStream_t s, const float* x, int64_t n, int64_t stride, int s
     run(s, g, n, segments, totals);
 }
 
+void SeqSum::run(hipStream_t s, const float* x, int64_t n, int64_t stride, int segments, float* totals,
+                 const double* block_sums) {
+    SeqSegments g;
+    g.array[0] = x;
+    g.per_array = segments;
+    g.stride = stride;
+    run(s, g, n, segments, totals, block_sums);
+}
+
 void SeqSum::reserve(int64_t n, int segments) {
-    const int nb = int(div_up(n, kSeqSumBlock));
+    const int nb = int(div_up(n, kSeqSumMinBlock));  // room for the shortest blocks
     bsum_.reset(sizeof(double) * size_t(nb) * segments);
     bins_.reset(sizeof(double) * size_t(nb) * segments * kBins);
     ebase_.reset(sizeof(int) * size_t(nb) * segments);
 }
 
-void SeqSum::run(hipStream_t s, const SeqSegments& g, int64_t n, int segments, float* totals) {
+void SeqSum::run(hipStream_t s, const SeqSegments& g, int64_t n, int segments, float* totals, const double* block_sums,
+                 int block) {
     if (n <= 0 || segments <= 0) return;
     if (segments > g.per_array * SeqSegments::kMaxArrays) fail(LDH_ERR_INVALID_ARGUMENT, "seqsum-segments");
-    const int nb = int(div_up(n, kSeqSumBlock));
+    if (block < kSeqSumMinBlock || block > kSeqSumBlock) fail(LDH_ERR_INVALID_ARGUMENT, "seqsum-block");
+    const int nb = int(div_up(n, block));
     reserve(n, segments);
     const double data_bytes = 4.0 * double(n) * segments;  // one read of the summed data
+    const double* bsum = block_sums != nullptr ? block_sums : bsum_.as<double>();
     {
-        ProfileScope ps("seqsum-block-sums", data_bytes);
-        hipLaunchKernelGGL(block_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, g, n, nb, bsum_.as<double>());
-        hipLaunchKernelGGL(prefix_kernel, dim3(segments), dim3(kThreads), 0, s, bsum_.as<double>(), nb,
-                           ebase_.as<int>());
+        ProfileScope ps("seqsum-block-sums", block_sums != nullptr ? 0.0 : data_bytes);
+        if (block_sums == nullptr)
+            hipLaunchKernelGGL(block_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, g, n, block, nb,
+                               bsum_.as<double>());
+        hipLaunchKernelGGL(prefix_kernel, dim3(segments), dim3(kThreads), 0, s, bsum, nb, ebase_.as<int>());
     }
     {
         ProfileScope ps("seqsum-binade-sums", data_bytes);
-        hipLaunchKernelGGL(binade_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, g, n, nb, ebase_.as<int>(),
-                           bins_.as<double>());
+        hipLaunchKernelGGL(binade_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, g, n, block, nb, bsum,
+                           ebase_.as<int>(), bins_.as<double>());
     }
     {
         // reads the block tables and replays a few blocks: latency-bound, no byte model
         ProfileScope ps("seqsum-walk", 0.0);
-        hipLaunchKernelGGL(walk_kernel, dim3(segments), dim3(64), 0, s, g, n, nb, bsum_.as<double>(), ebase_.as<int>(),
+        hipLaunchKernelGGL(walk_kernel, dim3(segments), dim3(64), 0, s, g, n, block, nb, bsum, ebase_.as<int>(),
                            bins_.as<double>(), totals);
     }
     LDH_CHECK_LAUNCH("seqsum-launch");

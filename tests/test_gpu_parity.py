@@ -76,6 +76,26 @@ def test_sequential_sum_reproduces_fp32_accumulation(ctx, oracle):
         assert np.float32(got) == np.float32(want), (name, got, want, x.size)
 
 
+@pytest.mark.parametrize("block", [512, 1020, 2047, 4092])
+def test_sequential_sums_any_block_length(ctx, oracle, block):
+    """Batched sums over shorter, unaligned blocks (the raw PSF sums use one row per block)."""
+    cases = seqsum_cases()
+    rng = np.random.default_rng(block)
+    rows = {
+        "airy": cases["airy"].reshape(1, -1),
+        "uniform": rng.random((3, 700_001)).astype(np.float32),
+        "eighths": cases["eighths"].reshape(1, -1),
+        "ones-20M": cases["ones-20M"].reshape(1, -1),
+        "big-last": cases["big-last"].reshape(1, -1),
+        "wide-range": cases["wide-range"].reshape(2, -1),
+    }
+    for name, x in rows.items():
+        got = ctx.sequential_sums(x, block)
+        for i in range(x.shape[0]):
+            want = np.float32(oracle.sequential_sum(x[i]))
+            assert got[i] == want, (name, i, block, got[i], want)
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 5, 6, 7])
 def test_pupil_amplitude(ctx, oracle, mode):
     p = params(aperture_mode=mode, pupil_resolution=200, blade_count=7, roundness=0.3, rotation_deg=17.0,
