@@ -8,7 +8,7 @@ namespace ldh {
 
 void FftTables::init(hipStream_t s) {
     std::vector<float2> h(kTwiddleEntries);
-    for (const int t : {kTw2, kTw3, kTw9}) {
+    for (const int t : {kTw2, kTw3, kTw9, kTw5}) {
         const int off = tw_offset(t);
         for (int j = 0; j < t; ++j) {
             // exp(-2 pi i j / t), computed in double and rounded once

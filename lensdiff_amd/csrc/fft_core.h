@@ -112,9 +112,32 @@ template <bool INV> __device__ __forceinline__ void dft3(float2& a0, float2& a1,
     a2 = make_float2(m.x - s * d.x, m.y - s * d.y);
 }
 
+// 5 points: X0 = a0 + t1 + t2, X1,4 = b1 -+ i d1, X2,3 = b2 -+ i d2 (forward) with
+// t1 = a1 + a4, t2 = a2 + a3, b1 = a0 + c1 t1 + c2 t2, b2 = a0 + c2 t1 + c1 t2,
+// d1 = s1 (a1 - a4) + s2 (a2 - a3), d2 = s2 (a1 - a4) - s1 (a2 - a3), ck = cos(2 pi k / 5),
+// sk = sin(2 pi k / 5).
+template <bool INV>
+__device__ __forceinline__ void dft5(float2& a0, float2& a1, float2& a2, float2& a3, float2& a4) {
+#pragma clang fp contract(fast)
+    const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+    const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
+    const float2 t1 = make_float2(a1.x + a4.x, a1.y + a4.y), t2 = make_float2(a2.x + a3.x, a2.y + a3.y);
+    const float2 t3 = make_float2(a1.x - a4.x, a1.y - a4.y), t4 = make_float2(a2.x - a3.x, a2.y - a3.y);
+    const float2 b1 = make_float2(a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y);
+    const float2 b2 = make_float2(a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y);
+    const float2 d1 = rot90<INV>(make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
+    const float2 d2 = rot90<INV>(make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
+    a0 = make_float2(a0.x + t1.x + t2.x, a0.y + t1.y + t2.y);
+    a1 = make_float2(b1.x + d1.x, b1.y + d1.y);
+    a4 = make_float2(b1.x - d1.x, b1.y - d1.y);
+    a2 = make_float2(b2.x + d2.x, b2.y + d2.y);
+    a3 = make_float2(b2.x - d2.x, b2.y - d2.y);
+}
+
 template <int R, bool INV> __device__ __forceinline__ void dft(float2* v) {
     if constexpr (R == 2) dft2<INV>(v[0], v[1]);
     else if constexpr (R == 3) dft3<INV>(v[0], v[1], v[2]);
+    else if constexpr (R == 5) dft5<INV>(v[0], v[1], v[2], v[3], v[4]);
     else if constexpr (R == 4) dft4<INV>(v[0], v[1], v[2], v[3]);
     else if constexpr (R == 8) dft8<INV>(v);
     else dft16<INV>(v);

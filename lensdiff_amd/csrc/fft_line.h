@@ -1,6 +1,6 @@
 // Single-line FFTs held in LDS for the whole transform (the PSF line kernels), planned as one
 // composite first stage and S stages of radix B = 16 or 8: M = R0 * B^S with R0 <= 48
-// (6144 = 24*16*16 = 12*8*8*8, 8192 = 32*16*16 = 16*8*8*8). Radix 16 means fewer passes over
+// (5120 = 20*16*16 = 10*8*8*8, 6144 = 24*16*16 = 12*8*8*8, 8192 = 32*16*16 = 16*8*8*8). Radix 16 means fewer passes over
 // LDS; radix 8 means fewer registers per thread and more threads per line; which is faster is
 // left to the autotuner. Stage inputs and outputs go through policies, so the first
 // stage can read its (mostly zero) input straight from global memory and the last stage can
@@ -14,6 +14,22 @@ namespace ldh {
 
 // exp(-2 pi i m / R) for the composite radices: cos and sin of 2 pi m / R rounded to float.
 template <int R> struct Roots;
+template <> struct Roots<5> {
+    static constexpr float c[5] = {1.000000000e+00f, 3.090169944e-01f, -8.090169944e-01f, -8.090169944e-01f, 3.090169944e-01f};
+    static constexpr float s[5] = {0.000000000e+00f, 9.510565163e-01f, 5.877852523e-01f, -5.877852523e-01f, -9.510565163e-01f};
+};
+template <> struct Roots<10> {
+    static constexpr float c[10] = {1.000000000e+00f, 8.090169944e-01f, 3.090169944e-01f, -3.090169944e-01f, -8.090169944e-01f, -1.000000000e+00f, -8.090169944e-01f, -3.090169944e-01f, 3.090169944e-01f, 8.090169944e-01f};
+    static constexpr float s[10] = {0.000000000e+00f, 5.877852523e-01f, 9.510565163e-01f, 9.510565163e-01f, 5.877852523e-01f, 1.224646799e-16f, -5.877852523e-01f, -9.510565163e-01f, -9.510565163e-01f, -5.877852523e-01f};
+};
+template <> struct Roots<20> {
+    static constexpr float c[20] = {1.000000000e+00f, 9.510565163e-01f, 8.090169944e-01f, 5.877852523e-01f, 3.090169944e-01f, 6.123233996e-17f, -3.090169944e-01f, -5.877852523e-01f, -8.090169944e-01f, -9.510565163e-01f, -1.000000000e+00f, -9.510565163e-01f, -8.090169944e-01f, -5.877852523e-01f, -3.090169944e-01f, -1.836970199e-16f, 3.090169944e-01f, 5.877852523e-01f, 8.090169944e-01f, 9.510565163e-01f};
+    static constexpr float s[20] = {0.000000000e+00f, 3.090169944e-01f, 5.877852523e-01f, 8.090169944e-01f, 9.510565163e-01f, 1.000000000e+00f, 9.510565163e-01f, 8.090169944e-01f, 5.877852523e-01f, 3.090169944e-01f, 1.224646799e-16f, -3.090169944e-01f, -5.877852523e-01f, -8.090169944e-01f, -9.510565163e-01f, -1.000000000e+00f, -9.510565163e-01f, -8.090169944e-01f, -5.877852523e-01f, -3.090169944e-01f};
+};
+template <> struct Roots<40> {
+    static constexpr float c[40] = {1.000000000e+00f, 9.876883406e-01f, 9.510565163e-01f, 8.910065242e-01f, 8.090169944e-01f, 7.071067812e-01f, 5.877852523e-01f, 4.539904997e-01f, 3.090169944e-01f, 1.564344650e-01f, 6.123233996e-17f, -1.564344650e-01f, -3.090169944e-01f, -4.539904997e-01f, -5.877852523e-01f, -7.071067812e-01f, -8.090169944e-01f, -8.910065242e-01f, -9.510565163e-01f, -9.876883406e-01f, -1.000000000e+00f, -9.876883406e-01f, -9.510565163e-01f, -8.910065242e-01f, -8.090169944e-01f, -7.071067812e-01f, -5.877852523e-01f, -4.539904997e-01f, -3.090169944e-01f, -1.564344650e-01f, -1.836970199e-16f, 1.564344650e-01f, 3.090169944e-01f, 4.539904997e-01f, 5.877852523e-01f, 7.071067812e-01f, 8.090169944e-01f, 8.910065242e-01f, 9.510565163e-01f, 9.876883406e-01f};
+    static constexpr float s[40] = {0.000000000e+00f, 1.564344650e-01f, 3.090169944e-01f, 4.539904997e-01f, 5.877852523e-01f, 7.071067812e-01f, 8.090169944e-01f, 8.910065242e-01f, 9.510565163e-01f, 9.876883406e-01f, 1.000000000e+00f, 9.876883406e-01f, 9.510565163e-01f, 8.910065242e-01f, 8.090169944e-01f, 7.071067812e-01f, 5.877852523e-01f, 4.539904997e-01f, 3.090169944e-01f, 1.564344650e-01f, 1.224646799e-16f, -1.564344650e-01f, -3.090169944e-01f, -4.539904997e-01f, -5.877852523e-01f, -7.071067812e-01f, -8.090169944e-01f, -8.910065242e-01f, -9.510565163e-01f, -9.876883406e-01f, -1.000000000e+00f, -9.876883406e-01f, -9.510565163e-01f, -8.910065242e-01f, -8.090169944e-01f, -7.071067812e-01f, -5.877852523e-01f, -4.539904997e-01f, -3.090169944e-01f, -1.564344650e-01f};
+};
 template <> struct Roots<6> {
     static constexpr float c[6] = {1.000000000e+00f, 5.000000000e-01f, -5.000000000e-01f, -1.000000000e+00f, -5.000000000e-01f, 5.000000000e-01f};
     static constexpr float s[6] = {0.000000000e+00f, 8.660254038e-01f, 8.660254038e-01f, 1.224646799e-16f, -8.660254038e-01f, -8.660254038e-01f};
@@ -81,8 +97,11 @@ template <int A, int B, bool INV> __device__ __forceinline__ void dft_ab(float2*
 }
 
 template <int R, bool INV> __device__ __forceinline__ void dft_any(float2* v) {
-    if constexpr (R == 2 || R == 3 || R == 4 || R == 8 || R == 16) dft<R, INV>(v);
+    if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16) dft<R, INV>(v);
     else if constexpr (R == 6) dft_ab<2, 3, INV>(v);
+    else if constexpr (R == 10) dft_ab<2, 5, INV>(v);
+    else if constexpr (R == 20) dft_ab<4, 5, INV>(v);
+    else if constexpr (R == 40) dft_ab<8, 5, INV>(v);
     else if constexpr (R == 9) dft_ab<3, 3, INV>(v);
     else if constexpr (R == 12) dft_ab<4, 3, INV>(v);
     else if constexpr (R == 18) dft_ab<2, 9, INV>(v);
@@ -103,8 +122,8 @@ constexpr int line_stages(int M, int B) {
 }
 constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 constexpr bool radix_ok(int r) {
-    return r == 2 || r == 3 || r == 4 || r == 6 || r == 8 || r == 9 || r == 12 || r == 16 || r == 18 || r == 24 ||
-           r == 32 || r == 36 || r == 48;
+    return r == 2 || r == 3 || r == 4 || r == 5 || r == 6 || r == 8 || r == 9 || r == 10 || r == 12 || r == 16 ||
+           r == 18 || r == 20 || r == 24 || r == 32 || r == 36 || r == 40 || r == 48;
 }
 template <int M, int B = 16> struct LinePlan {
     static constexpr int S = line_stages(M, B);

@@ -98,22 +98,25 @@ private:
 struct Profiler;
 
 // ---- FFT (fft.hip) ---------------------------------------------------------------------------
-// Transform lengths are 2^a * 3^b with b <= 2, up to kMaxFft. Twiddles come from one buffer
-// holding three tables, one per length family: exp(-2 pi i j / T) for T = 16384 (powers of two),
-// 12288 (3 * 2^a) and 9216 (9 * 2^a). A length N reads its family's table at stride T / N, so
-// the last stage of every transform reads consecutive entries.
+// Transform lengths are 2^a * 3^b with b <= 2, up to kMaxFft (the PSF line transforms also
+// take 5 * 2^a). Twiddles come from one buffer holding four tables, one per length family:
+// exp(-2 pi i j / T) for T = 16384 (powers of two), 12288 (3 * 2^a), 9216 (9 * 2^a) and 10240
+// (5 * 2^a). A length N reads its family's table at stride T / N, so the last stage of every
+// transform reads consecutive entries.
 constexpr int kMaxFftLog2 = 14;
 constexpr int kMaxFft = 1 << kMaxFftLog2;   // 16384 points
-constexpr int kTw2 = 16384, kTw3 = 12288, kTw9 = 9216;
-constexpr int kTwOff2 = 0, kTwOff3 = kTw2, kTwOff9 = kTw2 + kTw3;
-constexpr int kTwiddleEntries = kTw2 + kTw3 + kTw9;
+constexpr int kTw2 = 16384, kTw3 = 12288, kTw9 = 9216, kTw5 = 10240;
+constexpr int kTwOff2 = 0, kTwOff3 = kTw2, kTwOff9 = kTw2 + kTw3, kTwOff5 = kTw2 + kTw3 + kTw9;
+constexpr int kTwiddleEntries = kTw2 + kTw3 + kTw9 + kTw5;
 
-// odd part of n (1, 3 or 9 for supported lengths)
+// odd part of n (1, 3, 5 or 9 for supported lengths)
 __host__ __device__ constexpr int odd_part(int n) { return (n & 1) ? n : odd_part(n >> 1); }
 // Table length and offset of the family of length n.
-__host__ __device__ constexpr int tw_table(int n) { return odd_part(n) == 1 ? kTw2 : odd_part(n) == 3 ? kTw3 : kTw9; }
+__host__ __device__ constexpr int tw_table(int n) {
+    return odd_part(n) == 1 ? kTw2 : odd_part(n) == 3 ? kTw3 : odd_part(n) == 5 ? kTw5 : kTw9;
+}
 __host__ __device__ constexpr int tw_offset(int n) {
-    return odd_part(n) == 1 ? kTwOff2 : odd_part(n) == 3 ? kTwOff3 : kTwOff9;
+    return odd_part(n) == 1 ? kTwOff2 : odd_part(n) == 3 ? kTwOff3 : odd_part(n) == 5 ? kTwOff5 : kTwOff9;
 }
 
 // 2^a * 3^b, b <= 2, in [2, kMaxFft]
@@ -148,7 +151,7 @@ inline int grid_length_at_least(int v) {
 }
 
 struct FftTables {
-    DeviceBuffer twiddle;  // float2[kTwiddleEntries]: the three family tables
+    DeviceBuffer twiddle;  // float2[kTwiddleEntries]: the four family tables
     void init(hipStream_t s);
 };
 

@@ -15,7 +15,7 @@
 // pupil from offset floor((R-P)/2) to 0 multiplies each row transform by exp(-2 i pi_f kx off / R)
 // and each column transform by exp(-2 i pi_f ky off / R): unit-modulus factors that |F|^2
 // drops. With the input at 0 the circular convolution only has to hold lags -(P-1) .. R-1, so
-// the Bluestein length is the smallest FFT length M >= R + P - 1 (6144 for R = 4092, P = 1024,
+// the Bluestein length is the smallest line length M >= R + P - 1 (5120 for R = 4092, P = 1024,
 // where the reference uses 8192).
 // The sequential-sum normalisation then scales each raw PSF to unit energy.
 #include <cmath>
@@ -242,7 +242,7 @@ void host_fft_pow2(std::vector<double>& re, std::vector<double>& im) {
     }
 }
 
-// Forward DFT of length n = 2^a * q (q = 1, 3 or 9) in double: q interleaved radix-2 FFTs of
+// Forward DFT of length n = 2^a * q (q = 1, 3, 5 or 9) in double: q interleaved radix-2 FFTs of
 // length 2^a, then twiddles and length-q DFTs.
 void host_dft(std::vector<double>& re, std::vector<double>& im) {
     const int n = int(re.size());
@@ -329,13 +329,13 @@ bool launch_cols(int m, hipStream_t s, const float2* GT, int P, int R, int off, 
 }
 
 // Line lengths of the PSF kernels: powers of two (plain transforms of R, and Bluestein) and
-// 3 * 2^a, 9 * 2^a Bluestein lengths with a composite-first plan.
-#define LDH_PSF_LENGTHS(X)                                                                                 \
-    X(128) || X(144) || X(192) || X(256) || X(288) || X(384) || X(512) || X(576) || X(768) || X(1024) ||   \
-        X(1536) || X(2048) || X(2304) || X(3072) || X(4096) || X(4608) || X(6144) || X(8192) || X(9216) || \
-        X(12288) || X(16384)
-constexpr int kPsfLengths[] = {128,  144,  192,  256,  288,  384,  512,  576,  768,  1024, 1536,
-                               2048, 2304, 3072, 4096, 4608, 6144, 8192, 9216, 12288, 16384};
+// 3 * 2^a, 5 * 2^a, 9 * 2^a Bluestein lengths with a composite-first plan.
+#define LDH_PSF_LENGTHS(X)                                                                                  \
+    X(128) || X(144) || X(160) || X(192) || X(256) || X(288) || X(320) || X(384) || X(512) || X(576) ||     \
+        X(640) || X(768) || X(1024) || X(1280) || X(1536) || X(2048) || X(2304) || X(2560) || X(3072) ||    \
+        X(4096) || X(4608) || X(5120) || X(6144) || X(8192) || X(9216) || X(12288) || X(16384)
+constexpr int kPsfLengths[] = {128,  144,  160,  192,  256,  288,  320,  384,  512,  576,  640,   768,  1024, 1280,
+                               1536, 2048, 2304, 2560, 3072, 4096, 4608, 5120, 6144, 8192, 9216, 12288, 16384};
 
 int psf_length_at_least(int v) {
     for (int n : kPsfLengths)

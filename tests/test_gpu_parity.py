@@ -118,7 +118,8 @@ def test_pupil_phase(ctx, oracle):
     assert np.abs(a - b).max() < 2e-6
 
 
-@pytest.mark.parametrize("pupil,raw", [(64, 256), (64, 300), (128, 512), (96, 1020)])
+# Bluestein line lengths: 384 = 3*2^7, 1280 = 5*2^8, 2560 = 5*2^9, 5120 = 5*2^10 (the UHD bank's)
+@pytest.mark.parametrize("pupil,raw", [(64, 256), (64, 300), (128, 512), (96, 1020), (300, 2200), (1024, 4092)])
 def test_raw_psf(ctx, oracle, pupil, raw):
     p = params(pupil_resolution=pupil, aperture_mode=1, blade_count=6, roundness=0.2)
     p.set_noll({4: 0.3, 11: -0.2, 8: 0.1})
