@@ -178,8 +178,9 @@ def main():
         ctx.profiling(True)
         ms.append(round(ctx.bank_build_seconds(lens(5002)) * 1e3, 2))
         ctx.profiling(False)
-        print(json.dumps({"bank_build_ms": ms, "autotune": ctx.tuning_report(),
-                          "kernel_families": families(ctx.profile_read())}))
+        fams = families(ctx.profile_read())
+        print(json.dumps({"bank_build_ms": ms, "kernel_sides": ctx.bank_kernel_sizes(lens(5002)),
+                          "autotune": ctx.tuning_report(), "kernel_families": fams}))
         ctx.close()
         return
     from oracle.oracle import rel_l2

@@ -163,6 +163,17 @@ class Context:
                 out[name].append(a)
         return out
 
+    def bank_kernel_sizes(self, p) -> list:
+        """Cropped kernel side per bin of the bank for `p`."""
+        b = C.c_void_p()
+        self._check(self.lib.ldh_bank_acquire(self.h, C.byref(p), C.byref(b)))
+        try:
+            info = _abi.BankInfo()
+            self._check(self.lib.ldh_bank_info_get(self.h, b, C.byref(info)))
+            return [int(info.kernel_size[i]) for i in range(info.bin_count)]
+        finally:
+            self.lib.ldh_bank_release(self.h, b)
+
     def bank_build_seconds(self, p) -> float:
         """Wall time of building the bank for `p` on the device (synchronised), bank released."""
         b = C.c_void_p()

@@ -296,6 +296,23 @@ crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, float*
     store_block_sums<3>(share, slots);
 }
 
+// x[seg] *= 1 / totals[seg] (segments [3][count] of K*K) over the crop square of the segment's
+// bin only: crop_taper_kernel leaves every texel outside it at 0, which the scale keeps.
+__global__ void scale_crop_kernel(float* __restrict__ x, int K, const int* __restrict__ radius,
+                                  const float* __restrict__ totals) {
+    const int seg = blockIdx.y, count = gridDim.y / 3, bin = seg % count;
+    const float total = totals[seg];
+    if (total <= 0.0f) return;  // the reference leaves a non-positive total untouched
+    const float inv = 1.0f / total;
+    const int r = min(max(radius[bin], 1), K / 2), c = K / 2;
+    const int lo = c - r, side = min(K - 1, c + r) - lo + 1;  // lo >= 0; K even: c + r may reach K
+    float* base = x + int64_t(seg) * K * K + int64_t(lo) * K + lo;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < side * side; i += gridDim.x * blockDim.x) {
+        const int y = i / side, xx = i - y * side;
+        base[int64_t(y) * K + xx] *= inv;
+    }
+}
+
 __global__ void fill_zero_kernel(uint32_t* __restrict__ p, int64_t n) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
         p[i] = 0u;
@@ -433,7 +450,13 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
         LDH_CHECK_LAUNCH("crop-taper");
     }
     w.seqsum.run(s, out3, kk, kk, 3 * count, t_final, bsum);
-    w.seqsum.scale(s, out3, kk, kk, 3 * count, t_final);
+    {
+        // touches (2r+1)^2 texels per kernel, r known on the device only: no byte model
+        ProfileScope ps_("shape-scale", 0.0);
+        hipLaunchKernelGGL(scale_crop_kernel, dim3(unsigned(std::min<int64_t>(div_up(kk, 256), 1024)), unsigned(3 * count)),
+                           dim3(256), 0, s, out3, K, d_radius, t_final);
+        LDH_CHECK_LAUNCH("shape-scale");
+    }
 }
 
 std::vector<float> radial_profile(hipStream_t s, BankWorkspace& w, const float* psf, int R) {

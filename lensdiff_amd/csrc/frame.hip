@@ -273,15 +273,22 @@ __global__ void composite_kernel(const float4* __restrict__ src, const float* __
     dst[i] = make_float4(o[0], o[1], o[2], s.w);
 }
 
+// out[r] = sum_b m[r][b] * kernels[b] (rows <= 6), accumulated over b in order for every r;
+// each kernel texel is read once.
 __global__ void combine_kernel(const float* __restrict__ kernels, int bins, int64_t kk, const float* __restrict__ m,
                                int rows, float* __restrict__ out) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= kk) return;
-    for (int r = 0; r < rows; ++r) {
-        float acc = 0.0f;
-        for (int b = 0; b < bins; ++b) acc += m[r * bins + b] * kernels[int64_t(b) * kk + i];
-        out[int64_t(r) * kk + i] = acc;
+    float acc[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int b = 0; b < bins; ++b) {
+        const float k = kernels[int64_t(b) * kk + i];
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+            if (r < rows) acc[r] += m[r * bins + b] * k;
     }
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+        if (r < rows) out[int64_t(r) * kk + i] = acc[r];
 }
 
 inline dim3 grid1(int64_t n) { return dim3(unsigned(div_up(n, 256))); }
