@@ -295,10 +295,11 @@ ldh_status ldh_shape_kernels(ldh_context* ctx, const ldh_params* p, const float*
                 "invalid-argument");
         const int max_r = ldh::resolve_sizes(*p).max_radius;
         const int64_t kk = int64_t(K) * K;
-        ldh::DeviceBuffer db(sizeof(float) * kk), d3(sizeof(float) * kk * 3), dr(sizeof(int));
+        ldh::DeviceBuffer db(sizeof(float) * kk), d3(sizeof(float) * kk * 3), dr(sizeof(int)), dsc(sizeof(float) * 3);
         h2d(ctx, db.get(), base, sizeof(float) * kk);
         ldh::shape_kernels(ctx->stream, ctx->bw, db.as<float>(), nullptr, K, 1, p->anisotropy_emphasis, max_r, d3.as<float>(),
-                           dr.as<int>());
+                           dr.as<int>(), dsc.as<float>());
+        ldh::apply_kernel_scales(ctx->stream, d3.as<float>(), K, 1, dr.as<int>(), dsc.as<float>());
         std::vector<float> h(size_t(kk) * 3);
         int r = 0;
         LDH_HIP(hipMemcpyAsync(&r, dr.get(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "shape");
@@ -441,8 +442,10 @@ ldh_status ldh_bank_download(ldh_context* ctx, const ldh_bank* bank, int32_t bin
         require(capacity >= int64_t(n) * n, LDH_ERR_INVALID_ARGUMENT, "capacity-too-small");
         std::vector<float> h(size_t(K) * K);
         d2h(ctx, h.data(), b.kernel_ptr(which, bin), sizeof(float) * size_t(K) * K);
+        // the stored texels times the kernel's unit-sum scale: the normalised kernel
+        const float sc = b.host_scale(which, bin);
         for (int y = 0; y < n; ++y)
-            for (int x = 0; x < n; ++x) out[size_t(y) * n + x] = h[size_t(c - rc + y) * K + size_t(c - rc + x)];
+            for (int x = 0; x < n; ++x) out[size_t(y) * n + x] = h[size_t(c - rc + y) * K + size_t(c - rc + x)] * sc;
     });
 }
 

@@ -18,6 +18,9 @@ constexpr int kMaxKernelRadius = 2048;
 // bin's cropped support: kernel (which, bin) at store + which * which_stride + bin * K*K (which:
 // full, core, structure). A bank built alone owns `kernels` ([which][bin][K*K]); the banks of a
 // batched build share one store ([which][unit][K*K], each bank's bins consecutive units).
+// The stored kernels are not yet unit-sum: kernel (which, bin) holds texels t whose normalised
+// values are t * scale (scale = 1 / its sequential FP32 sum, or 1 for a non-positive sum, the
+// values LensDiffCpuReference.cpp:440-449 leaves), applied by every reader.
 struct DeviceBank {
     int bins = 0;
     int pupil = 0, raw = 0;
@@ -30,10 +33,24 @@ struct DeviceBank {
     std::shared_ptr<DeviceBuffer> shared;   // or the batch's store
     float* store = nullptr;
     int64_t which_stride = 0;
+    // unit-sum multipliers: scale (which, bin) at scales + which * scales_which_stride + bin
+    DeviceBuffer own_scales;
+    std::shared_ptr<DeviceBuffer> shared_scales;
+    float* scales = nullptr;
+    int64_t scales_which_stride = 0;
+    std::vector<float> host_scales;  // [which][bin]
     size_t bytes() const { return size_t(3) * bins * size_t(kernel) * kernel * sizeof(float); }
     const float* kernel_ptr(int which, int bin) const {
         return store + which * which_stride + int64_t(bin) * kernel * kernel;
     }
+    const float* scale_ptr(int which, int bin) const { return scales + which * scales_which_stride + bin; }
+    float host_scale(int which, int bin) const { return host_scales[size_t(which) * bins + size_t(bin)]; }
+};
+
+// A kernel as its readers see it: texels `data`, each multiplied by *scale when scale is set.
+struct KernelRef {
+    const float* data = nullptr;
+    const float* scale = nullptr;
 };
 
 // Scratch shared by bank builds on one stream.
@@ -64,8 +81,12 @@ void base_kernels(hipStream_t s, BankWorkspace& w, const float* raw, int R, cons
 // radius[count] (device). `max_radius` as LensDiffCpuReference.cpp:1633. base_totals (nullable):
 // per-kernel sums of a base that is not yet normalised (read as base / total).
 // d_gains (nullable): per-kernel anisotropy gain 1 + 4 * max(0, emphasis), else from `anisotropy`.
+// out3 is left unnormalised; out_scales (device [3][count]) receives the unit-sum multipliers.
 void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const float* base_totals, int K, int count,
-                   double anisotropy, int max_radius, float* out3, int* d_radius, const float* d_gains = nullptr);
+                   double anisotropy, int max_radius, float* out3, int* d_radius, float* out_scales,
+                   const float* d_gains = nullptr);
+// out3 *= out_scales in place (normalised kernels for a caller that copies them out).
+void apply_kernel_scales(hipStream_t s, float* out3, int K, int count, const int* d_radius, const float* scales);
 // Radial mean profile of an R x R PSF about its centre (radialMeanProfile, :865-887), computed
 // on the device and returned to the host.
 std::vector<float> radial_profile(hipStream_t s, BankWorkspace& w, const float* psf, int R);

@@ -296,14 +296,13 @@ crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, float*
     store_block_sums<3>(share, slots);
 }
 
-// x[seg] *= 1 / totals[seg] (segments [3][count] of K*K) over the crop square of the segment's
-// bin only: crop_taper_kernel leaves every texel outside it at 0, which the scale keeps.
+// x[seg] *= scales[seg] (segments [3][count] of K*K) over the crop square of the segment's bin
+// only: crop_taper_kernel leaves every texel outside it at 0, which the scale keeps.
 __global__ void scale_crop_kernel(float* __restrict__ x, int K, const int* __restrict__ radius,
-                                  const float* __restrict__ totals) {
+                                  const float* __restrict__ scales) {
     const int seg = blockIdx.y, count = gridDim.y / 3, bin = seg % count;
-    const float total = totals[seg];
-    if (total <= 0.0f) return;  // the reference leaves a non-positive total untouched
-    const float inv = 1.0f / total;
+    const float inv = scales[seg];
+    if (inv == 1.0f) return;
     const int r = min(max(radius[bin], 1), K / 2), c = K / 2;
     const int lo = c - r, side = min(K - 1, c + r) - lo + 1;  // lo >= 0; K even: c + r may reach K
     float* base = x + int64_t(seg) * K * K + int64_t(lo) * K + lo;
@@ -311,6 +310,11 @@ __global__ void scale_crop_kernel(float* __restrict__ x, int K, const int* __res
         const int y = i / side, xx = i - y * side;
         base[int64_t(y) * K + xx] *= inv;
     }
+}
+
+__global__ void inverse_totals_kernel(const float* __restrict__ totals, int n, float* __restrict__ scales) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) scales[i] = inverse_total(totals, i);
 }
 
 __global__ void fill_zero_kernel(uint32_t* __restrict__ p, int64_t n) {
@@ -382,8 +386,16 @@ void BankWorkspace::reserve_shaping(int K, int count, int max_radius) {
     seqsum.reserve(kk, 3 * count);
 }
 
+void apply_kernel_scales(hipStream_t s, float* out3, int K, int count, const int* d_radius, const float* scales) {
+    const int64_t kk = int64_t(K) * K;
+    hipLaunchKernelGGL(scale_crop_kernel, dim3(unsigned(std::min<int64_t>(div_up(kk, 256), 1024)), unsigned(3 * count)),
+                       dim3(256), 0, s, out3, K, d_radius, scales);
+    LDH_CHECK_LAUNCH("kernel-scales");
+}
+
 void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const float* base_totals, int K, int count,
-                   double anisotropy, int max_radius, float* out3, int* d_radius, const float* d_gains) {
+                   double anisotropy, int max_radius, float* out3, int* d_radius, float* out_scales,
+                   const float* d_gains) {
     const int64_t kk = int64_t(K) * K;
     const int rmax_mean = K / 2;
     w.reserve_shaping(K, count, max_radius);
@@ -450,13 +462,10 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
         LDH_CHECK_LAUNCH("crop-taper");
     }
     w.seqsum.run(s, out3, kk, kk, 3 * count, t_final, bsum);
-    {
-        // touches (2r+1)^2 texels per kernel, r known on the device only: no byte model
-        ProfileScope ps_("shape-scale", 0.0);
-        hipLaunchKernelGGL(scale_crop_kernel, dim3(unsigned(std::min<int64_t>(div_up(kk, 256), 1024)), unsigned(3 * count)),
-                           dim3(256), 0, s, out3, K, d_radius, t_final);
-        LDH_CHECK_LAUNCH("shape-scale");
-    }
+    // the unit-sum scales travel with the kernels; readers multiply as they load
+    hipLaunchKernelGGL(inverse_totals_kernel, dim3(unsigned(div_up(3 * count, 256))), dim3(256), 0, s, t_final,
+                       3 * count, out_scales);
+    LDH_CHECK_LAUNCH("kernel-scales");
 }
 
 std::vector<float> radial_profile(hipStream_t s, BankWorkspace& w, const float* psf, int R) {
@@ -604,10 +613,16 @@ std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWo
     bank->store = bank->kernels.as<float>();
     bank->which_stride = kk * L;
     w.radius.reset(sizeof(int) * L);
+    bank->own_scales.reset(sizeof(float) * 3 * L);
+    bank->scales = bank->own_scales.as<float>();
+    bank->scales_which_stride = L;
     shape_kernels(s, w, w.base.as<float>(), w.base_totals.as<float>(), K, L, p.anisotropy_emphasis, r,
-                  bank->kernels.as<float>(), w.radius.as<int>());
+                  bank->kernels.as<float>(), w.radius.as<int>(), bank->scales);
     bank->radius.resize(size_t(L));
+    bank->host_scales.resize(size_t(3) * L);
     LDH_HIP(hipMemcpyAsync(bank->radius.data(), w.radius.get(), sizeof(int) * L, hipMemcpyDeviceToHost, s), "bank-radius");
+    LDH_HIP(hipMemcpyAsync(bank->host_scales.data(), bank->scales, sizeof(float) * 3 * L, hipMemcpyDeviceToHost, s),
+            "bank-scales");
     LDH_HIP(hipStreamSynchronize(s), "bank-radius");
     bank->support = 4;
     for (int v : bank->radius) bank->support = std::max(bank->support, v);

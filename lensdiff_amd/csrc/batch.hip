@@ -30,7 +30,7 @@ struct Group {
     std::vector<float> scales, gains;
     PsfPlan plan;
     DeviceBuffer d_geoms, d_amp_index, d_coefs, d_scales, d_gains, amp, phase, radius;
-    std::shared_ptr<DeviceBuffer> store;
+    std::shared_ptr<DeviceBuffer> store, kernel_scales;  // [which][unit] unit-sum multipliers
 };
 
 template <class T>
@@ -51,7 +51,7 @@ void enqueue_group(hipStream_t s, const FftTables& t, BankWorkspace& w, Group& g
     base_kernels(s, w, w.raw.as<float>(), R, g.d_scales.as<float>(), w.totals.as<float>(), g.r, U, w.base.as<float>(),
                  w.base_totals.as<float>());
     shape_kernels(s, w, w.base.as<float>(), w.base_totals.as<float>(), K, U, 0.0, g.r, g.store->as<float>(),
-                  g.radius.as<int>(), g.d_gains.as<float>());
+                  g.radius.as<int>(), g.kernel_scales->as<float>(), g.d_gains.as<float>());
 }
 
 }  // namespace
@@ -154,6 +154,7 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
         g.phase.reset(sizeof(float) * size_t(g.P) * g.P * g.units);
         g.radius.reset(sizeof(int) * g.units);
         g.store = std::make_shared<DeviceBuffer>(sizeof(float) * kk * 3 * g.units);
+        g.kernel_scales = std::make_shared<DeviceBuffer>(sizeof(float) * 3 * g.units);
         // the workspace grows to the batch's largest group before any launch
         w.raw.reset(sizeof(float) * size_t(g.R) * g.R * g.units);
         w.totals.reset(sizeof(float) * g.units);
@@ -211,7 +212,10 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
     for (auto& gp : groups) {
         Group& g = *gp;
         std::vector<int> radius(static_cast<size_t>(g.units));
+        std::vector<float> kscales(static_cast<size_t>(3 * g.units));
         LDH_HIP(hipMemcpy(radius.data(), g.radius.get(), sizeof(int) * g.units, hipMemcpyDeviceToHost), "batch-radius");
+        LDH_HIP(hipMemcpy(kscales.data(), g.kernel_scales->get(), sizeof(float) * 3 * g.units, hipMemcpyDeviceToHost),
+                "batch-scales");
         const int64_t kk = int64_t(g.K) * g.K;
         for (size_t k = 0; k < g.requests.size(); ++k) {
             const int i = g.requests[k];
@@ -228,6 +232,13 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
             bank->shared = g.store;
             bank->store = g.store->as<float>() + int64_t(g.unit0[k]) * kk;
             bank->which_stride = kk * g.units;
+            bank->shared_scales = g.kernel_scales;
+            bank->scales = g.kernel_scales->as<float>() + g.unit0[k];
+            bank->scales_which_stride = g.units;
+            bank->host_scales.resize(size_t(3) * bank->bins);
+            for (int which = 0; which < 3; ++which)
+                for (int b = 0; b < bank->bins; ++b)
+                    bank->host_scales[size_t(which) * bank->bins + b] = kscales[size_t(which) * g.units + g.unit0[k] + b];
             banks[size_t(i)] = bank;
         }
     }

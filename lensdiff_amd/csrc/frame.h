@@ -36,14 +36,15 @@ struct KernelSpectra {
     DeviceBuffer data;    // float2 or __half2 [pairs][ph][pw]
     size_t bytes() const { return data.bytes(); }
 };
-// kernels: device pointers (each K x K), packed in order (0,1), (2,3), ...; `radius` bounds
-// the non-zero support of every kernel.
-void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<const float*>& kernels, int K,
+// kernels: K x K each (texels times their scale), packed in order (0,1), (2,3), ...; `radius`
+// bounds the non-zero support of every kernel.
+void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<KernelRef>& kernels, int K,
                           int radius, const Grid& g, bool fp16, KernelSpectra& out, DeviceBuffer& scratch);
-// Combined RGB kernels of a spectral bank: out[c] = sum_b M[c][b] k_b for the natural (c=0..2)
-// and, if with_style, style (c=3..5) matrices. out: device float[6][K*K].
-void combine_kernels(hipStream_t s, const float* kernels, int bins, int K, const SpectrumConfig& cfg,
-                     bool with_style, float* out, DeviceBuffer& scratch);
+// Combined RGB kernels of a spectral bank: out[c] = sum_b M[c][b] (k_b * scales[b]) for the
+// natural (c=0..2) and, if with_style, style (c=3..5) matrices. out: device float[6][K*K],
+// unit scale.
+void combine_kernels(hipStream_t s, const float* kernels, const float* scales, int bins, int K,
+                     const SpectrumConfig& cfg, bool with_style, float* out, DeviceBuffer& scratch);
 
 // Scratch and outputs of one frame on one stream.
 struct FrameWork {
@@ -62,8 +63,8 @@ enum class EffectMode { PerBin, Precombined };
 void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
                   const float4* src, float4* dst, int w, int h, EffectMode mode, const KernelSpectra* cached);
 
-// Which real kernels (device pointers) feed the effect, for building cached spectra.
-std::vector<const float*> effect_kernels(const ldh_params& p, const DeviceBank& bank, EffectMode mode,
-                                         const float* combined, int* which_out);
+// Which real kernels feed the effect, for building cached spectra.
+std::vector<KernelRef> effect_kernels(const ldh_params& p, const DeviceBank& bank, EffectMode mode,
+                                      const float* combined, int* which_out);
 
 }  // namespace ldh

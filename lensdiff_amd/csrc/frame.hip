@@ -92,6 +92,8 @@ struct FrameRows {
 struct KernelRows {
     const float* ka;
     const float* kb;
+    const float* sa;  // texel multipliers (nullable: 1)
+    const float* sb;
     float2* out;
     int K, c, r, pw, ph;
     __device__ float2 load(int row, int x) const {
@@ -99,7 +101,9 @@ struct KernelRows {
         const int dx = x < pw / 2 ? x : x - pw;
         if (dx < -r || dx > r || dy < -r || dy > r) return make_float2(0.0f, 0.0f);
         const size_t i = size_t(c + dy) * K + size_t(c + dx);
-        return make_float2(ka[i], kb ? kb[i] : 0.0f);
+        const float a = sa ? ka[i] * *sa : ka[i];
+        const float b = kb ? (sb ? kb[i] * *sb : kb[i]) : 0.0f;
+        return make_float2(a, b);
     }
     __device__ void store(int row, int x, float2 v) const { out[size_t(row) * pw + x] = v; }
 };
@@ -273,15 +277,15 @@ __global__ void composite_kernel(const float4* __restrict__ src, const float* __
     dst[i] = make_float4(o[0], o[1], o[2], s.w);
 }
 
-// out[r] = sum_b m[r][b] * kernels[b] (rows <= 6), accumulated over b in order for every r;
-// each kernel texel is read once.
-__global__ void combine_kernel(const float* __restrict__ kernels, int bins, int64_t kk, const float* __restrict__ m,
-                               int rows, float* __restrict__ out) {
+// out[r] = sum_b m[r][b] * (kernels[b] * scales[b]) (rows <= 6), accumulated over b in order for
+// every r; each kernel texel is read once.
+__global__ void combine_kernel(const float* __restrict__ kernels, const float* __restrict__ scales, int bins,
+                               int64_t kk, const float* __restrict__ m, int rows, float* __restrict__ out) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= kk) return;
     float acc[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     for (int b = 0; b < bins; ++b) {
-        const float k = kernels[int64_t(b) * kk + i];
+        const float k = kernels[int64_t(b) * kk + i] * scales[b];
 #pragma unroll
         for (int r = 0; r < 6; ++r)
             if (r < rows) acc[r] += m[r * bins + b] * k;
@@ -309,7 +313,7 @@ Grid make_grid(int w, int h, int support) {
     return g;
 }
 
-void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<const float*>& kernels, int K,
+void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<KernelRef>& kernels, int K,
                           int radius, const Grid& g, bool fp16, KernelSpectra& out, DeviceBuffer& scratch) {
     const int n = int(kernels.size());
     const int pairs = (n + 1) / 2;
@@ -323,8 +327,9 @@ void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<c
     if (fp16) scratch.reset(sizeof(float2) * cells);
     for (int p = 0; p < pairs; ++p) {
         float2* dst = fp16 ? scratch.as<float2>() : out.data.as<float2>() + cells * p;
-        const float* kb = 2 * p + 1 < n ? kernels[size_t(2 * p + 1)] : nullptr;
-        KernelRows op{kernels[size_t(2 * p)], kb, dst, K, K / 2, radius, g.pw, g.ph};
+        const KernelRef ka = kernels[size_t(2 * p)];
+        const KernelRef kb = 2 * p + 1 < n ? kernels[size_t(2 * p + 1)] : KernelRef{};
+        KernelRows op{ka.data, kb.data, ka.scale, kb.scale, dst, K, K / 2, radius, g.pw, g.ph};
         // only the rows [0, r] and [ph - r, ph) of the wrapped kernel are non-zero
         const int band = 2 * radius + 1;
         rows_pass(s, tw, g.pw, band, op, false, radius + 1, g.ph - band);
@@ -340,8 +345,8 @@ void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<c
     }
 }
 
-void combine_kernels(hipStream_t s, const float* kernels, int bins, int K, const SpectrumConfig& cfg, bool with_style,
-                     float* out, DeviceBuffer& scratch) {
+void combine_kernels(hipStream_t s, const float* kernels, const float* scales, int bins, int K,
+                     const SpectrumConfig& cfg, bool with_style, float* out, DeviceBuffer& scratch) {
     const int rows = with_style ? 6 : 3;
     std::vector<float> m(static_cast<size_t>(rows * bins));
     for (int r = 0; r < 3; ++r)
@@ -354,7 +359,8 @@ void combine_kernels(hipStream_t s, const float* kernels, int bins, int K, const
     const int64_t kk = int64_t(K) * K;
 {
     ProfileScope ps_("spectra-combine", 4.0 * double(kk) * (bins + rows));
-        hipLaunchKernelGGL(combine_kernel, grid1(kk), dim3(256), 0, s, kernels, bins, kk, scratch.as<float>(), rows, out);
+        hipLaunchKernelGGL(combine_kernel, grid1(kk), dim3(256), 0, s, kernels, scales, bins, kk, scratch.as<float>(),
+                           rows, out);
     LDH_CHECK_LAUNCH("combine-kernels");
     }
     LDH_HIP(hipStreamSynchronize(s), "combine");
@@ -390,19 +396,19 @@ Plan plan_of(const ldh_params& p) {
 
 }  // namespace
 
-std::vector<const float*> effect_kernels(const ldh_params& p, const DeviceBank& bank, EffectMode mode,
-                                         const float* combined, int* which_out) {
+std::vector<KernelRef> effect_kernels(const ldh_params& p, const DeviceBank& bank, EffectMode mode,
+                                      const float* combined, int* which_out) {
     const Plan pl = plan_of(p);
-    std::vector<const float*> ks;
+    std::vector<KernelRef> ks;
     const int which = pl.sets.front();
     if (which_out) *which_out = which;
     if (!pl.spectral) {
-        ks.push_back(bank.kernel_ptr(which, 0));
+        ks.push_back({bank.kernel_ptr(which, 0), bank.scale_ptr(which, 0)});
     } else if (mode == EffectMode::PerBin) {
-        for (int b = 0; b < bank.bins; ++b) ks.push_back(bank.kernel_ptr(which, b));
+        for (int b = 0; b < bank.bins; ++b) ks.push_back({bank.kernel_ptr(which, b), bank.scale_ptr(which, b)});
     } else {
         const int64_t kk = int64_t(bank.kernel) * bank.kernel;
-        for (int r = 0; r < 6; ++r) ks.push_back(combined + r * kk);
+        for (int r = 0; r < 6; ++r) ks.push_back({combined + r * kk, nullptr});
     }
     return ks;
 }
@@ -468,22 +474,22 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
     for (size_t si = 0; si < pl.sets.size(); ++si) {
         const int set = pl.sets[si];
         // kernels of this set
-        std::vector<const float*> ks;
+        std::vector<KernelRef> ks;
         const int64_t kk = int64_t(bank.kernel) * bank.kernel;
         const bool use_cache = cached != nullptr && si == 0 && pl.sets.size() == 1;
         if (use_cache) {
             // the cached spectra stand for the kernels; only their count matters here
-            ks.assign(size_t(cached->kernels), nullptr);
+            ks.assign(size_t(cached->kernels), KernelRef{});
         } else if (!pl.spectral) {
-            ks.push_back(bank.kernel_ptr(set, 0));
+            ks.push_back({bank.kernel_ptr(set, 0), bank.scale_ptr(set, 0)});
         } else if (mode == EffectMode::PerBin) {
-            for (int b = 0; b < n; ++b) ks.push_back(bank.kernel_ptr(set, b));
+            for (int b = 0; b < n; ++b) ks.push_back({bank.kernel_ptr(set, b), bank.scale_ptr(set, b)});
         } else {
             fw.combined.reset(sizeof(float) * kk * 6);
-            combine_kernels(s, bank.kernel_ptr(set, 0), n, bank.kernel, cfg, with_style, fw.combined.as<float>(),
-                            fw.kspec_tmp);
+            combine_kernels(s, bank.kernel_ptr(set, 0), bank.scale_ptr(set, 0), n, bank.kernel, cfg, with_style,
+                            fw.combined.as<float>(), fw.kspec_tmp);
             const int nk = with_style ? 6 : 3;
-            for (int r = 0; r < nk; ++r) ks.push_back(fw.combined.as<float>() + r * kk);
+            for (int r = 0; r < nk; ++r) ks.push_back({fw.combined.as<float>() + r * kk, nullptr});
         }
         const int pairs = pl.spectral ? int((ks.size() + 1) / 2) : 1;
         if (use_cache && (cached->grid.pw != g.pw || cached->grid.ph != g.ph || cached->pairs != pairs))
@@ -497,7 +503,7 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
                 kspec = half ? static_cast<const void*>(cached->data.as<__half2>() + cells * pi)
                               : static_cast<const void*>(cached->data.as<float2>() + cells * pi);
             } else {
-                std::vector<const float*> pair{ks[size_t(pl.spectral ? 2 * pi : 0)]};
+                std::vector<KernelRef> pair{ks[size_t(pl.spectral ? 2 * pi : 0)]};
                 if (pl.spectral && size_t(2 * pi + 1) < ks.size()) pair.push_back(ks[size_t(2 * pi + 1)]);
                 build_kernel_spectra(s, t, pair, bank.kernel, bank.support, g, false, fw.kspec, fw.kspec_tmp);
                 kspec = fw.kspec.data.get();

@@ -13,9 +13,10 @@
 //      FP64);
 //   4. a one-wavefront walk over the blocks in order: a block whose binade sum keeps the
 //      running sum inside its binade is added at once; a block where the running sum crosses
-//      into the next binade, or whose binade lies outside the four computed, is replayed: its
-//      64-element chunks are added through a lane scan of their rounded contributions, and the
-//      chunk holding the crossing is added element by element in true FP32.
+//      into the next binade, or whose binade lies outside the four computed, is replayed by the
+//      whole workgroup: its 16-element chunks are added through a scan of their rounded
+//      contributions, and the chunk holding the crossing is added element by element in true
+//      FP32.
 // An addend that lands exactly halfway between two multiples of u rounds by the parity of the
 // running sum, which the block sums cannot know: binade sums and chunks holding such a tie are
 // marked and always go the element-by-element way. The estimate only decides which blocks are
@@ -33,7 +34,6 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kPerThread = kSeqSumBlock / kThreads;
-constexpr int kChunk = kSeqSumBlock / 64;  // elements per lane in a replay
 constexpr int kBins = 4;                   // binades per block
 constexpr int kNoBase = INT_MIN;           // running sum estimated at zero: replay the block
 constexpr double kTied = -1.0;             // a binade sum holding a rounding tie
@@ -175,19 +175,20 @@ struct Run {
 
 __device__ __forceinline__ double binade_limit(int e) { return ldexp(1.0, e + 1); }
 
-// Sum of the rounded contributions of a lane's chunk at binade e (exact values at s = 0), or
-// kTied when one of them is a rounding tie.
+// Sum of the rounded contributions of a chunk of C elements at binade e (exact values at s = 0),
+// or kTied when one of them is a rounding tie.
+template <int C>
 __device__ __forceinline__ double chunk_contribution(const float* v, int e) {
     double acc = 0.0;
     if (e == kNoBase) {
 #pragma unroll
-        for (int j = 0; j < kChunk; ++j) acc += double(v[j]);
+        for (int j = 0; j < C; ++j) acc += double(v[j]);
         return acc;
     }
     const int shift = 23 - e;
     bool tie = false;
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) {
+    for (int j = 0; j < C; ++j) {
         const float t = ldexpf(v[j], shift);
         const float r = rintf(t);
         tie |= fabsf(t - r) == 0.5f;
@@ -196,61 +197,92 @@ __device__ __forceinline__ double chunk_contribution(const float* v, int e) {
     return tie ? kTied : ldexp(acc, e - 23);
 }
 
-// Adds one block element by element in effect: lane l holds chunk l (kChunk consecutive
+// The walk runs one workgroup of kWalkThreads threads per segment. Every wave executes the
+// block-level walk (identical data, identical decisions); the replay of a block spreads its
+// kSeqSumBlock elements over all threads, kWalkChunk consecutive elements each.
+constexpr int kWalkThreads = 256;
+constexpr int kWalkWaves = kWalkThreads / 64;
+constexpr int kWalkChunk = kSeqSumBlock / kWalkThreads;
+constexpr int kNone = 1 << 30;
+
+struct WalkShared {
+    double wave_total[kWalkWaves];
+    int first_over[kWalkWaves];
+    float sum;
+};
+
+// Adds one block element by element in effect: thread t holds chunk t (kWalkChunk consecutive
 // elements). Chunks that keep the running sum inside its binade are added through a scan of
 // their rounded contributions; the chunk where it leaves the binade (or where s leaves 0) is
-// added by its own lane in true FP32, in order; so is a chunk holding a rounding tie.
+// added by its own thread in true FP32, in order; so is a chunk holding a rounding tie. Every
+// thread of the workgroup calls it with the same r and leaves with the same r.
 __device__ void replay_block(const float* __restrict__ base, float sc, int64_t n, int bl, int64_t block, Run& r,
-                             int lane) {
-    const int64_t start = block * bl + int64_t(lane) * kChunk;
+                             WalkShared& sh) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t start = block * bl + int64_t(tid) * kWalkChunk;
     const int64_t end = min(n, block * bl + bl);
-    float v[kChunk];
-    if (start + kChunk <= end && (reinterpret_cast<uintptr_t>(base + start) & 15) == 0) {
+    float v[kWalkChunk];
+    if (start + kWalkChunk <= end && (reinterpret_cast<uintptr_t>(base + start) & 15) == 0) {
         const float4* p4 = reinterpret_cast<const float4*>(base + start);
 #pragma unroll
-        for (int j = 0; j < kChunk / 4; ++j) {
+        for (int j = 0; j < kWalkChunk / 4; ++j) {
             const float4 q = p4[j];
             v[4 * j] = q.x; v[4 * j + 1] = q.y; v[4 * j + 2] = q.z; v[4 * j + 3] = q.w;
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < kChunk; ++j) v[j] = start + j < end ? base[start + j] : 0.0f;
+        for (int j = 0; j < kWalkChunk; ++j) v[j] = start + j < end ? base[start + j] : 0.0f;
     }
     if (sc != 1.0f) {
 #pragma unroll
-        for (int j = 0; j < kChunk; ++j) v[j] *= sc;
+        for (int j = 0; j < kWalkChunk; ++j) v[j] *= sc;
     }
     int first = 0;
-    while (first < 64) {
-        double c = lane >= first ? chunk_contribution(v, r.e) : 0.0;
+    while (first < kWalkThreads) {
+        double c = tid >= first ? chunk_contribution<kWalkChunk>(v, r.e) : 0.0;
         const bool tie = c == kTied;
         if (tie) c = 0.0;
-        const double incl = wave_inclusive_scan(c, lane);
-        const bool over = tie || (lane >= first && (r.e == kNoBase ? incl > 0.0 : r.s + incl >= binade_limit(r.e)));
+        // workgroup inclusive scan: within the wave, then the totals of the waves before
+        double incl = wave_inclusive_scan(c, lane);
+        if (lane == 63) sh.wave_total[wave] = incl;
+        __syncthreads();
+        double total = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWalkWaves; ++w) {
+            if (w < wave) incl += sh.wave_total[w];
+            total += sh.wave_total[w];
+        }
+        const bool over = tie || (tid >= first && (r.e == kNoBase ? incl > 0.0 : r.s + incl >= binade_limit(r.e)));
         const unsigned long long mask = __ballot(over);
-        if (mask == 0ull) {
-            r.s += __shfl(incl, 63);
+        if (lane == 0) sh.first_over[wave] = mask ? wave * 64 + __ffsll((long long)mask) - 1 : kNone;
+        __syncthreads();
+        int l = kNone;
+#pragma unroll
+        for (int w = 0; w < kWalkWaves; ++w) l = min(l, sh.first_over[w]);
+        if (l == kNone) {
+            r.s += total;
             return;
         }
-        const int l = __ffsll((long long)mask) - 1;
-        const double before = __shfl(incl - c, l);
-        float sf = float(r.s + before);  // exact: multiples of u below the binade limit
-        if (lane == l) {
+        if (tid == l) {
+            float sf = float(r.s + (incl - c));  // exact: multiples of u below the binade limit
 #pragma unroll
-            for (int j = 0; j < kChunk; ++j) sf = sf + v[j];
+            for (int j = 0; j < kWalkChunk; ++j) sf = sf + v[j];
+            sh.sum = sf;
         }
-        sf = __shfl(sf, l);
+        __syncthreads();
+        const float sf = sh.sum;
         r.s = double(sf);
         r.e = sf > 0.0f ? ilogb(sf) : kNoBase;
         first = l + 1;
     }
 }
 
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(kWalkThreads)
 walk_kernel(SeqSegments g, int64_t n, int bl, int nb, const double* __restrict__ bsum, const int* __restrict__ ebase,
             const double* __restrict__ bins, float* __restrict__ out) {
+    __shared__ WalkShared sh;
     const int seg = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const float* base = g.base(seg);
     const float sc = g.scale(seg);
     const double* bs = bsum + int64_t(seg) * nb;
@@ -307,7 +339,7 @@ walk_kernel(SeqSegments g, int64_t n, int bl, int nb, const double* __restrict__
             }
             const int l = __ffsll((long long)mask) - 1;
             r.s += __shfl(incl - col, l);
-            replay_block(base, sc, n, bl, g0 + l, r, lane);
+            replay_block(base, sc, n, bl, g0 + l, r, sh);
             first = l + 1;
         }
         cur_sum = nxt_sum;
@@ -315,7 +347,7 @@ walk_kernel(SeqSegments g, int64_t n, int bl, int nb, const double* __restrict__
 #pragma unroll
         for (int q = 0; q < kBins; ++q) cur_b[q] = nxt_b[q];
     }
-    if (lane == 0) out[seg] = float(r.s);
+    if (threadIdx.x == 0) out[seg] = float(r.s);
 }
 
 __global__ void scale_by_inverse_kernel(float* __restrict__ x, int64_t n, int64_t stride,
@@ -379,8 +411,8 @@ void SeqSum::run(hipStream_t s, const SeqSegments& g, int64_t n, int segments, f
     {
         // reads the block tables and replays a few blocks: latency-bound, no byte model
         ProfileScope ps("seqsum-walk", 0.0);
-        hipLaunchKernelGGL(walk_kernel, dim3(segments), dim3(64), 0, s, g, n, block, nb, bsum, ebase_.as<int>(),
-                           bins_.as<double>(), totals);
+        hipLaunchKernelGGL(walk_kernel, dim3(segments), dim3(kWalkThreads), 0, s, g, n, block, nb, bsum,
+                           ebase_.as<int>(), bins_.as<double>(), totals);
     }
     LDH_CHECK_LAUNCH("seqsum-launch");
 }

@@ -240,8 +240,9 @@ void Server::worker(Slot* s) {
                         const bool sty = with_style(r->params);
                         const SpectrumConfig cfg = build_spectrum_config(r->params, bank->wavelengths);
                         e->combined.reset(sizeof(float) * size_t(bank->kernel) * bank->kernel * 6);
-                        combine_kernels(s->stream, bank->kernel_ptr(LDH_KERNEL_FULL, 0), bank->bins, bank->kernel, cfg,
-                                        sty, e->combined.as<float>(), s->tmp);
+                        combine_kernels(s->stream, bank->kernel_ptr(LDH_KERNEL_FULL, 0),
+                                        bank->scale_ptr(LDH_KERNEL_FULL, 0), bank->bins, bank->kernel, cfg, sty,
+                                        e->combined.as<float>(), s->tmp);
                         e->combined_style = sty;
                     }
                     LDH_HIP(hipStreamSynchronize(s->stream), "served-bank");
@@ -291,12 +292,12 @@ void Server::worker(Slot* s) {
                     });
                 }
                 if (mine) {
-                    std::vector<const float*> ks;
+                    std::vector<KernelRef> ks;
                     if (nk == 1) {
-                        ks.push_back(bank.kernel_ptr(LDH_KERNEL_FULL, 0));
+                        ks.push_back({bank.kernel_ptr(LDH_KERNEL_FULL, 0), bank.scale_ptr(LDH_KERNEL_FULL, 0)});
                     } else {
                         const size_t kk = size_t(bank.kernel) * bank.kernel;
-                        for (int i = 0; i < nk; ++i) ks.push_back(e->combined.as<float>() + kk * i);
+                        for (int i = 0; i < nk; ++i) ks.push_back({e->combined.as<float>() + kk * i, nullptr});
                     }
                     build_kernel_spectra(s->stream, fft_, ks, bank.kernel, bank.support, g, sk.fp16 != 0, *spec, s->tmp);
                     LDH_HIP(hipStreamSynchronize(s->stream), "served-spectra");
