@@ -145,6 +145,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-form measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times (HBM counter calibration) and exit")
     ap.add_argument("--bank-probe", action="store_true",
@@ -317,7 +318,8 @@ def main():
         "kernel_families_served": families(served_stats),
     }
     progress("latency pass done")
-    out["batched"] = batched_form(ctx, progress)
+    if not args.no_batched:
+        out["batched"] = batched_form(ctx, progress)
     if not args.no_cpu_baseline:
         progress("cpu baseline (reference, one core)")
         out["cpu_baseline"] = cpu_baseline(L(10), w, h, 31)

@@ -138,6 +138,9 @@ template <int M, int B = 16> struct LinePlan {
 // Stage policies. A stage reads elements base + r * STEP through in.get<STEP>(base, pbase, r)
 // and writes them through out.put<STEP>(base, pbase, r, v), with pbase = lds_pad(base); `lds`
 // tells whether a policy touches the line in LDS. Global policies only need operator().
+// An output policy with `prefetch` multiplies each element by a per-element constant from global
+// memory: the stage loads the constants (pre<STEP>(base, r)) before its butterflies, so their
+// latency overlaps the arithmetic, and writes with put_pre<STEP>(base, pbase, r, v, c).
 template <class F> struct GlobalIn {
     static constexpr bool lds = false;
     F f;
@@ -145,8 +148,13 @@ template <class F> struct GlobalIn {
 };
 template <class F> struct GlobalOut {
     static constexpr bool lds = false;
+    static constexpr bool prefetch = F::kPrefetch;
     F f;
     template <int STEP> __device__ void put(int base, int, int r, float2 v) const { f(base + r * STEP, v); }
+    template <int STEP> __device__ float2 pre(int base, int r) const { return f.pre(base + r * STEP); }
+    template <int STEP> __device__ void put_pre(int base, int, int r, float2 v, float2 c) const {
+        f.put_pre(base + r * STEP, v, c);
+    }
 };
 struct LdsIn {
     static constexpr bool lds = true;
@@ -157,6 +165,7 @@ struct LdsIn {
 };
 struct LdsOut {
     static constexpr bool lds = true;
+    static constexpr bool prefetch = false;
     float2* line;
     template <int STEP> __device__ void put(int base, int pbase, int r, float2 v) const {
         line[lds_pad_step<STEP>(base, pbase, r)] = v;
@@ -165,12 +174,19 @@ struct LdsOut {
 // Output multiplied by a spectrum (the Bluestein filter).
 struct LdsMulOut {
     static constexpr bool lds = true;
+    static constexpr bool prefetch = true;
     float2* line;
     const float2* __restrict__ f;
     template <int STEP> __device__ void put(int base, int pbase, int r, float2 v) const {
         line[lds_pad_step<STEP>(base, pbase, r)] = cmul(v, f[base + r * STEP]);
     }
+    template <int STEP> __device__ float2 pre(int base, int r) const { return f[base + r * STEP]; }
+    template <int STEP> __device__ void put_pre(int base, int pbase, int r, float2 v, float2 c) const {
+        line[lds_pad_step<STEP>(base, pbase, r)] = cmul(v, c);
+    }
 };
+
+__host__ __device__ constexpr int top_bit(int r) { return r >= 16 ? 16 : r >= 8 ? 8 : r >= 4 ? 4 : r >= 2 ? 2 : 1; }
 
 // One Stockham stage over a single line of N points: sub-length Ns, radix R.
 template <int N, int Ns, int R, int NT, bool INV, class In, class Out>
@@ -193,21 +209,39 @@ __device__ __forceinline__ void line_stage(const float2* __restrict__ tw, int ti
         const int u = tid + p * NT;
         if (BF % NT == 0 || u < BF) {
             const int k = u % Ns;
+            const int base = (u - k) * R + k;
+            float2 c[Out::prefetch ? R : 1];
+            if constexpr (Out::prefetch) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) c[r] = out.template pre<Ns>(base, r);
+            }
             if constexpr (Ns > 1) {
                 constexpr int stride = tw_table(N) / (Ns * R);
                 const float2* __restrict__ t = tw + tw_offset(N);
+                static_assert((R & (R - 1)) == 0, "tail stages are radix 2^b");
+                // W^(k r) for r = 2^b from the table (one load each, issued together), the other
+                // powers as products W^(k r) = W^(k (r - h)) W^(k h), h the top bit of r: at most
+                // three rounded products per twiddle.
+                float2 w[R];
+#pragma unroll
+                for (int h = 1; h < R; h *= 2) {
+                    w[h] = t[k * h * stride];
+                    if (INV) w[h].y = -w[h].y;
+                }
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    float2 w = t[k * r * stride];
-                    if (INV) w.y = -w.y;
-                    v[p][r] = cmul(v[p][r], w);
+                    const int h = top_bit(r);
+                    if (r != h) w[r] = cmul(w[r - h], w[h]);
+                    v[p][r] = cmul(v[p][r], w[r]);
                 }
             }
             dft_any<R, INV>(v[p]);
-            const int base = (u - k) * R + k;
             const int pbase = lds_pad(base);
 #pragma unroll
-            for (int r = 0; r < R; ++r) out.template put<Ns>(base, pbase, r, v[p][r]);
+            for (int r = 0; r < R; ++r) {
+                if constexpr (Out::prefetch) out.template put_pre<Ns>(base, pbase, r, v[p][r], c[r]);
+                else out.template put<Ns>(base, pbase, r, v[p][r]);
+            }
         }
     }
     if constexpr (Out::lds) __syncthreads();

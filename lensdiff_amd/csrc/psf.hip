@@ -49,48 +49,66 @@ struct RowInput {
     }
 };
 
-// Stage-0 input of the column transform of frequency column kx: the P row-transform values.
+// Stage-0 input of the column transform of frequency row ky: the P pass-1 values, which already
+// carry the chirp of their position (the row pass applies it). Branch-free: every lane loads
+// (out-of-range positions read element 0) so the stage's loads issue back to back.
 struct ColInput {
     const float2* __restrict__ col;
-    const float2* __restrict__ chirp;
-    int P, off, bluestein;
+    int P, off;
     __device__ float2 operator()(int j) const {
         const int y = j - off;
-        if (y < 0 || y >= P) return make_float2(0.0f, 0.0f);
-        float2 v = col[y];
-        if (bluestein) v = cmul(v, chirp[y]);
-        return v;
+        const bool in = y >= 0 && y < P;
+        const float2 v = col[in ? y : 0];
+        return in ? v : make_float2(0.0f, 0.0f);
     }
 };
 
 // Output k < R of a line: Bluestein unchirps and scales by 1/M.
+// with_chirp: the same with chirp[k] already loaded (c).
 struct LineValue {
     const float2* __restrict__ chirp;
     float inv_m;
     int bluestein;
-    __device__ float2 operator()(int k, float2 v) const {
-        if (bluestein) v = cmul(make_float2(v.x * inv_m, v.y * inv_m), chirp[k]);
+    __device__ float2 operator()(int k, float2 v) const { return with_chirp(v, bluestein ? chirp[k] : v); }
+    __device__ float2 with_chirp(float2 v, float2 c) const {
+        if (bluestein) v = cmul(make_float2(v.x * inv_m, v.y * inv_m), c);
         return v;
     }
+    // chirp[k] for outputs k < R (k >= R: never used, reads element 0)
+    __device__ float2 pre(int k, int R) const { return bluestein ? chirp[k < R ? k : 0] : make_float2(1.0f, 0.0f); }
 };
+// Output k < R of a row line, times `post` (the column transform's chirp at this line's
+// position, or 1).
 struct RowOutput {
+    static constexpr bool kPrefetch = true;
     float2* __restrict__ out;
     LineValue val;
     int R;
-    __device__ void operator()(int k, float2 v) const {
-        if (k < R) out[k] = val(k, v);
+    float2 post;
+    int bluestein;
+    __device__ void operator()(int k, float2 v) const { put_pre(k, v, pre(k)); }
+    __device__ float2 pre(int k) const { return val.pre(k, R); }
+    __device__ void put_pre(int k, float2 v, float2 c) const {
+        if (k < R) {
+            float2 f = val.with_chirp(v, c);
+            if (bluestein) f = cmul(f, post);
+            out[k] = f;
+        }
     }
 };
 // |F|^2 of output k into the centred row: column (k - R/2) mod R (fftShiftSquare, :303-314),
 // also added in FP64 to the thread's share of the row sum.
 struct ColOutput {
+    static constexpr bool kPrefetch = true;
     float* __restrict__ row;
     LineValue val;
     int R, shift;  // R - R/2
     double* acc;
-    __device__ void operator()(int k, float2 v) const {
+    __device__ void operator()(int k, float2 v) const { put_pre(k, v, pre(k)); }
+    __device__ float2 pre(int k) const { return val.pre(k, R); }
+    __device__ void put_pre(int k, float2 v, float2 c) const {
         if (k < R) {
-            const float2 f = val(k, v);
+            const float2 f = val.with_chirp(v, c);
             int x = k + shift;
             if (x >= R) x -= R;
             const float p = f.x * f.x + f.y * f.y;
@@ -127,7 +145,8 @@ psf_rows_kernel(const float* __restrict__ amp, const int* __restrict__ amp_index
     const int y = blockIdx.x, lam = blockIdx.y, tid = threadIdx.x;
     const size_t a0 = amp_index != nullptr ? size_t(amp_index[lam]) * P * P : 0;
     const RowInput in{amp + a0 + size_t(y) * P, phase + (size_t(lam) * P + y) * P, chirp, P, off, bluestein};
-    const RowOutput out{G + (size_t(lam) * P + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R};
+    const float2 post = bluestein ? chirp[y] : make_float2(1.0f, 0.0f);
+    const RowOutput out{G + (size_t(lam) * P + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R, post, bluestein};
     line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out);
 }
 
@@ -146,7 +165,7 @@ psf_cols_kernel(const float2* __restrict__ GT, int P, int R, int off, const floa
     int y = ky + shift;
     if (y >= R) y -= R;
     double acc = 0.0;
-    const ColInput in{GT + (size_t(lam) * R + ky) * P, chirp, P, off, bluestein};
+    const ColInput in{GT + (size_t(lam) * R + ky) * P, P, off};
     const ColOutput out{IT + (size_t(lam) * R + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R, shift,
                         &acc};
     line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out);
