@@ -207,6 +207,7 @@ constexpr int kNone = 1 << 30;
 
 struct WalkShared {
     double wave_total[kWalkWaves];
+    double incl;  // scan value of the chunk just resolved element by element
     int first_over[kWalkWaves];
     float sum;
 };
@@ -230,29 +231,46 @@ __device__ void replay_block(const float* __restrict__ base, float sc, int64_t n
             v[4 * j] = q.x; v[4 * j + 1] = q.y; v[4 * j + 2] = q.z; v[4 * j + 3] = q.w;
         }
     } else {
+        // a partial chunk: clamped loads (always in the block) issued together, then masked
 #pragma unroll
-        for (int j = 0; j < kWalkChunk; ++j) v[j] = start + j < end ? base[start + j] : 0.0f;
+        for (int j = 0; j < kWalkChunk; ++j) v[j] = base[min(start + j, end - 1)];
+#pragma unroll
+        for (int j = 0; j < kWalkChunk; ++j)
+            if (start + j >= end) v[j] = 0.0f;
     }
     if (sc != 1.0f) {
 #pragma unroll
         for (int j = 0; j < kWalkChunk; ++j) v[j] *= sc;
     }
+    // Chunk contributions and their scan hold for one binade (ce); a chunk resolved element by
+    // element that leaves the running sum in that binade (a rounding tie) only moves the origin
+    // of the scan (base), so the next chunks need no new contributions or scan.
     int first = 0;
+    int ce = kNoBase + 1;  // no contributions yet (never a binade or kNoBase)
+    double c = 0.0, incl = 0.0, total = 0.0, base_incl = 0.0;
+    bool tie = false;
     while (first < kWalkThreads) {
-        double c = tid >= first ? chunk_contribution<kWalkChunk>(v, r.e) : 0.0;
-        const bool tie = c == kTied;
-        if (tie) c = 0.0;
-        // workgroup inclusive scan: within the wave, then the totals of the waves before
-        double incl = wave_inclusive_scan(c, lane);
-        if (lane == 63) sh.wave_total[wave] = incl;
-        __syncthreads();
-        double total = 0.0;
+        if (r.e != ce) {
+            c = tid >= first ? chunk_contribution<kWalkChunk>(v, r.e) : 0.0;
+            tie = c == kTied;
+            if (tie) c = 0.0;
+            // workgroup inclusive scan: within the wave, then the totals of the waves before
+            incl = wave_inclusive_scan(c, lane);
+            if (lane == 63) sh.wave_total[wave] = incl;
+            __syncthreads();
+            total = 0.0;
 #pragma unroll
-        for (int w = 0; w < kWalkWaves; ++w) {
-            if (w < wave) incl += sh.wave_total[w];
-            total += sh.wave_total[w];
+            for (int w = 0; w < kWalkWaves; ++w) {
+                if (w < wave) incl += sh.wave_total[w];
+                total += sh.wave_total[w];
+            }
+            base_incl = 0.0;
+            ce = r.e;
         }
-        const bool over = tie || (tid >= first && (r.e == kNoBase ? incl > 0.0 : r.s + incl >= binade_limit(r.e)));
+        // the running sum after chunk t (t >= first, nothing over before it) is r.s + rel
+        const double rel = incl - base_incl;
+        const bool over =
+            tid >= first && (tie || (r.e == kNoBase ? rel > 0.0 : r.s + rel >= binade_limit(r.e)));
         const unsigned long long mask = __ballot(over);
         if (lane == 0) sh.first_over[wave] = mask ? wave * 64 + __ffsll((long long)mask) - 1 : kNone;
         __syncthreads();
@@ -260,19 +278,21 @@ __device__ void replay_block(const float* __restrict__ base, float sc, int64_t n
 #pragma unroll
         for (int w = 0; w < kWalkWaves; ++w) l = min(l, sh.first_over[w]);
         if (l == kNone) {
-            r.s += total;
+            r.s += total - base_incl;
             return;
         }
         if (tid == l) {
-            float sf = float(r.s + (incl - c));  // exact: multiples of u below the binade limit
+            float sf = float(r.s + (rel - c));  // exact: multiples of u below the binade limit
 #pragma unroll
             for (int j = 0; j < kWalkChunk; ++j) sf = sf + v[j];
             sh.sum = sf;
+            sh.incl = incl;
         }
         __syncthreads();
         const float sf = sh.sum;
         r.s = double(sf);
         r.e = sf > 0.0f ? ilogb(sf) : kNoBase;
+        base_incl = sh.incl;
         first = l + 1;
     }
 }
@@ -289,17 +309,18 @@ walk_kernel(SeqSegments g, int64_t n, int bl, int nb, const double* __restrict__
     const int* eb = ebase + int64_t(seg) * nb;
     const double* bn = bins + int64_t(seg) * nb * kBins;
     Run r{0.0, kNoBase};
-    // per-lane block data of the current group, loaded one group ahead
+    // per-lane block data of the current group, loaded one group ahead without branches (lanes
+    // past the last block read it and are masked off), so the loads stay in flight while the
+    // current group is walked
     auto load = [&](int blk, double& sum, int& e0, double (&b)[kBins]) {
-        sum = 0.0;
-        e0 = kNoBase;
-        if (blk < nb) {
-            sum = bs[blk];
-            e0 = eb[blk];
-            if (e0 != kNoBase) {
+        const int c = min(blk, nb - 1);
+        sum = bs[c];
+        e0 = eb[c];
 #pragma unroll
-                for (int q = 0; q < kBins; ++q) b[q] = bn[int64_t(blk) * kBins + q];
-            }
+        for (int q = 0; q < kBins; ++q) b[q] = bn[int64_t(c) * kBins + q];
+        if (blk >= nb) {
+            sum = 0.0;
+            e0 = kNoBase;
         }
     };
     double cur_sum, nxt_sum;
