@@ -119,13 +119,24 @@ ring_stats_kernel(const float* __restrict__ img, int size, float center, int rma
     const float sc = inverse_total(prescale, seg);
     const int64_t p0 = int64_t(blockIdx.x) * kRingPixelsPerBlock;
     const int64_t p1 = min(npx, p0 + kRingPixelsPerBlock);
-    for (int64_t p = p0 + threadIdx.x; p < p1; p += kRingThreads) {
-        const int x = int(p % size), y = int(p / size);
-        const int r = ring_of<CEIL>(x, y, center, rmax);
-        const float v = base[p] * sc;
-        atomicAdd(&s_sum[r], double(v));
-        if (counts) atomicAdd(&s_cnt[r], 1);
-        if (peaks) atomicMax(&s_pk[r], __float_as_uint(v));  // v >= 0: bit order is value order
+    // kBatch pixels per thread per step: their loads are issued together (clamped inside the
+    // workgroup's range), then each goes to its ring
+    constexpr int kBatch = 8;
+    for (int64_t q = p0 + threadIdx.x; q < p1; q += int64_t(kBatch) * kRingThreads) {
+        float raw[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) raw[k] = base[min(q + int64_t(k) * kRingThreads, p1 - 1)];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            const int64_t p = q + int64_t(k) * kRingThreads;
+            if (p >= p1) break;
+            const int x = int(p % size), y = int(p / size);
+            const int r = ring_of<CEIL>(x, y, center, rmax);
+            const float v = raw[k] * sc;
+            atomicAdd(&s_sum[r], double(v));
+            if (counts) atomicAdd(&s_cnt[r], 1);
+            if (peaks) atomicMax(&s_pk[r], __float_as_uint(v));  // v >= 0: bit order is value order
+        }
     }
     __syncthreads();
     for (int r = threadIdx.x; r < nr; r += kRingThreads) {

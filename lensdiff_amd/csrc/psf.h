@@ -11,8 +11,9 @@ constexpr int kMaxRawPsf = 8192;  // its Bluestein length (R + P - 1 <= 9215) fi
 
 // Stage buffers of the two line passes (shareable by plans of different sizes).
 struct PsfScratch {
-    DeviceBuffer g, gt, row_sums;
+    DeviceBuffer field, g, gt, row_sums;
     void reserve(int pupil, int raw, int count) {
+        field.reset(sizeof(float2) * size_t(pupil) * pupil * count);
         g.reset(sizeof(float2) * size_t(pupil) * raw * count);
         gt.reset(sizeof(float2) * size_t(pupil) * raw * count);
         row_sums.reset(sizeof(double) * size_t(raw) * count);

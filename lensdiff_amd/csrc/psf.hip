@@ -5,9 +5,9 @@
 // floor((R-P)/2)), so it is computed as two separable line stages, columns first:
 //   pass 1: for each pupil column x, the R-point DFT of its P embedded values -> H[x][ky]
 //   pass 2: for each ky, the R-point DFT over the P columns of H             -> F[ky][kx]
-// The pupil amplitude and phase are generated transposed ([x][y]) so pass 1 reads contiguous
-// lines, and pass 2's line is one row of F, whose |F|^2 lands straight in its row of the
-// centred raw PSF.
+// The pupil amplitude and phase are generated transposed ([x][y]) and turned into the complex
+// field (times the chirp) by one elementwise kernel, so pass 1 reads contiguous lines; pass 2's
+// line is one row of F, whose |F|^2 lands straight in its row of the centred raw PSF.
 // Each R-point DFT is done in LDS. For power-of-two R it is a plain Stockham FFT of length R.
 // Otherwise it is Bluestein's algorithm with the reference's chirp exp(-i pi j^2 / R) evaluated
 // in double with pi rounded to float (LensDiffCpuReference.cpp:244-274), which makes the
@@ -32,20 +32,36 @@ namespace {
 
 // Stage-0 input of the row transform of pupil row y: A * (cos 2 pi phi, sin 2 pi phi) where
 // A > 0, at line position x + off, times the chirp for Bluestein.
+// The complex pupil field of row-line y at position x: A * (cos 2 pi phi, sin 2 pi phi) where
+// A > 0, times the chirp for Bluestein (one elementwise kernel for all wavelengths).
+__global__ void pupil_field_kernel(const float* __restrict__ amp, const int* __restrict__ amp_index,
+                                   const float* __restrict__ phase, int P, const float2* __restrict__ chirp,
+                                   int bluestein, float2* __restrict__ field) {
+    const int lam = blockIdx.y;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t pp = int64_t(P) * P;
+    if (i >= pp) return;
+    const int64_t a0 = amp_index != nullptr ? int64_t(amp_index[lam]) * pp : 0;
+    const int x = int(i % P);
+    const float a = amp[a0 + i];
+    float rad = 0.0f;
+    if (a > 0.0f) rad = phase[int64_t(lam) * pp + i] * 2.0f * kPiF;
+    float2 v = make_float2(a * cosf(rad), a * sinf(rad));
+    if (bluestein) v = cmul(v, chirp[x]);
+    field[int64_t(lam) * pp + i] = v;
+}
+
+// Stage-0 input of the row transform of row-line y: its P field values at positions j - off.
+// Branch-free: every lane loads (out-of-range positions read element 0) so the stage's loads
+// issue back to back.
 struct RowInput {
-    const float* __restrict__ arow;
-    const float* __restrict__ prow;
-    const float2* __restrict__ chirp;
-    int P, off, bluestein;
+    const float2* __restrict__ row;
+    int P, off;
     __device__ float2 operator()(int j) const {
         const int x = j - off;
-        if (x < 0 || x >= P) return make_float2(0.0f, 0.0f);
-        const float a = arow[x];
-        float rad = 0.0f;
-        if (a > 0.0f) rad = prow[x] * 2.0f * kPiF;
-        float2 v = make_float2(a * cosf(rad), a * sinf(rad));
-        if (bluestein) v = cmul(v, chirp[x]);
-        return v;
+        const bool in = x >= 0 && x < P;
+        const float2 v = row[in ? x : 0];
+        return in ? v : make_float2(0.0f, 0.0f);
     }
 };
 
@@ -138,13 +154,11 @@ __device__ __forceinline__ void line_transform(float2* lds, const float2* __rest
 // allocation must allow (1: no constraint). The autotuner picks (B, W) for long lines.
 template <int M, int B, int W>
 __global__ void __launch_bounds__((LinePlan<M, B>::NT), W)
-psf_rows_kernel(const float* __restrict__ amp, const int* __restrict__ amp_index, const float* __restrict__ phase,
-                int P, int R, int off, const float2* __restrict__ chirp, const float2* __restrict__ bhat,
-                int bluestein, float2* __restrict__ G, const float2* __restrict__ tw) {
+psf_rows_kernel(const float2* __restrict__ field, int P, int R, int off, const float2* __restrict__ chirp,
+                const float2* __restrict__ bhat, int bluestein, float2* __restrict__ G, const float2* __restrict__ tw) {
     extern __shared__ float2 lds[];
     const int y = blockIdx.x, lam = blockIdx.y, tid = threadIdx.x;
-    const size_t a0 = amp_index != nullptr ? size_t(amp_index[lam]) * P * P : 0;
-    const RowInput in{amp + a0 + size_t(y) * P, phase + (size_t(lam) * P + y) * P, chirp, P, off, bluestein};
+    const RowInput in{field + (size_t(lam) * P + y) * P, P, off};
     const float2 post = bluestein ? chirp[y] : make_float2(1.0f, 0.0f);
     const RowOutput out{G + (size_t(lam) * P + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R, post, bluestein};
     line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out);
@@ -298,11 +312,11 @@ constexpr int kPsfVariants = 5;
 template <int M> constexpr int psf_variants() { return M >= 2048 ? kPsfVariants : 1; }
 
 template <int M, int B, int W>
-void launch_rows_v(hipStream_t s, const float* amp, const int* amp_index, const float* phase, int P, int R, int off,
-                   const float2* chirp, const float2* bhat, int bluestein, float2* G, const float2* tw, int L) {
+void launch_rows_v(hipStream_t s, const float2* field, int P, int R, int off, const float2* chirp, const float2* bhat,
+                   int bluestein, float2* G, const float2* tw, int L) {
     allow_large_lds(reinterpret_cast<const void*>(&psf_rows_kernel<M, B, W>), (M + M / 32 + 1) * 8);
-    hipLaunchKernelGGL((psf_rows_kernel<M, B, W>), dim3(P, L), dim3(LinePlan<M, B>::NT), (M + M / 32 + 1) * 8, s, amp,
-                       amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw);
+    hipLaunchKernelGGL((psf_rows_kernel<M, B, W>), dim3(P, L), dim3(LinePlan<M, B>::NT), (M + M / 32 + 1) * 8, s, field,
+                       P, R, off, chirp, bhat, bluestein, G, tw);
 }
 template <int M, int B, int W>
 void launch_cols_v(hipStream_t s, const float2* GT, int P, int R, int off, const float2* chirp, const float2* bhat,
@@ -317,17 +331,17 @@ std::string tune_key(const char* kind, int M, int L) {
 }
 
 template <int M>
-bool launch_rows(int m, hipStream_t s, const float* amp, const int* amp_index, const float* phase, int P, int R,
-                 int off, const float2* chirp, const float2* bhat, int bluestein, float2* G, const float2* tw, int L) {
+bool launch_rows(int m, hipStream_t s, const float2* field, int P, int R, int off, const float2* chirp,
+                 const float2* bhat, int bluestein, float2* G, const float2* tw, int L) {
     if (m != M) return false;
     Autotuner::instance().run(tune_key("psf-rows", M, L), psf_variants<M>(), s, [&](int v) {
         if constexpr (psf_variants<M>() > 1) {
-            if (v == 1) return launch_rows_v<M, 16, 3>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
-            if (v == 2) return launch_rows_v<M, 16, 4>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
-            if (v == 3) return launch_rows_v<M, 8, 1>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
-            if (v == 4) return launch_rows_v<M, 8, 3>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
+            if (v == 1) return launch_rows_v<M, 16, 3>(s, field, P, R, off, chirp, bhat, bluestein, G, tw, L);
+            if (v == 2) return launch_rows_v<M, 16, 4>(s, field, P, R, off, chirp, bhat, bluestein, G, tw, L);
+            if (v == 3) return launch_rows_v<M, 8, 1>(s, field, P, R, off, chirp, bhat, bluestein, G, tw, L);
+            if (v == 4) return launch_rows_v<M, 8, 3>(s, field, P, R, off, chirp, bhat, bluestein, G, tw, L);
         }
-        launch_rows_v<M, 16, 1>(s, amp, amp_index, phase, P, R, off, chirp, bhat, bluestein, G, tw, L);
+        launch_rows_v<M, 16, 1>(s, field, P, R, off, chirp, bhat, bluestein, G, tw, L);
     });
     return true;
 }
@@ -433,9 +447,16 @@ void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, cons
     const bool row_blocks = R >= kSeqSumMinBlock && R <= kSeqSumBlock;
     double* row_sums = row_blocks ? scratch.row_sums.as<double>() : nullptr;
     {
-        // algorithmic bytes: amplitude and phase in, row transforms out
+        // amplitude (shared by the wavelengths of a pupil) and phase in, complex field out
+        ProfileScope ps("psf-field", 12.0 * P * P * L);
+        hipLaunchKernelGGL(pupil_field_kernel, dim3(unsigned(div_up(int64_t(P) * P, 256)), unsigned(L)), dim3(256), 0, s,
+                           amp, amp_index, phase, P, chirp, blu, scratch.field.as<float2>());
+        LDH_CHECK_LAUNCH("psf-field");
+    }
+    {
+        // algorithmic bytes: the field in, row transforms out
         ProfileScope ps("psf-rows", (8.0 * P * P + 8.0 * P * R) * L);
-#define LDH_PSF_ROWS(MM) launch_rows<MM>(m_, s, amp, amp_index, phase, P, R, offset_, chirp, bhat, blu, g_.as<float2>(), tw, L)
+#define LDH_PSF_ROWS(MM) launch_rows<MM>(m_, s, scratch.field.as<float2>(), P, R, offset_, chirp, bhat, blu, g_.as<float2>(), tw, L)
         if (!(LDH_PSF_LENGTHS(LDH_PSF_ROWS))) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
 #undef LDH_PSF_ROWS
         LDH_CHECK_LAUNCH("psf-rows");
