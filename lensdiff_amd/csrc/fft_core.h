@@ -185,11 +185,28 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
             if constexpr (Ns > 1) {
                 constexpr int stride = tw_table(N) / (Ns * R);
                 const float2* __restrict__ t = tw + tw_offset(N);
+                if constexpr ((R & (R - 1)) == 0) {
+                    // W^(k r) for r = 2^b from the table (loads issued together), the other powers
+                    // as products W^(k (r - h)) W^(k h), h the top bit of r
+                    float2 w[R];
 #pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    float2 w = t[k * r * stride];
-                    if (INV) w.y = -w.y;
-                    v[p][r] = cmul(v[p][r], w);
+                    for (int h = 1; h < R; h *= 2) {
+                        w[h] = t[k * h * stride];
+                        if (INV) w[h].y = -w[h].y;
+                    }
+#pragma unroll
+                    for (int r = 1; r < R; ++r) {
+                        const int h = r >= 8 ? 8 : r >= 4 ? 4 : r >= 2 ? 2 : 1;
+                        if (r != h) w[r] = cmul(w[r - h], w[h]);
+                        v[p][r] = cmul(v[p][r], w[r]);
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 1; r < R; ++r) {
+                        float2 w = t[k * r * stride];
+                        if (INV) w.y = -w.y;
+                        v[p][r] = cmul(v[p][r], w);
+                    }
                 }
             }
             dft<R, INV>(v[p]);
@@ -245,11 +262,32 @@ fft_tile_kernel(IO io, const float2* __restrict__ tw, int ntiles) {
     extern __shared__ float2 lds[];
     const int tile = xcd_tile(blockIdx.x, ntiles);
     const int tid = threadIdx.x;
-    for (int e = tid; e < E; e += NT) {
-        int line, i;
-        if constexpr (ROWS) { line = e / N; i = e % N; }
-        else { line = e % NL; i = e / NL; }
-        lds[lds_pad(line * LS + i * IS)] = io.load(tile, line, i);
+    // loads in batches of up to kBatch per thread, all issued before their LDS writes
+    constexpr int IT = (E + NT - 1) / NT;
+    constexpr int kBatch = IT < 16 ? IT : 16;
+#pragma unroll
+    for (int c0 = 0; c0 < IT; c0 += kBatch) {
+        float2 t[kBatch];
+#pragma unroll
+        for (int c = 0; c < kBatch; ++c) {
+            const int e = tid + (c0 + c) * NT;
+            if ((E % NT == 0 && IT % kBatch == 0) || (c0 + c < IT && e < E)) {
+                int line, i;
+                if constexpr (ROWS) { line = e / N; i = e % N; }
+                else { line = e % NL; i = e / NL; }
+                t[c] = io.load(tile, line, i);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kBatch; ++c) {
+            const int e = tid + (c0 + c) * NT;
+            if ((E % NT == 0 && IT % kBatch == 0) || (c0 + c < IT && e < E)) {
+                int line, i;
+                if constexpr (ROWS) { line = e / N; i = e % N; }
+                else { line = e % NL; i = e / NL; }
+                lds[lds_pad(line * LS + i * IS)] = t[c];
+            }
+        }
     }
     __syncthreads();
     fft_stages<N, 1, NL, NT, INV, LS, IS>(lds, tw, tid);
