@@ -171,21 +171,6 @@ struct LdsOut {
         line[lds_pad_step<STEP>(base, pbase, r)] = v;
     }
 };
-// Output multiplied by a spectrum (the Bluestein filter).
-struct LdsMulOut {
-    static constexpr bool lds = true;
-    static constexpr bool prefetch = true;
-    float2* line;
-    const float2* __restrict__ f;
-    template <int STEP> __device__ void put(int base, int pbase, int r, float2 v) const {
-        line[lds_pad_step<STEP>(base, pbase, r)] = cmul(v, f[base + r * STEP]);
-    }
-    template <int STEP> __device__ float2 pre(int base, int r) const { return f[base + r * STEP]; }
-    template <int STEP> __device__ void put_pre(int base, int pbase, int r, float2 v, float2 c) const {
-        line[lds_pad_step<STEP>(base, pbase, r)] = cmul(v, c);
-    }
-};
-
 __host__ __device__ constexpr int top_bit(int r) { return r >= 16 ? 16 : r >= 8 ? 8 : r >= 4 ? 4 : r >= 2 ? 2 : 1; }
 
 // One Stockham stage over a single line of N points: sub-length Ns, radix R.
@@ -257,6 +242,29 @@ __device__ __forceinline__ void line_tail(const float2* __restrict__ tw, int tid
         line_stage<M, Ns, B, NT, INV>(tw, tid, li, lo);
         line_tail<M, B, Ns * B, NT, INV>(tw, tid, li, lo, out);
     }
+}
+
+// line[i] *= f[i] over the M points of a line in LDS (padded), NT threads: every thread's
+// loads are issued before its multiplies (a pass of its own, when the transforms' registers are
+// free). Ends with a barrier.
+template <int M, int NT>
+__device__ __forceinline__ void line_multiply(float2* line, const float2* __restrict__ f, int tid) {
+    constexpr int IT = (M + NT - 1) / NT;
+    float2 c[IT], v[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int i = tid + it * NT;
+        if (M % NT == 0 || i < M) {
+            c[it] = f[i];
+            v[it] = line[lds_pad(i)];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int i = tid + it * NT;
+        if (M % NT == 0 || i < M) line[lds_pad(i)] = cmul(v[it], c[it]);
+    }
+    __syncthreads();
 }
 
 // The whole line transform: stage 0 reads through `in`, the last stage writes through `out`,

@@ -135,7 +135,7 @@ struct ColOutput {
 };
 
 // The R-point transform of one line: a length-M FFT for power-of-two R, else Bluestein's
-// forward FFT, filter multiply (fused into its last stage) and inverse FFT.
+// forward FFT, filter multiply (a pass over the line in LDS) and inverse FFT.
 template <int M, int B, class In, class Out>
 __device__ __forceinline__ void line_transform(float2* lds, const float2* __restrict__ tw,
                                                const float2* __restrict__ bhat, bool bluestein, int tid,
@@ -143,7 +143,8 @@ __device__ __forceinline__ void line_transform(float2* lds, const float2* __rest
     const GlobalIn<In> gin{in};
     const GlobalOut<Out> gout{out};
     if (bluestein) {
-        line_fft<M, B, false>(lds, tw, tid, gin, LdsMulOut{lds, bhat});
+        line_fft<M, B, false>(lds, tw, tid, gin, LdsOut{lds});
+        line_multiply<M, LinePlan<M, B>::NT>(lds, bhat, tid);
         line_fft<M, B, true>(lds, tw, tid, LdsIn{lds}, gout);
     } else {
         line_fft<M, B, false>(lds, tw, tid, gin, gout);
