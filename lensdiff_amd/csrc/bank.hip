@@ -99,7 +99,8 @@ __device__ __forceinline__ int ring_of(int x, int y, float c, int rmax) {
 constexpr int kRingThreads = 256;
 constexpr int kRingPixelsPerBlock = 16384;
 
-// sums[r] += v, counts[r] += 1, peaks[r] = max(v) over the pixels of ring r, per segment.
+// sums[r] += v, peaks[r] = max(v) over the pixels of ring r, per segment; counts[r] += 1 for
+// segment 0 only (ring sizes are the same for every segment of one size).
 // prescale (nullable): per-segment totals; pixels are read as v * (1 / total), the values an
 // in-place normalisation would have left.
 template <bool CEIL>
@@ -117,32 +118,38 @@ ring_stats_kernel(const float* __restrict__ img, int size, float center, int rma
     const int64_t npx = int64_t(size) * size;
     const float* base = img + seg * npx;
     const float sc = inverse_total(prescale, seg);
-    const int64_t p0 = int64_t(blockIdx.x) * kRingPixelsPerBlock;
-    const int64_t p1 = min(npx, p0 + kRingPixelsPerBlock);
+    const int p0 = blockIdx.x * kRingPixelsPerBlock;  // size^2 < 2^31
+    const int p1 = int(min(npx, int64_t(p0) + kRingPixelsPerBlock));
+    const bool count = counts != nullptr && seg == 0;
+    // the thread's pixel as (x, y), advanced by kRingThreads per step without divisions
+    int x = (p0 + int(threadIdx.x)) % size, y = (p0 + int(threadIdx.x)) / size;
     // kBatch pixels per thread per step: their loads are issued together (clamped inside the
     // workgroup's range), then each goes to its ring
     constexpr int kBatch = 8;
-    for (int64_t q = p0 + threadIdx.x; q < p1; q += int64_t(kBatch) * kRingThreads) {
+    for (int q = p0 + int(threadIdx.x); q < p1; q += kBatch * kRingThreads) {
         float raw[kBatch];
 #pragma unroll
-        for (int k = 0; k < kBatch; ++k) raw[k] = base[min(q + int64_t(k) * kRingThreads, p1 - 1)];
+        for (int k = 0; k < kBatch; ++k) raw[k] = base[min(q + k * kRingThreads, p1 - 1)];
 #pragma unroll
         for (int k = 0; k < kBatch; ++k) {
-            const int64_t p = q + int64_t(k) * kRingThreads;
-            if (p >= p1) break;
-            const int x = int(p % size), y = int(p / size);
+            if (q + k * kRingThreads >= p1) break;
             const int r = ring_of<CEIL>(x, y, center, rmax);
             const float v = raw[k] * sc;
             atomicAdd(&s_sum[r], double(v));
-            if (counts) atomicAdd(&s_cnt[r], 1);
+            if (count) atomicAdd(&s_cnt[r], 1);
             if (peaks) atomicMax(&s_pk[r], __float_as_uint(v));  // v >= 0: bit order is value order
+            x += kRingThreads;
+            while (x >= size) {
+                x -= size;
+                ++y;
+            }
         }
     }
     __syncthreads();
     for (int r = threadIdx.x; r < nr; r += kRingThreads) {
         if (s_cnt[r] == 0 && s_sum[r] == 0.0 && s_pk[r] == 0u) continue;
         atomicAdd(&sums[int64_t(seg) * nr + r], s_sum[r]);
-        if (counts) atomicAdd(&counts[int64_t(seg) * nr + r], s_cnt[r]);
+        if (count) atomicAdd(&counts[r], s_cnt[r]);
         if (peaks) atomicMax(&peaks[int64_t(seg) * nr + r], s_pk[r]);
     }
 }
@@ -166,7 +173,7 @@ ring_mean_kernel(const double* __restrict__ sums, const int* __restrict__ counts
     for (int64_t i = start + threadIdx.x; i < end; i += kSumThreads) {
         const int r = ring_of<false>(int(i % K), int(i / K), c, rmax);
         const float s = float(sums[int64_t(seg) * nr + r]);
-        const int n = max(1, counts[int64_t(seg) * nr + r]);
+        const int n = max(1, counts[r]);
         const float v = s / float(n);
         out[int64_t(seg) * kk + i] = v;
         share[0] += double(v);
@@ -345,7 +352,7 @@ void ring_stats(hipStream_t s, const float* img, int size, float center, int rma
                            0, s, static_cast<uint32_t*>(p), words);
     };
     clear(sums, int64_t(nr) * segs * 2);
-    if (counts) clear(counts, int64_t(nr) * segs);
+    if (counts) clear(counts, int64_t(nr));  // segment 0's ring sizes serve every segment
     if (peaks) clear(peaks, int64_t(nr) * segs);
     const size_t lds = size_t(nr) * (sizeof(double) + sizeof(int) + sizeof(unsigned));
     const int64_t npx = int64_t(size) * size;
