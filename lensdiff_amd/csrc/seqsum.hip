@@ -43,11 +43,26 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__device__ __forceinline__ double wave_inclusive_scan(double v, int lane) {
-    for (int o = 1; o < 64; o <<= 1) {
-        const double t = __shfl_up(v, o);
-        if (lane >= o) v += t;
-    }
+// v moved across lanes by a DPP control (both 32-bit halves); lanes that the control or the row
+// mask leaves without a source read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Inclusive scan over the 64 lanes with DPP: shifts by 1, 2, 4, 8 within each row of 16, then
+// the last lane of row 0 / 2 into rows 1 / 3 (row_bcast:15) and of row 1 into rows 2 and 3
+// (row_bcast:31). The additions are exact here (sums of multiples of one power of two), so the
+// order does not matter.
+__device__ __forceinline__ double wave_inclusive_scan(double v, int) {
+    v += dpp_move<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_move<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_move<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_move<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
     return v;
 }
 
