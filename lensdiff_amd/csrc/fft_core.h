@@ -186,14 +186,13 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
                 constexpr int stride = tw_table(N) / (Ns * R);
                 const float2* __restrict__ t = tw + tw_offset(N);
                 if constexpr ((R & (R - 1)) == 0) {
-                    // W^(k r) for r = 2^b from the table (loads issued together), the other powers
-                    // as products W^(k (r - h)) W^(k h), h the top bit of r
+                    // W^k from the table, W^(2^b k) by squaring, the other powers as products
+                    // W^(k (r - h)) W^(k h), h the top bit of r
                     float2 w[R];
+                    w[1] = t[k * stride];
+                    if (INV) w[1].y = -w[1].y;
 #pragma unroll
-                    for (int h = 1; h < R; h *= 2) {
-                        w[h] = t[k * h * stride];
-                        if (INV) w[h].y = -w[h].y;
-                    }
+                    for (int h = 2; h < R; h *= 2) w[h] = cmul(w[h / 2], w[h / 2]);
 #pragma unroll
                     for (int r = 1; r < R; ++r) {
                         const int h = r >= 8 ? 8 : r >= 4 ? 4 : r >= 2 ? 2 : 1;

@@ -204,15 +204,14 @@ __device__ __forceinline__ void line_stage(const float2* __restrict__ tw, int ti
                 constexpr int stride = tw_table(N) / (Ns * R);
                 const float2* __restrict__ t = tw + tw_offset(N);
                 static_assert((R & (R - 1)) == 0, "tail stages are radix 2^b");
-                // W^(k r) for r = 2^b from the table (one load each, issued together), the other
-                // powers as products W^(k r) = W^(k (r - h)) W^(k h), h the top bit of r: at most
-                // three rounded products per twiddle.
+                // W^k from the table (one load), W^(2^b k) by squaring, the other powers as
+                // products W^(k r) = W^(k (r - h)) W^(k h), h the top bit of r: at most six
+                // rounded products per twiddle (loads, not arithmetic, limit these kernels).
                 float2 w[R];
+                w[1] = t[k * stride];
+                if (INV) w[1].y = -w[1].y;
 #pragma unroll
-                for (int h = 1; h < R; h *= 2) {
-                    w[h] = t[k * h * stride];
-                    if (INV) w[h].y = -w[h].y;
-                }
+                for (int h = 2; h < R; h *= 2) w[h] = cmul(w[h / 2], w[h / 2]);
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
                     const int h = top_bit(r);
