@@ -235,7 +235,7 @@ template <int N, bool ROWS> struct TileShape {
     // line groups for 1024 elements.
     static constexpr int NL = ROWS ? (N >= 1024 ? 1 : 1024 / N) : (kColWidth * (N >= 64 ? 1 : 64 / N));
     static constexpr int ELEMS = NL * N;
-    static constexpr int NT0 = ROWS ? ELEMS / 16 : ELEMS / 8;
+    static constexpr int NT0 = ELEMS / 8;  // 8 elements per thread (more threads hide the loads)
     static constexpr int NT1 = NT0 < 64 ? 64 : (NT0 > 512 ? 512 : NT0);
     static constexpr int NT = (NT1 + 63) / 64 * 64;
     static constexpr int LDS_BYTES = (ELEMS + ELEMS / 32 + 1) * 8;
