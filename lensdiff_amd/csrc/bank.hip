@@ -162,19 +162,44 @@ __device__ __forceinline__ double* block_sum_slot(double* sums, int array, int k
     return sums + (size_t(array) * kernels + blockIdx.y) * gridDim.x + blockIdx.x;
 }
 
+// Elements per thread of a block; each thread's element j is block element j * kSumThreads + tid.
+// Kernels that gather (ring means) or read several arrays per texel (crop + taper) issue all of
+// a thread's loads before using them (K*K < 2^31).
+constexpr int kPerSum = kSeqSumBlock / kSumThreads;
+
+// Advances (x, y) by kSumThreads texels along the rows of a K-wide image.
+__device__ __forceinline__ void advance_texel(int& x, int& y, int K) {
+    x += kSumThreads;
+    while (x >= K) {
+        x -= K;
+        ++y;
+    }
+}
+
 __global__ void __launch_bounds__(kSumThreads)
 ring_mean_kernel(const double* __restrict__ sums, const int* __restrict__ counts, int K, int rmax,
                  float* __restrict__ out, double* __restrict__ block_sums) {
     const int seg = blockIdx.y;
     const float c = float(K - 1) * 0.5f;
     const int nr = rmax + 1;
-    const int64_t kk = int64_t(K) * K, start = block_start(), end = min(kk, start + kSeqSumBlock);
+    const int kk = K * K, start = int(block_start()), end = min(kk, start + kSeqSumBlock);
+    const int i0 = start + int(threadIdx.x);
+    int x = i0 % K, y = i0 / K;
+    double sv[kPerSum];
+    int nv[kPerSum];
+#pragma unroll
+    for (int j = 0; j < kPerSum; ++j) {
+        const int r = ring_of<false>(x, y, c, rmax);
+        sv[j] = sums[int64_t(seg) * nr + r];
+        nv[j] = counts[r];
+        advance_texel(x, y, K);
+    }
     double share[1] = {0.0};
-    for (int64_t i = start + threadIdx.x; i < end; i += kSumThreads) {
-        const int r = ring_of<false>(int(i % K), int(i / K), c, rmax);
-        const float s = float(sums[int64_t(seg) * nr + r]);
-        const int n = max(1, counts[r]);
-        const float v = s / float(n);
+#pragma unroll
+    for (int j = 0; j < kPerSum; ++j) {
+        const int i = i0 + j * kSumThreads;
+        if (i >= end) break;
+        const float v = float(sv[j]) / float(max(1, nv[j]));
         out[int64_t(seg) * kk + i] = v;
         share[0] += double(v);
     }
@@ -287,9 +312,26 @@ crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, float*
     const float extent = fmaxf(1.0f, float(r));
     const float fade = fmaxf(6.0f, fminf(24.0f, extent * 0.04f));
     const float ramp = fmaxf(0.0f, extent - fade);
+    // loads of the texels inside the square first (all of the thread's), then the taper
+    const int i0 = int(start) + int(threadIdx.x);
+    const int64_t o0 = int64_t(bin) * kk;
+    float sv[3][kPerSum];
+    {
+        int x = i0 % K, y = i0 / K;
+#pragma unroll
+        for (int j = 0; j < kPerSum; ++j) {
+            const bool inside = i0 + j * kSumThreads < end && abs(x - c) <= r && abs(y - c) <= r;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) sv[k][j] = inside ? ts.src[k][o0 + i0 + j * kSumThreads] : 0.0f;
+            advance_texel(x, y, K);
+        }
+    }
     double share[3] = {0.0, 0.0, 0.0};
-    for (int64_t i = start + threadIdx.x; i < end; i += kSumThreads) {
-        const int x = int(i % K), y = int(i / K);
+    int x = i0 % K, y = i0 / K;
+#pragma unroll
+    for (int j = 0; j < kPerSum; ++j) {
+        const int i = i0 + j * kSumThreads;
+        if (i >= end) break;
         const bool inside = abs(x - c) <= r && abs(y - c) <= r;
         float w = 0.0f;
         if (inside) {
@@ -301,13 +343,13 @@ crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, float*
                 w = fmaxf(0.0f, cosf(sm * 1.57079637f));
             }
         }
-        const int64_t o = int64_t(bin) * kk + i;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const float v = inside ? ts.src[k][o] * inv[k] * w : 0.0f;
+            const float v = inside ? sv[k][j] * inv[k] * w : 0.0f;
             dst[(int64_t(k) * count + bin) * kk + i] = v;
             share[k] += double(v);
         }
+        advance_texel(x, y, K);
     }
     double* const slots[3] = {block_sum_slot(block_sums, 0, count), block_sum_slot(block_sums, 1, count),
                               block_sum_slot(block_sums, 2, count)};
