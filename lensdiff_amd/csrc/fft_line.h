@@ -173,9 +173,31 @@ struct LdsOut {
 };
 __host__ __device__ constexpr int top_bit(int r) { return r >= 16 ? 16 : r >= 8 ? 8 : r >= 4 ? 4 : r >= 2 ? 2 : 1; }
 
-// One Stockham stage over a single line of N points: sub-length Ns, radix R.
-template <int N, int Ns, int R, int NT, bool INV, class In, class Out>
-__device__ __forceinline__ void line_stage(const float2* __restrict__ tw, int tid, const In& in, const Out& out) {
+// W^k (forward sign) of each radix-B stage's butterfly for thread tid, k = tid % Ns: loaded
+// together before the first transform (and reused by a second one), so no stage waits for its
+// twiddle. Used where every thread does one butterfly per stage.
+template <int M, int B> struct StageTwiddles {
+    static constexpr int S = LinePlan<M, B>::S;
+    float2 w[S > 0 ? S : 1];
+};
+template <int M, int B>
+__device__ __forceinline__ StageTwiddles<M, B> stage_twiddles(const float2* __restrict__ tw, int tid) {
+    StageTwiddles<M, B> st;
+    const float2* __restrict__ t = tw + tw_offset(M);
+    int ns = LinePlan<M, B>::R0;
+#pragma unroll
+    for (int s = 0; s < StageTwiddles<M, B>::S; ++s) {
+        st.w[s] = t[(tid % ns) * (tw_table(M) / (ns * B))];
+        ns *= B;
+    }
+    return st;
+}
+
+// One Stockham stage over a single line of N points: sub-length Ns, radix R. wk: the stage's
+// W^k when PRE (one butterfly per thread), else loaded here.
+template <int N, int Ns, int R, int NT, bool INV, bool PRE = false, class In, class Out>
+__device__ __forceinline__ void line_stage(const float2* __restrict__ tw, int tid, const In& in, const Out& out,
+                                           float2 wk = make_float2(1.0f, 0.0f)) {
     constexpr int BF = N / R;
     constexpr int PER = (BF + NT - 1) / NT;
     float2 v[PER][R];
@@ -208,7 +230,8 @@ __device__ __forceinline__ void line_stage(const float2* __restrict__ tw, int ti
                 // products W^(k r) = W^(k (r - h)) W^(k h), h the top bit of r: at most six
                 // rounded products per twiddle (loads, not arithmetic, limit these kernels).
                 float2 w[R];
-                w[1] = t[k * stride];
+                if constexpr (PRE && PER == 1) w[1] = wk;
+                else w[1] = t[k * stride];
                 if (INV) w[1].y = -w[1].y;
 #pragma unroll
                 for (int h = 2; h < R; h *= 2) w[h] = cmul(w[h / 2], w[h / 2]);
@@ -231,15 +254,15 @@ __device__ __forceinline__ void line_stage(const float2* __restrict__ tw, int ti
     if constexpr (Out::lds) __syncthreads();
 }
 
-// Radix-B stages from sub-length Ns to M; the last writes through `out`.
-template <int M, int B, int Ns, int NT, bool INV, class Out>
+// Radix-B stages from sub-length Ns (stage SI of the tail) to M; the last writes through `out`.
+template <int M, int B, int Ns, int SI, int NT, bool INV, class Out>
 __device__ __forceinline__ void line_tail(const float2* __restrict__ tw, int tid, const LdsIn& li, const LdsOut& lo,
-                                          const Out& out) {
+                                          const Out& out, const StageTwiddles<M, B>& st) {
     if constexpr (Ns * B == M) {
-        line_stage<M, Ns, B, NT, INV>(tw, tid, li, out);
+        line_stage<M, Ns, B, NT, INV, true>(tw, tid, li, out, st.w[SI]);
     } else {
-        line_stage<M, Ns, B, NT, INV>(tw, tid, li, lo);
-        line_tail<M, B, Ns * B, NT, INV>(tw, tid, li, lo, out);
+        line_stage<M, Ns, B, NT, INV, true>(tw, tid, li, lo, st.w[SI]);
+        line_tail<M, B, Ns * B, SI + 1, NT, INV>(tw, tid, li, lo, out, st);
     }
 }
 
@@ -267,10 +290,10 @@ __device__ __forceinline__ void line_multiply(float2* line, const float2* __rest
 }
 
 // The whole line transform: stage 0 reads through `in`, the last stage writes through `out`,
-// the stages between keep the line in LDS (`line`, padded).
+// the stages between keep the line in LDS (`line`, padded). st: stage_twiddles<M, B>(tw, tid).
 template <int M, int B, bool INV, class In, class Out>
 __device__ __forceinline__ void line_fft(float2* line, const float2* __restrict__ tw, int tid, const In& in,
-                                         const Out& out) {
+                                         const Out& out, const StageTwiddles<M, B>& st) {
     using P = LinePlan<M, B>;
     static_assert(P::ok, "line length has no composite-first plan");
     constexpr int NT = P::NT, R0 = P::R0, S = P::S;
@@ -280,7 +303,7 @@ __device__ __forceinline__ void line_fft(float2* line, const float2* __restrict_
         line_stage<M, 1, R0, NT, INV>(tw, tid, in, out);
     } else {
         line_stage<M, 1, R0, NT, INV>(tw, tid, in, lo);
-        line_tail<M, B, R0, NT, INV>(tw, tid, li, lo, out);
+        line_tail<M, B, R0, 0, NT, INV>(tw, tid, li, lo, out, st);
     }
 }
 

@@ -142,12 +142,13 @@ __device__ __forceinline__ void line_transform(float2* lds, const float2* __rest
                                                const In& in, const Out& out) {
     const GlobalIn<In> gin{in};
     const GlobalOut<Out> gout{out};
+    const StageTwiddles<M, B> st = stage_twiddles<M, B>(tw, tid);  // shared by both transforms
     if (bluestein) {
-        line_fft<M, B, false>(lds, tw, tid, gin, LdsOut{lds});
+        line_fft<M, B, false>(lds, tw, tid, gin, LdsOut{lds}, st);
         line_multiply<M, LinePlan<M, B>::NT>(lds, bhat, tid);
-        line_fft<M, B, true>(lds, tw, tid, LdsIn{lds}, gout);
+        line_fft<M, B, true>(lds, tw, tid, LdsIn{lds}, gout, st);
     } else {
-        line_fft<M, B, false>(lds, tw, tid, gin, gout);
+        line_fft<M, B, false>(lds, tw, tid, gin, gout, st);
     }
 }
 
