@@ -186,7 +186,7 @@ psf_cols_kernel(const float2* __restrict__ GT, int P, int R, int off, const floa
                         &acc};
     line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out);
     if (row_sums == nullptr) return;
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    acc = wave_total(acc);
     if ((tid & 63) == 0) part[tid >> 6] = acc;
     __syncthreads();
     if (tid == 0) {

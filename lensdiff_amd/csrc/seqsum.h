@@ -27,6 +27,38 @@ struct SeqSegments {
     }
 };
 
+// v moved across lanes by a DPP control (both 32-bit halves); lanes that the control or the row
+// mask leaves without a source read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Inclusive scan over the 64 lanes with DPP: shifts by 1, 2, 4, 8 within each row of 16, then
+// the last lane of row 0 / 2 into rows 1 / 3 (row_bcast:15) and of row 1 into rows 2 and 3
+// (row_bcast:31). All 64 lanes must be active. The walk's additions are exact (multiples of one
+// power of two), so their order does not matter; the other callers form FP64 block sums, which
+// only place blocks and mark empty ones.
+__device__ __forceinline__ double wave_inclusive_scan(double v, int) {
+    v += dpp_move<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_move<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_move<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_move<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+// The sum over the 64 lanes, in every lane (the scan's last lane, read back as a scalar).
+__device__ __forceinline__ double wave_total(double v) {
+    v = wave_inclusive_scan(v, 0);
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+
 // Kernels that produce a summed array can hand the sum its block sums: a workgroup of
 // kSumThreads threads covers block blockIdx.x (kSeqSumBlock consecutive elements) of one
 // segment, each thread adds the values it writes into FP64 shares, and every thread then calls
@@ -40,8 +72,7 @@ __device__ __forceinline__ void store_block_sums(double (&share)[N], double* con
     __shared__ double part[N][kWaves];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        double v = share[j];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        const double v = wave_total(share[j]);
         if ((threadIdx.x & 63) == 0) part[j][threadIdx.x >> 6] = v;
     }
     __syncthreads();

@@ -38,33 +38,6 @@ constexpr int kBins = 4;                   // binades per block
 constexpr int kNoBase = INT_MIN;           // running sum estimated at zero: replay the block
 constexpr double kTied = -1.0;             // a binade sum holding a rounding tie
 
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
-// v moved across lanes by a DPP control (both 32-bit halves); lanes that the control or the row
-// mask leaves without a source read 0.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_move(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-
-// Inclusive scan over the 64 lanes with DPP: shifts by 1, 2, 4, 8 within each row of 16, then
-// the last lane of row 0 / 2 into rows 1 / 3 (row_bcast:15) and of row 1 into rows 2 and 3
-// (row_bcast:31). The additions are exact here (sums of multiples of one power of two), so the
-// order does not matter.
-__device__ __forceinline__ double wave_inclusive_scan(double v, int) {
-    v += dpp_move<0x111, 0xf>(v);  // row_shr:1
-    v += dpp_move<0x112, 0xf>(v);  // row_shr:2
-    v += dpp_move<0x114, 0xf>(v);  // row_shr:4
-    v += dpp_move<0x118, 0xf>(v);  // row_shr:8
-    v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-    v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
-    return v;
-}
 
 // Blocks are `bl` consecutive elements (bl <= kSeqSumBlock); block b covers
 // [b * bl, min(n, (b + 1) * bl)).
@@ -82,7 +55,7 @@ block_sums_kernel(SeqSegments g, int64_t n, int bl, int nb, double* __restrict__
         if (i < end) acc += double(base[i] * sc);
     }
     __shared__ double red[kThreads / 64];
-    acc = wave_sum(acc);
+    acc = wave_total(acc);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -161,7 +134,7 @@ binade_sums_kernel(SeqSegments g, int64_t n, int bl, int nb, const double* __res
             tie |= fabsf(t - r) == 0.5f;
             acc += double(r);
         }
-        acc = wave_sum(acc);
+        acc = wave_total(acc);
         const bool any_tie = __ballot(tie) != 0ull;
         if ((threadIdx.x & 63) == 0) {
             red[q][threadIdx.x >> 6] = acc;
