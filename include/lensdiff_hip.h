@@ -306,6 +306,11 @@ ldh_status ldh_profiling_reset(ldh_context* ctx);
  * into buf (buf may be NULL when capacity is 0); returns the full length. */
 int32_t ldh_tuning_report(char* buf, int32_t capacity);
 
+/* Pins every autotuned kernel family to one variant (variant >= 0; a family with fewer variants
+ * takes its last) or returns them to tuning on first use (variant = -1); clears the choices made
+ * so far (no counterpart in the reference: lets a caller or a test run each variant). */
+void ldh_tuning_force(int32_t variant);
+
 #ifdef __cplusplus
 }
 #endif

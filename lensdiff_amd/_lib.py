@@ -56,6 +56,7 @@ SIGNATURES = {
     "ldh_profiling_read": (C.c_int32, [_CTX, C.POINTER(_abi.KernelStat), C.c_int32]),
     "ldh_profiling_reset": (C.c_int32, [_CTX]),
     "ldh_tuning_report": (C.c_int32, [C.c_char_p, C.c_int32]),
+    "ldh_tuning_force": (None, [C.c_int32]),
 }
 
 _lib = None

@@ -267,6 +267,10 @@ class Context:
     def profile_reset(self):
         self._check(self.lib.ldh_profiling_reset(self.h))
 
+    def tuning_force(self, variant: int):
+        """Pins every autotuned kernel family to `variant` (-1: tune again on first use)."""
+        self.lib.ldh_tuning_force(int(variant))
+
     def tuning_report(self) -> dict:
         """The autotuner's choices: {key: {"variant": k, "ms": [per-variant times]}}."""
         n = self.lib.ldh_tuning_report(None, 0)

@@ -641,4 +641,6 @@ int32_t ldh_tuning_report(char* buf, int32_t capacity) {
     return int32_t(r.size());
 }
 
+void ldh_tuning_force(int32_t variant) { ldh::Autotuner::instance().force(int(variant)); }
+
 }  // extern "C"

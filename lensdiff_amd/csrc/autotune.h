@@ -1,5 +1,6 @@
-// Run-time choice among kernel variants that compute bit-identical results (same arithmetic per
-// element, different launch shapes or occupancy targets). The first launch for a key times every
+// Run-time choice among kernel variants that compute the same transform (different radix orders,
+// launch shapes or occupancy targets; results agree to FP32 rounding, and whichever variant ran
+// is the result). The first launch for a key times every
 // variant on the live workload with HIP events on the launch stream (each variant twice, the
 // second run counted, so code-object loading is not), keeps the fastest for the life of the
 // process, and later launches go straight to it. Keys name the kernel family, the line length
@@ -25,6 +26,9 @@ public:
     // One line per key: "key: vN  t0 t1 ... (ms)".
     std::string report() const;
     void reset();
+    // variant >= 0: every key launches that variant (the last one where a family has fewer)
+    // without timing; -1: back to tuning. Either way the table is cleared.
+    void force(int variant);
 
 private:
     struct Entry {
@@ -33,6 +37,7 @@ private:
     };
     mutable std::mutex mu_;
     std::map<std::string, Entry> table_;
+    int forced_ = -1;
 };
 
 }  // namespace ldh

@@ -18,6 +18,11 @@ int Autotuner::run(const std::string& key, int variants, hipStream_t s, const st
     (void)hipStreamIsCapturing(s, &cap);
     {
         std::lock_guard<std::mutex> g(mu_);
+        if (forced_ >= 0) {
+            const int v = forced_ < variants ? forced_ : variants - 1;
+            launch(v);
+            return v;
+        }
         auto it = table_.find(k);
         // inside a graph capture nothing may synchronise: an untuned key takes variant 0
         if (it != table_.end() || variants <= 1 || cap != hipStreamCaptureStatusNone) {
@@ -67,6 +72,12 @@ std::string Autotuner::report() const {
 void Autotuner::reset() {
     std::lock_guard<std::mutex> g(mu_);
     table_.clear();
+}
+
+void Autotuner::force(int variant) {
+    std::lock_guard<std::mutex> g(mu_);
+    table_.clear();
+    forced_ = variant < 0 ? -1 : variant;
 }
 
 }  // namespace ldh

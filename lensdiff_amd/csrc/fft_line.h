@@ -111,6 +111,41 @@ template <int R, bool INV> __device__ __forceinline__ void dft_any(float2* v) {
     else dft_ab<16, 3, INV>(v);  // 48
 }
 
+// The composite first stage of a line whose input is zero past `extent`: butterfly u reads
+// positions u + r * (M / R0), so when extent <= NZ * (M / R0) only its first NZ inputs can be
+// non-zero, and with Q = R0 / NZ
+//     X[Q k1 + k2] = sum_{n < NZ} (x[n] W_R0^(n k2)) W_NZ^(n k1)
+// (W_R0^Q = W_NZ): Q twiddled NZ-point DFTs replace the R0-point DFT. NZ is the smallest divisor
+// of R0 with 5 NZ >= R0 (for R >= 4P a Bluestein line of M >= R + P - 1 points holds its P inputs
+// in about the first fifth), for the radices with a root table.
+constexpr bool has_roots(int r) {
+    return r == 5 || r == 6 || r == 9 || r == 10 || r == 12 || r == 18 || r == 20 || r == 24 || r == 32 || r == 36 ||
+           r == 40 || r == 48;
+}
+constexpr int prune_nz(int r0) {
+    if (!has_roots(r0)) return r0;
+    for (int d = 1; d < r0; ++d)
+        if (r0 % d == 0 && 5 * d >= r0) return d;
+    return r0;
+}
+template <int R0, int NZ, bool INV> __device__ __forceinline__ void dft_pruned(float2* v) {
+#pragma clang fp contract(fast)
+    constexpr int Q = R0 / NZ;
+    static_assert(Q * NZ == R0, "NZ divides R0");
+    float2 x[NZ];
+#pragma unroll
+    for (int n = 0; n < NZ; ++n) x[n] = v[n];
+#pragma unroll
+    for (int k2 = 0; k2 < Q; ++k2) {
+        float2 a[NZ];
+#pragma unroll
+        for (int n = 0; n < NZ; ++n) a[n] = (n == 0 || k2 == 0) ? x[n] : cmul(x[n], root<R0, INV>(n * k2));
+        if constexpr (NZ > 1) dft_any<NZ, INV>(a);
+#pragma unroll
+        for (int k1 = 0; k1 < NZ; ++k1) v[Q * k1 + k2] = a[k1];
+    }
+}
+
 // M = R0 * B^S with the fewest stages S >= 1 and 2 <= R0 <= 48 (S = 0 when M <= 48).
 constexpr int line_stages(int M, int B) {
     int bs = 1;
@@ -195,9 +230,11 @@ __device__ __forceinline__ StageTwiddles<M, B> stage_twiddles(const float2* __re
 
 // One Stockham stage over a single line of N points: sub-length Ns, radix R. wk: the stage's
 // W^k when PRE (one butterfly per thread), else loaded here.
-template <int N, int Ns, int R, int NT, bool INV, bool PRE = false, class In, class Out>
+// NZ < R (first stage only): inputs r >= NZ are zero and not read.
+template <int N, int Ns, int R, int NT, bool INV, bool PRE = false, int NZ = R, class In, class Out>
 __device__ __forceinline__ void line_stage(const float2* __restrict__ tw, int tid, const In& in, const Out& out,
                                            float2 wk = make_float2(1.0f, 0.0f)) {
+    static_assert(NZ == R || Ns == 1, "only the first stage is pruned");
     constexpr int BF = N / R;
     constexpr int PER = (BF + NT - 1) / NT;
     float2 v[PER][R];
@@ -207,7 +244,7 @@ __device__ __forceinline__ void line_stage(const float2* __restrict__ tw, int ti
         if (BF % NT == 0 || u < BF) {
             const int pu = lds_pad(u);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[p][r] = in.template get<BF>(u, pu, r);
+            for (int r = 0; r < NZ; ++r) v[p][r] = in.template get<BF>(u, pu, r);
         }
     }
     if constexpr (In::lds && Out::lds) __syncthreads();  // every read before the in-place writes
@@ -242,7 +279,8 @@ __device__ __forceinline__ void line_stage(const float2* __restrict__ tw, int ti
                     v[p][r] = cmul(v[p][r], w[r]);
                 }
             }
-            dft_any<R, INV>(v[p]);
+            if constexpr (NZ < R) dft_pruned<R, NZ, INV>(v[p]);
+            else dft_any<R, INV>(v[p]);
             const int pbase = lds_pad(base);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -291,9 +329,10 @@ __device__ __forceinline__ void line_multiply(float2* line, const float2* __rest
 
 // The whole line transform: stage 0 reads through `in`, the last stage writes through `out`,
 // the stages between keep the line in LDS (`line`, padded). st: stage_twiddles<M, B>(tw, tid).
+// extent: the input is zero at positions >= extent (uniform over the workgroup).
 template <int M, int B, bool INV, class In, class Out>
 __device__ __forceinline__ void line_fft(float2* line, const float2* __restrict__ tw, int tid, const In& in,
-                                         const Out& out, const StageTwiddles<M, B>& st) {
+                                         const Out& out, const StageTwiddles<M, B>& st, int extent = M) {
     using P = LinePlan<M, B>;
     static_assert(P::ok, "line length has no composite-first plan");
     constexpr int NT = P::NT, R0 = P::R0, S = P::S;
@@ -302,7 +341,13 @@ __device__ __forceinline__ void line_fft(float2* line, const float2* __restrict_
     if constexpr (S == 0) {
         line_stage<M, 1, R0, NT, INV>(tw, tid, in, out);
     } else {
-        line_stage<M, 1, R0, NT, INV>(tw, tid, in, lo);
+        constexpr int NZ = prune_nz(R0);
+        if constexpr (NZ < R0) {
+            if (extent <= NZ * (M / R0)) line_stage<M, 1, R0, NT, INV, false, NZ>(tw, tid, in, lo);
+            else line_stage<M, 1, R0, NT, INV>(tw, tid, in, lo);
+        } else {
+            line_stage<M, 1, R0, NT, INV>(tw, tid, in, lo);
+        }
         line_tail<M, B, R0, 0, NT, INV>(tw, tid, li, lo, out, st);
     }
 }

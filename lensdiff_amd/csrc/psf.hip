@@ -113,21 +113,19 @@ struct RowOutput {
     }
 };
 // |F|^2 of output k into the centred row: column (k - R/2) mod R (fftShiftSquare, :303-314),
-// also added in FP64 to the thread's share of the row sum.
+// also added in FP64 to the thread's share of the row sum. Bluestein's output chirp has unit
+// modulus, so |F|^2 is |v|^2 / M^2 without it (scale2 = 1 / M^2, or 1 for a plain FFT).
 struct ColOutput {
-    static constexpr bool kPrefetch = true;
+    static constexpr bool kPrefetch = false;
     float* __restrict__ row;
-    LineValue val;
+    float scale2;
     int R, shift;  // R - R/2
     double* acc;
-    __device__ void operator()(int k, float2 v) const { put_pre(k, v, pre(k)); }
-    __device__ float2 pre(int k) const { return val.pre(k, R); }
-    __device__ void put_pre(int k, float2 v, float2 c) const {
+    __device__ void operator()(int k, float2 v) const {
         if (k < R) {
-            const float2 f = val.with_chirp(v, c);
             int x = k + shift;
             if (x >= R) x -= R;
-            const float p = f.x * f.x + f.y * f.y;
+            const float p = fmaf(v.x, v.x, v.y * v.y) * scale2;
             row[x] = p;
             *acc += double(p);
         }
@@ -139,16 +137,16 @@ struct ColOutput {
 template <int M, int B, class In, class Out>
 __device__ __forceinline__ void line_transform(float2* lds, const float2* __restrict__ tw,
                                                const float2* __restrict__ bhat, bool bluestein, int tid,
-                                               const In& in, const Out& out) {
+                                               const In& in, const Out& out, int extent) {
     const GlobalIn<In> gin{in};
     const GlobalOut<Out> gout{out};
     const StageTwiddles<M, B> st = stage_twiddles<M, B>(tw, tid);  // shared by both transforms
     if (bluestein) {
-        line_fft<M, B, false>(lds, tw, tid, gin, LdsOut{lds}, st);
+        line_fft<M, B, false>(lds, tw, tid, gin, LdsOut{lds}, st, extent);
         line_multiply<M, LinePlan<M, B>::NT>(lds, bhat, tid);
         line_fft<M, B, true>(lds, tw, tid, LdsIn{lds}, gout, st);
     } else {
-        line_fft<M, B, false>(lds, tw, tid, gin, gout, st);
+        line_fft<M, B, false>(lds, tw, tid, gin, gout, st, extent);
     }
 }
 
@@ -163,7 +161,7 @@ psf_rows_kernel(const float2* __restrict__ field, int P, int R, int off, const f
     const RowInput in{field + (size_t(lam) * P + y) * P, P, off};
     const float2 post = bluestein ? chirp[y] : make_float2(1.0f, 0.0f);
     const RowOutput out{G + (size_t(lam) * P + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R, post, bluestein};
-    line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out);
+    line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out, off + P);
 }
 
 // Also writes the FP64 sum of each output row to row_sums[lam * R + y] (nullable): the block
@@ -182,9 +180,9 @@ psf_cols_kernel(const float2* __restrict__ GT, int P, int R, int off, const floa
     if (y >= R) y -= R;
     double acc = 0.0;
     const ColInput in{GT + (size_t(lam) * R + ky) * P, P, off};
-    const ColOutput out{IT + (size_t(lam) * R + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R, shift,
-                        &acc};
-    line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out);
+    const float inv_m = 1.0f / float(M);
+    const ColOutput out{IT + (size_t(lam) * R + y) * R, bluestein ? inv_m * inv_m : 1.0f, R, shift, &acc};
+    line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out, off + P);
     if (row_sums == nullptr) return;
     acc = wave_total(acc);
     if ((tid & 63) == 0) part[tid >> 6] = acc;

@@ -133,6 +133,28 @@ def test_raw_psf(ctx, oracle, pupil, raw):
     assert abs(gs - ws) / ws < 1e-5, (gs, ws)
 
 
+# every autotuner variant of the line kernels (radix 16 or 8 after the first stage, with the
+# first stage pruned to the pupil's extent) on lines long enough to be tuned
+@pytest.mark.parametrize("pupil,raw", [(300, 2200), (1024, 4092)])
+def test_raw_psf_every_variant(ctx, oracle, pupil, raw):
+    p = params(pupil_resolution=pupil, aperture_mode=1, blade_count=7, roundness=0.1)
+    p.set_noll({4: -0.25, 6: 0.15, 11: 0.2})
+    amp = oracle.pupil_amplitude(p)
+    ph = oracle.pupil_phase(p)
+    want = oracle.raw_psf(amp, ph, raw)
+    got = []
+    try:
+        for v in range(5):
+            ctx.tuning_force(v)
+            got.append(ctx.raw_psf(amp, ph, raw))
+    finally:
+        ctx.tuning_force(-1)
+    for v, g in enumerate(got):
+        assert rel_l2(g, want) < TOL, v
+        # the variants differ only in the order of FP32 operations
+        assert rel_l2(g, got[0]) < 1e-5, v
+
+
 def test_first_minimum_radius(ctx, oracle):
     p = params(pupil_resolution=128, aperture_mode=0, roundness=1.0)
     amp = oracle.pupil_amplitude(p)
