@@ -97,8 +97,16 @@ __device__ __forceinline__ int ring_of(int x, int y, float c, int rmax) {
 }
 
 constexpr int kRingThreads = 256;
-constexpr int kRingRun = 8;             // consecutive pixels of one row per thread
-constexpr int kRingRowsPerBlock = 16;   // rows per workgroup
+constexpr int kRingRun = 8;                   // consecutive pixels of one row per thread
+constexpr int kRingSpan = 64 * kRingRun;      // pixels of a row one wave takes at a time
+constexpr int kRingRowsPerBlock = 16;         // rows per workgroup
+
+// Orders a wave's LDS writes before its later LDS reads by other lanes of the same wave.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // sums[r] += v, peaks[r] = max(v) over the pixels of ring r, per segment; counts[r] += 1 for
 // segment 0 only (ring sizes are the same for every segment of one size).
@@ -108,23 +116,18 @@ constexpr int kRingRowsPerBlock = 16;   // rows per workgroup
 // ring in registers before one LDS atomic per ring it meets: where a row runs along a ring (far
 // above or below the centre, or the clamped corners) the lanes of a wave would otherwise all hit
 // the same LDS word, and where the ring changes every pixel, lanes kRingRun pixels apart hit
-// different words.
+// different words. A wave loads a span of kRingSpan pixels coalesced (lane l, load k: pixel
+// 64 k + l) and hands the runs out through LDS (lane l reads pixels 8 l .. 8 l + 7).
 // MIRROR (centre = (size - 1) / 2): pixel (x, y) and its mirrors (size-1-x, y), (x, size-1-y),
-// (size-1-x, size-1-y) have the same dx^2 and dy^2, so the same ring; the thread walks the
-// top-left quadrant (with the centre row and column) and adds each distinct mirror once.
-// Each workgroup stores its whole histogram to the partials (RingPartials); ring_reduce_kernel
-// adds them up in workgroup order. (Global atomics from every workgroup to the same few thousand
-// words serialise per word.)
-struct RingPartials {
-    double* sums;      // [segment][workgroup][ring]
-    int* counts;       // [workgroup][ring], segment 0; null: no counts
-    unsigned* peaks;   // [segment][workgroup][ring]; null: no peaks
-};
+// (size-1-x, size-1-y) have the same dx^2 and dy^2, so the same ring; the waves walk the top-left
+// quadrant (with the centre row and column) and add each distinct mirror once.
 template <bool CEIL, bool MIRROR>
 __global__ void __launch_bounds__(kRingThreads)
 ring_stats_kernel(const float* __restrict__ img, int size, float center, int rmax, const float* __restrict__ prescale,
-                  RingPartials part) {
+                  double* __restrict__ sums, int* __restrict__ counts, unsigned* __restrict__ peaks) {
     extern __shared__ unsigned char smem[];
+    // one span per wave, one pad word per 8 (stride-1 writes and stride-9 reads hit distinct banks)
+    __shared__ float stage[kRingThreads / 64][kRingSpan + kRingSpan / 8];
     const int nr = rmax + 1;
     double* s_sum = reinterpret_cast<double*>(smem);
     int* s_cnt = reinterpret_cast<int*>(s_sum + nr);
@@ -135,37 +138,54 @@ ring_stats_kernel(const float* __restrict__ img, int size, float center, int rma
     const int64_t npx = int64_t(size) * size;
     const float* base = img + seg * npx;
     const float sc = inverse_total(prescale, seg);
-    const bool count = part.counts != nullptr && seg == 0;
-    const bool peaks = part.peaks != nullptr;
+    const bool count = counts != nullptr && seg == 0;
     const int w = MIRROR ? (size + 1) / 2 : size;  // columns and rows walked
     const int y0 = blockIdx.x * kRingRowsPerBlock;
     const int y1 = min(w, y0 + kRingRowsPerBlock);
-    const int runs = (w + kRingRun - 1) / kRingRun;  // runs per row
+    const int spans = (w + kRingSpan - 1) / kRingSpan;  // spans per row
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+    float* st = stage[wave];
     auto flush = [&](int r, double acc, int n, unsigned pk) {
         atomicAdd(&s_sum[r], acc);
         if (count) atomicAdd(&s_cnt[r], n);
         if (peaks) atomicMax(&s_pk[r], pk);
     };
-    for (int i = int(threadIdx.x); i < (y1 - y0) * runs; i += kRingThreads) {
-        const int y = y0 + i / runs;
-        const int xs = (i % runs) * kRingRun;
+    constexpr int kCopies = MIRROR ? 4 : 1;
+    // tasks (row, span) go to waves round-robin: every lane of a wave has the same trip count
+    for (int task = wave; task < (y1 - y0) * spans; task += kRingThreads / 64) {
+        const int y = y0 + task / spans;
+        const int x0 = (task % spans) * kRingSpan;
         const int ym = size - 1 - y;
-        const float* row = base + int64_t(y) * size;
-        const float* rowm = base + int64_t(ym) * size;
-        // the run's loads issued together, clamped inside the walked columns
-        constexpr int kCopies = MIRROR ? 4 : 1;
-        float v[kCopies][kRingRun];
+        // every copy's loads issued together, clamped inside the walked columns
+        float t[kCopies][kRingRun];
 #pragma unroll
-        for (int k = 0; k < kRingRun; ++k) {
-            const int x = min(xs + k, w - 1);
-            v[0][k] = row[x];
-            if constexpr (MIRROR) {
-                v[1][k] = row[size - 1 - x];
-                v[2][k] = rowm[x];
-                v[3][k] = rowm[size - 1 - x];
+        for (int c = 0; c < kCopies; ++c) {
+            const float* src = base + int64_t((c & 2) ? ym : y) * size;
+#pragma unroll
+            for (int k = 0; k < kRingRun; ++k) {
+                const int x = min(x0 + 64 * k + lane, w - 1);
+                t[c][k] = src[(c & 1) ? size - 1 - x : x];
             }
         }
-        const int n = min(kRingRun, w - xs);
+        float v[kCopies][kRingRun];
+#pragma unroll
+        for (int c = 0; c < kCopies; ++c) {
+#pragma unroll
+            for (int k = 0; k < kRingRun; ++k) {
+                const int i = 64 * k + lane;
+                st[i + i / 8] = t[c][k];
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int j = 0; j < kRingRun; ++j) {
+                const int i = kRingRun * lane + j;
+                v[c][j] = st[i + i / 8];
+            }
+            wave_lds_sync();
+        }
+        const int xs = x0 + kRingRun * lane;
+        const int n = min(kRingRun, w - xs);  // <= 0 past the row's end
+        if (n <= 0) continue;
         int rc = ring_of<CEIL>(xs, y, center, rmax);
         double acc = 0.0;
         int cnt = 0;
@@ -198,33 +218,12 @@ ring_stats_kernel(const float* __restrict__ img, int size, float center, int rma
         flush(rc, acc, cnt, pk);
     }
     __syncthreads();
-    const int64_t at = (int64_t(seg) * gridDim.x + blockIdx.x) * nr;
     for (int r = threadIdx.x; r < nr; r += kRingThreads) {
-        part.sums[at + r] = s_sum[r];
-        if (count) part.counts[int64_t(blockIdx.x) * nr + r] = s_cnt[r];
-        if (peaks) part.peaks[at + r] = s_pk[r];
+        if (s_cnt[r] == 0 && s_sum[r] == 0.0 && s_pk[r] == 0u) continue;
+        atomicAdd(&sums[int64_t(seg) * nr + r], s_sum[r]);
+        if (count) atomicAdd(&counts[r], s_cnt[r]);
+        if (peaks) atomicMax(&peaks[int64_t(seg) * nr + r], s_pk[r]);
     }
-}
-
-// sums[seg][r], counts[r] and peaks[seg][r] from the partials of nb workgroups, added in order.
-__global__ void __launch_bounds__(256)
-ring_reduce_kernel(RingPartials part, int nb, int nr, double* __restrict__ sums, int* __restrict__ counts,
-                   unsigned* __restrict__ peaks) {
-    const int seg = blockIdx.y;
-    const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= nr) return;
-    const int64_t at = int64_t(seg) * nb * nr + r;
-    double s = 0.0;
-    unsigned pk = 0u;
-    int c = 0;
-    for (int b = 0; b < nb; ++b) {
-        s += part.sums[at + int64_t(b) * nr];
-        if (peaks) pk = max(pk, part.peaks[at + int64_t(b) * nr]);
-        if (counts && seg == 0) c += part.counts[int64_t(b) * nr + r];
-    }
-    sums[int64_t(seg) * nr + r] = s;
-    if (peaks) peaks[int64_t(seg) * nr + r] = pk;
-    if (counts && seg == 0) counts[r] = c;
 }
 
 // The elementwise shaping kernels below run workgroups of kSumThreads threads over blocks of
@@ -457,45 +456,35 @@ __global__ void fill_zero_kernel(uint32_t* __restrict__ p, int64_t n) {
 
 inline dim3 blocks_grid(int64_t n, int segs) { return dim3(unsigned(seqsum_blocks(n)), unsigned(segs)); }
 
-// Rows walked and workgroups of one ring-statistics launch, and the bytes of its partials.
-inline bool ring_mirror(int size, float center) { return center == float(size - 1) * 0.5f; }
-inline int ring_blocks(int size, bool mirror) { return div_up(mirror ? (size + 1) / 2 : size, kRingRowsPerBlock); }
-inline size_t ring_partial_bytes(int size, bool mirror, int nr, int segs) {
-    // sums and peaks per segment, counts once
-    return size_t(ring_blocks(size, mirror)) * size_t(nr) * (size_t(segs) * (sizeof(double) + sizeof(unsigned)) + sizeof(int));
-}
-
 template <bool CEIL>
-void ring_stats(hipStream_t s, DeviceBuffer& scratch, const float* img, int size, float center, int rmax, int segs,
-                const float* prescale, double* sums, int* counts, unsigned* peaks) {
+void ring_stats(hipStream_t s, const float* img, int size, float center, int rmax, int segs, const float* prescale,
+                double* sums, int* counts, unsigned* peaks) {
     const int nr = rmax + 1;
-    const bool mirror = ring_mirror(size, center);
-    const int nb = ring_blocks(size, mirror);
-    scratch.reset(ring_partial_bytes(size, mirror, nr, segs));  // reserved ahead of graph captures
-    RingPartials part;
-    part.sums = scratch.as<double>();
-    part.peaks = peaks ? reinterpret_cast<unsigned*>(part.sums + int64_t(nb) * nr * segs) : nullptr;
-    part.counts = counts ? reinterpret_cast<int*>(reinterpret_cast<unsigned*>(part.sums + int64_t(nb) * nr * segs) +
-                                                  int64_t(nb) * nr * segs)
-                         : nullptr;
+    // cleared by a kernel, not hipMemsetAsync, so the sequence can be captured into a graph
+    auto clear = [&](void* p, int64_t words) {
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(unsigned(std::min<int64_t>(div_up(words, 256), 1024))), dim3(256),
+                           0, s, static_cast<uint32_t*>(p), words);
+    };
+    clear(sums, int64_t(nr) * segs * 2);
+    if (counts) clear(counts, int64_t(nr));  // segment 0's ring sizes serve every segment
+    if (peaks) clear(peaks, int64_t(nr) * segs);
     const size_t lds = size_t(nr) * (sizeof(double) + sizeof(int) + sizeof(unsigned));
     const int64_t npx = int64_t(size) * size;
-    using Kernel = void (*)(const float*, int, float, int, const float*, RingPartials);
+    const bool mirror = center == float(size - 1) * 0.5f;
+    using Kernel = void (*)(const float*, int, float, int, const float*, double*, int*, unsigned*);
     const Kernel both[2] = {&ring_stats_kernel<CEIL, false>, &ring_stats_kernel<CEIL, true>};
     static bool attr_set = false;
     if (!attr_set) {
         for (const Kernel k : both)
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      160 * 1024);
+                                      128 * 1024);
         attr_set = true;
     }
     ProfileScope ps("shape-ring-stats", 4.0 * double(npx) * segs);
-    hipLaunchKernelGGL(both[mirror ? 1 : 0], dim3(unsigned(nb), unsigned(segs)), dim3(kRingThreads), lds, s, img,
-                       size, center, rmax, prescale, part);
+    const int walked = mirror ? (size + 1) / 2 : size;
+    hipLaunchKernelGGL(both[mirror ? 1 : 0], dim3(unsigned(div_up(walked, kRingRowsPerBlock)), unsigned(segs)),
+                       dim3(kRingThreads), lds, s, img, size, center, rmax, prescale, sums, counts, peaks);
     LDH_CHECK_LAUNCH("ring-stats");
-    hipLaunchKernelGGL(ring_reduce_kernel, dim3(unsigned(div_up(nr, 256)), unsigned(segs)), dim3(256), 0, s, part, nb,
-                       nr, sums, counts, peaks);
-    LDH_CHECK_LAUNCH("ring-reduce");
 }
 
 }  // namespace
@@ -531,7 +520,6 @@ void BankWorkspace::reserve_shaping(int K, int count, int max_radius) {
     block_sums.reset(sizeof(double) * size_t(seqsum_blocks(kk)) * 3 * count);
     ring_cnt.reset(sizeof(int) * nr * count);
     ring_peak.reset(sizeof(unsigned) * nr * count);
-    ring_part.reset(ring_partial_bytes(K, ring_mirror(K, float(K - 1) * 0.5f), nr, count));
     seqsum.reserve(kk, 3 * count);
 }
 
@@ -556,7 +544,7 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
     // Each normalisation below only sums its array; readers apply 1 / total as they load, which
     // gives the values an in-place normalisation would have left.
     // azimuthal mean of the normalised base
-    ring_stats<false>(s, w.ring_part, base, K, float(K - 1) * 0.5f, rmax_mean, count, base_totals, w.ring_sum.as<double>(),
+    ring_stats<false>(s, base, K, float(K - 1) * 0.5f, rmax_mean, count, base_totals, w.ring_sum.as<double>(),
                       w.ring_cnt.as<int>(), nullptr);
     double* bsum = w.block_sums.as<double>();
     const dim3 grid = blocks_grid(kk, count);
@@ -594,7 +582,7 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
     w.seqsum.run(s, g, kk, 2 * count, t_struct, bsum);  // t_struct, then t_run
     // adaptive support radius of the normalised shaped kernel
     const int rmax_sup = std::min(max_radius, K / 2);
-    ring_stats<true>(s, w.ring_part, w.shaped.as<float>(), K, float(K - 1) * 0.5f, rmax_sup, count, t_shaped,
+    ring_stats<true>(s, w.shaped.as<float>(), K, float(K - 1) * 0.5f, rmax_sup, count, t_shaped,
                      w.ring_sum.as<double>(), nullptr, w.ring_peak.as<unsigned>());
     const size_t sup_lds = size_t(rmax_sup + 1) * (sizeof(double) + sizeof(unsigned));
 {
@@ -623,7 +611,7 @@ std::vector<float> radial_profile(hipStream_t s, BankWorkspace& w, const float* 
     w.ring_sum.reset(sizeof(double) * nr);
     w.ring_cnt.reset(sizeof(int) * nr);
     const float center = (R & 1) == 0 ? float(R) * 0.5f : float(R - 1) * 0.5f;
-    ring_stats<false>(s, w.ring_part, psf, R, center, rmax, 1, nullptr, w.ring_sum.as<double>(), w.ring_cnt.as<int>(), nullptr);
+    ring_stats<false>(s, psf, R, center, rmax, 1, nullptr, w.ring_sum.as<double>(), w.ring_cnt.as<int>(), nullptr);
     std::vector<double> sums(static_cast<size_t>(nr));
     std::vector<int> counts(static_cast<size_t>(nr));
     LDH_HIP(hipMemcpyAsync(sums.data(), w.ring_sum.get(), sizeof(double) * nr, hipMemcpyDeviceToHost, s), "profile");
