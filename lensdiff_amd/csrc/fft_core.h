@@ -158,8 +158,14 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
     constexpr int BF = N / R;
     constexpr int TOTAL = NL * BF;
     constexpr int PER = (TOTAL + NT - 1) / NT;
+    constexpr bool kPow2 = (R & (R - 1)) == 0;
+    constexpr int stride = tw_table(N) / (Ns * R);
+    const float2* __restrict__ t = tw + tw_offset(N);
     float2 v[PER][R];
     int line_of[PER], j_of[PER];
+    // the stage's table twiddles are loaded first, so they arrive while the LDS reads and the
+    // barrier complete (one per butterfly for power-of-two radices, else R - 1)
+    float2 wt[PER][Ns > 1 ? (kPow2 ? 1 : R - 1) : 1];
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
         const int u = tid + p * NT;
@@ -168,6 +174,20 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
         else { line = u % NL; j = u / NL; }
         line_of[p] = line;
         j_of[p] = j;
+        if constexpr (Ns > 1) {
+            // every lane loads (a lane past TOTAL reads entry 0), so the loads issue together
+            const int k = (TOTAL % NT == 0 || u < TOTAL) ? j % Ns : 0;
+            if constexpr (kPow2) wt[p][0] = t[k * stride];
+            else {
+#pragma unroll
+                for (int r = 1; r < R; ++r) wt[p][r - 1] = t[k * r * stride];
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int u = tid + p * NT;
+        const int line = line_of[p], j = j_of[p];
         if (TOTAL % NT == 0 || u < TOTAL) {
             const int b = line * LS + j * IS;
             const int pb = lds_pad(b);
@@ -183,13 +203,11 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
             const int j = j_of[p];
             const int k = j % Ns;
             if constexpr (Ns > 1) {
-                constexpr int stride = tw_table(N) / (Ns * R);
-                const float2* __restrict__ t = tw + tw_offset(N);
-                if constexpr ((R & (R - 1)) == 0) {
+                if constexpr (kPow2) {
                     // W^k from the table, W^(2^b k) by squaring, the other powers as products
                     // W^(k (r - h)) W^(k h), h the top bit of r
                     float2 w[R];
-                    w[1] = t[k * stride];
+                    w[1] = wt[p][0];
                     if (INV) w[1].y = -w[1].y;
 #pragma unroll
                     for (int h = 2; h < R; h *= 2) w[h] = cmul(w[h / 2], w[h / 2]);
@@ -202,7 +220,7 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
                 } else {
 #pragma unroll
                     for (int r = 1; r < R; ++r) {
-                        float2 w = t[k * r * stride];
+                        float2 w = wt[p][r - 1];
                         if (INV) w.y = -w.y;
                         v[p][r] = cmul(v[p][r], w);
                     }
@@ -261,12 +279,14 @@ fft_tile_kernel(IO io, const float2* __restrict__ tw, int ntiles) {
     extern __shared__ float2 lds[];
     const int tile = xcd_tile(blockIdx.x, ntiles);
     const int tid = threadIdx.x;
-    // loads in batches of up to kBatch per thread, all issued before their LDS writes
+    // loads in batches of up to kBatch per thread: every fetch of a batch (memory only) is issued
+    // before the first finish (the arithmetic, which may branch on launch-uniform settings) and
+    // the LDS writes
     constexpr int IT = (E + NT - 1) / NT;
     constexpr int kBatch = IT < 16 ? IT : 16;
 #pragma unroll
     for (int c0 = 0; c0 < IT; c0 += kBatch) {
-        float2 t[kBatch];
+        typename IO::Raw t[kBatch];
 #pragma unroll
         for (int c = 0; c < kBatch; ++c) {
             const int e = tid + (c0 + c) * NT;
@@ -274,7 +294,7 @@ fft_tile_kernel(IO io, const float2* __restrict__ tw, int ntiles) {
                 int line, i;
                 if constexpr (ROWS) { line = e / N; i = e % N; }
                 else { line = e % NL; i = e / NL; }
-                t[c] = io.load(tile, line, i);
+                t[c] = io.fetch(tile, line, i);
             }
         }
 #pragma unroll
@@ -284,17 +304,35 @@ fft_tile_kernel(IO io, const float2* __restrict__ tw, int ntiles) {
                 int line, i;
                 if constexpr (ROWS) { line = e / N; i = e % N; }
                 else { line = e % NL; i = e / NL; }
-                lds[lds_pad(line * LS + i * IS)] = t[c];
+                lds[lds_pad(line * LS + i * IS)] = io.finish(t[c]);
             }
         }
     }
     __syncthreads();
+    // store-side twiddles loaded before the stages, so their latency overlaps the transform
+    float2 w[IO::kStoreTwiddle ? IT : 1];
+    if constexpr (IO::kStoreTwiddle) {
+#pragma unroll
+        for (int c = 0; c < IT; ++c) {
+            const int e = tid + c * NT;
+            if (E % NT == 0 || e < E) {
+                int line, i;
+                if constexpr (ROWS) { line = e / N; i = e % N; }
+                else { line = e % NL; i = e / NL; }
+                w[c] = io.twiddle(tile, line, i);
+            }
+        }
+    }
     fft_stages<N, 1, NL, NT, INV, LS, IS>(lds, tw, tid);
-    for (int e = tid; e < E; e += NT) {
-        int line, i;
-        if constexpr (ROWS) { line = e / N; i = e % N; }
-        else { line = e % NL; i = e / NL; }
-        io.store(tile, line, i, lds[lds_pad(line * LS + i * IS)]);
+#pragma unroll
+    for (int c = 0; c < IT; ++c) {
+        const int e = tid + c * NT;
+        if (E % NT == 0 || e < E) {
+            int line, i;
+            if constexpr (ROWS) { line = e / N; i = e % N; }
+            else { line = e % NL; i = e / NL; }
+            io.store(tile, line, i, lds[lds_pad(line * LS + i * IS)], w[IO::kStoreTwiddle ? c : 0]);
+        }
     }
 }
 

@@ -72,21 +72,33 @@ struct ColIO {
         *col = cg * kColWidth + line % kColWidth;
         return g * g_row_mul + i * i_row_mul;
     }
-    __device__ float2 load(int tile, int line, int i) const {
+    // Every lane loads: a point that reads as zero (past the last line group, or in the zero band)
+    // loads element 0 of the source instead and is replaced by 0, so the loads of a batch issue
+    // back to back instead of each behind its own branch.
+    using Raw = float2;
+    __device__ float2 finish(float2 v) const { return v; }
+    __device__ float2 fetch(int tile, int line, int i) const {
         int g, col;
         const int row = row_of(tile, line, i, &g, &col);
-        if (g >= gcount || (row >= rows.zero_lo && row < rows.zero_hi)) return make_float2(0.0f, 0.0f);
-        return op(src, size_t(row) * size_t(pitch) + size_t(col));
+        const bool zero = g >= gcount || (row >= rows.zero_lo && row < rows.zero_hi);
+        const float2 v = op(src, zero ? size_t(0) : size_t(row) * size_t(pitch) + size_t(col));
+        return zero ? make_float2(0.0f, 0.0f) : v;
     }
-    __device__ void store(int tile, int line, int i, float2 v) const {
+    // The split's twiddle W_N^(g i) of a stored point (conjugated for TW < 0), loaded by the tile
+    // kernel before its stages; anything for a point that is not stored.
+    static constexpr bool kStoreTwiddle = TW != 0;
+    __device__ float2 twiddle(int tile, int line, int i) const {
+        int g, col;
+        (void)row_of(tile, line, i, &g, &col);
+        float2 w = tw[(g < gcount ? g : 0) * i * tw_stride];
+        if (TW < 0) w.y = -w.y;
+        return w;
+    }
+    __device__ void store(int tile, int line, int i, float2 v, float2 w) const {
         int g, col;
         const int row = row_of(tile, line, i, &g, &col);
         if (g >= gcount || row >= rows.store_rows) return;
-        if constexpr (TW != 0) {
-            float2 w = tw[g * i * tw_stride];
-            if (TW < 0) w.y = -w.y;
-            v = cmul(v, w);
-        }
+        if constexpr (TW != 0) v = cmul(v, w);
         data[size_t(row) * size_t(pitch) + size_t(col)] = v;
     }
 };
@@ -165,11 +177,24 @@ struct RowIO {
         const int l = tile * lines_per_tile + line;
         return l < split ? l : l + gap;
     }
-    __device__ float2 load(int tile, int line, int i) const {
+    // Row ops split a point's load into fetch (memory only, no branches) and finish (the
+    // arithmetic). A line past the last (a partial last tile) fetches row 0 and reads as zero.
+    struct Raw {
+        typename RowOp::Raw r;
+        bool in;
+    };
+    __device__ Raw fetch(int tile, int line, int i) const {
         const int l = tile * lines_per_tile + line;
-        return l < nrows ? op.load(row_of(tile, line), i) : make_float2(0.0f, 0.0f);
+        const bool in = l < nrows;
+        return Raw{op.fetch(in ? row_of(tile, line) : 0, i), in};
     }
-    __device__ void store(int tile, int line, int i, float2 v) const {
+    __device__ float2 finish(const Raw& t) const {
+        const float2 v = op.finish(t.r);
+        return t.in ? v : make_float2(0.0f, 0.0f);
+    }
+    static constexpr bool kStoreTwiddle = false;
+    __device__ float2 twiddle(int, int, int) const { return make_float2(1.0f, 0.0f); }
+    __device__ void store(int tile, int line, int i, float2 v, float2) const {
         const int l = tile * lines_per_tile + line;
         if (l < nrows) op.store(row_of(tile, line), i, v);
     }
@@ -189,7 +214,9 @@ void rows_pass(hipStream_t s, const float2* tw, int width, int nrows, const RowO
 struct PlainRows {
     float2* data;
     int pitch;
-    __device__ float2 load(int row, int x) const { return data[size_t(row) * pitch + x]; }
+    using Raw = float2;
+    __device__ float2 fetch(int row, int x) const { return data[size_t(row) * pitch + x]; }
+    __device__ float2 finish(float2 v) const { return v; }
     __device__ void store(int row, int x, float2 v) const { data[size_t(row) * pitch + x] = v; }
 };
 

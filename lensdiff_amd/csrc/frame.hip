@@ -72,11 +72,22 @@ struct FrameRows {
     int driver;     // 1: scalar driver; 0: channels ca (+ i cb)
     int ca, cb;     // cb < 0: zero imaginary part
     float scale;
-    __device__ float2 load(int row, int x) const {
-        if (row >= h || x >= w) return make_float2(0.0f, 0.0f);
-        const size_t i = size_t(row) * w + x;
-        const float4 q = decoded(src, i, tf);
-        const float m = mask[i];
+    // fetch: the pixel and its mask value, branch-free (a point outside the frame reads pixel 0);
+    // finish: decode and weigh (0 outside the frame).
+    struct Raw {
+        float4 q;
+        float m;
+        bool in;
+    };
+    __device__ Raw fetch(int row, int x) const {
+        const bool in = row < h && x < w;
+        const size_t i = in ? size_t(row) * w + x : 0;
+        return Raw{src[i], mask[i], in};
+    }
+    __device__ float2 finish(const Raw& t) const {
+        if (!t.in) return make_float2(0.0f, 0.0f);
+        const float4 q = make_float4(decode_channel(t.q.x, tf), decode_channel(t.q.y, tf), decode_channel(t.q.z, tf), t.q.w);
+        const float m = t.m;
         if (driver) {
             const float sig = extraction == LDH_EXTRACT_LUMA ? luma709(q.x, q.y, q.z) : max3(q.x, q.y, q.z);
             return make_float2(sig * m * scale, 0.0f);
@@ -96,13 +107,24 @@ struct KernelRows {
     const float* sb;
     float2* out;
     int K, c, r, pw, ph;
-    __device__ float2 load(int row, int x) const {
+    // fetch: the texels of both kernels, branch-free (a point outside the support reads the centre
+    // texel; a missing second kernel reads the first); finish: scales, and 0 outside the support.
+    struct Raw {
+        float a, b;
+        bool in;
+    };
+    __device__ Raw fetch(int row, int x) const {
         const int dy = row < ph / 2 ? row : row - ph;
         const int dx = x < pw / 2 ? x : x - pw;
-        if (dx < -r || dx > r || dy < -r || dy > r) return make_float2(0.0f, 0.0f);
-        const size_t i = size_t(c + dy) * K + size_t(c + dx);
-        const float a = sa ? ka[i] * *sa : ka[i];
-        const float b = kb ? (sb ? kb[i] * *sb : kb[i]) : 0.0f;
+        const bool in = dx >= -r && dx <= r && dy >= -r && dy <= r;
+        const size_t i = in ? size_t(c + dy) * K + size_t(c + dx) : size_t(c) * K + size_t(c);
+        const float* kbp = kb ? kb : ka;
+        return Raw{ka[i], kbp[i], in};
+    }
+    __device__ float2 finish(const Raw& t) const {
+        if (!t.in) return make_float2(0.0f, 0.0f);
+        const float a = sa ? t.a * *sa : t.a;
+        const float b = kb ? (sb ? t.b * *sb : t.b) : 0.0f;
         return make_float2(a, b);
     }
     __device__ void store(int row, int x, float2 v) const { out[size_t(row) * pw + x] = v; }
@@ -115,7 +137,9 @@ struct ExtractRows {
     float* ob;
     int w, h, pw;
     float scale;
-    __device__ float2 load(int row, int x) const { return data[size_t(row) * pw + x]; }
+    using Raw = float2;
+    __device__ float2 fetch(int row, int x) const { return data[size_t(row) * pw + x]; }
+    __device__ float2 finish(float2 v) const { return v; }
     __device__ void store(int row, int x, float2 v) const {
         if (row >= h || x >= w) return;
         const size_t i = size_t(row) * w + x;
