@@ -152,20 +152,25 @@ template <int N, int Ns> struct StageRadix {
 };
 
 // One Stockham stage over all NL lines of the tile, executed by NT threads.
-template <int N, int Ns, int NL, int NT, bool INV, int LS, int IS>
-__device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict__ tw, int tid) {
+// twl (TWL): the N twiddles W_N^j, j < N, in LDS; else the stage reads the global table.
+template <int N, int Ns, int NL, int NT, bool INV, int LS, int IS, bool TWL>
+__device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict__ tw, const float2* twl, int tid) {
     constexpr int R = StageRadix<N, Ns>::value;
     constexpr int BF = N / R;
     constexpr int TOTAL = NL * BF;
     constexpr int PER = (TOTAL + NT - 1) / NT;
     constexpr bool kPow2 = (R & (R - 1)) == 0;
     constexpr int stride = tw_table(N) / (Ns * R);
+    constexpr int q = N / (Ns * R);  // W_(Ns R)^k = W_N^(k q)
+    // one table twiddle per butterfly and its powers by squaring and products, except for an
+    // odd radix with the table in LDS (every power read from the table)
+    constexpr bool kOne = kPow2 || !TWL;
     const float2* __restrict__ t = tw + tw_offset(N);
     float2 v[PER][R];
     int line_of[PER], j_of[PER];
-    // the stage's table twiddles are loaded first, so they arrive while the LDS reads and the
-    // barrier complete (one per butterfly for power-of-two radices, else R - 1)
-    float2 wt[PER][Ns > 1 ? (kPow2 ? 1 : R - 1) : 1];
+    // the stage's twiddles are read first, so they arrive while the LDS reads and the barrier
+    // complete
+    float2 wt[PER][Ns > 1 ? (kOne ? 1 : R - 1) : 1];
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
         const int u = tid + p * NT;
@@ -177,10 +182,10 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
         if constexpr (Ns > 1) {
             // every lane loads (a lane past TOTAL reads entry 0), so the loads issue together
             const int k = (TOTAL % NT == 0 || u < TOTAL) ? j % Ns : 0;
-            if constexpr (kPow2) wt[p][0] = t[k * stride];
+            if constexpr (kOne) wt[p][0] = TWL ? twl[k * q] : t[k * stride];
             else {
 #pragma unroll
-                for (int r = 1; r < R; ++r) wt[p][r - 1] = t[k * r * stride];
+                for (int r = 1; r < R; ++r) wt[p][r - 1] = twl[k * r * q];
             }
         }
     }
@@ -203,7 +208,7 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
             const int j = j_of[p];
             const int k = j % Ns;
             if constexpr (Ns > 1) {
-                if constexpr (kPow2) {
+                if constexpr (kOne) {
                     // W^k from the table, W^(2^b k) by squaring, the other powers as products
                     // W^(k (r - h)) W^(k h), h the top bit of r
                     float2 w[R];
@@ -236,11 +241,11 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
     __syncthreads();
 }
 
-template <int N, int Ns, int NL, int NT, bool INV, int LS, int IS>
-__device__ __forceinline__ void fft_stages(float2* lds, const float2* __restrict__ tw, int tid) {
+template <int N, int Ns, int NL, int NT, bool INV, int LS, int IS, bool TWL>
+__device__ __forceinline__ void fft_stages(float2* lds, const float2* __restrict__ tw, const float2* twl, int tid) {
     if constexpr (Ns < N) {
-        fft_stage<N, Ns, NL, NT, INV, LS, IS>(lds, tw, tid);
-        fft_stages<N, Ns * StageRadix<N, Ns>::value, NL, NT, INV, LS, IS>(lds, tw, tid);
+        fft_stage<N, Ns, NL, NT, INV, LS, IS, TWL>(lds, tw, twl, tid);
+        fft_stages<N, Ns * StageRadix<N, Ns>::value, NL, NT, INV, LS, IS, TWL>(lds, tw, twl, tid);
     }
 }
 
@@ -256,7 +261,12 @@ template <int N, bool ROWS> struct TileShape {
     static constexpr int NT0 = ELEMS / 8;  // 8 elements per thread (more threads hide the loads)
     static constexpr int NT1 = NT0 < 64 ? 64 : (NT0 > 512 ? 512 : NT0);
     static constexpr int NT = (NT1 + 63) / 64 * 64;
-    static constexpr int LDS_BYTES = (ELEMS + ELEMS / 32 + 1) * 8;
+    static constexpr int DATA_WORDS = ELEMS + ELEMS / 32 + 1;
+    // column tiles whose N has a factor 3 (radix-3 stages read two twiddles per butterfly) keep
+    // the N twiddles W_N^j in LDS after the data; power-of-two tiles read one per butterfly from
+    // the global table
+    static constexpr bool TWL = !ROWS && (N & (N - 1)) != 0;
+    static constexpr int LDS_BYTES = (DATA_WORDS + (TWL ? N : 0)) * 8;
 };
 
 // Maps the hardware workgroup id to a logical tile so that consecutive logical tiles run on
@@ -279,6 +289,12 @@ fft_tile_kernel(IO io, const float2* __restrict__ tw, int ntiles) {
     extern __shared__ float2 lds[];
     const int tile = xcd_tile(blockIdx.x, ntiles);
     const int tid = threadIdx.x;
+    // the column tile's N twiddles: read first (one per thread), stored to LDS with the data
+    static_assert(!S::TWL || N <= NT, "one twiddle per thread");
+    float2 tw_mine = make_float2(1.0f, 0.0f);
+    if constexpr (S::TWL) {
+        if (tid < N) tw_mine = tw[tw_offset(N) + tid * (tw_table(N) / N)];
+    }
     // loads in batches of up to kBatch per thread: every fetch of a batch (memory only) is issued
     // before the first finish (the arithmetic, which may branch on launch-uniform settings) and
     // the LDS writes
@@ -308,22 +324,44 @@ fft_tile_kernel(IO io, const float2* __restrict__ tw, int ntiles) {
             }
         }
     }
+    float2* twl = lds + S::DATA_WORDS;
+    if constexpr (S::TWL) {
+        if (tid < N) twl[tid] = tw_mine;
+    }
     __syncthreads();
     // store-side twiddles loaded before the stages, so their latency overlaps the transform
     float2 w[IO::kStoreTwiddle ? IT : 1];
     if constexpr (IO::kStoreTwiddle) {
+        if constexpr (!ROWS && NT % NL == 0) {
+            // a thread's points share their line (so g) and step i by D = NT / NL: W^(g i) for
+            // i = i0 + c D is W^(g i0) times (W^(g D))^c, two table reads and products
+            constexpr int D = NT / NL;
+            const int line = tid % NL, i0 = tid / NL;
+            const float2 base = io.twiddle(tile, line, i0);
+            float2 pw[IT];
+            pw[0] = make_float2(1.0f, 0.0f);
+            if constexpr (IT > 1) pw[1] = io.twiddle(tile, line, D);
 #pragma unroll
-        for (int c = 0; c < IT; ++c) {
-            const int e = tid + c * NT;
-            if (E % NT == 0 || e < E) {
-                int line, i;
-                if constexpr (ROWS) { line = e / N; i = e % N; }
-                else { line = e % NL; i = e / NL; }
-                w[c] = io.twiddle(tile, line, i);
+            for (int c = 2; c < IT; ++c) {
+                const int h = c >= 8 ? 8 : c >= 4 ? 4 : c >= 2 ? 2 : 1;
+                pw[c] = c == h ? cmul(pw[h / 2], pw[h / 2]) : cmul(pw[c - h], pw[h]);
+            }
+#pragma unroll
+            for (int c = 0; c < IT; ++c) w[c] = c == 0 ? base : cmul(base, pw[c]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < IT; ++c) {
+                const int e = tid + c * NT;
+                if (E % NT == 0 || e < E) {
+                    int line, i;
+                    if constexpr (ROWS) { line = e / N; i = e % N; }
+                    else { line = e % NL; i = e / NL; }
+                    w[c] = io.twiddle(tile, line, i);
+                }
             }
         }
     }
-    fft_stages<N, 1, NL, NT, INV, LS, IS>(lds, tw, tid);
+    fft_stages<N, 1, NL, NT, INV, LS, IS, S::TWL>(lds, tw, twl, tid);
 #pragma unroll
     for (int c = 0; c < IT; ++c) {
         const int e = tid + c * NT;
