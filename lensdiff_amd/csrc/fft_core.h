@@ -295,32 +295,51 @@ fft_tile_kernel(IO io, const float2* __restrict__ tw, int ntiles) {
     if constexpr (S::TWL) {
         if (tid < N) tw_mine = tw[tw_offset(N) + tid * (tw_table(N) / N)];
     }
-    // loads in batches of up to kBatch per thread: every fetch of a batch (memory only) is issued
-    // before the first finish (the arithmetic, which may branch on launch-uniform settings) and
-    // the LDS writes
+    // Column tiles move adjacent column pairs (16 bytes per lane and access): pair p of the tile
+    // is columns 2 (p % NP), 2 (p % NP) + 1 of point p / NP. Needs every thread to own the same
+    // number of pairs and keep its column pair (NT a multiple of NP).
+    constexpr int NP = NL / 2;
+    constexpr bool kPairs = !ROWS && IO::kPairs && (E / 2) % NT == 0 && NT % NP == 0;
     constexpr int IT = (E + NT - 1) / NT;
-    constexpr int kBatch = IT < 16 ? IT : 16;
+    constexpr int IP = kPairs ? E / 2 / NT : 1;  // pairs per thread
+    if constexpr (kPairs) {
+        const int lp = (tid % NP) * 2, i0 = tid / NP;
+        float4 t[IP];
 #pragma unroll
-    for (int c0 = 0; c0 < IT; c0 += kBatch) {
-        typename IO::Raw t[kBatch];
+        for (int c = 0; c < IP; ++c) t[c] = io.fetch_pair(tile, lp, i0 + c * (NT / NP));
 #pragma unroll
-        for (int c = 0; c < kBatch; ++c) {
-            const int e = tid + (c0 + c) * NT;
-            if ((E % NT == 0 && IT % kBatch == 0) || (c0 + c < IT && e < E)) {
-                int line, i;
-                if constexpr (ROWS) { line = e / N; i = e % N; }
-                else { line = e % NL; i = e / NL; }
-                t[c] = io.fetch(tile, line, i);
-            }
+        for (int c = 0; c < IP; ++c) {
+            const int at = lp + (i0 + c * (NT / NP)) * NL;
+            lds[lds_pad(at)] = make_float2(t[c].x, t[c].y);
+            lds[lds_pad(at + 1)] = make_float2(t[c].z, t[c].w);
         }
+    } else {
+        // loads in batches of up to kBatch per thread: every fetch of a batch (memory only) is issued
+        // before the first finish (the arithmetic, which may branch on launch-uniform settings) and
+        // the LDS writes
+        constexpr int kBatch = IT < 16 ? IT : 16;
 #pragma unroll
-        for (int c = 0; c < kBatch; ++c) {
-            const int e = tid + (c0 + c) * NT;
-            if ((E % NT == 0 && IT % kBatch == 0) || (c0 + c < IT && e < E)) {
-                int line, i;
-                if constexpr (ROWS) { line = e / N; i = e % N; }
-                else { line = e % NL; i = e / NL; }
-                lds[lds_pad(line * LS + i * IS)] = io.finish(t[c]);
+        for (int c0 = 0; c0 < IT; c0 += kBatch) {
+            typename IO::Raw t[kBatch];
+#pragma unroll
+            for (int c = 0; c < kBatch; ++c) {
+                const int e = tid + (c0 + c) * NT;
+                if ((E % NT == 0 && IT % kBatch == 0) || (c0 + c < IT && e < E)) {
+                    int line, i;
+                    if constexpr (ROWS) { line = e / N; i = e % N; }
+                    else { line = e % NL; i = e / NL; }
+                    t[c] = io.fetch(tile, line, i);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kBatch; ++c) {
+                const int e = tid + (c0 + c) * NT;
+                if ((E % NT == 0 && IT % kBatch == 0) || (c0 + c < IT && e < E)) {
+                    int line, i;
+                    if constexpr (ROWS) { line = e / N; i = e % N; }
+                    else { line = e % NL; i = e / NL; }
+                    lds[lds_pad(line * LS + i * IS)] = io.finish(t[c]);
+                }
             }
         }
     }
@@ -331,7 +350,22 @@ fft_tile_kernel(IO io, const float2* __restrict__ tw, int ntiles) {
     __syncthreads();
     // store-side twiddles loaded before the stages, so their latency overlaps the transform
     float2 w[IO::kStoreTwiddle ? IT : 1];
-    if constexpr (IO::kStoreTwiddle) {
+    if constexpr (IO::kStoreTwiddle && kPairs) {
+        // a thread's pairs keep their columns (so g) and step i by D = NT / NP
+        constexpr int D = NT / NP;
+        const int lp = (tid % NP) * 2, i0 = tid / NP;
+        const float2 base = io.twiddle(tile, lp, i0);
+        float2 pw[IP];
+        pw[0] = make_float2(1.0f, 0.0f);
+        if constexpr (IP > 1) pw[1] = io.twiddle(tile, lp, D);
+#pragma unroll
+        for (int c = 2; c < IP; ++c) {
+            const int h = c >= 8 ? 8 : c >= 4 ? 4 : c >= 2 ? 2 : 1;
+            pw[c] = c == h ? cmul(pw[h / 2], pw[h / 2]) : cmul(pw[c - h], pw[h]);
+        }
+#pragma unroll
+        for (int c = 0; c < IP; ++c) w[c] = c == 0 ? base : cmul(base, pw[c]);
+    } else if constexpr (IO::kStoreTwiddle) {
         if constexpr (!ROWS && NT % NL == 0) {
             // a thread's points share their line (so g) and step i by D = NT / NL: W^(g i) for
             // i = i0 + c D is W^(g i0) times (W^(g D))^c, two table reads and products
@@ -362,14 +396,24 @@ fft_tile_kernel(IO io, const float2* __restrict__ tw, int ntiles) {
         }
     }
     fft_stages<N, 1, NL, NT, INV, LS, IS, S::TWL>(lds, tw, twl, tid);
+    if constexpr (kPairs) {
+        const int lp = (tid % NP) * 2, i0 = tid / NP;
 #pragma unroll
-    for (int c = 0; c < IT; ++c) {
-        const int e = tid + c * NT;
-        if (E % NT == 0 || e < E) {
-            int line, i;
-            if constexpr (ROWS) { line = e / N; i = e % N; }
-            else { line = e % NL; i = e / NL; }
-            io.store(tile, line, i, lds[lds_pad(line * LS + i * IS)], w[IO::kStoreTwiddle ? c : 0]);
+        for (int c = 0; c < IP; ++c) {
+            const int i = i0 + c * (NT / NP);
+            const int at = lp + i * NL;
+            io.store_pair(tile, lp, i, lds[lds_pad(at)], lds[lds_pad(at + 1)], w[IO::kStoreTwiddle ? c : 0]);
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < IT; ++c) {
+            const int e = tid + c * NT;
+            if (E % NT == 0 || e < E) {
+                int line, i;
+                if constexpr (ROWS) { line = e / N; i = e % N; }
+                else { line = e % NL; i = e / NL; }
+                io.store(tile, line, i, lds[lds_pad(line * LS + i * IS)], w[IO::kStoreTwiddle ? c : 0]);
+            }
         }
     }
 }

@@ -22,15 +22,24 @@ namespace ldh {
 
 // ---- element ops ---------------------------------------------------------------------------
 // extra_bytes: bytes an op reads per element besides the element itself (byte model).
+// pair(d, idx): elements idx and idx + 1 (idx even: one 16-byte load per array).
 struct PlainLoad {
     static constexpr double extra_bytes = 0.0;
     __device__ float2 operator()(const float2* d, size_t idx) const { return d[idx]; }
+    __device__ float4 pair(const float2* d, size_t idx) const { return *reinterpret_cast<const float4*>(d + idx); }
 };
 // Spectrum multiply fused into the first inverse pass: v = d[idx] * k[idx].
 struct MulSpectrumLoad {
     static constexpr double extra_bytes = 8.0;
     const float2* k;
     __device__ float2 operator()(const float2* d, size_t idx) const { return cmul(d[idx], k[idx]); }
+    __device__ float4 pair(const float2* d, size_t idx) const {
+        const float4 a = *reinterpret_cast<const float4*>(d + idx);
+        const float4 b = *reinterpret_cast<const float4*>(k + idx);
+        const float2 lo = cmul(make_float2(a.x, a.y), make_float2(b.x, b.y));
+        const float2 hi = cmul(make_float2(a.z, a.w), make_float2(b.z, b.w));
+        return make_float4(lo.x, lo.y, hi.x, hi.y);
+    }
 };
 // Same with the kernel spectrum stored as FP16 pairs (FP32 arithmetic).
 struct MulSpectrumHalfLoad {
@@ -39,6 +48,13 @@ struct MulSpectrumHalfLoad {
     __device__ float2 operator()(const float2* d, size_t idx) const {
         const float2 kk = __half22float2(k[idx]);
         return cmul(d[idx], kk);
+    }
+    __device__ float4 pair(const float2* d, size_t idx) const {
+        const float4 a = *reinterpret_cast<const float4*>(d + idx);
+        const __half2 k0 = k[idx], k1 = k[idx + 1];
+        const float2 lo = cmul(make_float2(a.x, a.y), __half22float2(k0));
+        const float2 hi = cmul(make_float2(a.z, a.w), __half22float2(k1));
+        return make_float4(lo.x, lo.y, hi.x, hi.y);
     }
 };
 
@@ -84,6 +100,25 @@ struct ColIO {
         const float2 v = op(src, zero ? size_t(0) : size_t(row) * size_t(pitch) + size_t(col));
         return zero ? make_float2(0.0f, 0.0f) : v;
     }
+    // The same for the two adjacent columns line, line + 1 (line even): one 16-byte access.
+    static constexpr bool kPairs = true;
+    __device__ float4 fetch_pair(int tile, int line, int i) const {
+        int g, col;
+        const int row = row_of(tile, line, i, &g, &col);
+        const bool zero = g >= gcount || (row >= rows.zero_lo && row < rows.zero_hi);
+        const float4 v = op.pair(src, zero ? size_t(0) : size_t(row) * size_t(pitch) + size_t(col));
+        return zero ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : v;
+    }
+    __device__ void store_pair(int tile, int line, int i, float2 a, float2 b, float2 w) const {
+        int g, col;
+        const int row = row_of(tile, line, i, &g, &col);
+        if (g >= gcount || row >= rows.store_rows) return;
+        if constexpr (TW != 0) {
+            a = cmul(a, w);
+            b = cmul(b, w);
+        }
+        *reinterpret_cast<float4*>(data + size_t(row) * size_t(pitch) + size_t(col)) = make_float4(a.x, a.y, b.x, b.y);
+    }
     // The split's twiddle W_N^(g i) of a stored point (conjugated for TW < 0), loaded by the tile
     // kernel before its stages; anything for a point that is not stored.
     static constexpr bool kStoreTwiddle = TW != 0;
@@ -108,6 +143,9 @@ void col_pass(hipStream_t s, const float2* tw, const float2* src, float2* data, 
               int g_row_mul, int i_row_mul, int height, const Load& op, ColRows rows, bool inverse) {
     const int nl = col_tile_lines(npts);
     const int groups = nl / kColWidth;
+    // column tiles move 16-byte pairs of adjacent elements
+    if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(data)) & 15) != 0)
+        fail(LDH_ERR_INVALID_ARGUMENT, "fft-column-data-not-16-byte-aligned");
     ColIO<Load, TW> io{src, data, tw + tw_offset(height), width, width / kColWidth, groups, gcount, g_row_mul,
                        i_row_mul, tw_table(height) / height, rows, op};
     const int ntiles = (width / kColWidth) * int(div_up(gcount, groups));
@@ -193,6 +231,9 @@ struct RowIO {
         return t.in ? v : make_float2(0.0f, 0.0f);
     }
     static constexpr bool kStoreTwiddle = false;
+    static constexpr bool kPairs = false;
+    __device__ float4 fetch_pair(int, int, int) const { return make_float4(0.0f, 0.0f, 0.0f, 0.0f); }
+    __device__ void store_pair(int, int, int, float2, float2, float2) const {}
     __device__ float2 twiddle(int, int, int) const { return make_float2(1.0f, 0.0f); }
     __device__ void store(int tile, int line, int i, float2 v, float2) const {
         const int l = tile * lines_per_tile + line;
