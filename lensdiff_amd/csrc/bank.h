@@ -102,6 +102,9 @@ float clear_circle_first_zero(hipStream_t s, const FftTables& t, BankWorkspace& 
 // buildPsfBankGlobalOnly, LensDiffCpuReference.cpp:1702-1734).
 void bank_wavelength_params(const ldh_params& p, const Sizes& sz, const std::vector<float>& wl, float first_zero,
                             std::vector<PhaseCoef>& coefs, std::vector<float>& scales);
+// Whether coefficient sets differ only in defocus (c[0]) and `enabled`, so that one texel
+// evaluation of the other phase terms serves all of them.
+bool defocus_only_coefs(const std::vector<PhaseCoef>& coefs);
 
 // Batched build of many banks (batch.hip). mode: LDH_BATCH_* (include/lensdiff_hip.h).
 struct BatchTiming {

@@ -9,6 +9,7 @@
 // through SeqSum so the scale matches the reference's sequential FP32 accumulation.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "bank.h"
 #include "host_params.h"
@@ -685,6 +686,16 @@ float clear_circle_first_zero(hipStream_t s, const FftTables& t, BankWorkspace& 
     return fz;
 }
 
+bool defocus_only_coefs(const std::vector<PhaseCoef>& coefs) {
+    for (size_t i = 1; i < coefs.size(); ++i) {
+        PhaseCoef a = coefs[0], b = coefs[i];
+        a.c[0] = b.c[0] = 0.0f;
+        a.enabled = b.enabled = 0;
+        if (std::memcmp(&a, &b, sizeof(PhaseCoef)) != 0) return false;
+    }
+    return true;
+}
+
 void bank_wavelength_params(const ldh_params& p, const Sizes& sz, const std::vector<float>& wl, float first_zero,
                             std::vector<PhaseCoef>& coefs, std::vector<float>& scales) {
     const int L = int(wl.size());
@@ -729,16 +740,22 @@ std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWo
     w.scales.reset(sizeof(float) * L);
     LDH_HIP(hipMemcpyAsync(w.coefs.get(), coefs.data(), sizeof(PhaseCoef) * L, hipMemcpyHostToDevice, s), "bank-upload");
     LDH_HIP(hipMemcpyAsync(w.scales.get(), scales.data(), sizeof(float) * L, hipMemcpyHostToDevice, s), "bank-upload");
-    w.phase.reset(sizeof(float) * size_t(P) * P * L);
-    pupil_phase(s, w.coefs.as<PhaseCoef>(), P, L, w.phase.as<float>(), true);
+    // the wavelengths' coefficients differ only in defocus: their phases are formed with the
+    // fields (one evaluation of the other terms per texel), unless some other term differs
+    const bool defocus_only = defocus_only_coefs(coefs);
+    if (!defocus_only) {
+        w.phase.reset(sizeof(float) * size_t(P) * P * L);
+        pupil_phase(s, w.coefs.as<PhaseCoef>(), P, L, w.phase.as<float>(), true);
+    }
 
     // raw PSFs, then per-wavelength resampled base kernels
     w.plan.prepare(s, P, R);
     w.raw.reset(sizeof(float) * size_t(R) * R * L);
     w.totals.reset(sizeof(float) * L);
     // the unit-sum scale of each raw PSF is applied inside the resample
-    w.plan.raw_psfs(s, t, w.amp.as<float>(), w.phase.as<float>(), L, w.raw.as<float>(), w.totals.as<float>(), w.seqsum,
-                    w.psf, false);
+    w.plan.raw_psfs(s, t, w.amp.as<float>(), defocus_only ? nullptr : w.phase.as<float>(), L, w.raw.as<float>(),
+                    w.totals.as<float>(), w.seqsum, w.psf, false, nullptr,
+                    defocus_only ? w.coefs.as<PhaseCoef>() : nullptr);
     const int64_t kk = int64_t(K) * K;
     w.base.reset(sizeof(float) * kk * L);
     w.base_totals.reset(sizeof(float) * L);

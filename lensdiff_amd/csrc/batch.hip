@@ -29,7 +29,8 @@ struct Group {
     std::vector<PhaseCoef> coefs;
     std::vector<float> scales, gains;
     PsfPlan plan;
-    DeviceBuffer d_geoms, d_amp_index, d_coefs, d_scales, d_gains, amp, phase, radius;
+    DeviceBuffer d_geoms, d_amp_index, d_coefs, d_scales, d_gains, d_unit0, amp, phase, radius;
+    bool defocus_only = true;  // every request's coefficient sets differ only in defocus
     std::shared_ptr<DeviceBuffer> store, kernel_scales;  // [which][unit] unit-sum multipliers
 };
 
@@ -44,10 +45,15 @@ void upload(DeviceBuffer& d, const std::vector<T>& h, hipStream_t s) {
 void enqueue_group(hipStream_t s, const FftTables& t, BankWorkspace& w, Group& g) {
     const int P = g.P, R = g.R, U = g.units, K = g.K;
     pupil_amplitudes(s, g.d_geoms.as<PupilGeom>(), P, int(g.requests.size()), g.amp.as<float>(), true);
-    pupil_phase(s, g.d_coefs.as<PhaseCoef>(), P, U, g.phase.as<float>(), true);
     // raw PSFs (unit sums applied inside the resample), base kernels, shaping
-    g.plan.raw_psfs(s, t, g.amp.as<float>(), g.phase.as<float>(), U, w.raw.as<float>(), w.totals.as<float>(), w.seqsum,
-                    w.psf, false, g.d_amp_index.as<int>());
+    if (g.defocus_only) {
+        g.plan.raw_psfs(s, t, g.amp.as<float>(), nullptr, U, w.raw.as<float>(), w.totals.as<float>(), w.seqsum, w.psf,
+                        false, nullptr, g.d_coefs.as<PhaseCoef>(), g.d_unit0.as<int>(), int(g.requests.size()));
+    } else {
+        pupil_phase(s, g.d_coefs.as<PhaseCoef>(), P, U, g.phase.as<float>(), true);
+        g.plan.raw_psfs(s, t, g.amp.as<float>(), g.phase.as<float>(), U, w.raw.as<float>(), w.totals.as<float>(),
+                        w.seqsum, w.psf, false, g.d_amp_index.as<int>());
+    }
     base_kernels(s, w, w.raw.as<float>(), R, g.d_scales.as<float>(), w.totals.as<float>(), g.r, U, w.base.as<float>(),
                  w.base_totals.as<float>());
     shape_kernels(s, w, w.base.as<float>(), w.base_totals.as<float>(), K, U, 0.0, g.r, g.store->as<float>(),
@@ -133,6 +139,7 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
             std::vector<float> scales;
             bank_wavelength_params(p, resolve_sizes(p), wls[size_t(i)], first_zero, coefs, scales);
             g.unit0.push_back(g.units);
+            g.defocus_only = g.defocus_only && defocus_only_coefs(coefs);
             g.geoms.push_back(make_pupil_geom(p));
             const float gain = 1.0f + std::max(0.0f, float(p.anisotropy_emphasis)) * 4.0f;
             for (size_t b = 0; b < coefs.size(); ++b) {
@@ -147,6 +154,9 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
         upload(g.d_geoms, g.geoms, s);
         upload(g.d_amp_index, g.amp_index, s);
         upload(g.d_coefs, g.coefs, s);
+        std::vector<int> bounds = g.unit0;
+        bounds.push_back(g.units);
+        upload(g.d_unit0, bounds, s);
         upload(g.d_scales, g.scales, s);
         upload(g.d_gains, g.gains, s);
         const int64_t kk = int64_t(g.K) * g.K;

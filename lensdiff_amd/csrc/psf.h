@@ -29,9 +29,15 @@ public:
     // sequential-FP32 sums. With `normalize` each PSF is scaled to unit sum, else left for the
     // consumer to scale by 1/total.
     // amp_index (device, nullable): PSF l uses amplitude amp + amp_index[l] * P * P.
+    // defocus_coefs (device, nullable): the `count` phase coefficient sets, for `pupils` pupils
+    // whose sets differ only in defocus and `enabled` within a pupil; the phases are then formed
+    // with the fields and neither `phase` nor amp_index is read: pupil k has amplitude
+    // amp + k * P * P and the units unit0[k] .. unit0[k + 1] - 1 (device, pupils + 1 entries; null
+    // for one pupil).
     void raw_psfs(hipStream_t s, const FftTables& t, const float* amp, const float* phase, int count,
                   float* out, float* totals, SeqSum& seqsum, PsfScratch& scratch, bool normalize = true,
-                  const int* amp_index = nullptr);
+                  const int* amp_index = nullptr, const PhaseCoef* defocus_coefs = nullptr,
+                  const int* unit0 = nullptr, int pupils = 1);
     int pupil() const { return pupil_; }
     int raw() const { return raw_; }
     int bluestein_length() const { return m_; }

@@ -47,8 +47,35 @@ __global__ void pupil_field_kernel(const float* __restrict__ amp, const int* __r
     float rad = 0.0f;
     if (a > 0.0f) rad = phase[int64_t(lam) * pp + i] * 2.0f * kPiF;
     float2 v = make_float2(a * cosf(rad), a * sinf(rad));
-    if (bluestein) v = cmul(v, chirp[x]);
+    if (bluestein) v = cmul_fixed(v, chirp[x]);
     field[int64_t(lam) * pp + i] = v;
+}
+
+// The complex fields of the wavelengths of one or more pupils (workgroup row k: pupil k, its
+// amplitude amp + k * P * P and units unit0[k] .. unit0[k + 1] - 1; unit0 null: one pupil of
+// `units` units), where a pupil's phase coefficient sets differ only in defocus (the chromatic
+// focus offset): each texel's other phase terms are formed once (phase_texel), and each unit's
+// phase, amplitude * (cos, sin)(2 pi phase) and chirp follow as in pupil_phase_kernel and
+// pupil_field_kernel. Transposed layout ([x][y]) as there.
+__global__ void pupil_field_defocus_kernel(const float* __restrict__ amp, const PhaseCoef* __restrict__ coefs,
+                                           const int* __restrict__ unit0, int units, int P,
+                                           const float2* __restrict__ chirp, int bluestein, float2* __restrict__ field) {
+    const int k = blockIdx.y;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t pp = int64_t(P) * P;
+    if (i >= pp) return;
+    const int u0 = unit0 ? unit0[k] : 0, u1 = unit0 ? unit0[k + 1] : units;
+    const int x = int(i / P), y = int(i % P);
+    const float a = amp[int64_t(k) * pp + i];
+    const float2 c = bluestein ? chirp[y] : make_float2(1.0f, 0.0f);
+    const PhaseTexel t = phase_texel(coefs[u0], x, y, P);
+    for (int u = u0; u < u1; ++u) {
+        float rad = 0.0f;
+        if (a > 0.0f) rad = t.phase(coefs[u]) * 2.0f * kPiF;
+        float2 v = make_float2(a * cosf(rad), a * sinf(rad));
+        if (bluestein) v = cmul_fixed(v, c);  // as in pupil_field_kernel, bit for bit
+        field[int64_t(u) * pp + i] = v;
+    }
 }
 
 // Stage-0 input of the row transform of row-line y: its P field values at positions j - off.
@@ -433,7 +460,7 @@ void pupil_phase(hipStream_t s, const PhaseCoef* d_coefs, int n, int count, floa
 
 void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, const float* phase, int count,
                        float* out, float* totals, SeqSum& seqsum, PsfScratch& scratch, bool normalize,
-                       const int* amp_index) {
+                       const int* amp_index, const PhaseCoef* defocus_coefs, const int* unit0, int pupils) {
     const int P = pupil_, R = raw_, L = count;
     scratch.reserve(P, R, L);
     DeviceBuffer& g_ = scratch.g;
@@ -448,9 +475,15 @@ void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, cons
     double* row_sums = row_blocks ? scratch.row_sums.as<double>() : nullptr;
     {
         // amplitude (shared by the wavelengths of a pupil) and phase in, complex field out
-        ProfileScope ps("psf-field", 12.0 * P * P * L);
-        hipLaunchKernelGGL(pupil_field_kernel, dim3(unsigned(div_up(int64_t(P) * P, 256)), unsigned(L)), dim3(256), 0, s,
-                           amp, amp_index, phase, P, chirp, blu, scratch.field.as<float2>());
+        if (defocus_coefs != nullptr) {
+            ProfileScope ps("psf-field", (4.0 * pupils + 8.0 * L) * P * P);
+            hipLaunchKernelGGL(pupil_field_defocus_kernel, dim3(unsigned(div_up(int64_t(P) * P, 256)), unsigned(pupils)),
+                               dim3(256), 0, s, amp, defocus_coefs, unit0, L, P, chirp, blu, scratch.field.as<float2>());
+        } else {
+            ProfileScope ps("psf-field", 12.0 * P * P * L);
+            hipLaunchKernelGGL(pupil_field_kernel, dim3(unsigned(div_up(int64_t(P) * P, 256)), unsigned(L)), dim3(256), 0,
+                               s, amp, amp_index, phase, P, chirp, blu, scratch.field.as<float2>());
+        }
         LDH_CHECK_LAUNCH("psf-field");
     }
     {

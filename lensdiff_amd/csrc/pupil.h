@@ -198,6 +198,71 @@ __device__ inline float pupil_amplitude_at(const PupilGeom& g, int x, int y) {
     return apodization_d(g.apodization, radius / outer);
 }
 
+// The terms of phase_waves_at at one texel, for coefficient sets that differ only in defocus
+// (c[0]) and `enabled`: the other products c_j z_j and the extra terms' sum are formed once, and
+// phase(c0) adds them in the same order as phase_waves_at, so the result is the same float.
+struct PhaseTexel {
+    bool inside;
+    float z4;
+    float prod[13];  // c[1..13] times their terms, in phase_waves_at's order
+    float extra;     // the extra terms' sum (0 without them)
+    bool has_extra;
+    __device__ float phase(const PhaseCoef& p) const {
+        if (!p.enabled || !inside) return 0.0f;
+        float w = p.c[0] * z4;
+#pragma unroll
+        for (int j = 0; j < 13; ++j) w = w + prod[j];
+        if (has_extra) w += extra;
+        return w;
+    }
+};
+__device__ inline PhaseTexel phase_texel(const PhaseCoef& p, int x, int y, int size) {
+    PhaseTexel t{};
+    t.has_extra = p.has_extra != 0;
+    const float outer = 0.86f;
+    const float nx = ((float(x) + 0.5f) / float(size) - 0.5f) * 2.0f;
+    const float ny = ((float(y) + 0.5f) / float(size) - 0.5f) * 2.0f;
+    const float radius = sqrtf(nx * nx + ny * ny);
+    const float rx = nx * p.cos_rot - ny * p.sin_rot;
+    const float ry = nx * p.sin_rot + ny * p.cos_rot;
+    const float sx = rx / outer - p.decenter_x;
+    const float sy = ry / outer - p.decenter_y;
+    const float r2 = sx * sx + sy * sy;
+    t.inside = !(radius > outer) && !(r2 > 1.0f);
+    const float n3 = 1.7320508075688772f, n6 = 2.4494897427831781f, n8 = 2.8284271247461901f;
+    const float n5 = 2.2360679774997898f, n10 = 3.1622776601683795f, n12 = 3.4641016151377544f;
+    t.z4 = n3 * (2.0f * r2 - 1.0f);
+    const float z6 = n6 * (sx * sx - sy * sy);
+    const float z5 = n6 * (2.0f * sx * sy);
+    const float z8 = n8 * ((3.0f * r2 - 2.0f) * sx);
+    const float z7 = n8 * ((3.0f * r2 - 2.0f) * sy);
+    const float z11 = n5 * (6.0f * r2 * r2 - 6.0f * r2 + 1.0f);
+    const float z10 = n8 * (sx * sx * sx - 3.0f * sx * sy * sy);
+    const float z9 = n8 * (3.0f * sx * sx * sy - sy * sy * sy);
+    const float sa = (4.0f * r2 - 3.0f);
+    const float z12 = n10 * sa * (sx * sx - sy * sy);
+    const float z13 = n10 * sa * (2.0f * sx * sy);
+    const float z14 = n10 * (sx * sx * sx * sx - 6.0f * sx * sx * sy * sy + sy * sy * sy * sy);
+    const float z15 = n10 * (4.0f * sx * sy * (sx * sx - sy * sy));
+    const float sc = (10.0f * r2 * r2 - 12.0f * r2 + 3.0f);
+    const float z16 = n12 * sc * sx;
+    const float z17 = n12 * sc * sy;
+    const float z[13] = {z6, z5, z8, z7, z11, z10, z9, z12, z13, z14, z15, z16, z17};
+#pragma unroll
+    for (int j = 0; j < 13; ++j) t.prod[j] = p.c[j + 1] * z[j];
+    if (t.has_extra) {
+        const float c3 = sx * sx * sx - 3.0f * sx * sy * sy;
+        const float s3 = 3.0f * sx * sx * sy - sy * sy * sy;
+        const float x2 = sx * sx, y2 = sy * sy;
+        const float c5 = sx * (x2 * x2 - 10.0f * x2 * y2 + 5.0f * y2 * y2);
+        const float s5 = sy * (5.0f * x2 * x2 - 10.0f * x2 * y2 + y2 * y2);
+        const float r53 = (5.0f * r2 - 4.0f);
+        t.extra = p.extra[0] * (2.0f * sx) + p.extra[1] * (2.0f * sy) + p.extra[2] * (n12 * r53 * c3) +
+                  p.extra[3] * (n12 * r53 * s3) + p.extra[4] * (n12 * c5) + p.extra[5] * (n12 * s5);
+    }
+    return t;
+}
+
 __device__ inline float phase_waves_at(const PhaseCoef& p, int x, int y, int size) {
     if (!p.enabled) return 0.0f;
     const float outer = 0.86f;

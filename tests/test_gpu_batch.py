@@ -45,14 +45,17 @@ def test_batch_modes_build_identical_banks(ctx):
 def test_batched_bank_matches_oracle(ctx, oracle):
     reqs = small_batch()
     banks, _ = build(ctx, reqs, _abi.LDH_BATCH_GRAPH)
-    flips = 0
+    flips = kernels = 0
     for i in (0, 4, 8):
         want = oracle.bank(reqs[i])
         got = banks[i]
         assert abs(got["support"] - want["support"]) <= 1
-        for name in ("full", "core", "structure"):
-            for g, w in zip(got[name], want[name]):
-                flips += g.shape != w.shape
-                assert kernel_rel_l2(g, w) < 1e-4, (i, name)
-    # a radius may sit on its threshold; most do not
-    assert flips <= 3, flips
+        for b in range(len(want["full"])):
+            kernels += 1
+            # one crop radius per wavelength sizes its full, core and structure kernels alike
+            flips += any(got[name][b].shape != want[name][b].shape for name in ("full", "core", "structure"))
+            for name in ("full", "core", "structure"):
+                assert kernel_rel_l2(got[name][b], want[name][b]) < 1e-4, (i, name, b)
+    # a crop radius is an energy threshold on ring sums: one sitting on it moves with the last bit
+    # of the FP32 field (the pupil chirp product), so a few of the radii may differ by one step
+    assert flips <= max(2, kernels // 10), (flips, kernels)
