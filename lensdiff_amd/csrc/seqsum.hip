@@ -9,11 +9,11 @@
 //   1. per-block FP64 sums;
 //   2. their exclusive prefix per segment, which places the running sum of each block to well
 //      within a binade; each block gets a base binade eb just below that estimate;
-//   3. per block, the sums of rounded contributions for the four binades eb .. eb+3 (exact in
+//   3. per block, the sums of rounded contributions for the two binades eb, eb+1 (exact in
 //      FP64);
 //   4. a one-wavefront walk over the blocks in order: a block whose binade sum keeps the
 //      running sum inside its binade is added at once; a block where the running sum crosses
-//      into the next binade, or whose binade lies outside the four computed, is replayed by the
+//      into the next binade, or whose binade lies outside the two computed, is replayed by the
 //      whole workgroup: its 16-element chunks are added through a scan of their rounded
 //      contributions, and the chunk holding the crossing is added element by element in true
 //      FP32.
@@ -34,7 +34,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kPerThread = kSeqSumBlock / kThreads;
-constexpr int kBins = 4;                   // binades per block
+constexpr int kBins = 2;                   // binades per block (the running sum starts in eb or eb+1)
 constexpr int kNoBase = INT_MIN;           // running sum estimated at zero: replay the block
 constexpr double kTied = -1.0;             // a binade sum holding a rounding tie
 
@@ -67,7 +67,8 @@ block_sums_kernel(SeqSegments g, int64_t n, int bl, int nb, double* __restrict__
 
 // Exclusive prefix of the block sums of one segment; base binade of each block placed 1/64
 // below the prefix so that the running sum (within a fraction of a percent of the exact prefix)
-// starts inside the four binades computed for the block.
+// starts inside the two binades computed for the block (a block whose running sum starts higher is
+// replayed, which only costs time).
 __global__ void __launch_bounds__(kThreads)
 prefix_kernel(const double* __restrict__ bsum, int nb, int* __restrict__ ebase) {
     const int seg = blockIdx.x, t = threadIdx.x;
@@ -313,7 +314,7 @@ walk_kernel(SeqSegments g, int64_t n, int bl, int nb, const double* __restrict__
     };
     double cur_sum, nxt_sum;
     int cur_e0, nxt_e0;
-    double cur_b[kBins] = {0.0, 0.0, 0.0, 0.0}, nxt_b[kBins] = {0.0, 0.0, 0.0, 0.0};
+    double cur_b[kBins] = {}, nxt_b[kBins] = {};
     load(lane, cur_sum, cur_e0, cur_b);
 #pragma unroll 1
     for (int g0 = 0; g0 < nb; g0 += 64) {
@@ -328,9 +329,9 @@ walk_kernel(SeqSegments g, int64_t n, int bl, int nb, const double* __restrict__
                 const int q = r.e - cur_e0;
                 if (r.e != kNoBase && cur_e0 != kNoBase && q >= 0 && q < kBins) {
                     col = cur_b[0];
-                    if (q == 1) col = cur_b[1];
-                    if (q == 2) col = cur_b[2];
-                    if (q == 3) col = cur_b[3];
+#pragma unroll
+                    for (int k = 1; k < kBins; ++k)
+                        if (q == k) col = cur_b[k];
                     if (col == kTied) {
                         col = 0.0;
                         force = true;
