@@ -106,37 +106,18 @@ struct ColInput {
     }
 };
 
-// Output k < R of a line: Bluestein unchirps and scales by 1/M.
-// with_chirp: the same with chirp[k] already loaded (c).
-struct LineValue {
-    const float2* __restrict__ chirp;
-    float inv_m;
-    int bluestein;
-    __device__ float2 operator()(int k, float2 v) const { return with_chirp(v, bluestein ? chirp[k] : v); }
-    __device__ float2 with_chirp(float2 v, float2 c) const {
-        if (bluestein) v = cmul(make_float2(v.x * inv_m, v.y * inv_m), c);
-        return v;
-    }
-    // chirp[k] for outputs k < R (k >= R: never used, reads element 0)
-    __device__ float2 pre(int k, int R) const { return bluestein ? chirp[k < R ? k : 0] : make_float2(1.0f, 0.0f); }
-};
-// Output k < R of a row line, times `post` (the column transform's chirp at this line's
-// position, or 1).
+// Output k < R of a row line times `post`: the column transform's chirp at this line's position
+// times 1/M (Bluestein), or 1. Bluestein's output chirp c_k is left out: every value the column
+// transform of frequency row k reads carries the same c_k, so it only multiplies that row of F by
+// a unit-modulus factor, which |F|^2 drops.
 struct RowOutput {
-    static constexpr bool kPrefetch = true;
+    static constexpr bool kPrefetch = false;
     float2* __restrict__ out;
-    LineValue val;
     int R;
     float2 post;
     int bluestein;
-    __device__ void operator()(int k, float2 v) const { put_pre(k, v, pre(k)); }
-    __device__ float2 pre(int k) const { return val.pre(k, R); }
-    __device__ void put_pre(int k, float2 v, float2 c) const {
-        if (k < R) {
-            float2 f = val.with_chirp(v, c);
-            if (bluestein) f = cmul(f, post);
-            out[k] = f;
-        }
+    __device__ void operator()(int k, float2 v) const {
+        if (k < R) out[k] = bluestein ? cmul(v, post) : v;
     }
 };
 // |F|^2 of output k into the centred row: column (k - R/2) mod R (fftShiftSquare, :303-314),
@@ -186,8 +167,9 @@ psf_rows_kernel(const float2* __restrict__ field, int P, int R, int off, const f
     extern __shared__ float2 lds[];
     const int y = blockIdx.x, lam = blockIdx.y, tid = threadIdx.x;
     const RowInput in{field + (size_t(lam) * P + y) * P, P, off};
-    const float2 post = bluestein ? chirp[y] : make_float2(1.0f, 0.0f);
-    const RowOutput out{G + (size_t(lam) * P + y) * R, LineValue{chirp, 1.0f / float(M), bluestein}, R, post, bluestein};
+    const float inv_m = 1.0f / float(M);
+    const float2 post = bluestein ? make_float2(chirp[y].x * inv_m, chirp[y].y * inv_m) : make_float2(1.0f, 0.0f);
+    const RowOutput out{G + (size_t(lam) * P + y) * R, R, post, bluestein};
     line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out, off + P);
 }
 
