@@ -59,7 +59,7 @@ struct BankWorkspace {
     PsfScratch psf;
     SeqSum seqsum;
     DeviceBuffer amp, phase, coefs, raw, totals, base, base_totals, shape_totals, scales, mean, shaped, structure,
-        ring_sum, ring_cnt, ring_peak, radius, ref_amp, ref_phase, ref_raw, ref_total;
+        ring_sum, ring_cnt, ring_peak, ring_mean, radius, ref_amp, ref_phase, ref_raw, ref_total;
     DeviceBuffer block_sums;  // producers' block sums handed to the sequential sums, [3][count][blocks]
     // first-zero radius of the clear-circle PSF per (P, R) (GetLensDiffReferenceRawPsfCached)
     std::vector<std::pair<std::pair<int, int>, float>> first_zero;

@@ -249,22 +249,29 @@ __device__ __forceinline__ void advance_texel(int& x, int& y, int K) {
     }
 }
 
+// Mean of each ring, float(sum) / float(count) (count at least 1), per segment.
+__global__ void ring_means_kernel(const double* __restrict__ sums, const int* __restrict__ counts, int nr,
+                                  float* __restrict__ means) {
+    const int seg = blockIdx.y;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    means[int64_t(seg) * nr + r] = float(sums[int64_t(seg) * nr + r]) / float(max(1, counts[r]));
+}
+
+// The azimuthal-mean image: each texel takes its ring's mean (the ring means precomputed once).
 __global__ void __launch_bounds__(kSumThreads)
-ring_mean_kernel(const double* __restrict__ sums, const int* __restrict__ counts, int K, int rmax,
-                 float* __restrict__ out, double* __restrict__ block_sums) {
+ring_mean_kernel(const float* __restrict__ means, int K, int rmax, float* __restrict__ out,
+                 double* __restrict__ block_sums) {
     const int seg = blockIdx.y;
     const float c = float(K - 1) * 0.5f;
     const int nr = rmax + 1;
     const int kk = K * K, start = int(block_start()), end = min(kk, start + kSeqSumBlock);
     const int i0 = start + int(threadIdx.x);
     int x = i0 % K, y = i0 / K;
-    double sv[kPerSum];
-    int nv[kPerSum];
+    float mv[kPerSum];
 #pragma unroll
     for (int j = 0; j < kPerSum; ++j) {
-        const int r = ring_of<false>(x, y, c, rmax);
-        sv[j] = sums[int64_t(seg) * nr + r];
-        nv[j] = counts[r];
+        mv[j] = means[int64_t(seg) * nr + ring_of<false>(x, y, c, rmax)];
         advance_texel(x, y, K);
     }
     double share[1] = {0.0};
@@ -272,9 +279,8 @@ ring_mean_kernel(const double* __restrict__ sums, const int* __restrict__ counts
     for (int j = 0; j < kPerSum; ++j) {
         const int i = i0 + j * kSumThreads;
         if (i >= end) break;
-        const float v = float(sv[j]) / float(max(1, nv[j]));
-        out[int64_t(seg) * kk + i] = v;
-        share[0] += double(v);
+        out[int64_t(seg) * kk + i] = mv[j];
+        share[0] += double(mv[j]);
     }
     double* const dst[1] = {block_sum_slot(block_sums, 0, gridDim.y)};
     store_block_sums<1>(share, dst);
@@ -385,15 +391,20 @@ crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, float*
     const float extent = fmaxf(1.0f, float(r));
     const float fade = fmaxf(6.0f, fminf(24.0f, extent * 0.04f));
     const float ramp = fmaxf(0.0f, extent - fade);
-    // loads of the texels inside the square first (all of the thread's), then the taper
+    // loads of the texels the taper keeps first (all of the thread's), then the taper: a texel of
+    // the square whose distance is at least extent + 1 (so rr >= extent) gets weight 0 and is
+    // not read
     const int i0 = int(start) + int(threadIdx.x);
     const int64_t o0 = int64_t(bin) * kk;
+    const float far2 = (extent + 1.0f) * (extent + 1.0f);
     float sv[3][kPerSum];
     {
         int x = i0 % K, y = i0 / K;
 #pragma unroll
         for (int j = 0; j < kPerSum; ++j) {
-            const bool inside = i0 + j * kSumThreads < end && abs(x - c) <= r && abs(y - c) <= r;
+            const float fx = float(x - c), fy = float(y - c);
+            const bool inside = i0 + j * kSumThreads < end && abs(x - c) <= r && abs(y - c) <= r &&
+                                fx * fx + fy * fy < far2;
 #pragma unroll
             for (int k = 0; k < 3; ++k) sv[k][j] = inside ? ts.src[k][o0 + i0 + j * kSumThreads] : 0.0f;
             advance_texel(x, y, K);
@@ -521,6 +532,7 @@ void BankWorkspace::reserve_shaping(int K, int count, int max_radius) {
     block_sums.reset(sizeof(double) * size_t(seqsum_blocks(kk)) * 3 * count);
     ring_cnt.reset(sizeof(int) * nr * count);
     ring_peak.reset(sizeof(unsigned) * nr * count);
+    ring_mean.reset(sizeof(float) * nr * count);
     seqsum.reserve(kk, 3 * count);
 }
 
@@ -551,8 +563,11 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
     const dim3 grid = blocks_grid(kk, count);
     {
         ProfileScope ps_("shape-ring-mean", 4.0 * double(kk) * count);
-        hipLaunchKernelGGL(ring_mean_kernel, grid, dim3(kSumThreads), 0, s, w.ring_sum.as<double>(),
-                           w.ring_cnt.as<int>(), K, rmax_mean, w.mean.as<float>(), bsum);
+        float* means = w.ring_mean.as<float>();
+        hipLaunchKernelGGL(ring_means_kernel, dim3(unsigned(div_up(rmax_mean + 1, 256)), unsigned(count)), dim3(256), 0,
+                           s, w.ring_sum.as<double>(), w.ring_cnt.as<int>(), rmax_mean + 1, means);
+        hipLaunchKernelGGL(ring_mean_kernel, grid, dim3(kSumThreads), 0, s, means, K, rmax_mean, w.mean.as<float>(),
+                           bsum);
         LDH_CHECK_LAUNCH("ring-mean");
     }
     w.seqsum.run(s, w.mean.as<float>(), kk, kk, count, t_mean, bsum);
