@@ -332,8 +332,13 @@ residual_kernel(const float* __restrict__ shaped, const float* __restrict__ shap
 }
 
 // estimateAdaptiveSupportRadius + paddedAdaptiveSupportRadius (LensDiffCpuReference.cpp:984-1036),
-// one workgroup per bin: the ring energies and peaks are staged in LDS by the workgroup, then
-// one thread runs the reference's sequential scan (its float running sum) over LDS.
+// one workgroup per bin: the ring energies and peaks are staged in LDS by the workgroup, then one
+// wave finishes. outsidePeak[r] = max(peak[r..rmax]) is a suffix-max scan in 64-ring chunks from
+// the end. The reference's float running sum cum_r = cum_(r-1) + float(e_r) is kept exactly: a
+// chunk's 64 sums are formed one after the other from the chunk's values read across lanes
+// (readlane, no memory round trip per ring), then every lane tests its ring (captured =
+// cum_r / total >= 0.9999 and outsidePeak[r + 1] <= 2e-4 peak) and the first ring that passes,
+// the reference's break, is found by a ballot.
 constexpr int kSupportThreads = 256;
 __global__ void __launch_bounds__(kSupportThreads)
 support_radius_kernel(const double* __restrict__ energy, const unsigned* __restrict__ peaks,
@@ -348,23 +353,49 @@ support_radius_kernel(const double* __restrict__ energy, const unsigned* __restr
         pk[r] = peaks[int64_t(seg) * nr + r];
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int r = rmax - 1; r >= 0; --r) pk[r] = max(pk[r], pk[r + 1]);  // outsidePeak[r]
+    if (threadIdx.x >= 64) return;
+    const int lane = int(threadIdx.x);
+    // suffix maxima, last chunk first; carry: the maximum beyond the chunk
+    unsigned carry = 0u;
+    for (int base = (rmax / 64) * 64; base >= 0; base -= 64) {
+        const int r = base + lane;
+        unsigned v = r <= rmax ? pk[r] : 0u;
+#pragma unroll
+        for (int d = 1; d < 64; d *= 2) {
+            const unsigned t = __shfl_down(v, d);
+            if (lane + d < 64) v = max(v, t);
+        }
+        v = max(v, carry);
+        if (r <= rmax) pk[r] = v;
+        carry = __shfl(v, 0);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const float total = totals[seg];
     const float gpeak = __uint_as_float(pk[0]);
     int est = max(4, rmax);
     if (total > 1e-6f && gpeak > 1e-6f) {
         float cum = 0.0f;
-        for (int r = 0; r <= rmax; ++r) {
-            cum += float(e[r]);
-            const float captured = cum / total;
+        for (int base = 0; base <= rmax; base += 64) {
+            const int r = base + lane;
+            const float ev = r <= rmax ? float(e[r]) : 0.0f;
+            float mine = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 64; ++j) {
+                cum += __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, ev), j));
+                if (lane == j) mine = cum;
+            }
+            const float captured = mine / total;
             const float remaining = r + 1 <= rmax ? __uint_as_float(pk[r + 1]) : 0.0f;
-            if (captured >= 0.9999f && remaining <= gpeak * 2e-4f) {
-                est = max(4, r + 1);
+            const unsigned long long hit = __ballot(r <= rmax && captured >= 0.9999f && remaining <= gpeak * 2e-4f);
+            if (hit != 0ull) {
+                est = max(4, base + __ffsll(static_cast<long long>(hit)) - 1 + 1);
                 break;
             }
         }
     }
+    if (lane != 0) return;
     const int pad = max(6, min(24, int(ceilf(float(max(1, est)) * 0.05f))));
     const int hi = max(4, max_radius);
     radius[seg] = min(max(est + pad, 4), hi);
