@@ -312,7 +312,7 @@ void host_dft(std::vector<double>& re, std::vector<double>& im) {
     }
 }
 
-// Variants of the line kernels: (B, W) = (16, 1), (16, 3), (16, 4), (8, 1), (8, 3).
+// Variants of the line kernels: (B, W) = (16, 1), (16, 3), (16, 4), (8, 8), (8, 3).
 // Tuning keys carry the line length and the batch (wavelength count rounded up to a power of
 // two, 32 and more alike): a one-wavelength launch does not fill the device the way a
 // 31-wavelength bank does.
@@ -347,7 +347,7 @@ bool launch_rows(int m, hipStream_t s, const float2* field, int P, int R, int of
         if constexpr (psf_variants<M>() > 1) {
             if (v == 1) return launch_rows_v<M, 16, 3>(s, field, P, R, off, chirp, bhat, bluestein, G, tw, L);
             if (v == 2) return launch_rows_v<M, 16, 4>(s, field, P, R, off, chirp, bhat, bluestein, G, tw, L);
-            if (v == 3) return launch_rows_v<M, 8, 1>(s, field, P, R, off, chirp, bhat, bluestein, G, tw, L);
+            if (v == 3) return launch_rows_v<M, 8, 8>(s, field, P, R, off, chirp, bhat, bluestein, G, tw, L);
             if (v == 4) return launch_rows_v<M, 8, 3>(s, field, P, R, off, chirp, bhat, bluestein, G, tw, L);
         }
         launch_rows_v<M, 16, 1>(s, field, P, R, off, chirp, bhat, bluestein, G, tw, L);
@@ -362,7 +362,7 @@ bool launch_cols(int m, hipStream_t s, const float2* GT, int P, int R, int off, 
         if constexpr (psf_variants<M>() > 1) {
             if (v == 1) return launch_cols_v<M, 16, 3>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
             if (v == 2) return launch_cols_v<M, 16, 4>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
-            if (v == 3) return launch_cols_v<M, 8, 1>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
+            if (v == 3) return launch_cols_v<M, 8, 8>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
             if (v == 4) return launch_cols_v<M, 8, 3>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
         }
         launch_cols_v<M, 16, 1>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
