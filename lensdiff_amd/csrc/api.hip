@@ -38,6 +38,9 @@ struct ldh_context {
     bool cached = false;
     ldh::BankKey cached_key;
     std::shared_ptr<ldh::DeviceBank> cached_bank;
+    // the nine zone banks of the last field-phase render, keyed by their zones' bank keys
+    std::vector<ldh::BankKey> zone_keys;
+    std::vector<std::shared_ptr<ldh::DeviceBank>> zone_banks;
     // host staging
     float* pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -113,9 +116,48 @@ std::shared_ptr<ldh::DeviceBank> ensure_bank(ldh_context* ctx, const ldh_params&
     return b;
 }
 
-void check_params(const ldh_params* p) {
+// The zone banks of a field-phase render (EnsureLensDiffPsfBank's zone caches,
+// LensDiffCpuReference.cpp:1756-1790), built together in shared launches (the zones have equal
+// sizes) and kept until the zones' parameters change.
+std::vector<const ldh::DeviceBank*> ensure_zone_banks(ldh_context* ctx, const ldh_params& p) {
+    std::vector<ldh_params> zones;
+    std::vector<ldh::BankKey> keys;
+    for (int z = 0; z < ldh::kFieldZones; ++z) {
+        zones.push_back(ldh::field_zone_params(p, z % 3, z / 3));
+        // a zone's bank is built from its resolved global terms alone (buildPsfBankGlobalOnly)
+        zones.back().phase_field_strength = 0.0;
+        keys.push_back(ldh::make_bank_key(zones.back()));
+    }
+    if (keys != ctx->zone_keys || ctx->zone_banks.size() != size_t(ldh::kFieldZones)) {
+        ldh::ProfileScope ps("bank-build", 0.0);
+        ctx->zone_keys.clear();
+        ctx->zone_banks.clear();
+        ctx->zone_banks = ldh::build_banks_batched(ctx->stream, ctx->fft, ctx->bw, zones, LDH_BATCH_GROUPED, nullptr);
+        ctx->zone_keys = keys;
+    }
+    std::vector<const ldh::DeviceBank*> out;
+    for (const auto& b : ctx->zone_banks) out.push_back(b.get());
+    return out;
+}
+
+// Renders the frame at d_src into d_dst with the bank or, for field-dependent phase, the zone banks.
+void render_with_banks(ldh_context* ctx, const ldh_params& p, const float4* d_src, float4* d_dst, int w, int h) {
+    const ldh::EffectMode mode =
+        p.conv_mode == LDH_CONV_PRECOMBINED ? ldh::EffectMode::Precombined : ldh::EffectMode::PerBin;
+    if (ldh::has_field_phase(p)) {
+        const auto banks = ensure_zone_banks(ctx, p);
+        ldh::ProfileScope ps("frame", 0.0);
+        ldh::render_frame_zones(ctx->stream, ctx->fft, ctx->fw, p, banks, d_src, d_dst, w, h, mode);
+        return;
+    }
+    auto bank = ensure_bank(ctx, p);
+    ldh::ProfileScope ps("frame", 0.0);
+    ldh::render_frame(ctx->stream, ctx->fft, ctx->fw, p, *bank, d_src, d_dst, w, h, mode, nullptr);
+}
+
+void check_params(const ldh_params* p, bool allow_field_zones = false) {
     require(p != nullptr, LDH_ERR_INVALID_ARGUMENT, "missing-params");
-    const std::string why = ldh::unsupported_reason(*p);
+    const std::string why = ldh::unsupported_reason(*p, allow_field_zones);
     if (!why.empty()) throw ldh::Error{LDH_ERR_UNSUPPORTED, why};
 }
 
@@ -454,13 +496,8 @@ ldh_status ldh_render_device(ldh_context* ctx, const ldh_params* p, const float*
     return guarded(ctx, [&] {
         require(d_src && d_dst, LDH_ERR_MISSING_BUFFER, "missing-source-or-destination-buffer");
         require(w > 0 && h > 0, LDH_ERR_INVALID_BOUNDS, "invalid-source-bounds");
-        check_params(p);
-        auto bank = ensure_bank(ctx, *p);
-        const ldh::EffectMode mode =
-            p->conv_mode == LDH_CONV_PRECOMBINED ? ldh::EffectMode::Precombined : ldh::EffectMode::PerBin;
-        ldh::ProfileScope ps("frame", 0.0);
-        ldh::render_frame(ctx->stream, ctx->fft, ctx->fw, *p, *bank, reinterpret_cast<const float4*>(d_src),
-                          reinterpret_cast<float4*>(d_dst), w, h, mode, nullptr);
+        check_params(p, true);
+        render_with_banks(ctx, *p, reinterpret_cast<const float4*>(d_src), reinterpret_cast<float4*>(d_dst), w, h);
     });
 }
 
@@ -476,7 +513,7 @@ ldh_status ldh_render(ldh_context* ctx, const ldh_render_request* req, const ldh
         require(w > 0 && h > 0, LDH_ERR_INVALID_BOUNDS, "invalid-source-bounds");
         require(req->src.bytes_per_component == 4 && req->src.components >= 3 && req->dst.bytes_per_component == 4 &&
                     req->dst.components >= 3, LDH_ERR_UNSUPPORTED, "unsupported-pixel-format");
-        check_params(p);
+        check_params(p, true);
         const size_t bytes = sizeof(float) * 4 * size_t(w) * h;
         float* host = pinned(ctx, bytes);
         const auto* sbase = static_cast<const uint8_t*>(req->src.data);
@@ -496,14 +533,7 @@ ldh_status ldh_render(ldh_context* ctx, const ldh_render_request* req, const ldh
         ctx->dsrc.reset(bytes);
         ctx->ddst.reset(bytes);
         h2d(ctx, ctx->dsrc.get(), host, bytes);
-        auto bank = ensure_bank(ctx, *p);
-        const ldh::EffectMode mode =
-            p->conv_mode == LDH_CONV_PRECOMBINED ? ldh::EffectMode::Precombined : ldh::EffectMode::PerBin;
-        {
-            ldh::ProfileScope ps("frame", 0.0);
-            ldh::render_frame(ctx->stream, ctx->fft, ctx->fw, *p, *bank, ctx->dsrc.as<float4>(), ctx->ddst.as<float4>(),
-                              w, h, mode, nullptr);
-        }
+        render_with_banks(ctx, *p, ctx->dsrc.as<float4>(), ctx->ddst.as<float4>(), w, h);
         d2h(ctx, host, ctx->ddst.get(), bytes);
         const ldh_rect db = req->dst.bounds, rw = req->render_window;
         const int x1 = std::max(rw.x1, db.x1), y1 = std::max(rw.y1, db.y1);

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "bank.h"
+#include "host_params.h"
 #include "ldh_internal.h"
 #include "seqsum.h"
 
@@ -49,7 +50,7 @@ void combine_kernels(hipStream_t s, const float* kernels, const float* scales, i
 // Scratch and outputs of one frame on one stream.
 struct FrameWork {
     DeviceBuffer src, dst, mask, image_spec, out_spec, planes, effect, lum, totals, factors, matrices, kspec_tmp,
-        combined, max_buf;
+        combined, max_buf, zone;
     KernelSpectra kspec;  // per-frame kernel spectra (plain path)
     SeqSum seqsum;
 };
@@ -62,6 +63,14 @@ enum class EffectMode { PerBin, Precombined };
 // to compute them in `fw.kspec` for this frame.
 void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
                   const float4* src, float4* dst, int w, int h, EffectMode mode, const KernelSpectra* cached);
+
+// Renders one frame with field-dependent phase (the zone loop of RunLensDiffCpuReference,
+// LensDiffCpuReference.cpp:2160-2170): banks[zy * 3 + zx] is the bank of zone (zx, zy); each
+// zone's effect, core and structure planes are blended into the frame's with the zone's tent
+// weights (accumulateWeightedImage, :1452-1475) before the Preserve rescale and composite.
+void render_frame_zones(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p,
+                        const std::vector<const DeviceBank*>& banks, const float4* src, float4* dst, int w, int h,
+                        EffectMode mode);
 
 // Which real kernels feed the effect, for building cached spectra.
 std::vector<KernelRef> effect_kernels(const ldh_params& p, const DeviceBank& bank, EffectMode mode,

@@ -219,7 +219,9 @@ __device__ __forceinline__ float soft_shoulder(float v, float sh) {
 }
 
 // Split look: effect = shoulder(core) * core_gain + shoulder(structure) * structure_gain.
-__global__ void split_combine_kernel(const float* __restrict__ core, const float* __restrict__ structure, int64_t npx,
+// The shouldered core and structure replace their planes, as applyShoulder leaves the
+// reference's images (LensDiffCpuReference.cpp:2126-2132).
+__global__ void split_combine_kernel(float* __restrict__ core, float* __restrict__ structure, int64_t npx,
                                      float core_sh, float struct_sh, float core_gain, float struct_gain,
                                      float* __restrict__ eff) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -227,10 +229,39 @@ __global__ void split_combine_kernel(const float* __restrict__ core, const float
     for (int c = 0; c < 3; ++c) {
         float a = core[int64_t(c) * npx + i];
         float b = structure[int64_t(c) * npx + i];
-        if (core_sh > 0.0f) a = soft_shoulder(a, core_sh);
-        if (struct_sh > 0.0f) b = soft_shoulder(b, struct_sh);
+        if (core_sh > 0.0f) {
+            a = soft_shoulder(a, core_sh);
+            core[int64_t(c) * npx + i] = a;
+        }
+        if (struct_sh > 0.0f) {
+            b = soft_shoulder(b, struct_sh);
+            structure[int64_t(c) * npx + i] = b;
+        }
         eff[int64_t(c) * npx + i] = a * core_gain + b * struct_gain;
     }
+}
+
+__global__ void zero_kernel(float* __restrict__ x, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        x[i] = 0.0f;
+}
+
+// accumulateWeightedImage (LensDiffCpuReference.cpp:1452-1475) for zone (zx, zy): acc += zone *
+// wx * wy over `planes` planes, the tent weights centred on x = zx (w-1)/2 and y = zy (h-1)/2;
+// pixels where the weight is zero are not touched.
+__global__ void zone_accumulate_kernel(const float* __restrict__ zone, float* __restrict__ acc, int planes, int w,
+                                       int h, int zx, int zy) {
+    const int64_t npx = int64_t(w) * h;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    const int x = int(i % w), y = int(i / w);
+    const float py = (float(y) / float(max(1, h - 1))) * 2.0f;
+    const float wy = fmaxf(0.0f, 1.0f - fabsf(py - float(zy)));
+    const float px = (float(x) / float(max(1, w - 1))) * 2.0f;
+    const float wx = fmaxf(0.0f, 1.0f - fabsf(px - float(zx)));
+    const float wt = wx * wy;
+    if (!(wy > 0.0f) || !(wt > 0.0f)) return;
+    for (int c = 0; c < planes; ++c) acc[int64_t(c) * npx + i] += zone[int64_t(c) * npx + i] * wt;
 }
 
 // Luma of the redistributed image (segment 0) and of the effect (segment 1), for Preserve.
@@ -437,20 +468,10 @@ std::vector<KernelRef> effect_kernels(const ldh_params& p, const DeviceBank& ban
     return ks;
 }
 
-void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
-                  const float4* src, float4* dst, int w, int h, EffectMode mode, const KernelSpectra* cached) {
-    const Plan pl = plan_of(p);
-    const int64_t npx = int64_t(w) * h;
-    const Grid g = make_grid(w, h, bank.support);
-    const size_t cells = g.cells();
-    const float2* tw = t.twiddle.as<float2>();
-    const int n = bank.bins;
-    const SpectrumConfig cfg = build_spectrum_config(p, bank.wavelengths);
-    const float force = std::max(0.0f, std::min(float(p.spectrum_force), 1.0f));
-    const bool with_style = p.spectrum_style != LDH_STYLE_NATURAL && force > 0.0f;
-    if (mode == EffectMode::Precombined && (!pl.spectral || pl.split || pl.sets.front() != kFull))
-        mode = EffectMode::PerBin;  // pre-combination applies to the spectral full-kernel effect
+namespace {
 
+// Step 1: the selection mask of the frame into fw.mask.
+void select_stage(hipStream_t s, FrameWork& fw, const ldh_params& p, const float4* src, int64_t npx) {
     // 1. selection mask
     fw.mask.reset(sizeof(float) * npx);
     SelectParams sp;
@@ -465,6 +486,23 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
         hipLaunchKernelGGL(selection_kernel, grid1(npx), dim3(256), 0, s, src, npx, sp, fw.mask.as<float>());
     LDH_CHECK_LAUNCH("selection");
     }
+}
+
+// Steps 2-4 for one bank: the effect planes [effect, core, structure] (3 each; only the effect
+// planes unless the look is split) into effect9 (npx * 9 floats).
+void effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
+                  const float4* src, int w, int h, EffectMode mode, const KernelSpectra* cached, float* effect9) {
+    const Plan pl = plan_of(p);
+    const int64_t npx = int64_t(w) * h;
+    const Grid g = make_grid(w, h, bank.support);
+    const size_t cells = g.cells();
+    const float2* tw = t.twiddle.as<float2>();
+    const int n = bank.bins;
+    const SpectrumConfig cfg = build_spectrum_config(p, bank.wavelengths);
+    const float force = std::max(0.0f, std::min(float(p.spectrum_force), 1.0f));
+    const bool with_style = p.spectrum_style != LDH_STYLE_NATURAL && force > 0.0f;
+    if (mode == EffectMode::Precombined && (!pl.spectral || pl.split || pl.sets.front() != kFull))
+        mode = EffectMode::PerBin;  // pre-combination applies to the spectral full-kernel effect
     const float redist_scale = 1.0f - float(std::max(0.0, std::min(p.core_preserve, 1.0)));
 
     // 2. image spectra: mono -> (R + iG), (B); spectral -> driver
@@ -486,7 +524,6 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
     fw.out_spec.reset(sizeof(float2) * cells);
     const int nplanes = pl.spectral ? (mode == EffectMode::Precombined ? 6 : n) : 3;
     fw.planes.reset(sizeof(float) * npx * std::max(nplanes, 3));
-    fw.effect.reset(sizeof(float) * npx * 9);
     fw.matrices.reset(sizeof(float) * 6 * n);
     LDH_HIP(hipMemcpyAsync(fw.matrices.as<float>(), cfg.natural.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, s),
             "matrices");
@@ -555,7 +592,7 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
             }
         }
         // effect planes of this set
-        float* eff = fw.effect.as<float>() + npx * 3 * (pl.split ? 1 + int(si) : 0);
+        float* eff = effect9 + npx * 3 * (pl.split ? 1 + int(si) : 0);
         if (!pl.spectral) {
             LDH_HIP(hipMemcpyAsync(eff, planes, sizeof(float) * npx * 3, hipMemcpyDeviceToDevice, s), "effect-copy");
         } else if (mode == EffectMode::PerBin) {
@@ -570,14 +607,22 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
         }
         LDH_CHECK_LAUNCH("effect-map");
     }
-    float* effect = fw.effect.as<float>();
     if (pl.split) {
-        const float* core = effect + npx * 3;
-        const float* structure = effect + npx * 6;
+        float* core = effect9 + npx * 3;
+        float* structure = effect9 + npx * 6;
         hipLaunchKernelGGL(split_combine_kernel, grid1(npx), dim3(256), 0, s, core, structure, npx,
                            float(p.core_shoulder), float(p.structure_shoulder), float(std::max(0.0, p.core_gain)),
-                           float(std::max(0.0, p.structure_gain)), effect);
+                           float(std::max(0.0, p.structure_gain)), effect9);
         LDH_CHECK_LAUNCH("split-combine");
+    }
+}
+
+// Steps 5-6 on the effect planes in fw.effect: Preserve rescale and composite.
+void finish_stage(hipStream_t s, FrameWork& fw, const ldh_params& p, const float4* src, float4* dst, int64_t npx) {
+    const Plan pl = plan_of(p);
+    const float redist_scale = 1.0f - float(std::max(0.0, std::min(p.core_preserve, 1.0)));
+    const float* effect = fw.effect.as<float>();
+    if (pl.split) {
         if (p.debug_view == LDH_VIEW_CORE) effect = effect + npx * 3;
         else if (p.debug_view == LDH_VIEW_STRUCTURE) effect = effect + npx * 6;
     }
@@ -627,6 +672,39 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
                        mask_max, npx, cp, dst);
     LDH_CHECK_LAUNCH("composite");
     }
+}
+
+}  // namespace
+
+void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
+                  const float4* src, float4* dst, int w, int h, EffectMode mode, const KernelSpectra* cached) {
+    const int64_t npx = int64_t(w) * h;
+    select_stage(s, fw, p, src, npx);
+    fw.effect.reset(sizeof(float) * npx * 9);
+    effect_stage(s, t, fw, p, bank, src, w, h, mode, cached, fw.effect.as<float>());
+    finish_stage(s, fw, p, src, dst, npx);
+}
+
+void render_frame_zones(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p,
+                        const std::vector<const DeviceBank*>& banks, const float4* src, float4* dst, int w, int h,
+                        EffectMode mode) {
+    if (banks.size() != size_t(kFieldZones)) fail(LDH_ERR_INVALID_ARGUMENT, "field-zone-bank-count");
+    const int64_t npx = int64_t(w) * h;
+    const int planes = plan_of(p).split ? 9 : 3;
+    select_stage(s, fw, p, src, npx);
+    fw.effect.reset(sizeof(float) * npx * 9);
+    fw.zone.reset(sizeof(float) * npx * 9);
+    hipLaunchKernelGGL(zero_kernel, dim3(unsigned(std::min<int64_t>(div_up(npx * planes, 256), 4096))), dim3(256), 0, s,
+                       fw.effect.as<float>(), npx * planes);
+    LDH_CHECK_LAUNCH("zone-clear");
+    for (int z = 0; z < kFieldZones; ++z) {
+        effect_stage(s, t, fw, p, *banks[size_t(z)], src, w, h, mode, nullptr, fw.zone.as<float>());
+        ProfileScope ps_("frame-zone-blend", 12.0 * planes * double(npx));
+        hipLaunchKernelGGL(zone_accumulate_kernel, grid1(npx), dim3(256), 0, s, fw.zone.as<float>(),
+                           fw.effect.as<float>(), planes, w, h, z % 3, z / 3);
+        LDH_CHECK_LAUNCH("zone-accumulate");
+    }
+    finish_stage(s, fw, p, src, dst, npx);
 }
 
 }  // namespace ldh

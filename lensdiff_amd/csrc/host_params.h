@@ -30,7 +30,16 @@ struct BankKey {
 };
 BankKey make_bank_key(const ldh_params& p);
 
-// Rejects parameters outside this path with a kebab-case tag, or returns "".
-std::string unsupported_reason(const ldh_params& p);
+// Rejects parameters outside this path with a kebab-case tag, or returns "". Field-dependent
+// phase is accepted only where the caller renders field zones (allow_field_zones).
+std::string unsupported_reason(const ldh_params& p, bool allow_field_zones = false);
+
+// Field-dependent phase (HasLensDiffFieldPhase, LensDiffPhase.cpp:40-57): the frame is rendered
+// with one bank per zone of a 3x3 grid over the frame.
+constexpr int kFieldZones = 9;
+bool has_field_phase(const ldh_params& p);
+// The parameters of zone (zx, zy), zx, zy in 0..2 (ResolveLensDiffFieldZoneParams at normalised
+// position (zx - 1, zy - 1), LensDiffPhase.cpp:98-168).
+ldh_params field_zone_params(const ldh_params& p, int zx, int zy);
 
 }  // namespace ldh

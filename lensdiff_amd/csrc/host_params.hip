@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <iterator>
 
 #include "host_params.h"
 
@@ -121,19 +122,59 @@ BankKey make_bank_key(const ldh_params& p) {
     return k;
 }
 
-std::string unsupported_reason(const ldh_params& p) {
-    if (p.aperture_mode == LDH_APERTURE_CUSTOM) return "custom-aperture-not-supported";
-    if (p.aperture_mode < 0 || p.aperture_mode > LDH_APERTURE_SNOWFLAKE) return "invalid-aperture-mode";
-    if (p.phase_mode == LDH_PHASE_ENABLED && std::abs(p.phase_field_strength) > 1e-6) {
-        const double f[14] = {p.phase_field_defocus, p.phase_field_astig_radial, p.phase_field_astig_tangential,
+bool has_field_phase(const ldh_params& p) {
+    if (p.phase_mode != LDH_PHASE_ENABLED || !(std::abs(p.phase_field_strength) > 1e-6)) return false;
+    const double terms[14] = {p.phase_field_defocus, p.phase_field_astig_radial, p.phase_field_astig_tangential,
                               p.phase_field_coma_radial, p.phase_field_coma_tangential, p.phase_field_spherical,
                               p.phase_field_trefoil_radial, p.phase_field_trefoil_tangential,
                               p.phase_field_secondary_astig_radial, p.phase_field_secondary_astig_tangential,
                               p.phase_field_quadrafoil_radial, p.phase_field_quadrafoil_tangential,
                               p.phase_field_secondary_coma_radial, p.phase_field_secondary_coma_tangential};
-        for (double v : f)
-            if (std::abs(v) > 1e-6) return "field-zones-not-supported";
+    return std::any_of(std::begin(terms), std::end(terms), [](double v) { return std::abs(v) > 1e-6; });
+}
+
+ldh_params field_zone_params(const ldh_params& p, int zx, int zy) {
+    ldh_params z = p;
+    if (!has_field_phase(p)) return z;
+    const float nx = float(zx - 1), ny = float(zy - 1);
+    const double radial = std::min(1.0, std::sqrt(double(nx) * nx + double(ny) * ny) / std::sqrt(2.0));
+    const double bias = std::min(std::max(p.phase_field_edge_bias, 0.0), 1.0);
+    const double scale = p.phase_field_strength * std::pow(radial, 1.0 + bias * 2.0);
+    if (std::abs(scale) <= 1e-9) return z;
+    const double angle = std::atan2(double(ny), double(nx));
+    z.phase_defocus += p.phase_field_defocus * scale;
+    z.phase_spherical += p.phase_field_spherical * scale;
+    // (radial, tangential) field pairs rotated by harmonic * angle into the (x, y) pupil terms
+    struct Pair {
+        double radial, tangential;
+        int harmonic;
+        double *x, *y;
+    };
+    const Pair pairs[6] = {
+        {p.phase_field_astig_radial, p.phase_field_astig_tangential, 2, &z.phase_astigmatism0, &z.phase_astigmatism45},
+        {p.phase_field_coma_radial, p.phase_field_coma_tangential, 1, &z.phase_coma_x, &z.phase_coma_y},
+        {p.phase_field_trefoil_radial, p.phase_field_trefoil_tangential, 3, &z.phase_trefoil_x, &z.phase_trefoil_y},
+        {p.phase_field_secondary_astig_radial, p.phase_field_secondary_astig_tangential, 2,
+         &z.phase_secondary_astigmatism0, &z.phase_secondary_astigmatism45},
+        {p.phase_field_quadrafoil_radial, p.phase_field_quadrafoil_tangential, 4, &z.phase_quadrafoil0,
+         &z.phase_quadrafoil45},
+        {p.phase_field_secondary_coma_radial, p.phase_field_secondary_coma_tangential, 1, &z.phase_secondary_coma_x,
+         &z.phase_secondary_coma_y},
+    };
+    for (const Pair& q : pairs) {
+        const double r = q.radial * scale, t = q.tangential * scale;
+        const double a = double(q.harmonic) * angle;
+        const double c = std::cos(a), s = std::sin(a);
+        *q.x += r * c - t * s;
+        *q.y += r * s + t * c;
     }
+    return z;
+}
+
+std::string unsupported_reason(const ldh_params& p, bool allow_field_zones) {
+    if (p.aperture_mode == LDH_APERTURE_CUSTOM) return "custom-aperture-not-supported";
+    if (p.aperture_mode < 0 || p.aperture_mode > LDH_APERTURE_SNOWFLAKE) return "invalid-aperture-mode";
+    if (!allow_field_zones && has_field_phase(p)) return "field-zones-not-supported";
     if (p.wavelength_count < 0 || p.wavelength_count > LDH_MAX_WAVELENGTHS) return "invalid-wavelength-count";
     switch (p.debug_view) {
         case LDH_VIEW_FINAL: case LDH_VIEW_SELECTION: case LDH_VIEW_EFFECT: case LDH_VIEW_CORE:

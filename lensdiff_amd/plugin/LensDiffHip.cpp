@@ -122,7 +122,10 @@ ldh_context* shared_context(std::string* error) {
 bool fill_cache(ldh_context* ctx, const LensDiffParams& params, LensDiffPsfBankCache& cache, std::string* error) {
     const LensDiffPsfBankKey key = MakeLensDiffPsfBankKey(params);
     if (cache.valid && cache.key == key) return true;
-    const ldh_params q = to_ldh(params);
+    ldh_params q = to_ldh(params);
+    // the cache's bins are the global bank (buildPsfBankGlobalOnly, LensDiffCpuReference.cpp:1769);
+    // the zone banks of field-dependent phase stay inside the context
+    q.phase_field_strength = 0.0;
     ldh_bank* bank = nullptr;
     if (ldh_bank_acquire(ctx, &q, &bank) != LDH_OK) {
         if (error) *error = ldh_last_error(ctx);

@@ -277,6 +277,57 @@ inline bool nonflat_phase(const ldh_params& p) {
             has_extra_terms(p));
 }
 
+// ---- Field zones (LensDiffPhase.cpp:40-57, :98-168) ---------------------------------------
+// Field-dependent phase: the frame is rendered with nine banks, one per zone of a 3x3 grid.
+inline bool field_phase(const ldh_params& p) {
+    if (p.phase_mode != LDH_PHASE_ENABLED || std::abs(p.phase_field_strength) <= 1e-6) return false;
+    const double f[14] = {p.phase_field_defocus, p.phase_field_astig_radial, p.phase_field_astig_tangential,
+                          p.phase_field_coma_radial, p.phase_field_coma_tangential, p.phase_field_spherical,
+                          p.phase_field_trefoil_radial, p.phase_field_trefoil_tangential,
+                          p.phase_field_secondary_astig_radial, p.phase_field_secondary_astig_tangential,
+                          p.phase_field_quadrafoil_radial, p.phase_field_quadrafoil_tangential,
+                          p.phase_field_secondary_coma_radial, p.phase_field_secondary_coma_tangential};
+    for (double v : f)
+        if (std::abs(v) > 1e-6) return true;
+    return false;
+}
+
+// A (radial, tangential) pair of field terms rotated by harmonic * angle onto the pupil's
+// (x, y) pair of the same Zernike order.
+inline void add_rotated(double radial, double tangential, double angle, int harmonic, double& x, double& y) {
+    const double a = double(harmonic) * angle;
+    const double c = std::cos(a), s = std::sin(a);
+    x += radial * c - tangential * s;
+    y += radial * s + tangential * c;
+}
+
+// Parameters of the zone at normalised field position (nx, ny) in {-1, 0, 1}^2: the global
+// terms plus the field terms scaled by strength * r^(1 + 2 edge_bias), r = |(nx, ny)| / sqrt 2.
+inline ldh_params field_zone_params(const ldh_params& p, float nx, float ny) {
+    ldh_params z = p;
+    if (!field_phase(p)) return z;
+    const double r = std::min(1.0, std::sqrt(double(nx) * nx + double(ny) * ny) / std::sqrt(2.0));
+    const double bias = clampv(p.phase_field_edge_bias, 0.0, 1.0);
+    const double k = p.phase_field_strength * std::pow(r, 1.0 + bias * 2.0);
+    if (std::abs(k) <= 1e-9) return z;
+    const double angle = std::atan2(double(ny), double(nx));
+    z.phase_defocus += p.phase_field_defocus * k;
+    z.phase_spherical += p.phase_field_spherical * k;
+    add_rotated(p.phase_field_astig_radial * k, p.phase_field_astig_tangential * k, angle, 2, z.phase_astigmatism0,
+                z.phase_astigmatism45);
+    add_rotated(p.phase_field_coma_radial * k, p.phase_field_coma_tangential * k, angle, 1, z.phase_coma_x,
+                z.phase_coma_y);
+    add_rotated(p.phase_field_trefoil_radial * k, p.phase_field_trefoil_tangential * k, angle, 3, z.phase_trefoil_x,
+                z.phase_trefoil_y);
+    add_rotated(p.phase_field_secondary_astig_radial * k, p.phase_field_secondary_astig_tangential * k, angle, 2,
+                z.phase_secondary_astigmatism0, z.phase_secondary_astigmatism45);
+    add_rotated(p.phase_field_quadrafoil_radial * k, p.phase_field_quadrafoil_tangential * k, angle, 4,
+                z.phase_quadrafoil0, z.phase_quadrafoil45);
+    add_rotated(p.phase_field_secondary_coma_radial * k, p.phase_field_secondary_coma_tangential * k, angle, 1,
+                z.phase_secondary_coma_x, z.phase_secondary_coma_y);
+    return z;
+}
+
 // Waves at normalised pupil point (px, py). The 14 reference terms are summed in the
 // reference's order (LensDiffPhase.cpp:199-212); the extension terms (Noll 1976 normalisation,
 // Z2 = 2x, Z3 = 2y, Z18/19 = sqrt(12)(5r^2-4) r^3 cos/sin 3t, Z20/21 = sqrt(12) r^5 cos/sin 5t)
@@ -1315,6 +1366,28 @@ template <typename T> T encode_channel(T x, int tf) {
     return x;
 }
 
+// accumulateWeightedImage (LensDiffCpuReference.cpp:1452-1475): dst += src * wx * wy for the
+// zone's tent weights over the frame (zone centres at x = 0, (w-1)/2, w-1 and likewise in y);
+// pixels with a zero weight are left alone, the others take src's alpha.
+template <typename T> void accumulate_weighted(const Rgba<T>& src, Rgba<T>& dst, int zx, int zy) {
+    const T dx = T(std::max(1, src.w - 1)), dy = T(std::max(1, src.h - 1));
+    for (int y = 0; y < src.h; ++y) {
+        const T py = (T(y) / dy) * T(2);
+        const T wy = std::max(T(0), T(1) - std::abs(py - T(zy)));
+        if (wy <= T(0)) continue;
+        for (int x = 0; x < src.w; ++x) {
+            const T px = (T(x) / dx) * T(2);
+            const T wx = std::max(T(0), T(1) - std::abs(px - T(zx)));
+            const T wt = wx * wy;
+            if (wt <= T(0)) continue;
+            const T* s = src.px(x, y);
+            T* d = dst.px(x, y);
+            for (int c = 0; c < 3; ++c) d[c] += s[c] * wt;
+            d[3] = s[3];
+        }
+    }
+}
+
 // The effect and composite of RunLensDiffCpuReference (LensDiffCpuReference.cpp:2086-2220) on
 // a decoded, tightly packed RGBA image. Returns the output image (before encode) or the debug
 // view; `ok` false for parameters outside this path.
@@ -1344,33 +1417,51 @@ Rgba<T> render_core(const ldh_params& p, const Rgba<T>& src, bool* ok) {
                                                         : T(std::max(0.0, p.effect_gain));
     const T comp = p.energy_mode == LDH_ENERGY_PRESERVE ? gain : T(std::max(0.0, p.core_compensation));
 
+    // renderEffectFromBins (LensDiffCpuReference.cpp:2113-2154) for one bank's bins
+    auto effect_from = [&](const std::vector<Bin<T>>& bins, Rgba<T>& core, Rgba<T>& structure, Rgba<T>& effect) {
+        const Bin<T>& b0 = bins.front();
+        const bool mono = p.wavelength_count <= 0 && p.spectral_mode == LDH_SPECTRAL_MONO;
+        if (mono) {
+            if (p.look_mode == LDH_LOOK_SPLIT) {
+                core = convolve_rgb(redist, b0.core);
+                structure = convolve_rgb(redist, b0.structure);
+                apply_shoulder(core, T(p.core_shoulder));
+                apply_shoulder(structure, T(p.structure_shoulder));
+                effect = add_rgb(core, structure, T(std::max(0.0, p.core_gain)), T(std::max(0.0, p.structure_gain)));
+            } else {
+                effect = convolve_rgb(redist, b0.full);
+                if (p.debug_view == LDH_VIEW_CORE) core = convolve_rgb(redist, b0.core);
+                if (p.debug_view == LDH_VIEW_STRUCTURE) structure = convolve_rgb(redist, b0.structure);
+            }
+        } else {
+            if (p.look_mode == LDH_LOOK_SPLIT) {
+                core = spectral_effect(driver, bins, p, nat_m, sty_m, Which::Core);
+                structure = spectral_effect(driver, bins, p, nat_m, sty_m, Which::Structure);
+                apply_shoulder(core, T(p.core_shoulder));
+                apply_shoulder(structure, T(p.structure_shoulder));
+                effect = add_rgb(core, structure, T(std::max(0.0, p.core_gain)), T(std::max(0.0, p.structure_gain)));
+            } else {
+                effect = spectral_effect(driver, bins, p, nat_m, sty_m, Which::Full);
+                if (p.debug_view == LDH_VIEW_CORE) core = spectral_effect(driver, bins, p, nat_m, sty_m, Which::Core);
+                if (p.debug_view == LDH_VIEW_STRUCTURE) structure = spectral_effect(driver, bins, p, nat_m, sty_m, Which::Structure);
+            }
+        }
+    };
     Rgba<T> core(work.w, work.h), structure(work.w, work.h), effect(work.w, work.h);
-    const Bin<T>& b0 = bank->bins.front();
-    const bool mono = p.wavelength_count <= 0 && p.spectral_mode == LDH_SPECTRAL_MONO;
-    if (mono) {
-        if (p.look_mode == LDH_LOOK_SPLIT) {
-            core = convolve_rgb(redist, b0.core);
-            structure = convolve_rgb(redist, b0.structure);
-            apply_shoulder(core, T(p.core_shoulder));
-            apply_shoulder(structure, T(p.structure_shoulder));
-            effect = add_rgb(core, structure, T(std::max(0.0, p.core_gain)), T(std::max(0.0, p.structure_gain)));
-        } else {
-            effect = convolve_rgb(redist, b0.full);
-            if (p.debug_view == LDH_VIEW_CORE) core = convolve_rgb(redist, b0.core);
-            if (p.debug_view == LDH_VIEW_STRUCTURE) structure = convolve_rgb(redist, b0.structure);
-        }
+    if (!field_phase(p)) {
+        effect_from(bank->bins, core, structure, effect);
     } else {
-        if (p.look_mode == LDH_LOOK_SPLIT) {
-            core = spectral_effect(driver, bank->bins, p, nat_m, sty_m, Which::Core);
-            structure = spectral_effect(driver, bank->bins, p, nat_m, sty_m, Which::Structure);
-            apply_shoulder(core, T(p.core_shoulder));
-            apply_shoulder(structure, T(p.structure_shoulder));
-            effect = add_rgb(core, structure, T(std::max(0.0, p.core_gain)), T(std::max(0.0, p.structure_gain)));
-        } else {
-            effect = spectral_effect(driver, bank->bins, p, nat_m, sty_m, Which::Full);
-            if (p.debug_view == LDH_VIEW_CORE) core = spectral_effect(driver, bank->bins, p, nat_m, sty_m, Which::Core);
-            if (p.debug_view == LDH_VIEW_STRUCTURE) structure = spectral_effect(driver, bank->bins, p, nat_m, sty_m, Which::Structure);
-        }
+        // nine zone banks, each zone's images blended in with its tent weights
+        // (EnsureLensDiffPsfBank :1756-1790, the zone loop :2160-2170)
+        for (int zy = 0; zy < 3; ++zy)
+            for (int zx = 0; zx < 3; ++zx) {
+                std::unique_ptr<Bank<T>> zb(build_bank<T>(field_zone_params(p, float(zx - 1), float(zy - 1))));
+                Rgba<T> zc(work.w, work.h), zs(work.w, work.h), ze(work.w, work.h);
+                effect_from(zb->bins, zc, zs, ze);
+                accumulate_weighted(zc, core, zx, zy);
+                accumulate_weighted(zs, structure, zx, zy);
+                accumulate_weighted(ze, effect, zx, zy);
+            }
     }
     if (p.energy_mode == LDH_ENERGY_PRESERVE) {
         const T in_e = luma_total(redist), eff_e = luma_total(effect);

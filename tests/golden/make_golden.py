@@ -29,6 +29,11 @@ def params(**kw):
     return p
 
 
+FIELD = dict(phase_field_strength=1.0, phase_field_edge_bias=0.3, phase_field_defocus=0.15,
+             phase_field_coma_radial=0.3, phase_field_astig_tangential=0.2, phase_field_trefoil_radial=0.1,
+             phase_field_quadrafoil_tangential=0.05, phase_field_secondary_coma_radial=-0.07)
+
+
 def pack_params(p) -> np.ndarray:
     return np.frombuffer(bytes(p), np.uint8).copy()
 
@@ -124,6 +129,9 @@ def main():
         "mono_split_preserve": dict(look_mode=1, energy_mode=0, core_preserve=0.1, core_shoulder=2.0),
         "tristimulus": dict(spectral_mode=1, chromatic_focus=0.3),
         "spectral9_split": dict(spectral_mode=3, look_mode=1),
+        # field-dependent phase: nine zone banks blended with tent weights
+        "field_zones_split": dict(look_mode=1, **FIELD),
+        "field_zones_tristimulus": dict(spectral_mode=1, chromatic_focus=0.3, energy_mode=1, **FIELD),
     }
     frame = workloads.frame(48, 40, 0.01, seed=7)
     out["frame_in"] = frame

@@ -213,6 +213,11 @@ def small_frame(w, h, seed=0):
     return workloads.frame(w, h, 0.004, seed=seed)
 
 
+# field terms of every order, strong enough that the corner zones differ visibly from the centre
+FIELD = dict(phase_field_strength=1.0, phase_field_edge_bias=0.3, phase_field_defocus=0.15,
+             phase_field_coma_radial=0.3, phase_field_astig_tangential=0.2, phase_field_trefoil_radial=0.1,
+             phase_field_quadrafoil_tangential=0.05, phase_field_secondary_coma_radial=-0.07)
+
 RENDER_CASES = {
     "mono-physical-augment": dict(),
     "mono-split-preserve": dict(look_mode=1, energy_mode=0, core_preserve=0.1, core_shoulder=2.0,
@@ -223,6 +228,15 @@ RENDER_CASES = {
     "spectral9-split-luma": dict(spectral_mode=3, look_mode=1, extraction_mode=1),
     "list16": dict(wavelengths=np.linspace(400, 700, 16)),
     "davinci-transfer": dict(input_transfer=1, spectral_mode=1),
+    # the split look's core view shows the shouldered core
+    "split-core-view-shoulder": dict(look_mode=1, core_shoulder=1.5, debug_view=_abi.VIEW_CORE),
+    # field-dependent phase: nine zone banks, zone images blended with tent weights
+    "field-zones-split": dict(look_mode=1, core_shoulder=1.0, **FIELD),
+    "field-zones-tristimulus-preserve": dict(spectral_mode=1, chromatic_focus=0.3,
+                                             energy_mode=_abi.ENERGY_PRESERVE, **FIELD),
+    "field-zones-structure-view": dict(look_mode=1, structure_shoulder=1.0, debug_view=_abi.VIEW_STRUCTURE,
+                                       **FIELD),
+    "field-zones-mono-full": dict(look_mode=0, energy_mode=_abi.ENERGY_AUGMENT, **FIELD),
 }
 
 

@@ -108,7 +108,8 @@ def test_convolve(g, oracle):
     assert np.array_equal(oracle.convolve(g["conv_plane"], g["conv_kernel"]), g["conv_out"])
 
 
-@pytest.mark.parametrize("name", ["mono_physical_augment", "mono_split_preserve", "tristimulus", "spectral9_split"])
+@pytest.mark.parametrize("name", ["mono_physical_augment", "mono_split_preserve", "tristimulus", "spectral9_split",
+                                  "field_zones_split", "field_zones_tristimulus"])
 def test_frame(g, oracle, name):
     got = oracle.render_image(unpack(g["frame_%s_params" % name]), g["frame_in"])
     assert np.array_equal(got, g["frame_%s" % name])
