@@ -148,6 +148,26 @@ struct ExtractRows {
     }
 };
 
+// Row pass of the forward transform of two real planes packed as a + i b (b nullable: 0), zero
+// outside the frame; fetch is branch-free (a point outside reads pixel 0).
+struct PlaneRows {
+    const float* a;
+    const float* b;
+    float2* out;
+    int w, h, pw;
+    struct Raw {
+        float a, b;
+        bool in;
+    };
+    __device__ Raw fetch(int row, int x) const {
+        const bool in = row < h && x < w;
+        const size_t i = in ? size_t(row) * w + x : 0;
+        return Raw{a[i], b ? b[i] : 0.0f, in};
+    }
+    __device__ float2 finish(const Raw& t) const { return t.in ? make_float2(t.a, t.b) : make_float2(0.0f, 0.0f); }
+    __device__ void store(int row, int x, float2 v) const { out[size_t(row) * pw + x] = v; }
+};
+
 __global__ void to_half_kernel(const float2* __restrict__ in, __half2* __restrict__ out, int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) out[i] = __float22half2_rn(in[i]);
@@ -295,6 +315,55 @@ struct CompositeParams {
     float comp, gain;   // core compensation, effect gain
     float max_sub;      // (1 - protect) / scale
 };
+
+// eff *= factor[0] where the Preserve rescale applies (factor[1] > 0): scaleImage on the effect
+// before the post passes read it (LensDiffCpuReference.cpp:2172-2178).
+__global__ void scale_planes_kernel(float* __restrict__ eff, int64_t n, const float* __restrict__ factor) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || !(factor[1] > 0.0f)) return;
+    eff[i] *= factor[0];
+}
+
+// eff = eff * 1 + add * gain over 3 planes (addImages, :2185).
+__global__ void add_scaled_kernel(float* __restrict__ eff, const float* __restrict__ add, int64_t n, float gain) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    eff[i] = eff[i] * 1.0f + add[i] * gain;
+}
+
+// Bilinear sample of one plane at (fx, fy) clamped to the frame (sampleRgbaBilinear, :1477-1497).
+__device__ __forceinline__ float sample_plane(const float* __restrict__ pl, int w, int h, float fx, float fy) {
+    const float x = fminf(fmaxf(fx, 0.0f), float(max(0, w - 1)));
+    const float y = fminf(fmaxf(fy, 0.0f), float(max(0, h - 1)));
+    const int x0 = int(floorf(x)), y0 = int(floorf(y));
+    const int x1 = min(x0 + 1, w - 1), y1 = min(y0 + 1, h - 1);
+    const float tx = x - float(x0), ty = y - float(y0);
+    const float a = pl[int64_t(y0) * w + x0] * (1.0f - tx) + pl[int64_t(y0) * w + x1] * tx;
+    const float b = pl[int64_t(y1) * w + x0] * (1.0f - tx) + pl[int64_t(y1) * w + x1] * tx;
+    return a * (1.0f - ty) + b * ty;
+}
+
+// applyCreativeFringe (:1499-1534): red sampled `amount` pixels outward along the ray from the
+// frame centre, blue as far inward, green in place; preview = |fringed - effect| per channel.
+__global__ void fringe_kernel(const float* __restrict__ eff, float* __restrict__ out, float* __restrict__ preview,
+                              int w, int h, float amount) {
+    const int64_t npx = int64_t(w) * h;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    const int x = int(i % w), y = int(i / w);
+    const float cx = float(max(0, w - 1)) * 0.5f, cy = float(max(0, h - 1)) * 0.5f;
+    const float dx = float(x) - cx, dy = float(y) - cy;
+    const float len = sqrtf(dx * dx + dy * dy);
+    const float il = len > 1e-6f ? 1.0f / len : 0.0f;
+    const float sx = dx * il * amount, sy = dy * il * amount;
+    const float v[3] = {sample_plane(eff, w, h, float(x) + sx, float(y) + sy),
+                        sample_plane(eff + npx, w, h, float(x), float(y)),
+                        sample_plane(eff + 2 * npx, w, h, float(x) - sx, float(y) - sy)};
+    for (int c = 0; c < 3; ++c) {
+        out[int64_t(c) * npx + i] = v[c];
+        preview[int64_t(c) * npx + i] = fabsf(v[c] - eff[int64_t(c) * npx + i]);
+    }
+}
 
 __global__ void composite_kernel(const float4* __restrict__ src, const float* __restrict__ mask,
                                  const float* __restrict__ eff, const float* __restrict__ factor,
@@ -618,7 +687,64 @@ void effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
 }
 
 // Steps 5-6 on the effect planes in fw.effect: Preserve rescale and composite.
-void finish_stage(hipStream_t s, FrameWork& fw, const ldh_params& p, const float4* src, float4* dst, int64_t npx) {
+// buildGaussianKernel (LensDiffCpuReference.cpp:1680-1700) on the host: unnormalised weights
+// exp(-(x^2+y^2) / (2 sigma^2)) summed in raster order in float, then each divided by the sum.
+std::vector<float> gaussian_kernel(float radius_px, int& size) {
+    const int r = std::max(1, int(std::ceil(std::max(0.5f, radius_px))));
+    size = 2 * r + 1;
+    std::vector<float> k(size_t(size) * size);
+    const float sigma = std::max(0.5f, radius_px * 0.5f);
+    const float two_s2 = 2.0f * sigma * sigma;
+    float sum = 0.0f;
+    for (int y = -r; y <= r; ++y)
+        for (int x = -r; x <= r; ++x) {
+            const float v = std::exp(-(float(x * x + y * y) / two_s2));
+            k[size_t(y + r) * size + size_t(x + r)] = v;
+            sum += v;
+        }
+    if (sum > 0.0f)
+        for (float& v : k) v /= sum;
+    return k;
+}
+
+// The scatter image (convolveRgba of the effect with the Gaussian, :2183-2184): effect planes
+// (R + iG) and (B) transformed on the smallest grid for the kernel's support, multiplied by the
+// kernel's spectrum in the first inverse pass, cropped and clamped at zero into out3.
+void scatter_pass(hipStream_t s, const FftTables& t, FrameWork& fw, float radius_px, const float* eff, int w, int h,
+                  float* out3) {
+    int K = 0;
+    const std::vector<float> k = gaussian_kernel(radius_px, K);
+    const int r = K / 2;
+    const int64_t npx = int64_t(w) * h;
+    fw.gauss.reset(sizeof(float) * k.size());
+    LDH_HIP(hipMemcpyAsync(fw.gauss.get(), k.data(), sizeof(float) * k.size(), hipMemcpyHostToDevice, s),
+            "scatter-kernel");
+    const Grid g = make_grid(w, h, r);
+    const size_t cells = g.cells();
+    const float2* tw = t.twiddle.as<float2>();
+    ProfileScope ps_("frame-scatter", 0.0);
+    build_kernel_spectra(s, t, {KernelRef{fw.gauss.as<float>(), nullptr}}, K, r, g, false, fw.kspec, fw.kspec_tmp);
+    fw.image_spec.reset(sizeof(float2) * cells * 2);
+    fw.out_spec.reset(sizeof(float2) * cells);
+    for (int j = 0; j < 2; ++j) {
+        float2* spec = fw.image_spec.as<float2>() + cells * j;
+        PlaneRows op{eff + npx * (j == 0 ? 0 : 2), j == 0 ? eff + npx : nullptr, spec, w, h, g.pw};
+        rows_pass(s, tw, g.pw, h, op, false);
+        ColRows zero;
+        zero.zero_lo = h;
+        zero.zero_hi = g.ph;
+        cols_forward(s, tw, spec, g.pw, g.ph, PlainLoad{}, zero);
+        float2* o = fw.out_spec.as<float2>();
+        cols_inverse(s, tw, spec, o, g.pw, g.ph, MulSpectrumLoad{fw.kspec.data.as<float2>()}, h);
+        ExtractRows ex{o, out3 + npx * (j == 0 ? 0 : 2), j == 0 ? out3 + npx : nullptr, w, h, g.pw,
+                       1.0f / float(cells)};
+        rows_pass(s, tw, g.pw, h, ex, true);
+    }
+}
+
+void finish_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const float4* src,
+                  float4* dst, int w, int h) {
+    const int64_t npx = int64_t(w) * h;
     const Plan pl = plan_of(p);
     const float redist_scale = 1.0f - float(std::max(0.0, std::min(p.core_preserve, 1.0)));
     const float* effect = fw.effect.as<float>();
@@ -644,6 +770,49 @@ void finish_stage(hipStream_t s, FrameWork& fw, const ldh_params& p, const float
         factor = fw.factors.as<float>();
     }
 
+    // 5b. scatter and creative fringe on the rescaled effect (:2179-2190); they change only the
+    // final image and the effect, scatter and fringe views
+    const bool scatter = p.scatter_amount > 1e-6 && scatter_radius_px(p) > 0.25;
+    const float fringe = float(std::max(0.0, creative_fringe_px(p)));
+    const bool post_view = p.debug_view == LDH_VIEW_SCATTER || p.debug_view == LDH_VIEW_CREATIVE_FRINGE;
+    const bool post_out = p.debug_view == LDH_VIEW_FINAL || p.debug_view == LDH_VIEW_EFFECT || post_view;
+    const float* post_effect = fw.effect.as<float>();  // the effect after the post passes
+    const float* view_planes = nullptr;                // the scatter or fringe view
+    if (post_out && (scatter || fringe > 1e-6f || post_view)) {
+        fw.post.reset(sizeof(float) * npx * 9);  // [scatter image][fringe preview][fringed effect]
+        float* eff = fw.effect.as<float>();
+        float* scat = fw.post.as<float>();
+        float* preview = scat + npx * 3;
+        float* fringed = scat + npx * 6;
+        if (factor) {
+            hipLaunchKernelGGL(scale_planes_kernel, grid1(npx * 3), dim3(256), 0, s, eff, npx * 3, factor);
+            LDH_CHECK_LAUNCH("preserve-scale");
+            factor = nullptr;
+        }
+        if (scatter) {
+            scatter_pass(s, t, fw, float(scatter_radius_px(p)), eff, w, h, scat);
+            hipLaunchKernelGGL(add_scaled_kernel, grid1(npx * 3), dim3(256), 0, s, eff, scat, npx * 3,
+                               float(std::max(0.0, p.scatter_amount)));
+            LDH_CHECK_LAUNCH("scatter-add");
+        } else if (p.debug_view == LDH_VIEW_SCATTER) {
+            hipLaunchKernelGGL(zero_kernel, dim3(unsigned(std::min<int64_t>(div_up(npx * 3, 256), 4096))), dim3(256), 0,
+                               s, scat, npx * 3);
+            LDH_CHECK_LAUNCH("scatter-clear");
+        }
+        if (fringe > 1e-6f) {
+            ProfileScope ps_("frame-fringe", 40.0 * double(npx));
+            hipLaunchKernelGGL(fringe_kernel, grid1(npx), dim3(256), 0, s, eff, fringed, preview, w, h, fringe);
+            LDH_CHECK_LAUNCH("fringe");
+            post_effect = fringed;
+        } else if (p.debug_view == LDH_VIEW_CREATIVE_FRINGE) {
+            hipLaunchKernelGGL(zero_kernel, dim3(unsigned(std::min<int64_t>(div_up(npx * 3, 256), 4096))), dim3(256), 0,
+                               s, preview, npx * 3);
+            LDH_CHECK_LAUNCH("fringe-clear");
+        }
+        if (p.debug_view == LDH_VIEW_SCATTER) view_planes = scat;
+        if (p.debug_view == LDH_VIEW_CREATIVE_FRINGE) view_planes = preview;
+    }
+
     // 6. composite
     const unsigned* mask_max = nullptr;
     if (p.debug_view == LDH_VIEW_SELECTION) {
@@ -661,10 +830,14 @@ void finish_stage(hipStream_t s, FrameWork& fw, const ldh_params& p, const float
     cp.gain = gain;
     cp.max_sub = redist_scale > 1e-6f ? (1.0f - protect) / redist_scale : 0.0f;
     const float* eff_out = (p.debug_view == LDH_VIEW_FINAL || p.debug_view == LDH_VIEW_EFFECT || !pl.split)
-                               ? fw.effect.as<float>() : effect;
+                               ? post_effect : effect;
     if (p.debug_view == LDH_VIEW_CORE || p.debug_view == LDH_VIEW_STRUCTURE) {
         eff_out = effect;
         factor = nullptr;  // the reference rescales only the effect image
+    }
+    if (view_planes) {
+        eff_out = view_planes;
+        factor = nullptr;
     }
 {
     ProfileScope ps_("frame-composite", 56.0 * double(npx));
@@ -682,7 +855,7 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
     select_stage(s, fw, p, src, npx);
     fw.effect.reset(sizeof(float) * npx * 9);
     effect_stage(s, t, fw, p, bank, src, w, h, mode, cached, fw.effect.as<float>());
-    finish_stage(s, fw, p, src, dst, npx);
+    finish_stage(s, t, fw, p, src, dst, w, h);
 }
 
 void render_frame_zones(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p,
@@ -704,7 +877,7 @@ void render_frame_zones(hipStream_t s, const FftTables& t, FrameWork& fw, const 
                            fw.effect.as<float>(), planes, w, h, z % 3, z / 3);
         LDH_CHECK_LAUNCH("zone-accumulate");
     }
-    finish_stage(s, fw, p, src, dst, npx);
+    finish_stage(s, t, fw, p, src, dst, w, h);
 }
 
 }  // namespace ldh

@@ -34,6 +34,11 @@ BankKey make_bank_key(const ldh_params& p);
 // phase is accepted only where the caller renders field zones (allow_field_zones).
 std::string unsupported_reason(const ldh_params& p, bool allow_field_zones = false);
 
+// Scatter radius and creative-fringe shift in pixels (ResolveLensDiffScatterRadiusPx,
+// ResolveLensDiffCreativeFringePx, LensDiffTypes.h:322-328).
+double scatter_radius_px(const ldh_params& p);
+double creative_fringe_px(const ldh_params& p);
+
 // Field-dependent phase (HasLensDiffFieldPhase, LensDiffPhase.cpp:40-57): the frame is rendered
 // with one bank per zone of a 3x3 grid over the frame.
 constexpr int kFieldZones = 9;

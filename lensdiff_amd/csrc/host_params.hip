@@ -122,6 +122,9 @@ BankKey make_bank_key(const ldh_params& p) {
     return k;
 }
 
+double scatter_radius_px(const ldh_params& p) { return percent_to_px(p.scatter_radius, optics_short_side(p)); }
+double creative_fringe_px(const ldh_params& p) { return percent_to_px(p.creative_fringe, optics_short_side(p)); }
+
 bool has_field_phase(const ldh_params& p) {
     if (p.phase_mode != LDH_PHASE_ENABLED || !(std::abs(p.phase_field_strength) > 1e-6)) return false;
     const double terms[14] = {p.phase_field_defocus, p.phase_field_astig_radial, p.phase_field_astig_tangential,
@@ -178,12 +181,9 @@ std::string unsupported_reason(const ldh_params& p, bool allow_field_zones) {
     if (p.wavelength_count < 0 || p.wavelength_count > LDH_MAX_WAVELENGTHS) return "invalid-wavelength-count";
     switch (p.debug_view) {
         case LDH_VIEW_FINAL: case LDH_VIEW_SELECTION: case LDH_VIEW_EFFECT: case LDH_VIEW_CORE:
-        case LDH_VIEW_STRUCTURE: break;
+        case LDH_VIEW_STRUCTURE: case LDH_VIEW_CREATIVE_FRINGE: case LDH_VIEW_SCATTER: break;
         default: return "debug-view-not-supported";
     }
-    if (p.scatter_amount > 1e-6 && percent_to_px(p.scatter_radius, optics_short_side(p)) > 0.25)
-        return "scatter-not-supported";
-    if (percent_to_px(p.creative_fringe, optics_short_side(p)) > 1e-6) return "creative-fringe-not-supported";
     if (p.resolution_aware) {
         const double ws = double(optics_short_side(p)) / double(std::max(1, p.frame_short_side_px));
         if (std::abs(ws - 1.0) > 1e-6) return "resolution-aware-not-supported";

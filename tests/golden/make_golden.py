@@ -132,6 +132,11 @@ def main():
         # field-dependent phase: nine zone banks blended with tent weights
         "field_zones_split": dict(look_mode=1, **FIELD),
         "field_zones_tristimulus": dict(spectral_mode=1, chromatic_focus=0.3, energy_mode=1, **FIELD),
+        # post passes on the effect: Gaussian scatter and creative fringe, and their views
+        "scatter_fringe_split": dict(look_mode=1, scatter_amount=0.3, scatter_radius=2.5, creative_fringe=1.5),
+        "scatter_view": dict(scatter_amount=0.5, scatter_radius=3.0, debug_view=_abi.VIEW_SCATTER),
+        "fringe_view_preserve": dict(creative_fringe=2.0, energy_mode=_abi.ENERGY_PRESERVE,
+                                     debug_view=_abi.VIEW_CREATIVE_FRINGE),
     }
     frame = workloads.frame(48, 40, 0.01, seed=7)
     out["frame_in"] = frame

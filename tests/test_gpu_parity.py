@@ -237,6 +237,15 @@ RENDER_CASES = {
     "field-zones-structure-view": dict(look_mode=1, structure_shoulder=1.0, debug_view=_abi.VIEW_STRUCTURE,
                                        **FIELD),
     "field-zones-mono-full": dict(look_mode=0, energy_mode=_abi.ENERGY_AUGMENT, **FIELD),
+    # post passes: Gaussian scatter and creative fringe on the (rescaled) effect, and their views
+    "scatter-fringe-split": dict(look_mode=1, scatter_amount=0.3, scatter_radius=2.5, creative_fringe=1.5),
+    "scatter-preserve": dict(energy_mode=_abi.ENERGY_PRESERVE, scatter_amount=0.6, scatter_radius=4.0),
+    "scatter-view": dict(scatter_amount=0.5, scatter_radius=3.0, debug_view=_abi.VIEW_SCATTER),
+    "scatter-view-off": dict(debug_view=_abi.VIEW_SCATTER),
+    "fringe-view": dict(creative_fringe=2.0, debug_view=_abi.VIEW_CREATIVE_FRINGE),
+    "effect-view-post-tristimulus": dict(spectral_mode=1, scatter_amount=0.4, scatter_radius=2.0,
+                                         creative_fringe=1.0, debug_view=_abi.VIEW_EFFECT),
+    "field-zones-scatter": dict(scatter_amount=0.3, scatter_radius=2.0, **FIELD),
 }
 
 

@@ -109,7 +109,8 @@ def test_convolve(g, oracle):
 
 
 @pytest.mark.parametrize("name", ["mono_physical_augment", "mono_split_preserve", "tristimulus", "spectral9_split",
-                                  "field_zones_split", "field_zones_tristimulus"])
+                                  "field_zones_split", "field_zones_tristimulus", "scatter_fringe_split",
+                                  "scatter_view", "fringe_view_preserve"])
 def test_frame(g, oracle, name):
     got = oracle.render_image(unpack(g["frame_%s_params" % name]), g["frame_in"])
     assert np.array_equal(got, g["frame_%s" % name])
