@@ -50,7 +50,7 @@ void combine_kernels(hipStream_t s, const float* kernels, const float* scales, i
 // Scratch and outputs of one frame on one stream.
 struct FrameWork {
     DeviceBuffer src, dst, mask, image_spec, out_spec, planes, effect, lum, totals, factors, matrices, kspec_tmp,
-        combined, max_buf, zone, post, gauss;
+        combined, max_buf, zone, post, gauss, work, redist, display;
     KernelSpectra kspec;  // per-frame kernel spectra (plain path)
     SeqSum seqsum;
 };

@@ -239,11 +239,12 @@ __device__ __forceinline__ float soft_shoulder(float v, float sh) {
 }
 
 // Split look: effect = shoulder(core) * core_gain + shoulder(structure) * structure_gain.
-// The shouldered core and structure replace their planes, as applyShoulder leaves the
-// reference's images (LensDiffCpuReference.cpp:2126-2132).
+// With write_back, the shouldered core and structure replace their planes, as applyShoulder
+// leaves the reference's images (LensDiffCpuReference.cpp:2126-2132) for the core and structure
+// views.
 __global__ void split_combine_kernel(float* __restrict__ core, float* __restrict__ structure, int64_t npx,
                                      float core_sh, float struct_sh, float core_gain, float struct_gain,
-                                     float* __restrict__ eff) {
+                                     int write_back, float* __restrict__ eff) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= npx) return;
     for (int c = 0; c < 3; ++c) {
@@ -251,11 +252,11 @@ __global__ void split_combine_kernel(float* __restrict__ core, float* __restrict
         float b = structure[int64_t(c) * npx + i];
         if (core_sh > 0.0f) {
             a = soft_shoulder(a, core_sh);
-            core[int64_t(c) * npx + i] = a;
+            if (write_back) core[int64_t(c) * npx + i] = a;
         }
         if (struct_sh > 0.0f) {
             b = soft_shoulder(b, struct_sh);
-            structure[int64_t(c) * npx + i] = b;
+            if (write_back) structure[int64_t(c) * npx + i] = b;
         }
         eff[int64_t(c) * npx + i] = a * core_gain + b * struct_gain;
     }
@@ -307,6 +308,91 @@ __global__ void max_kernel(const float* __restrict__ x, int64_t n, unsigned* __r
         m = max(m, __float_as_uint(fmaxf(x[i], 0.0f)));
     for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o));
     if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// Bilinear sample at (x, y) in the form of sampleScalarBilinear / sampleRgbaBilinearImage
+// (LensDiffCpuReference.cpp:355-394): zero outside [0, w-1] x [0, h-1].
+struct Bilinear {
+    bool in = false;
+    int64_t i00 = 0, i10 = 0, i01 = 0, i11 = 0;
+    float tx = 0.0f, ty = 0.0f;
+    __device__ Bilinear(float x, float y, int w, int h) {
+        if (x < 0.0f || y < 0.0f || x > float(w - 1) || y > float(h - 1)) return;
+        in = true;
+        const int x0 = int(floorf(x)), y0 = int(floorf(y));
+        const int x1 = min(x0 + 1, w - 1), y1 = min(y0 + 1, h - 1);
+        tx = x - float(x0);
+        ty = y - float(y0);
+        i00 = int64_t(y0) * w + x0;
+        i10 = int64_t(y0) * w + x1;
+        i01 = int64_t(y1) * w + x0;
+        i11 = int64_t(y1) * w + x1;
+    }
+    __device__ float mix(float v00, float v10, float v01, float v11) const {
+        const float a = v00 + (v10 - v00) * tx;
+        const float b = v01 + (v11 - v01) * tx;
+        return a + (b - a) * ty;
+    }
+};
+
+// Source position of output texel x when w texels are resampled to ow (resampleRgbaImage,
+// resampleScalarImage, :396-434).
+__device__ __forceinline__ float resample_at(int x, int w, int ow) {
+    return (float(x) + 0.5f) * (float(w) / float(max(1, ow))) - 0.5f;
+}
+
+// The decoded frame (loadSourceRgba) resampled to the working size ow x oh
+// (LensDiffCpuReference.cpp:2090-2093).
+__global__ void resample_frame_kernel(const float4* __restrict__ src, int w, int h, int tf, float4* __restrict__ out,
+                                      int ow, int oh) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= int64_t(ow) * oh) return;
+    const int x = int(i % ow), y = int(i / ow);
+    if (ow == w && oh == h) {  // resampleRgbaImage returns the image unchanged at its own size
+        out[i] = decoded(src, size_t(i), tf);
+        return;
+    }
+    const Bilinear b(resample_at(x, w, ow), resample_at(y, h, oh), w, h);
+    if (!b.in) {
+        out[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const float4 p00 = decoded(src, size_t(b.i00), tf), p10 = decoded(src, size_t(b.i10), tf);
+    const float4 p01 = decoded(src, size_t(b.i01), tf), p11 = decoded(src, size_t(b.i11), tf);
+    out[i] = make_float4(b.mix(p00.x, p10.x, p01.x, p11.x), b.mix(p00.y, p10.y, p01.y, p11.y),
+                         b.mix(p00.z, p10.z, p01.z, p11.z), b.mix(p00.w, p10.w, p01.w, p11.w));
+}
+
+// `planes` planes of w x h resampled to ow x oh (the display images of a resolution-aware
+// render, :2192-2198).
+__global__ void resample_planes_kernel(const float* __restrict__ in, int w, int h, int planes,
+                                       float* __restrict__ out, int ow, int oh) {
+    const int64_t n = int64_t(ow) * oh;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = int(i % ow), y = int(i / ow);
+    const int64_t nin = int64_t(w) * h;
+    if (ow == w && oh == h) {
+        for (int c = 0; c < planes; ++c) out[int64_t(c) * n + i] = in[int64_t(c) * nin + i];
+        return;
+    }
+    const Bilinear b(resample_at(x, w, ow), resample_at(y, h, oh), w, h);
+    for (int c = 0; c < planes; ++c) {
+        const float* pl = in + int64_t(c) * nin;
+        out[int64_t(c) * n + i] = b.in ? b.mix(pl[b.i00], pl[b.i10], pl[b.i01], pl[b.i11]) : 0.0f;
+    }
+}
+
+// The redistributed image (applyMask, :1251-1265) as 3 planes: frame * (mask * scale).
+__global__ void redist_planes_kernel(const float4* __restrict__ work, const float* __restrict__ mask, float scale,
+                                     int64_t n, float* __restrict__ out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 q = work[i];
+    const float k = mask[i] * scale;
+    out[i] = q.x * k;
+    out[n + i] = q.y * k;
+    out[2 * n + i] = q.z * k;
 }
 
 struct CompositeParams {
@@ -365,10 +451,12 @@ __global__ void fringe_kernel(const float* __restrict__ eff, float* __restrict__
     }
 }
 
+// redist (nullable): the redistributed image as 3 planes (a resolution-aware render resamples it
+// from the working frame); otherwise it is formed here from the frame and the mask.
 __global__ void composite_kernel(const float4* __restrict__ src, const float* __restrict__ mask,
                                  const float* __restrict__ eff, const float* __restrict__ factor,
                                  const unsigned* __restrict__ mask_max, int64_t npx, CompositeParams cp,
-                                 float4* __restrict__ dst) {
+                                 float4* __restrict__ dst, const float* __restrict__ redist) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= npx) return;
     const float4 s = decoded(src, size_t(i), cp.tf);
@@ -389,11 +477,11 @@ __global__ void composite_kernel(const float4* __restrict__ src, const float* __
         dst[i] = make_float4(e[0], e[1], e[2], 1.0f);
         return;
     }
-    const float m = mask[i] * cp.scale;
+    const float m = redist ? 0.0f : mask[i] * cp.scale;
     const float sv[3] = {s.x, s.y, s.z};
     float o[3];
     for (int c = 0; c < 3; ++c) {
-        const float r = sv[c] * m;
+        const float r = redist ? redist[int64_t(c) * npx + i] : sv[c] * m;
         const float raw = fmaxf(0.0f, sv[c] - cp.comp * r + cp.gain * e[c]);
         const float flo = fmaxf(0.0f, sv[c] - r * cp.max_sub);
         o[c] = encode_channel(fmaxf(raw, flo), cp.tf);
@@ -500,6 +588,10 @@ struct Plan {
     bool split = false;
     std::vector<int> sets;  // kernel sets convolved, in order
 };
+
+bool core_or_structure_view(const ldh_params& p) {
+    return p.debug_view == LDH_VIEW_CORE || p.debug_view == LDH_VIEW_STRUCTURE;
+}
 
 Plan plan_of(const ldh_params& p) {
     Plan pl;
@@ -681,7 +773,7 @@ void effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
         float* structure = effect9 + npx * 6;
         hipLaunchKernelGGL(split_combine_kernel, grid1(npx), dim3(256), 0, s, core, structure, npx,
                            float(p.core_shoulder), float(p.structure_shoulder), float(std::max(0.0, p.core_gain)),
-                           float(std::max(0.0, p.structure_gain)), effect9);
+                           float(std::max(0.0, p.structure_gain)), core_or_structure_view(p) ? 1 : 0, effect9);
         LDH_CHECK_LAUNCH("split-combine");
     }
 }
@@ -742,8 +834,18 @@ void scatter_pass(hipStream_t s, const FftTables& t, FrameWork& fw, float radius
     }
 }
 
+// The frame a resolution-aware render was resampled from: its images are resampled back to it
+// and composited there.
+struct Display {
+    const float4* src;
+    int w, h, tf;
+};
+
+// Steps 5-6 on the effect planes in fw.effect of the (working) frame src, w x h. With `disp`, the
+// effect (or view) planes, the redistributed image and the mask are resampled to the display
+// frame and the composite runs there (LensDiffCpuReference.cpp:2192-2220).
 void finish_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const float4* src,
-                  float4* dst, int w, int h) {
+                  float4* dst, int w, int h, const Display* disp) {
     const int64_t npx = int64_t(w) * h;
     const Plan pl = plan_of(p);
     const float redist_scale = 1.0f - float(std::max(0.0, std::min(p.core_preserve, 1.0)));
@@ -778,7 +880,7 @@ void finish_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
     const bool post_out = p.debug_view == LDH_VIEW_FINAL || p.debug_view == LDH_VIEW_EFFECT || post_view;
     const float* post_effect = fw.effect.as<float>();  // the effect after the post passes
     const float* view_planes = nullptr;                // the scatter or fringe view
-    if (post_out && (scatter || fringe > 1e-6f || post_view)) {
+    if (post_out && (scatter || fringe > 1e-6f || post_view || disp)) {
         fw.post.reset(sizeof(float) * npx * 9);  // [scatter image][fringe preview][fringed effect]
         float* eff = fw.effect.as<float>();
         float* scat = fw.post.as<float>();
@@ -815,7 +917,7 @@ void finish_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
 
     // 6. composite
     const unsigned* mask_max = nullptr;
-    if (p.debug_view == LDH_VIEW_SELECTION) {
+    if (p.debug_view == LDH_VIEW_SELECTION && !disp) {
         fw.max_buf.reset(sizeof(unsigned));
         LDH_HIP(hipMemsetAsync(fw.max_buf.get(), 0, sizeof(unsigned), s), "mask-max");
         hipLaunchKernelGGL(max_kernel, dim3(256), dim3(256), 0, s, fw.mask.as<float>(), npx, fw.max_buf.as<unsigned>());
@@ -839,45 +941,116 @@ void finish_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
         eff_out = view_planes;
         factor = nullptr;
     }
+    if (disp) {
+        // the display images at the frame's size; the Preserve factor is already in the effect
+        const int64_t n = int64_t(disp->w) * disp->h;
+        fw.display.reset(sizeof(float) * n * 7);  // [effect or view][redistributed][mask]
+        float* d_eff = fw.display.as<float>();
+        float* d_red = d_eff + n * 3;
+        float* d_mask = d_eff + n * 6;
+        ProfileScope ps_("frame-display-resample", 4.0 * 7.0 * double(n));
+        if (p.debug_view == LDH_VIEW_SELECTION) {
+            hipLaunchKernelGGL(resample_planes_kernel, grid1(n), dim3(256), 0, s, fw.mask.as<float>(), w, h, 1, d_mask,
+                               disp->w, disp->h);
+            fw.max_buf.reset(sizeof(unsigned));
+            LDH_HIP(hipMemsetAsync(fw.max_buf.get(), 0, sizeof(unsigned), s), "mask-max");
+            hipLaunchKernelGGL(max_kernel, dim3(256), dim3(256), 0, s, d_mask, n, fw.max_buf.as<unsigned>());
+            mask_max = fw.max_buf.as<unsigned>();
+        } else {
+            hipLaunchKernelGGL(resample_planes_kernel, grid1(n), dim3(256), 0, s, eff_out, w, h, 3, d_eff, disp->w,
+                               disp->h);
+        }
+        if (p.debug_view == LDH_VIEW_FINAL) {
+            fw.redist.reset(sizeof(float) * npx * 3);
+            hipLaunchKernelGGL(redist_planes_kernel, grid1(npx), dim3(256), 0, s, src, fw.mask.as<float>(), redist_scale,
+                               npx, fw.redist.as<float>());
+            hipLaunchKernelGGL(resample_planes_kernel, grid1(n), dim3(256), 0, s, fw.redist.as<float>(), w, h, 3, d_red,
+                               disp->w, disp->h);
+        }
+        LDH_CHECK_LAUNCH("display-resample");
+        cp.tf = disp->tf;
+        hipLaunchKernelGGL(composite_kernel, grid1(n), dim3(256), 0, s, disp->src, d_mask, d_eff,
+                           static_cast<const float*>(nullptr), mask_max, n, cp, dst, d_red);
+        LDH_CHECK_LAUNCH("composite");
+        return;
+    }
 {
     ProfileScope ps_("frame-composite", 56.0 * double(npx));
         hipLaunchKernelGGL(composite_kernel, grid1(npx), dim3(256), 0, s, src, fw.mask.as<float>(), eff_out, factor,
-                       mask_max, npx, cp, dst);
+                       mask_max, npx, cp, dst, static_cast<const float*>(nullptr));
     LDH_CHECK_LAUNCH("composite");
     }
 }
 
 }  // namespace
 
-void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
-                  const float4* src, float4* dst, int w, int h, EffectMode mode, const KernelSpectra* cached) {
+namespace {
+
+// Steps 1-4 on the frame src (w x h): the selection mask, then the effect planes of the bank or,
+// with zone banks, of every zone blended into fw.effect.
+void effect_planes(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank* bank,
+                   const std::vector<const DeviceBank*>* zones, const float4* src, int w, int h, EffectMode mode,
+                   const KernelSpectra* cached) {
     const int64_t npx = int64_t(w) * h;
     select_stage(s, fw, p, src, npx);
     fw.effect.reset(sizeof(float) * npx * 9);
-    effect_stage(s, t, fw, p, bank, src, w, h, mode, cached, fw.effect.as<float>());
-    finish_stage(s, t, fw, p, src, dst, w, h);
+    if (!zones) {
+        effect_stage(s, t, fw, p, *bank, src, w, h, mode, cached, fw.effect.as<float>());
+        return;
+    }
+    // the core and structure planes of a split look reach the output only in their views
+    const int planes = plan_of(p).split && core_or_structure_view(p) ? 9 : 3;
+    fw.zone.reset(sizeof(float) * npx * 9);
+    hipLaunchKernelGGL(zero_kernel, dim3(unsigned(std::min<int64_t>(div_up(npx * planes, 256), 4096))), dim3(256), 0, s,
+                       fw.effect.as<float>(), npx * planes);
+    LDH_CHECK_LAUNCH("zone-clear");
+    for (int z = 0; z < kFieldZones; ++z) {
+        effect_stage(s, t, fw, p, *(*zones)[size_t(z)], src, w, h, mode, nullptr, fw.zone.as<float>());
+        ProfileScope ps_("frame-zone-blend", 12.0 * planes * double(npx));
+        hipLaunchKernelGGL(zone_accumulate_kernel, grid1(npx), dim3(256), 0, s, fw.zone.as<float>(),
+                           fw.effect.as<float>(), planes, w, h, z % 3, z / 3);
+        LDH_CHECK_LAUNCH("zone-accumulate");
+    }
+}
+
+// The whole frame; a resolution-aware render (working scale optics / frame short side away from
+// 1, LensDiffCpuReference.cpp:2088-2093) runs steps 1-5 on the decoded frame resampled to the
+// working size and composites its resampled images at the frame's size.
+void render_impl(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank* bank,
+                 const std::vector<const DeviceBank*>* zones, const float4* src, float4* dst, int w, int h,
+                 EffectMode mode, const KernelSpectra* cached) {
+    const double ws = working_scale(p);
+    if (!resolution_aware_active(p)) {
+        effect_planes(s, t, fw, p, bank, zones, src, w, h, mode, cached);
+        finish_stage(s, t, fw, p, src, dst, w, h, nullptr);
+        return;
+    }
+    const int ww = std::max(1, int(std::lround(w * ws))), wh = std::max(1, int(std::lround(h * ws)));
+    if (cached && (ww != w || wh != h)) fail(LDH_ERR_INVALID_ARGUMENT, "cached-spectra-mismatch");
+    const int64_t nw = int64_t(ww) * wh;
+    fw.work.reset(sizeof(float4) * nw);
+    hipLaunchKernelGGL(resample_frame_kernel, grid1(nw), dim3(256), 0, s, src, w, h, p.input_transfer,
+                       fw.work.as<float4>(), ww, wh);
+    LDH_CHECK_LAUNCH("working-resample");
+    ldh_params lin = p;  // the working frame is already decoded
+    lin.input_transfer = LDH_TRANSFER_LINEAR;
+    effect_planes(s, t, fw, lin, bank, zones, fw.work.as<float4>(), ww, wh, mode, cached);
+    const Display d{src, w, h, p.input_transfer};
+    finish_stage(s, t, fw, lin, fw.work.as<float4>(), dst, ww, wh, &d);
+}
+
+}  // namespace
+
+void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
+                  const float4* src, float4* dst, int w, int h, EffectMode mode, const KernelSpectra* cached) {
+    render_impl(s, t, fw, p, &bank, nullptr, src, dst, w, h, mode, cached);
 }
 
 void render_frame_zones(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p,
                         const std::vector<const DeviceBank*>& banks, const float4* src, float4* dst, int w, int h,
                         EffectMode mode) {
     if (banks.size() != size_t(kFieldZones)) fail(LDH_ERR_INVALID_ARGUMENT, "field-zone-bank-count");
-    const int64_t npx = int64_t(w) * h;
-    const int planes = plan_of(p).split ? 9 : 3;
-    select_stage(s, fw, p, src, npx);
-    fw.effect.reset(sizeof(float) * npx * 9);
-    fw.zone.reset(sizeof(float) * npx * 9);
-    hipLaunchKernelGGL(zero_kernel, dim3(unsigned(std::min<int64_t>(div_up(npx * planes, 256), 4096))), dim3(256), 0, s,
-                       fw.effect.as<float>(), npx * planes);
-    LDH_CHECK_LAUNCH("zone-clear");
-    for (int z = 0; z < kFieldZones; ++z) {
-        effect_stage(s, t, fw, p, *banks[size_t(z)], src, w, h, mode, nullptr, fw.zone.as<float>());
-        ProfileScope ps_("frame-zone-blend", 12.0 * planes * double(npx));
-        hipLaunchKernelGGL(zone_accumulate_kernel, grid1(npx), dim3(256), 0, s, fw.zone.as<float>(),
-                           fw.effect.as<float>(), planes, w, h, z % 3, z / 3);
-        LDH_CHECK_LAUNCH("zone-accumulate");
-    }
-    finish_stage(s, t, fw, p, src, dst, w, h);
+    render_impl(s, t, fw, p, nullptr, &banks, src, dst, w, h, mode, nullptr);
 }
 
 }  // namespace ldh

@@ -34,6 +34,12 @@ BankKey make_bank_key(const ldh_params& p);
 // phase is accepted only where the caller renders field zones (allow_field_zones).
 std::string unsupported_reason(const ldh_params& p, bool allow_field_zones = false);
 
+// Effect working scale optics / frame short side (ResolveLensDiffEffectWorkingScale,
+// LensDiffTypes.h:303-306); a resolution-aware render resamples the frame by it when it is away
+// from 1.
+double working_scale(const ldh_params& p);
+bool resolution_aware_active(const ldh_params& p);
+
 // Scatter radius and creative-fringe shift in pixels (ResolveLensDiffScatterRadiusPx,
 // ResolveLensDiffCreativeFringePx, LensDiffTypes.h:322-328).
 double scatter_radius_px(const ldh_params& p);

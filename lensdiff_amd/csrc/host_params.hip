@@ -122,6 +122,13 @@ BankKey make_bank_key(const ldh_params& p) {
     return k;
 }
 
+double working_scale(const ldh_params& p) {
+    return double(optics_short_side(p)) / double(std::max(1, p.frame_short_side_px));
+}
+bool resolution_aware_active(const ldh_params& p) {
+    return p.resolution_aware && std::abs(working_scale(p) - 1.0) > 1e-6;
+}
+
 double scatter_radius_px(const ldh_params& p) { return percent_to_px(p.scatter_radius, optics_short_side(p)); }
 double creative_fringe_px(const ldh_params& p) { return percent_to_px(p.creative_fringe, optics_short_side(p)); }
 
@@ -183,10 +190,6 @@ std::string unsupported_reason(const ldh_params& p, bool allow_field_zones) {
         case LDH_VIEW_FINAL: case LDH_VIEW_SELECTION: case LDH_VIEW_EFFECT: case LDH_VIEW_CORE:
         case LDH_VIEW_STRUCTURE: case LDH_VIEW_CREATIVE_FRINGE: case LDH_VIEW_SCATTER: break;
         default: return "debug-view-not-supported";
-    }
-    if (p.resolution_aware) {
-        const double ws = double(optics_short_side(p)) / double(std::max(1, p.frame_short_side_px));
-        if (std::abs(ws - 1.0) > 1e-6) return "resolution-aware-not-supported";
     }
     return "";
 }

@@ -231,7 +231,9 @@ void Server::worker(Slot* s) {
         std::unique_lock<std::mutex> prof_lock(s->prof_mu);
         ProfilerBinding bind(s->prof);
         try {
-            const std::string why = unsupported_reason(r->params);
+            std::string why = unsupported_reason(r->params);
+            // the served spectra are kept for the frame's own grid
+            if (why.empty() && resolution_aware_active(r->params)) why = "resolution-aware-not-supported";
             if (!why.empty()) throw Error{LDH_ERR_UNSUPPORTED, why};
             if (builder) {
                 try {

@@ -149,6 +149,26 @@ def main():
             setattr(p, k, v)
         out["frame_%s_params" % name] = pack_params(p)
         out["frame_%s" % name] = ref.render_image(p, frame)
+    # resolution-aware: the frame resampled to the working size (optics short side 2160 over the
+    # frame's) and the display images resampled back
+    for name, fss, kw in (("ra_up", 1080, dict(spectral_mode=1)),
+                          ("ra_down_split_preserve", 4320, dict(look_mode=1, energy_mode=0)),
+                          ("ra_selection", 1350, dict(debug_view=_abi.VIEW_SELECTION)),
+                          ("ra_scatter_fringe", 1080, dict(scatter_amount=0.4, scatter_radius=0.1,
+                                                           creative_fringe=0.05, input_transfer=1))):
+        p = workloads.base_params(40)
+        p.pupil_resolution = 64
+        p.resolution_aware = 1
+        p.frame_short_side_px = fss
+        p.max_kernel_radius_px = 0.8
+        p.diffraction_scale_px = 0.1
+        p.set_noll({4: 0.25, 11: -0.15})
+        for k, v in kw.items():
+            setattr(p, k, v)
+        img = workloads.davinci_intermediate_encode(frame) if p.input_transfer == 1 else frame
+        out["frame_%s_params" % name] = pack_params(p)
+        out["frame_%s_in" % name] = img
+        out["frame_%s" % name] = ref.render_image(p, img)
     np.savez_compressed(OUT, **out)
     print("wrote %s (%d arrays, %.1f KiB)" % (OUT, len(out), os.path.getsize(OUT) / 1024))
 

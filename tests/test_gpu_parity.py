@@ -319,3 +319,38 @@ def test_render_on_mixed_radix_grid(ctx, oracle):
     got = ctx.render_image(p, img)
     want = oracle.render_image(p, img)
     assert rel_l2(got, want) < TOL, rel_l2(got, want)
+
+
+# resolution-aware renders: working scale 2160 / frame short side (2, 0.5, 1.6, 0.72, 2)
+RA_CASES = {
+    "up-tristimulus": (1080, dict(spectral_mode=1)),
+    "down-split-preserve": (4320, dict(look_mode=1, energy_mode=_abi.ENERGY_PRESERVE, core_shoulder=1.0)),
+    "selection-view": (1350, dict(debug_view=_abi.VIEW_SELECTION)),
+    "scatter-fringe-davinci": (3000, dict(scatter_amount=0.4, scatter_radius=0.1, creative_fringe=0.05,
+                                          input_transfer=_abi.TRANSFER_DAVINCI_INTERMEDIATE)),
+    "field-zones-effect-view": (1080, dict(debug_view=_abi.VIEW_EFFECT, **FIELD)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(RA_CASES))
+def test_resolution_aware_matches_oracle(ctx, oracle, case):
+    """The frame resampled to the working size, rendered there, and its images resampled back."""
+    fss, kw = RA_CASES[case]
+    p = workloads.base_params(64)
+    p.pupil_resolution = 64
+    p.resolution_aware = 1
+    p.frame_short_side_px = fss
+    p.max_kernel_radius_px = 0.8
+    p.diffraction_scale_px = 0.1
+    p.set_noll({4: 0.25, 11: -0.15})
+    for k, v in kw.items():
+        setattr(p, k, v)
+    img = small_frame(96, 64, seed=4)
+    if p.input_transfer == _abi.TRANSFER_DAVINCI_INTERMEDIATE:
+        img = workloads.davinci_intermediate_encode(img)
+    got = ctx.render_image(p, img)
+    want = oracle.render_image(p, img)
+    if p.debug_view == _abi.VIEW_SELECTION:
+        assert np.array_equal(got, want.astype(np.float32))
+    else:
+        assert rel_l2(got, want) < TOL, rel_l2(got, want)

@@ -108,6 +108,12 @@ def test_convolve(g, oracle):
     assert np.array_equal(oracle.convolve(g["conv_plane"], g["conv_kernel"]), g["conv_out"])
 
 
+@pytest.mark.parametrize("name", ["ra_up", "ra_down_split_preserve", "ra_selection", "ra_scatter_fringe"])
+def test_resolution_aware_frame(g, oracle, name):
+    got = oracle.render_image(unpack(g["frame_%s_params" % name]), g["frame_%s_in" % name])
+    assert np.array_equal(got, g["frame_%s" % name])
+
+
 @pytest.mark.parametrize("name", ["mono_physical_augment", "mono_split_preserve", "tristimulus", "spectral9_split",
                                   "field_zones_split", "field_zones_tristimulus", "scatter_fringe_split",
                                   "scatter_view", "fringe_view_preserve"])
