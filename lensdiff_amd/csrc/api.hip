@@ -144,6 +144,17 @@ std::vector<const ldh::DeviceBank*> ensure_zone_banks(ldh_context* ctx, const ld
 void render_with_banks(ldh_context* ctx, const ldh_params& p, const float4* d_src, float4* d_dst, int w, int h) {
     const ldh::EffectMode mode =
         p.conv_mode == LDH_CONV_PRECOMBINED ? ldh::EffectMode::Precombined : ldh::EffectMode::PerBin;
+    if (ldh::is_static_view(p.debug_view)) {
+        // the global bank, or the edge zone (2, 2) for the phase-edge and field-PSF views of a
+        // field-dependent phase (BuildLensDiffStaticDebugRgba's edgeZone)
+        const bool edge = ldh::has_field_phase(p) &&
+                          (p.debug_view == LDH_VIEW_PHASE_EDGE || p.debug_view == LDH_VIEW_FIELD_PSF);
+        ldh_params q = edge ? ldh::field_zone_params(p, 2, 2) : p;
+        q.phase_field_strength = 0.0;
+        auto bank = ensure_bank(ctx, q);
+        ldh::render_static_view(ctx->stream, ctx->fw, q, p.debug_view, *bank, d_dst, w, h);
+        return;
+    }
     if (ldh::has_field_phase(p)) {
         const auto banks = ensure_zone_banks(ctx, p);
         ldh::ProfileScope ps("frame", 0.0);

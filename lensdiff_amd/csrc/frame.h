@@ -6,6 +6,7 @@
 
 #include "bank.h"
 #include "host_params.h"
+#include "psf.h"
 #include "ldh_internal.h"
 #include "seqsum.h"
 
@@ -50,7 +51,7 @@ void combine_kernels(hipStream_t s, const float* kernels, const float* scales, i
 // Scratch and outputs of one frame on one stream.
 struct FrameWork {
     DeviceBuffer src, dst, mask, image_spec, out_spec, planes, effect, lum, totals, factors, matrices, kspec_tmp,
-        combined, max_buf, zone, post, gauss, work, redist, display;
+        combined, max_buf, zone, post, gauss, work, redist, display, view_pupil, view_coef;
     KernelSpectra kspec;  // per-frame kernel spectra (plain path)
     SeqSum seqsum;
 };
@@ -71,6 +72,15 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
 void render_frame_zones(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p,
                         const std::vector<const DeviceBank*>& banks, const float4* src, float4* dst, int w, int h,
                         EffectMode mode);
+
+// The static debug views (BuildLensDiffStaticDebugRgba, LensDiffCpuReference.cpp:1909-1994)
+// at w x h into dst: pupil, phase and phase edge from the pupil of `p`, PSF and field PSF from the
+// first bin of `bank`, the chromatic split from its last, middle and first bins. The caller
+// passes the edge zone's parameters and bank for the phase-edge and field-PSF views of a
+// field-dependent phase.
+bool is_static_view(int view);
+void render_static_view(hipStream_t s, FrameWork& fw, const ldh_params& p, int view, const DeviceBank& bank,
+                        float4* dst, int w, int h);
 
 // Which real kernels feed the effect, for building cached spectra.
 std::vector<KernelRef> effect_kernels(const ldh_params& p, const DeviceBank& bank, EffectMode mode,

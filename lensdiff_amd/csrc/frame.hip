@@ -310,6 +310,95 @@ __global__ void max_kernel(const float* __restrict__ x, int64_t n, unsigned* __r
     if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
+// max |x| as float bits (non-negative floats order like their bit patterns)
+__global__ void max_abs_kernel(const float* __restrict__ x, int64_t n, unsigned* __restrict__ out) {
+    unsigned m = 0u;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        m = max(m, __float_as_uint(fabsf(x[i])));
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// An n x n square for the static views: texel (x, y) at data[y * pitch + x], multiplied by
+// *scale when scale is set; maxbits holds the bits of the largest stored texel.
+struct SquareSrc {
+    const float* data = nullptr;
+    int n = 1, pitch = 1;
+    const float* scale = nullptr;
+    const unsigned* maxbits = nullptr;
+};
+
+// sampleSquareBilinear (LensDiffCpuReference.cpp:316-335) of a square, zero outside it.
+__device__ float sample_square(const SquareSrc& q, float sc, float x, float y) {
+    const int n = q.n;
+    if (x < 0.0f || y < 0.0f || x > float(n - 1) || y > float(n - 1)) return 0.0f;
+    const int x0 = int(floorf(x)), y0 = int(floorf(y));
+    const int x1 = min(x0 + 1, n - 1), y1 = min(y0 + 1, n - 1);
+    const float tx = x - float(x0), ty = y - float(y0);
+    const float v00 = q.data[int64_t(y0) * q.pitch + x0] * sc, v10 = q.data[int64_t(y0) * q.pitch + x1] * sc;
+    const float v01 = q.data[int64_t(y1) * q.pitch + x0] * sc, v11 = q.data[int64_t(y1) * q.pitch + x1] * sc;
+    const float a = v00 + (v10 - v00) * tx, b = v01 + (v11 - v01) * tx;
+    return a + (b - a) * ty;
+}
+
+// Where output texel (x, y) falls in the centred square of side min(ow, oh) mapped onto an n x n
+// image (visualizeSquareCentered, :1416-1450), or false outside the square.
+__device__ bool square_coords(int x, int y, int ow, int oh, int n, float& sx, float& sy) {
+    const int side = min(ow, oh), xo = (ow - side) / 2, yo = (oh - side) / 2;
+    if (x < xo || x >= xo + side || y < yo || y >= yo + side) return false;
+    sx = (float(x - xo) / float(max(1, side - 1))) * float(n - 1);
+    sy = (float(y - yo) / float(max(1, side - 1))) * float(n - 1);
+    return true;
+}
+
+// visualizeSquareCentered: channel c of dst from square src[c] (the same square three times for
+// the grey views; last, middle and first bin for the chromatic split, :1951-1972), v / max or
+// log1p(1024 v) / log1p(1024 max); alpha 1.
+__global__ void square_view_kernel(SquareSrc r, SquareSrc g, SquareSrc b, int log_view, float4* __restrict__ dst,
+                                   int ow, int oh) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= int64_t(ow) * oh) return;
+    const int x = int(i % ow), y = int(i / ow);
+    const SquareSrc* src[3] = {&r, &g, &b};
+    float d[3];
+    for (int c = 0; c < 3; ++c) {
+        const SquareSrc& q = *src[c];
+        const float sc = q.scale ? *q.scale : 1.0f;
+        const float mx = __uint_as_float(*q.maxbits) * sc;
+        float sx, sy;
+        d[c] = 0.0f;
+        if (!square_coords(x, y, ow, oh, q.n, sx, sy)) continue;
+        const float v = sample_square(q, sc, sx, sy);
+        if (log_view) {
+            const float lmax = log1pf(fmaxf(0.0f, mx) * 1024.0f);
+            d[c] = lmax > 0.0f ? log1pf(fmaxf(0.0f, v) * 1024.0f) / lmax : 0.0f;
+        } else {
+            d[c] = mx > 0.0f ? v / mx : 0.0f;
+        }
+    }
+    dst[i] = make_float4(d[0], d[1], d[2], 1.0f);
+}
+
+// visualizeSignedPhaseCentered (:1536-1593): inside the pupil (amplitude > 1e-4), neutral grey
+// moved towards warm for positive and cool for negative waves by |phase| / max |phase|.
+__global__ void phase_view_kernel(SquareSrc phase, SquareSrc pupil, float4* __restrict__ dst, int ow, int oh) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= int64_t(ow) * oh) return;
+    const int x = int(i % ow), y = int(i / ow);
+    float4 o = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    float sx, sy;
+    if (square_coords(x, y, ow, oh, phase.n, sx, sy) && sample_square(pupil, 1.0f, sx, sy) > 1e-4f) {
+        const float mabs = __uint_as_float(*phase.maxbits);
+        const float v = sample_square(phase, 1.0f, sx, sy);
+        const float u = mabs > 0.0f ? fminf(fmaxf(v / mabs, -1.0f), 1.0f) : 0.0f;
+        const float m = fabsf(u), g = 0.18f;
+        if (m <= 1e-6f) o = make_float4(g, g, g, 1.0f);
+        else if (u > 0.0f) o = make_float4(g + 0.82f * m, g + 0.28f * m, fmaxf(0.0f, g - 0.12f * m), 1.0f);
+        else o = make_float4(fmaxf(0.0f, g - 0.10f * m), g + 0.34f * m, g + 0.80f * m, 1.0f);
+    }
+    dst[i] = o;
+}
+
 // Bilinear sample at (x, y) in the form of sampleScalarBilinear / sampleRgbaBilinearImage
 // (LensDiffCpuReference.cpp:355-394): zero outside [0, w-1] x [0, h-1].
 struct Bilinear {
@@ -1051,6 +1140,61 @@ void render_frame_zones(hipStream_t s, const FftTables& t, FrameWork& fw, const 
                         EffectMode mode) {
     if (banks.size() != size_t(kFieldZones)) fail(LDH_ERR_INVALID_ARGUMENT, "field-zone-bank-count");
     render_impl(s, t, fw, p, nullptr, &banks, src, dst, w, h, mode, nullptr);
+}
+
+bool is_static_view(int view) {
+    return view == LDH_VIEW_PUPIL || view == LDH_VIEW_PSF || view == LDH_VIEW_PHASE || view == LDH_VIEW_PHASE_EDGE ||
+           view == LDH_VIEW_FIELD_PSF || view == LDH_VIEW_CHROMATIC_SPLIT;
+}
+
+void render_static_view(hipStream_t s, FrameWork& fw, const ldh_params& p, int view, const DeviceBank& bank,
+                        float4* dst, int w, int h) {
+    const int64_t npx = int64_t(w) * h;
+    fw.max_buf.reset(sizeof(unsigned) * 3);
+    unsigned* mx = fw.max_buf.as<unsigned>();
+    LDH_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned) * 3, s), "view-max");
+    ProfileScope ps_("frame-static-view", 16.0 * double(npx));
+    if (view == LDH_VIEW_PUPIL || view == LDH_VIEW_PHASE || view == LDH_VIEW_PHASE_EDGE) {
+        // the pupil amplitude and phase waves at the bank's pupil size (cache.pupilDisplay /
+        // phaseDisplay, :1627-1630)
+        const PupilGeom geom = make_pupil_geom(p);
+        const int n = geom.size;
+        fw.view_pupil.reset(sizeof(float) * size_t(n) * n * 2);
+        float* amp = fw.view_pupil.as<float>();
+        float* ph = amp + size_t(n) * n;
+        pupil_amplitude(s, geom, amp, false);
+        SquareSrc a{amp, n, n, nullptr, mx};
+        hipLaunchKernelGGL(max_kernel, dim3(64), dim3(256), 0, s, amp, int64_t(n) * n, mx);
+        if (view == LDH_VIEW_PUPIL) {
+            hipLaunchKernelGGL(square_view_kernel, grid1(npx), dim3(256), 0, s, a, a, a, 0, dst, w, h);
+        } else {
+            const PhaseCoef c = make_phase_coef(p);
+            fw.view_coef.reset(sizeof(c));
+            LDH_HIP(hipMemcpyAsync(fw.view_coef.get(), &c, sizeof(c), hipMemcpyHostToDevice, s), "view-phase");
+            pupil_phase(s, fw.view_coef.as<PhaseCoef>(), n, 1, ph, false);
+            hipLaunchKernelGGL(max_abs_kernel, dim3(64), dim3(256), 0, s, ph, int64_t(n) * n, mx + 1);
+            SquareSrc q{ph, n, n, nullptr, mx + 1};
+            hipLaunchKernelGGL(phase_view_kernel, grid1(npx), dim3(256), 0, s, q, a, dst, w, h);
+        }
+        LDH_CHECK_LAUNCH("static-view");
+        return;
+    }
+    // kernels of the bank: the first bin's full kernel, or last / middle / first for the split
+    auto square = [&](int bin, unsigned* bits) {
+        const int K = bank.kernel, c = K / 2;
+        const int rc = std::min(std::max(bank.radius[size_t(bin)], 1), c);
+        const float* k = bank.kernel_ptr(kFull, bin);
+        hipLaunchKernelGGL(max_kernel, dim3(256), dim3(256), 0, s, k, int64_t(K) * K, bits);
+        return SquareSrc{k + int64_t(c - rc) * K + (c - rc), 2 * rc + 1, K, bank.scale_ptr(kFull, bin), bits};
+    };
+    if (view == LDH_VIEW_CHROMATIC_SPLIT) {
+        const SquareSrc r = square(bank.bins - 1, mx), g = square(bank.bins / 2, mx + 1), b = square(0, mx + 2);
+        hipLaunchKernelGGL(square_view_kernel, grid1(npx), dim3(256), 0, s, r, g, b, 1, dst, w, h);
+    } else {
+        const SquareSrc k = square(0, mx);
+        hipLaunchKernelGGL(square_view_kernel, grid1(npx), dim3(256), 0, s, k, k, k, 1, dst, w, h);
+    }
+    LDH_CHECK_LAUNCH("static-view");
 }
 
 }  // namespace ldh

@@ -234,6 +234,7 @@ void Server::worker(Slot* s) {
             std::string why = unsupported_reason(r->params);
             // the served spectra are kept for the frame's own grid
             if (why.empty() && resolution_aware_active(r->params)) why = "resolution-aware-not-supported";
+            if (why.empty() && is_static_view(r->params.debug_view)) why = "debug-view-not-supported";
             if (!why.empty()) throw Error{LDH_ERR_UNSUPPORTED, why};
             if (builder) {
                 try {

@@ -1366,6 +1366,119 @@ template <typename T> T encode_channel(T x, int tf) {
     return x;
 }
 
+// ---- Static debug views (BuildLensDiffStaticDebugRgba, LensDiffCpuReference.cpp:1909-1994) ----
+// sampleSquareBilinear (:316-335): zero outside the square.
+template <typename T> T sample_square(const std::vector<T>& img, int n, T x, T y) {
+    if (x < T(0) || y < T(0) || x > T(n - 1) || y > T(n - 1)) return T(0);
+    const int x0 = int(std::floor(x)), y0 = int(std::floor(y));
+    const int x1 = std::min(x0 + 1, n - 1), y1 = std::min(y0 + 1, n - 1);
+    const T tx = x - T(x0), ty = y - T(y0);
+    const T v00 = img[size_t(y0) * n + x0], v10 = img[size_t(y0) * n + x1];
+    const T v01 = img[size_t(y1) * n + x0], v11 = img[size_t(y1) * n + x1];
+    const T a = v00 + (v10 - v00) * tx, b = v01 + (v11 - v01) * tx;
+    return a + (b - a) * ty;
+}
+
+// Where output texel (x, y) of the centred square falls in an n x n image, or false outside the
+// square (the mapping of visualizeSquareCentered / visualizeSignedPhaseCentered).
+template <typename T> bool square_coords(int x, int y, int ow, int oh, int n, T& sx, T& sy) {
+    const int side = std::min(ow, oh), xo = (ow - side) / 2, yo = (oh - side) / 2;
+    if (x < xo || x >= xo + side || y < yo || y >= yo + side) return false;
+    const T nx = T(x - xo) / T(std::max(1, side - 1)), ny = T(y - yo) / T(std::max(1, side - 1));
+    sx = nx * T(n - 1);
+    sy = ny * T(n - 1);
+    return true;
+}
+
+// visualizeSquareCentered (:1416-1450): grey, v / max or log1p(1024 v) / log1p(1024 max).
+template <typename T> Rgba<T> view_square(const std::vector<T>& sq, int n, int ow, int oh, bool log_view) {
+    Rgba<T> out(ow, oh);
+    T mx = T(0);
+    for (T v : sq) mx = std::max(mx, v);
+    const T lmax = std::log1p(std::max(T(0), mx) * T(1024));
+    for (int y = 0; y < oh; ++y)
+        for (int x = 0; x < ow; ++x) {
+            T d = T(0), sx, sy;
+            if (square_coords(x, y, ow, oh, n, sx, sy)) {
+                const T v = sample_square(sq, n, sx, sy);
+                if (log_view) d = lmax > T(0) ? std::log1p(std::max(T(0), v) * T(1024)) / lmax : T(0);
+                else d = mx > T(0) ? v / mx : T(0);
+            }
+            T* q = out.px(x, y);
+            q[0] = q[1] = q[2] = d;
+            q[3] = T(1);
+        }
+    return out;
+}
+
+// visualizeSignedPhaseCentered (:1536-1593): inside the pupil, neutral grey pushed towards warm
+// for positive and cool for negative waves, by |phase| / max |phase|.
+template <typename T>
+Rgba<T> view_phase(const std::vector<T>& ph, const std::vector<T>& pupil, int n, int ow, int oh) {
+    Rgba<T> out(ow, oh);
+    T mabs = T(0);
+    for (T v : ph) mabs = std::max(mabs, std::abs(v));
+    for (int y = 0; y < oh; ++y)
+        for (int x = 0; x < ow; ++x) {
+            T* q = out.px(x, y);
+            q[0] = q[1] = q[2] = T(0);
+            q[3] = T(1);
+            T sx, sy;
+            if (!square_coords(x, y, ow, oh, n, sx, sy)) continue;
+            if (sample_square(pupil, n, sx, sy) <= T(1e-4f)) continue;
+            const T v = sample_square(ph, n, sx, sy);
+            const T u = mabs > T(0) ? clampv(v / mabs, T(-1), T(1)) : T(0);
+            const T m = std::abs(u), g = T(0.18f);
+            if (m <= T(1e-6f)) {
+                q[0] = q[1] = q[2] = g;
+            } else if (u > T(0)) {
+                q[0] = g + T(0.82f) * m;
+                q[1] = g + T(0.28f) * m;
+                q[2] = std::max(T(0), g - T(0.12f) * m);
+            } else {
+                q[0] = std::max(T(0), g - T(0.10f) * m);
+                q[1] = g + T(0.34f) * m;
+                q[2] = g + T(0.80f) * m;
+            }
+        }
+    return out;
+}
+
+inline bool static_view_kind(int view) {
+    return view == LDH_VIEW_PUPIL || view == LDH_VIEW_PSF || view == LDH_VIEW_PHASE || view == LDH_VIEW_PHASE_EDGE ||
+           view == LDH_VIEW_FIELD_PSF || view == LDH_VIEW_CHROMATIC_SPLIT;
+}
+
+// The static view of the bank (global parameters) or, for the phase-edge and field-PSF views with
+// field-dependent phase, of the last zone (2, 2), at ow x oh.
+template <typename T> Rgba<T> static_view(const ldh_params& p, const Bank<T>& bank, int ow, int oh) {
+    const int n = pupil_size_of(p.pupil_resolution);
+    const bool edge = field_phase(p) && (p.debug_view == LDH_VIEW_PHASE_EDGE || p.debug_view == LDH_VIEW_FIELD_PSF);
+    const ldh_params q = edge ? field_zone_params(p, 1.0f, 1.0f) : p;
+    switch (p.debug_view) {
+        case LDH_VIEW_PUPIL: return view_square(pupil_amplitude<T>(p), n, ow, oh, false);
+        case LDH_VIEW_PHASE:
+        case LDH_VIEW_PHASE_EDGE: return view_phase(pupil_phase<T>(q), pupil_amplitude<T>(q), n, ow, oh);
+        case LDH_VIEW_PSF: return view_square(bank.bins.front().full.v, bank.bins.front().full.size, ow, oh, true);
+        case LDH_VIEW_FIELD_PSF: {
+            if (!edge) return view_square(bank.bins.front().full.v, bank.bins.front().full.size, ow, oh, true);
+            std::unique_ptr<Bank<T>> zb(build_bank<T>(q));
+            return view_square(zb->bins.front().full.v, zb->bins.front().full.size, ow, oh, true);
+        }
+        default: {  // chromatic split: last bin in red, middle in green, first in blue
+            const auto& bins = bank.bins;
+            const Kernel<T>* k[3] = {&bins.back().full, &bins[bins.size() / 2].full, &bins.front().full};
+            Rgba<T> out(ow, oh);
+            for (int c = 0; c < 3; ++c) {
+                const Rgba<T> im = view_square(k[c]->v, k[c]->size, ow, oh, true);
+                for (size_t i = 0; i < size_t(ow) * oh; ++i) out.v[i * 4 + size_t(c)] = im.v[i * 4 + size_t(c)];
+            }
+            for (size_t i = 0; i < size_t(ow) * oh; ++i) out.v[i * 4 + 3] = T(1);
+            return out;
+        }
+    }
+}
+
 // accumulateWeightedImage (LensDiffCpuReference.cpp:1452-1475): dst += src * wx * wy for the
 // zone's tent weights over the frame (zone centres at x = 0, (w-1)/2, w-1 and likewise in y);
 // pixels with a zero weight are left alone, the others take src's alpha.
@@ -1395,6 +1508,8 @@ template <typename T>
 Rgba<T> render_core(const ldh_params& p, const Rgba<T>& src, bool* ok) {
     *ok = true;
     std::unique_ptr<Bank<T>> bank(build_bank<T>(p));
+    // static views replace the output at the frame's size (renderDebugView, :1800-1807)
+    if (static_view_kind(p.debug_view)) return static_view(p, *bank, src.w, src.h);
     std::vector<float> wl;
     for (const auto& b : bank->bins) wl.push_back(b.wavelength);
     const int n = int(wl.size());

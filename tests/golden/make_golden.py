@@ -169,6 +169,21 @@ def main():
         out["frame_%s_params" % name] = pack_params(p)
         out["frame_%s_in" % name] = img
         out["frame_%s" % name] = ref.render_image(p, img)
+    # static debug views (pupil, PSF, phase, edge-zone phase and PSF, chromatic split)
+    for name, kw in (("view_pupil", dict(debug_view=_abi.VIEW_PUPIL)),
+                     ("view_psf_spectral", dict(debug_view=_abi.VIEW_PSF, spectral_mode=2)),
+                     ("view_phase", dict(debug_view=_abi.VIEW_PHASE)),
+                     ("view_phase_edge_field", dict(debug_view=_abi.VIEW_PHASE_EDGE, **FIELD)),
+                     ("view_field_psf_field", dict(debug_view=_abi.VIEW_FIELD_PSF, **FIELD)),
+                     ("view_chromatic_split", dict(debug_view=_abi.VIEW_CHROMATIC_SPLIT, spectral_mode=3))):
+        p = workloads.base_params(40)
+        p.pupil_resolution = 64
+        p.max_kernel_radius_px = 20.0
+        p.set_noll({4: 0.25, 11: -0.15, 7: 0.1})
+        for k, v in kw.items():
+            setattr(p, k, v)
+        out["frame_%s_params" % name] = pack_params(p)
+        out["frame_%s" % name] = ref.render_image(p, frame)
     np.savez_compressed(OUT, **out)
     print("wrote %s (%d arrays, %.1f KiB)" % (OUT, len(out), os.path.getsize(OUT) / 1024))
 
