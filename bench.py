@@ -151,6 +151,8 @@ def main():
     ap.add_argument("--bank-probe", action="store_true",
                     help="only build two fresh 31-wavelength UHD banks (profiling the PSF path) and exit")
     ap.add_argument("--batch-probe", action="store_true", help="only run the batched form and exit")
+    ap.add_argument("--field-probe", action="store_true",
+                    help="only time a UHD frame with field-dependent phase (nine zone banks) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -182,6 +184,32 @@ def main():
         fams = families(ctx.profile_read())
         print(json.dumps({"bank_build_ms": ms, "kernel_sides": ctx.bank_kernel_sizes(lens(5002)),
                           "autotune": ctx.tuning_report(), "kernel_families": fams}))
+        ctx.close()
+        return
+    if args.field_probe:
+        # one UHD frame with field-dependent phase: nine zone banks (31 wavelengths each) built in
+        # shared launches, nine zone convolutions blended; then the same frame with the zone banks
+        # cached, and the frame without field phase for comparison
+        ctx = Context(0)
+        f = workloads.c2_frame()
+        h, w = f.shape[:2]
+        d_src, d_dst = ctx.alloc(f.nbytes), ctx.alloc(f.nbytes)
+        ctx.upload(d_src, f)
+        p = lens(7000)
+        p.phase_mode = 1
+        p.phase_field_strength, p.phase_field_edge_bias = 1.0, 0.3
+        p.phase_field_coma_radial, p.phase_field_astig_tangential, p.phase_field_defocus = 0.3, 0.2, 0.1
+
+        def timed(q):
+            t0 = time.perf_counter()
+            ctx.render_device(q, d_src, d_dst, w, h)
+            ctx.synchronize()
+            return round((time.perf_counter() - t0) * 1e3, 2)
+        out = {"field_zone_frame_first_ms": timed(p), "field_zone_frame_cached_ms": [timed(p) for _ in range(3)]}
+        q = lens(7000)
+        timed(q)
+        out["global_frame_cached_ms"] = [timed(q) for _ in range(3)]
+        print(json.dumps(out))
         ctx.close()
         return
     from oracle.oracle import rel_l2
