@@ -399,6 +399,64 @@ __global__ void phase_view_kernel(SquareSrc phase, SquareSrc pupil, float4* __re
     dst[i] = o;
 }
 
+// OTF view (computeOtfMagnitude, LensDiffCpuReference.cpp:1403-1414): the n x n DFT of the kernel
+// wrapped to the origin, as the reference's FFT computes it for odd n (Bluestein with its chirp
+// angle pi i^2 / n taken with pi rounded to float, which is the DFT with kernel
+// exp(-2 i pi_f x k / n)). Evaluated directly, one output per thread: the twiddle w^x advances by
+// one complex multiplication per term and the sums run in double. Rows first: thread kx of
+// workgroup (kx block, wrapped row py) sums row (py + c) % n of the normalised kernel, staged in
+// LDS in wrapped order.
+__global__ void otf_rows_kernel(const float* __restrict__ k, int pitch, int n, const float* __restrict__ scale,
+                                float2* __restrict__ out) {
+    extern __shared__ float otf_row[];
+    const int py = blockIdx.y, c = n / 2;
+    const float sc = scale ? *scale : 1.0f;
+    const float* row = k + int64_t((py + c) % n) * pitch;
+    for (int px = threadIdx.x; px < n; px += blockDim.x) otf_row[px] = row[(px + c) % n] * sc;
+    __syncthreads();
+    const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (kx >= n) return;
+    const double theta = -2.0 * double(float(M_PI)) * double(kx) / double(n);
+    double sn, cs;
+    sincos(theta, &sn, &cs);
+    double wr = 1.0, wi = 0.0, ar = 0.0, ai = 0.0;
+    for (int px = 0; px < n; ++px) {
+        const double v = otf_row[px];
+        ar = fma(v, wr, ar);
+        ai = fma(v, wi, ai);
+        const double t = wr * cs - wi * sn;
+        wi = wr * sn + wi * cs;
+        wr = t;
+    }
+    out[int64_t(py) * n + kx] = make_float2(float(ar), float(ai));
+}
+
+// Columns: thread ky of workgroup (ky block, column kx) sums column kx of the row transforms
+// (staged in LDS) against w^py, and writes |X| at its fftShiftSquare position (:303-314).
+__global__ void otf_cols_kernel(const float2* __restrict__ rows, int n, float* __restrict__ mag) {
+    extern __shared__ float2 otf_col[];
+    const int kx = blockIdx.y;
+    for (int py = threadIdx.x; py < n; py += blockDim.x) otf_col[py] = rows[int64_t(py) * n + kx];
+    __syncthreads();
+    const int ky = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ky >= n) return;
+    const double theta = -2.0 * double(float(M_PI)) * double(ky) / double(n);
+    double sn, cs;
+    sincos(theta, &sn, &cs);
+    double wr = 1.0, wi = 0.0, ar = 0.0, ai = 0.0;
+    for (int py = 0; py < n; ++py) {
+        const double vr = otf_col[py].x, vi = otf_col[py].y;
+        ar = fma(vr, wr, fma(-vi, wi, ar));
+        ai = fma(vr, wi, fma(vi, wr, ai));
+        const double t = wr * cs - wi * sn;
+        wi = wr * sn + wi * cs;
+        wr = t;
+    }
+    const int half = n / 2;
+    const int ys = (ky - half + n) % n, xs = (kx - half + n) % n;
+    mag[int64_t(ys) * n + xs] = float(sqrt(ar * ar + ai * ai));
+}
+
 // Bilinear sample at (x, y) in the form of sampleScalarBilinear / sampleRgbaBilinearImage
 // (LensDiffCpuReference.cpp:355-394): zero outside [0, w-1] x [0, h-1].
 struct Bilinear {
@@ -1143,8 +1201,8 @@ void render_frame_zones(hipStream_t s, const FftTables& t, FrameWork& fw, const 
 }
 
 bool is_static_view(int view) {
-    return view == LDH_VIEW_PUPIL || view == LDH_VIEW_PSF || view == LDH_VIEW_PHASE || view == LDH_VIEW_PHASE_EDGE ||
-           view == LDH_VIEW_FIELD_PSF || view == LDH_VIEW_CHROMATIC_SPLIT;
+    return view == LDH_VIEW_PUPIL || view == LDH_VIEW_PSF || view == LDH_VIEW_OTF || view == LDH_VIEW_PHASE ||
+           view == LDH_VIEW_PHASE_EDGE || view == LDH_VIEW_FIELD_PSF || view == LDH_VIEW_CHROMATIC_SPLIT;
 }
 
 void render_static_view(hipStream_t s, FrameWork& fw, const ldh_params& p, int view, const DeviceBank& bank,
@@ -1187,6 +1245,22 @@ void render_static_view(hipStream_t s, FrameWork& fw, const ldh_params& p, int v
         hipLaunchKernelGGL(max_kernel, dim3(256), dim3(256), 0, s, k, int64_t(K) * K, bits);
         return SquareSrc{k + int64_t(c - rc) * K + (c - rc), 2 * rc + 1, K, bank.scale_ptr(kFull, bin), bits};
     };
+    if (view == LDH_VIEW_OTF) {
+        const SquareSrc k = square(0, mx);
+        const int n = k.n;
+        fw.view_pupil.reset(sizeof(float2) * size_t(n) * n + sizeof(float) * size_t(n) * n);
+        float2* rows = fw.view_pupil.as<float2>();
+        float* mag = reinterpret_cast<float*>(rows + size_t(n) * n);
+        const unsigned nb = unsigned(div_up(n, 256));
+        hipLaunchKernelGGL(otf_rows_kernel, dim3(nb, unsigned(n)), dim3(256), sizeof(float) * n, s, k.data, k.pitch, n,
+                           k.scale, rows);
+        hipLaunchKernelGGL(otf_cols_kernel, dim3(nb, unsigned(n)), dim3(256), sizeof(float2) * n, s, rows, n, mag);
+        hipLaunchKernelGGL(max_kernel, dim3(256), dim3(256), 0, s, mag, int64_t(n) * n, mx + 1);
+        const SquareSrc m{mag, n, n, nullptr, mx + 1};
+        hipLaunchKernelGGL(square_view_kernel, grid1(npx), dim3(256), 0, s, m, m, m, 1, dst, w, h);
+        LDH_CHECK_LAUNCH("static-view");
+        return;
+    }
     if (view == LDH_VIEW_CHROMATIC_SPLIT) {
         const SquareSrc r = square(bank.bins - 1, mx), g = square(bank.bins / 2, mx + 1), b = square(0, mx + 2);
         hipLaunchKernelGGL(square_view_kernel, grid1(npx), dim3(256), 0, s, r, g, b, 1, dst, w, h);

@@ -189,7 +189,7 @@ std::string unsupported_reason(const ldh_params& p, bool allow_field_zones) {
     switch (p.debug_view) {
         case LDH_VIEW_FINAL: case LDH_VIEW_SELECTION: case LDH_VIEW_EFFECT: case LDH_VIEW_CORE:
         case LDH_VIEW_STRUCTURE: case LDH_VIEW_CREATIVE_FRINGE: case LDH_VIEW_SCATTER: case LDH_VIEW_PUPIL:
-        case LDH_VIEW_PSF: case LDH_VIEW_PHASE: case LDH_VIEW_PHASE_EDGE: case LDH_VIEW_FIELD_PSF:
+        case LDH_VIEW_PSF: case LDH_VIEW_OTF: case LDH_VIEW_PHASE: case LDH_VIEW_PHASE_EDGE: case LDH_VIEW_FIELD_PSF:
         case LDH_VIEW_CHROMATIC_SPLIT: break;
         default: return "debug-view-not-supported";
     }

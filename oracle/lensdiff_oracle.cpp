@@ -1444,9 +1444,26 @@ Rgba<T> view_phase(const std::vector<T>& ph, const std::vector<T>& pupil, int n,
     return out;
 }
 
+// computeOtfMagnitude (:1403-1414): |DFT| of the kernel wrapped to the origin on its own n x n
+// grid (paddedSpectrumFromKernel, :1153-1165), centred by fftShiftSquare (:303-314).
+template <typename T> std::vector<T> otf_magnitude(const Kernel<T>& k) {
+    const int n = std::max(1, k.size), c = k.size / 2;
+    std::vector<std::complex<T>> d(size_t(n) * n);
+    for (int y = 0; y < k.size; ++y)
+        for (int x = 0; x < k.size; ++x)
+            d[size_t((y - c + n) % n) * n + size_t((x - c + n) % n)] = std::complex<T>(k.v[size_t(y) * k.size + x], T(0));
+    fft_2d(d, n, n, false);
+    std::vector<T> out(size_t(n) * n);
+    const int half = n / 2;
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x)
+            out[size_t(y) * n + x] = std::abs(d[size_t((y + half) % n) * n + size_t((x + half) % n)]);
+    return out;
+}
+
 inline bool static_view_kind(int view) {
-    return view == LDH_VIEW_PUPIL || view == LDH_VIEW_PSF || view == LDH_VIEW_PHASE || view == LDH_VIEW_PHASE_EDGE ||
-           view == LDH_VIEW_FIELD_PSF || view == LDH_VIEW_CHROMATIC_SPLIT;
+    return view == LDH_VIEW_PUPIL || view == LDH_VIEW_PSF || view == LDH_VIEW_OTF || view == LDH_VIEW_PHASE ||
+           view == LDH_VIEW_PHASE_EDGE || view == LDH_VIEW_FIELD_PSF || view == LDH_VIEW_CHROMATIC_SPLIT;
 }
 
 // The static view of the bank (global parameters) or, for the phase-edge and field-PSF views with
@@ -1460,6 +1477,10 @@ template <typename T> Rgba<T> static_view(const ldh_params& p, const Bank<T>& ba
         case LDH_VIEW_PHASE:
         case LDH_VIEW_PHASE_EDGE: return view_phase(pupil_phase<T>(q), pupil_amplitude<T>(q), n, ow, oh);
         case LDH_VIEW_PSF: return view_square(bank.bins.front().full.v, bank.bins.front().full.size, ow, oh, true);
+        case LDH_VIEW_OTF: {
+            const Kernel<T>& k = bank.bins.front().full;
+            return view_square(otf_magnitude(k), std::max(1, k.size), ow, oh, true);
+        }
         case LDH_VIEW_FIELD_PSF: {
             if (!edge) return view_square(bank.bins.front().full.v, bank.bins.front().full.size, ow, oh, true);
             std::unique_ptr<Bank<T>> zb(build_bank<T>(q));

@@ -172,6 +172,7 @@ def main():
     # static debug views (pupil, PSF, phase, edge-zone phase and PSF, chromatic split)
     for name, kw in (("view_pupil", dict(debug_view=_abi.VIEW_PUPIL)),
                      ("view_psf_spectral", dict(debug_view=_abi.VIEW_PSF, spectral_mode=2)),
+                     ("view_otf", dict(debug_view=_abi.VIEW_OTF)),
                      ("view_phase", dict(debug_view=_abi.VIEW_PHASE)),
                      ("view_phase_edge_field", dict(debug_view=_abi.VIEW_PHASE_EDGE, **FIELD)),
                      ("view_field_psf_field", dict(debug_view=_abi.VIEW_FIELD_PSF, **FIELD)),

@@ -360,6 +360,8 @@ def test_resolution_aware_matches_oracle(ctx, oracle, case):
 STATIC_VIEWS = {
     "pupil": dict(debug_view=_abi.VIEW_PUPIL),
     "psf-spectral": dict(debug_view=_abi.VIEW_PSF, spectral_mode=2),
+    "otf": dict(debug_view=_abi.VIEW_OTF),
+    "otf-wide": dict(debug_view=_abi.VIEW_OTF, max_kernel_radius_px=60.0),
     "phase": dict(debug_view=_abi.VIEW_PHASE),
     "phase-edge-field": dict(debug_view=_abi.VIEW_PHASE_EDGE, **FIELD),
     "field-psf-field": dict(debug_view=_abi.VIEW_FIELD_PSF, **FIELD),

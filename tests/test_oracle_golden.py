@@ -116,7 +116,7 @@ def test_resolution_aware_frame(g, oracle, name):
 
 @pytest.mark.parametrize("name", ["mono_physical_augment", "mono_split_preserve", "tristimulus", "spectral9_split",
                                   "field_zones_split", "field_zones_tristimulus", "scatter_fringe_split",
-                                  "scatter_view", "fringe_view_preserve", "view_pupil", "view_psf_spectral",
+                                  "scatter_view", "fringe_view_preserve", "view_pupil", "view_psf_spectral", "view_otf",
                                   "view_phase", "view_phase_edge_field", "view_field_psf_field",
                                   "view_chromatic_split"])
 def test_frame(g, oracle, name):
