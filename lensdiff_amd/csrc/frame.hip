@@ -1173,7 +1173,6 @@ void render_impl(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_par
         return;
     }
     const int ww = std::max(1, int(std::lround(w * ws))), wh = std::max(1, int(std::lround(h * ws)));
-    if (cached && (ww != w || wh != h)) fail(LDH_ERR_INVALID_ARGUMENT, "cached-spectra-mismatch");
     const int64_t nw = int64_t(ww) * wh;
     fw.work.reset(sizeof(float4) * nw);
     hipLaunchKernelGGL(resample_frame_kernel, grid1(nw), dim3(256), 0, s, src, w, h, p.input_transfer,

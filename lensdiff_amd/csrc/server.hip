@@ -233,8 +233,6 @@ void Server::worker(Slot* s) {
         try {
             std::string why = unsupported_reason(r->params);
             // the served spectra are kept for the frame's own grid
-            if (why.empty() && resolution_aware_active(r->params)) why = "resolution-aware-not-supported";
-            if (why.empty() && is_static_view(r->params.debug_view)) why = "debug-view-not-supported";
             if (!why.empty()) throw Error{LDH_ERR_UNSUPPORTED, why};
             if (builder) {
                 try {
@@ -267,56 +265,69 @@ void Server::worker(Slot* s) {
                 if (!e->ready) throw Error{LDH_ERR_DEVICE, e->failed ? e->error : "server-stopping"};
             }
             const DeviceBank& bank = *e->bank;
-            const int nk = cached_kernel_count(r->params);
-            std::shared_ptr<KernelSpectra> spec;
-            if (nk > 0 && (nk == 1 || e->combined_style == with_style(r->params))) {
-                const Grid g = make_grid(r->w, r->h, bank.support);
-                const SpectraKey sk{g.pw, g.ph, nk, r->params.spectra_fp16 != 0 ? 1 : 0};
-                bool mine = false;
-                {
-                    std::unique_lock<std::mutex> lk(mu_);
-                    CacheEntry::Spectra* found = nullptr;
-                    for (auto& sp : e->spectra)
-                        if (sp.key == sk) found = &sp;
-                    if (!found) {
-                        e->spectra.push_back({sk, std::make_shared<KernelSpectra>(), false});
-                        found = &e->spectra.back();
-                        mine = true;
-                        stats_.spectra_misses += 1;
-                    } else {
-                        stats_.spectra_hits += 1;
-                        r->spectra_hit = true;
+            if (is_static_view(r->params.debug_view)) {
+                // drawn from the resident bank or the pupil (field phase is not served)
+                render_static_view(s->stream, s->fw, r->params, r->params.debug_view, bank, r->dst, r->w, r->h);
+            } else {
+                const int nk = cached_kernel_count(r->params);
+                std::shared_ptr<KernelSpectra> spec;
+                if (nk > 0 && (nk == 1 || e->combined_style == with_style(r->params))) {
+                    // a resolution-aware frame is convolved at its working size
+                    int gw = r->w, gh = r->h;
+                    if (resolution_aware_active(r->params)) {
+                        const double ws = working_scale(r->params);
+                        gw = std::max(1, int(std::lround(r->w * ws)));
+                        gh = std::max(1, int(std::lround(r->h * ws)));
                     }
-                    spec = found->spec;
-                    if (!mine) cv_.wait(lk, [&] {
+                    const Grid g = make_grid(gw, gh, bank.support);
+                    const SpectraKey sk{g.pw, g.ph, nk, r->params.spectra_fp16 != 0 ? 1 : 0};
+                    bool mine = false;
+                    {
+                        std::unique_lock<std::mutex> lk(mu_);
+                        CacheEntry::Spectra* found = nullptr;
                         for (auto& sp : e->spectra)
-                            if (sp.key == sk) return sp.ready;
-                        return true;
-                    });
-                }
-                if (mine) {
-                    std::vector<KernelRef> ks;
-                    if (nk == 1) {
-                        ks.push_back({bank.kernel_ptr(LDH_KERNEL_FULL, 0), bank.scale_ptr(LDH_KERNEL_FULL, 0)});
-                    } else {
-                        const size_t kk = size_t(bank.kernel) * bank.kernel;
-                        for (int i = 0; i < nk; ++i) ks.push_back({e->combined.as<float>() + kk * i, nullptr});
+                            if (sp.key == sk) found = &sp;
+                        if (!found) {
+                            e->spectra.push_back({sk, std::make_shared<KernelSpectra>(), false});
+                            found = &e->spectra.back();
+                            mine = true;
+                            stats_.spectra_misses += 1;
+                        } else {
+                            stats_.spectra_hits += 1;
+                            r->spectra_hit = true;
+                        }
+                        spec = found->spec;
+                        if (!mine) cv_.wait(lk, [&] {
+                            for (auto& sp : e->spectra)
+                                if (sp.key == sk) return sp.ready;
+                            return true;
+                        });
                     }
-                    build_kernel_spectra(s->stream, fft_, ks, bank.kernel, bank.support, g, sk.fp16 != 0, *spec, s->tmp);
-                    LDH_HIP(hipStreamSynchronize(s->stream), "served-spectra");
-                    std::lock_guard<std::mutex> lk(mu_);
-                    for (auto& sp : e->spectra)
-                        if (sp.key == sk) sp.ready = true;
-                    cv_.notify_all();
+                    if (mine) {
+                        std::vector<KernelRef> ks;
+                        if (nk == 1) {
+                            ks.push_back({bank.kernel_ptr(LDH_KERNEL_FULL, 0), bank.scale_ptr(LDH_KERNEL_FULL, 0)});
+                        } else {
+                            const size_t kk = size_t(bank.kernel) * bank.kernel;
+                            for (int i = 0; i < nk; ++i) ks.push_back({e->combined.as<float>() + kk * i, nullptr});
+                        }
+                        build_kernel_spectra(s->stream, fft_, ks, bank.kernel, bank.support, g, sk.fp16 != 0, *spec,
+                                             s->tmp);
+                        LDH_HIP(hipStreamSynchronize(s->stream), "served-spectra");
+                        std::lock_guard<std::mutex> lk(mu_);
+                        for (auto& sp : e->spectra)
+                            if (sp.key == sk) sp.ready = true;
+                        cv_.notify_all();
+                    }
                 }
+                ldh_params p = r->params;
+                if (spec && nk > 1) {
+                    p.conv_mode = LDH_CONV_PRECOMBINED;
+                    s->fw.combined.reset(0);  // cached spectra carry the combined kernels
+                }
+                render_frame(s->stream, fft_, s->fw, p, bank, r->src, r->dst, r->w, r->h,
+                             spec && nk > 1 ? EffectMode::Precombined : EffectMode::PerBin, spec.get());
             }
-            ldh_params p = r->params;
-            if (spec && nk > 1) {
-                p.conv_mode = LDH_CONV_PRECOMBINED;
-                s->fw.combined.reset(0);  // cached spectra carry the combined kernels
-            }
-            render_frame(s->stream, fft_, s->fw, p, bank, r->src, r->dst, r->w, r->h,
-                         spec && nk > 1 ? EffectMode::Precombined : EffectMode::PerBin, spec.get());
             LDH_HIP(hipStreamSynchronize(s->stream), "served-frame");
             r->status = LDH_OK;
         } catch (const Error& err) {

@@ -91,3 +91,41 @@ def test_served_rejects_unsupported_and_reports(ctx):
         ctx.server_stop()
         ctx.free(s)
         ctx.free(d)
+
+
+def test_served_static_views_and_working_scale_match_plain(ctx):
+    """A served static view is drawn from the resident bank like the plain one; a served
+    resolution-aware frame convolves at its working size with cached spectra."""
+    img = workloads.frame(96, 64, 0.01, seed=3)
+    cases = []
+    for view in (_abi.VIEW_PSF, _abi.VIEW_OTF, _abi.VIEW_PUPIL):
+        p = workloads.base_params(64)
+        p.pupil_resolution = 64
+        p.max_kernel_radius_px = 20.0
+        p.spectral_mode = 1
+        p.debug_view = view
+        cases.append(p)
+    for mode in (0, 1):  # mono (one cached kernel) and tristimulus (pre-combined spectra)
+        p = workloads.base_params(64)
+        p.pupil_resolution = 64
+        p.resolution_aware = 1
+        p.frame_short_side_px = 1350
+        p.max_kernel_radius_px = 0.8
+        p.diffraction_scale_px = 0.1
+        p.spectral_mode = mode
+        cases.append(p)
+    plain = [ctx.render_image(p, img) for p in cases]
+    s = ctx.alloc(img.nbytes)
+    d = ctx.alloc(img.nbytes)
+    ctx.upload(s, img)
+    ctx.server_start(1, 1 << 30)
+    try:
+        for p, want in zip(cases, plain):
+            ctx.wait(ctx.submit(p, s, d, 96, 64))
+            got = np.empty_like(img)
+            ctx.download(d, got)
+            assert rel_l2(got, want) <= 1e-5, (p.debug_view, p.spectral_mode)
+    finally:
+        ctx.server_stop()
+        ctx.free(s)
+        ctx.free(d)
