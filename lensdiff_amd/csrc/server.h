@@ -43,6 +43,8 @@ struct SpectraKey {
 struct CacheEntry {
     BankKey key;
     std::shared_ptr<DeviceBank> bank;
+    // the nine zone banks of a field-dependent phase, zone (zx, zy) at zy * 3 + zx (bank unset)
+    std::vector<std::shared_ptr<DeviceBank>> zones;
     DeviceBuffer combined;  // precombined natural (+ style) kernels, float[6][K*K]
     bool combined_style = false;
     struct Spectra {

@@ -27,6 +27,7 @@ bool with_style(const ldh_params& p) {
 // Real kernels whose spectra serve this request: precombined RGB (3, or 6 with style) for a
 // spectral bank, the single full kernel for a mono bank; 0 = not cacheable (split look).
 int cached_kernel_count(const ldh_params& p) {
+    if (has_field_phase(p)) return 0;  // nine zone convolutions, per bin
     if (spectral_fullkernel(p)) return with_style(p) ? 6 : 3;
     if (!is_spectral(p) && p.look_mode == LDH_LOOK_PHYSICAL && p.debug_view != LDH_VIEW_CORE &&
         p.debug_view != LDH_VIEW_STRUCTURE)
@@ -37,7 +38,41 @@ int cached_kernel_count(const ldh_params& p) {
 size_t bank_bytes_estimate(const ldh_params& p) {
     const Sizes s = resolve_sizes(p);
     const size_t kk = size_t(2 * s.max_radius + 1) * size_t(2 * s.max_radius + 1);
-    return kk * 4 * (3 * wavelength_list(p).size() + 6);
+    return kk * 4 * (3 * wavelength_list(p).size() + 6) * (has_field_phase(p) ? kFieldZones : 1);
+}
+
+// The parameters of the nine zone banks of a field-dependent phase, each built from its resolved
+// global terms (buildPsfBankGlobalOnly).
+std::vector<ldh_params> zone_requests(const ldh_params& p) {
+    std::vector<ldh_params> zones;
+    for (int z = 0; z < kFieldZones; ++z) {
+        zones.push_back(field_zone_params(p, z % 3, z / 3));
+        zones.back().phase_field_strength = 0.0;
+    }
+    return zones;
+}
+
+// Cache identity of a request: its bank key, or its zones' keys in order.
+BankKey served_key(const ldh_params& p) {
+    if (!has_field_phase(p)) return make_bank_key(p);
+    BankKey k;
+    k.values.push_back(double(kFieldZones));
+    for (const ldh_params& z : zone_requests(p)) {
+        const BankKey zk = make_bank_key(z);
+        k.values.insert(k.values.end(), zk.values.begin(), zk.values.end());
+    }
+    return k;
+}
+
+// The grid size a request convolves at: its working size when resolution-aware.
+void served_frame_size(const ldh_params& p, int w, int h, int& gw, int& gh) {
+    gw = w;
+    gh = h;
+    if (resolution_aware_active(p)) {
+        const double ws = working_scale(p);
+        gw = std::max(1, int(std::lround(w * ws)));
+        gh = std::max(1, int(std::lround(h * ws)));
+    }
 }
 size_t spectra_bytes(const Grid& g, int kernels, bool fp16) {
     return g.cells() * size_t((kernels + 1) / 2) * (fp16 ? 4 : 8);
@@ -46,6 +81,8 @@ size_t spectra_bytes(const Grid& g, int kernels, bool fp16) {
 
 size_t CacheEntry::bytes() const {
     size_t b = (bank ? bank->bytes() : 0) + combined.bytes();
+    for (const auto& z : zones)
+        if (z) b += z->bytes();
     for (const auto& s : spectra)
         if (s.spec) b += s.spec->bytes();
     return b;
@@ -181,7 +218,7 @@ ServedRequest* Server::admit(Slot* s, std::shared_ptr<CacheEntry>* entry, bool* 
         if (stopping_) return nullptr;
         if (!queue_.empty()) {
             ServedRequest* r = queue_.front();
-            const BankKey key = make_bank_key(r->params);
+            const BankKey key = served_key(r->params);
             std::shared_ptr<CacheEntry> e;
             for (auto it = cache_.begin(); it != cache_.end(); ++it)
                 if ((*it)->key == key) {
@@ -193,7 +230,9 @@ ServedRequest* Server::admit(Slot* s, std::shared_ptr<CacheEntry>* entry, bool* 
             const int nk = cached_kernel_count(r->params);
             if (nk > 0) {
                 const int support = e && e->ready ? e->bank->support : resolve_sizes(r->params).max_radius;
-                need += spectra_bytes(make_grid(r->w, r->h, support), nk, r->params.spectra_fp16 != 0);
+                int gw = 0, gh = 0;
+                served_frame_size(r->params, r->w, r->h, gw, gh);
+                need += spectra_bytes(make_grid(gw, gh, support), nk, r->params.spectra_fp16 != 0);
             }
             bool in_flight = false;
             for (const auto& c : cache_) in_flight = in_flight || c->pins > 0;
@@ -231,13 +270,19 @@ void Server::worker(Slot* s) {
         std::unique_lock<std::mutex> prof_lock(s->prof_mu);
         ProfilerBinding bind(s->prof);
         try {
-            std::string why = unsupported_reason(r->params);
+            std::string why = unsupported_reason(r->params, true);
             // the served spectra are kept for the frame's own grid
             if (!why.empty()) throw Error{LDH_ERR_UNSUPPORTED, why};
             if (builder) {
                 try {
-                    std::shared_ptr<DeviceBank> bank(build_bank(s->stream, fft_, s->bw, r->params).release());
-                    if (spectral_fullkernel(r->params)) {
+                    std::shared_ptr<DeviceBank> bank;
+                    std::vector<std::shared_ptr<DeviceBank>> zones;
+                    if (has_field_phase(r->params))
+                        zones = build_banks_batched(s->stream, fft_, s->bw, zone_requests(r->params), LDH_BATCH_GROUPED,
+                                                    nullptr);
+                    else
+                        bank = std::shared_ptr<DeviceBank>(build_bank(s->stream, fft_, s->bw, r->params).release());
+                    if (bank && spectral_fullkernel(r->params)) {
                         const bool sty = with_style(r->params);
                         const SpectrumConfig cfg = build_spectrum_config(r->params, bank->wavelengths);
                         e->combined.reset(sizeof(float) * size_t(bank->kernel) * bank->kernel * 6);
@@ -249,6 +294,7 @@ void Server::worker(Slot* s) {
                     LDH_HIP(hipStreamSynchronize(s->stream), "served-bank");
                     std::lock_guard<std::mutex> lk(mu_);
                     e->bank = bank;
+                    e->zones = zones;
                     e->ready = true;
                 } catch (const Error& err) {
                     std::lock_guard<std::mutex> lk(mu_);
@@ -264,21 +310,32 @@ void Server::worker(Slot* s) {
                 cv_.wait(lk, [&] { return e->ready || e->failed || stopping_; });
                 if (!e->ready) throw Error{LDH_ERR_DEVICE, e->failed ? e->error : "server-stopping"};
             }
-            const DeviceBank& bank = *e->bank;
-            if (is_static_view(r->params.debug_view)) {
-                // drawn from the resident bank or the pupil (field phase is not served)
-                render_static_view(s->stream, s->fw, r->params, r->params.debug_view, bank, r->dst, r->w, r->h);
+            if (!e->zones.empty()) {
+                if (is_static_view(r->params.debug_view)) {
+                    // zone (1, 1) holds the global terms; the edge views show zone (2, 2)
+                    const bool edge =
+                        r->params.debug_view == LDH_VIEW_PHASE_EDGE || r->params.debug_view == LDH_VIEW_FIELD_PSF;
+                    const ldh_params q = edge ? field_zone_params(r->params, 2, 2) : r->params;
+                    render_static_view(s->stream, s->fw, q, r->params.debug_view, *e->zones[edge ? 8 : 4], r->dst,
+                                       r->w, r->h);
+                } else {
+                    std::vector<const DeviceBank*> zb;
+                    for (const auto& z : e->zones) zb.push_back(z.get());
+                    ldh_params p = r->params;
+                    p.conv_mode = LDH_CONV_PER_BIN;
+                    render_frame_zones(s->stream, fft_, s->fw, p, zb, r->src, r->dst, r->w, r->h, EffectMode::PerBin);
+                }
+            } else if (is_static_view(r->params.debug_view)) {
+                // drawn from the resident bank or the pupil
+                render_static_view(s->stream, s->fw, r->params, r->params.debug_view, *e->bank, r->dst, r->w, r->h);
             } else {
+                const DeviceBank& bank = *e->bank;
                 const int nk = cached_kernel_count(r->params);
                 std::shared_ptr<KernelSpectra> spec;
                 if (nk > 0 && (nk == 1 || e->combined_style == with_style(r->params))) {
                     // a resolution-aware frame is convolved at its working size
-                    int gw = r->w, gh = r->h;
-                    if (resolution_aware_active(r->params)) {
-                        const double ws = working_scale(r->params);
-                        gw = std::max(1, int(std::lround(r->w * ws)));
-                        gh = std::max(1, int(std::lround(r->h * ws)));
-                    }
+                    int gw = 0, gh = 0;
+                    served_frame_size(r->params, r->w, r->h, gw, gh);
                     const Grid g = make_grid(gw, gh, bank.support);
                     const SpectraKey sk{g.pw, g.ph, nk, r->params.spectra_fp16 != 0 ? 1 : 0};
                     bool mine = false;

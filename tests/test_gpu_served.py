@@ -129,3 +129,36 @@ def test_served_static_views_and_working_scale_match_plain(ctx):
         ctx.server_stop()
         ctx.free(s)
         ctx.free(d)
+
+
+def test_served_field_zones_match_plain(ctx):
+    """A served request with field-dependent phase keeps its nine zone banks resident and renders
+    the zone blend like the plain path; its edge views come from zone (2, 2)."""
+    img = workloads.frame(96, 64, 0.01, seed=8)
+    cases = []
+    for view in (_abi.VIEW_FINAL, _abi.VIEW_FIELD_PSF, _abi.VIEW_PHASE_EDGE):
+        p = workloads.base_params(64)
+        p.pupil_resolution = 64
+        p.max_kernel_radius_px = 20.0
+        p.spectral_mode = 1
+        p.set_noll({4: 0.25, 11: -0.15})
+        p.phase_field_strength, p.phase_field_edge_bias = 1.0, 0.3
+        p.phase_field_coma_radial, p.phase_field_astig_tangential = 0.3, 0.2
+        p.debug_view = view
+        cases.append(p)
+    plain = [ctx.render_image(p, img) for p in cases]
+    s = ctx.alloc(img.nbytes)
+    d = ctx.alloc(img.nbytes)
+    ctx.upload(s, img)
+    ctx.server_start(1, 1 << 30)
+    try:
+        for i, (p, want) in enumerate(zip(cases, plain)):
+            st = ctx.wait(ctx.submit(p, s, d, 96, 64))
+            assert st.bank_hit == (i > 0)  # one entry holds the zone banks for every view
+            got = np.empty_like(img)
+            ctx.download(d, got)
+            assert rel_l2(got, want) <= 1e-6, p.debug_view
+    finally:
+        ctx.server_stop()
+        ctx.free(s)
+        ctx.free(d)
