@@ -424,9 +424,13 @@ ldh_status ldh_fft2d(ldh_context* ctx, float* data, int32_t w, int32_t h, int32_
 ldh_status ldh_bank_acquire(ldh_context* ctx, const ldh_params* p, ldh_bank** out) {
     return guarded(ctx, [&] {
         require(out != nullptr, LDH_ERR_INVALID_ARGUMENT, "missing-argument");
-        check_params(p);
+        check_params(p, true);
+        // the bins of LensDiffPsfBankCache: the global bank, also for a field-dependent phase
+        // (buildPsfBankGlobalOnly, LensDiffCpuReference.cpp:1769)
+        ldh_params q = *p;
+        q.phase_field_strength = 0.0;
         auto* b = new ldh_bank();
-        b->bank = ensure_bank(ctx, *p);
+        b->bank = ensure_bank(ctx, q);
         b->key = ctx->cached_key;
         *out = b;
     });
@@ -439,6 +443,9 @@ ldh_status ldh_bank_build_batch(ldh_context* ctx, const ldh_params* params, int3
                 LDH_ERR_INVALID_ARGUMENT, "invalid-argument");
         for (int32_t i = 0; i < count; ++i) banks[i] = nullptr;
         std::vector<ldh_params> reqs(params, params + count);
+        // a field-dependent phase keeps its global bank here, as the reference's cache.bins
+        // (buildPsfBankGlobalOnly); its zone banks live with the renders
+        for (ldh_params& q : reqs) q.phase_field_strength = 0.0;
         ldh::BatchTiming tm;
         // a capture must not record timing events: bind a disabled profiler
         ldh::Profiler none;

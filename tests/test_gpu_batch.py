@@ -59,3 +59,29 @@ def test_batched_bank_matches_oracle(ctx, oracle):
     # a crop radius is an energy threshold on ring sums: one sitting on it moves with the last bit
     # of the FP32 field (the pupil chirp product), so a few of the radii may differ by one step
     assert flips <= max(2, kernels // 10), (flips, kernels)
+
+
+def test_field_phase_banks_are_the_global_bank(ctx):
+    """A bank handle or batched bank of a field-dependent phase holds the global bank, the bins of
+    the reference's cache (its zone banks serve the renders)."""
+    reqs = small_batch()[:2]
+    field = []
+    for p in reqs:
+        q = type(p).from_buffer_copy(p)
+        q.phase_mode = _abi.PHASE_ENABLED
+        q.phase_field_strength, q.phase_field_coma_radial = 1.0, 0.3
+        field.append(q)
+    glob = [type(q).from_buffer_copy(q) for q in field]
+    for q in glob:
+        q.phase_field_strength = 0.0
+    plain, _ = build(ctx, field, _abi.LDH_BATCH_GROUPED)
+    want, _ = build(ctx, glob, _abi.LDH_BATCH_GROUPED)
+    for a, b in zip(plain, want):
+        for name in ("full", "core", "structure"):
+            for x, y in zip(a[name], b[name]):
+                assert np.array_equal(x, y), name
+    # the single-bank handle of the same request
+    one = ctx.bank(field[0])
+    for name in ("full", "core", "structure"):
+        for x, y in zip(one[name], want[0][name]):
+            assert np.array_equal(x, y), name
