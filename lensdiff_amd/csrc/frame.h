@@ -53,6 +53,8 @@ struct FrameWork {
     DeviceBuffer src, dst, mask, image_spec, out_spec, planes, effect, lum, totals, factors, matrices, kspec_tmp,
         combined, max_buf, zone, post, gauss, work, redist, display, view_pupil, view_coef;
     KernelSpectra kspec;  // per-frame kernel spectra (plain path)
+    float gauss_radius = -1.0f;  // the scatter kernel held in `gauss` (radius in px, side)
+    int gauss_size = 0;
     SeqSum seqsum;
 };
 

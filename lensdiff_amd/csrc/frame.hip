@@ -951,13 +951,20 @@ std::vector<float> gaussian_kernel(float radius_px, int& size) {
 // kernel's spectrum in the first inverse pass, cropped and clamped at zero into out3.
 void scatter_pass(hipStream_t s, const FftTables& t, FrameWork& fw, float radius_px, const float* eff, int w, int h,
                   float* out3) {
-    int K = 0;
-    const std::vector<float> k = gaussian_kernel(radius_px, K);
+    // the kernel is built on the host once per radius and kept on the device
+    if (fw.gauss_radius != radius_px || fw.gauss_size == 0) {
+        int K = 0;
+        const std::vector<float> k = gaussian_kernel(radius_px, K);
+        fw.gauss.reset(sizeof(float) * k.size());
+        LDH_HIP(hipMemcpyAsync(fw.gauss.get(), k.data(), sizeof(float) * k.size(), hipMemcpyHostToDevice, s),
+                "scatter-kernel");
+        LDH_HIP(hipStreamSynchronize(s), "scatter-kernel");
+        fw.gauss_radius = radius_px;
+        fw.gauss_size = K;
+    }
+    const int K = fw.gauss_size;
     const int r = K / 2;
     const int64_t npx = int64_t(w) * h;
-    fw.gauss.reset(sizeof(float) * k.size());
-    LDH_HIP(hipMemcpyAsync(fw.gauss.get(), k.data(), sizeof(float) * k.size(), hipMemcpyHostToDevice, s),
-            "scatter-kernel");
     const Grid g = make_grid(w, h, r);
     const size_t cells = g.cells();
     const float2* tw = t.twiddle.as<float2>();
