@@ -321,9 +321,12 @@ void Server::worker(Slot* s) {
                 } else {
                     std::vector<const DeviceBank*> zb;
                     for (const auto& z : e->zones) zb.push_back(z.get());
+                    // a spectral full-kernel look convolves each zone with its pre-combined RGB kernels
                     ldh_params p = r->params;
-                    p.conv_mode = LDH_CONV_PER_BIN;
-                    render_frame_zones(s->stream, fft_, s->fw, p, zb, r->src, r->dst, r->w, r->h, EffectMode::PerBin);
+                    const bool pre = spectral_fullkernel(p);
+                    p.conv_mode = pre ? LDH_CONV_PRECOMBINED : LDH_CONV_PER_BIN;
+                    render_frame_zones(s->stream, fft_, s->fw, p, zb, r->src, r->dst, r->w, r->h,
+                                       pre ? EffectMode::Precombined : EffectMode::PerBin);
                 }
             } else if (is_static_view(r->params.debug_view)) {
                 // drawn from the resident bank or the pupil

@@ -133,7 +133,8 @@ def test_served_static_views_and_working_scale_match_plain(ctx):
 
 def test_served_field_zones_match_plain(ctx):
     """A served request with field-dependent phase keeps its nine zone banks resident and renders
-    the zone blend like the plain path; its edge views come from zone (2, 2)."""
+    the zone blend like the plain path (pre-combined kernels for the spectral look); its edge views
+    come from zone (2, 2)."""
     img = workloads.frame(96, 64, 0.01, seed=8)
     cases = []
     for view in (_abi.VIEW_FINAL, _abi.VIEW_FIELD_PSF, _abi.VIEW_PHASE_EDGE):
@@ -157,7 +158,8 @@ def test_served_field_zones_match_plain(ctx):
             assert st.bank_hit == (i > 0)  # one entry holds the zone banks for every view
             got = np.empty_like(img)
             ctx.download(d, got)
-            assert rel_l2(got, want) <= 1e-6, p.debug_view
+            # the served zones convolve with pre-combined kernels, the plain ones per bin
+            assert rel_l2(got, want) <= 1e-5, p.debug_view
     finally:
         ctx.server_stop()
         ctx.free(s)
