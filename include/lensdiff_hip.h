@@ -311,6 +311,12 @@ int32_t ldh_tuning_report(char* buf, int32_t capacity);
  * so far (no counterpart in the reference: lets a caller or a test run each variant). */
 void ldh_tuning_force(int32_t variant);
 
+/* Test hook (no counterpart in the reference): the next `count` times the library reaches the
+ * named failure site it fails there as a device error would, with the tag "injected-<site>".
+ * Sites: "served-bank" (a served request building its bank), "served-spectra" (a served request
+ * building the kernel spectra of its FFT grid). count = 0 disarms the site. Process-wide. */
+void ldh_debug_inject_failure(const char* site, int32_t count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,7 @@ SIGNATURES = {
     "ldh_profiling_reset": (C.c_int32, [_CTX]),
     "ldh_tuning_report": (C.c_int32, [C.c_char_p, C.c_int32]),
     "ldh_tuning_force": (None, [C.c_int32]),
+    "ldh_debug_inject_failure": (None, [C.c_char_p, C.c_int32]),
 }
 
 _lib = None

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 
@@ -17,6 +18,19 @@
 #include "host_params.h"
 #include "profiler.h"
 #include "server.h"
+
+namespace ldh {
+std::mutex g_debug_mu;
+std::map<std::string, int> g_debug_failures;
+
+void debug_failure_point(const char* site) {
+    std::lock_guard<std::mutex> lk(g_debug_mu);
+    auto it = g_debug_failures.find(site);
+    if (it == g_debug_failures.end()) return;
+    if (--it->second <= 0) g_debug_failures.erase(it);
+    throw Error{LDH_ERR_DEVICE, std::string("injected-") + site};
+}
+}  // namespace ldh
 
 struct ldh_bank {
     std::shared_ptr<ldh::DeviceBank> bank;
@@ -690,5 +704,14 @@ int32_t ldh_tuning_report(char* buf, int32_t capacity) {
 }
 
 void ldh_tuning_force(int32_t variant) { ldh::Autotuner::instance().force(int(variant)); }
+
+void ldh_debug_inject_failure(const char* site, int32_t count) {
+    if (site == nullptr) return;
+    std::lock_guard<std::mutex> lk(ldh::g_debug_mu);
+    if (count <= 0)
+        ldh::g_debug_failures.erase(site);
+    else
+        ldh::g_debug_failures[site] = int(count);
+}
 
 }  // extern "C"

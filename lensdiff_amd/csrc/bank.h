@@ -8,6 +8,7 @@
 #include "host_params.h"
 #include "ldh_internal.h"
 #include "psf.h"
+#include "rings.h"
 #include "seqsum.h"
 
 namespace ldh {
@@ -59,12 +60,19 @@ struct BankWorkspace {
     PsfScratch psf;
     SeqSum seqsum;
     DeviceBuffer amp, phase, coefs, raw, totals, base, base_totals, shape_totals, scales, mean, shaped, structure,
-        ring_sum, ring_cnt, ring_peak, ring_mean, radius, ref_amp, ref_phase, ref_raw, ref_total;
+        ring_sum, ring_peak, ring_mean, radius, ref_amp, ref_phase, ref_raw, ref_total;
+    // ring orders of the kernel sizes seen (azimuthal-mean and support rings), and their scratch
+    std::vector<std::unique_ptr<RingPlan>> ring_plans;
+    RingScratch ring_scratch;
+    // The plan for (size, center, rmax, ceil), built on first use (a host step: call
+    // reserve_shaping before a graph capture).
+    const RingPlan& ring_plan(hipStream_t s, int size, float center, int rmax, bool ceil);
     DeviceBuffer block_sums;  // producers' block sums handed to the sequential sums, [3][count][blocks]
     // first-zero radius of the clear-circle PSF per (P, R) (GetLensDiffReferenceRawPsfCached)
     std::vector<std::pair<std::pair<int, int>, float>> first_zero;
-    // Grows the shaping buffers for `count` kernels of side K ahead of a graph capture.
-    void reserve_shaping(int K, int count, int max_radius);
+    // Grows the shaping buffers for `count` kernels of side K, and builds their ring plans, ahead
+    // of a graph capture.
+    void reserve_shaping(hipStream_t s, int K, int count, int max_radius);
 };
 
 float first_minimum_from_profile(const std::vector<float>& profile, int size);

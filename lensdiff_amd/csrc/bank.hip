@@ -4,8 +4,8 @@
 // paddedAdaptiveSupportRadius, cropKernelToRadius, applySupportBoundaryTaper, :957-1151),
 // all batched over the wavelengths of one bank.
 //
-// Ring statistics (azimuthal sums/counts, support-radius energies/peaks) use per-workgroup LDS
-// histograms flushed once per workgroup, with FP64 sums. Every unit-sum normalisation goes
+// Ring sums (azimuthal means, support-radius energies, the radial profile) are the reference's
+// float sums in raster order, reproduced exactly by rings.hip. Every unit-sum normalisation goes
 // through SeqSum so the scale matches the reference's sequential FP32 accumulation.
 #include <algorithm>
 #include <cmath>
@@ -97,136 +97,6 @@ __device__ __forceinline__ int ring_of(int x, int y, float c, int rmax) {
     return min(rmax, r);
 }
 
-constexpr int kRingThreads = 256;
-constexpr int kRingRun = 8;                   // consecutive pixels of one row per thread
-constexpr int kRingSpan = 64 * kRingRun;      // pixels of a row one wave takes at a time
-constexpr int kRingRowsPerBlock = 16;         // rows per workgroup
-
-// Orders a wave's LDS writes before its later LDS reads by other lanes of the same wave.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// sums[r] += v, peaks[r] = max(v) over the pixels of ring r, per segment; counts[r] += 1 for
-// segment 0 only (ring sizes are the same for every segment of one size).
-// prescale (nullable): per-segment totals; pixels are read as v * (1 / total), the values an
-// in-place normalisation would have left.
-// Each thread takes runs of kRingRun consecutive pixels of a row and adds a run's pixels of one
-// ring in registers before one LDS atomic per ring it meets: where a row runs along a ring (far
-// above or below the centre, or the clamped corners) the lanes of a wave would otherwise all hit
-// the same LDS word, and where the ring changes every pixel, lanes kRingRun pixels apart hit
-// different words. A wave loads a span of kRingSpan pixels coalesced (lane l, load k: pixel
-// 64 k + l) and hands the runs out through LDS (lane l reads pixels 8 l .. 8 l + 7).
-// MIRROR (centre = (size - 1) / 2): pixel (x, y) and its mirrors (size-1-x, y), (x, size-1-y),
-// (size-1-x, size-1-y) have the same dx^2 and dy^2, so the same ring; the waves walk the top-left
-// quadrant (with the centre row and column) and add each distinct mirror once.
-template <bool CEIL, bool MIRROR>
-__global__ void __launch_bounds__(kRingThreads)
-ring_stats_kernel(const float* __restrict__ img, int size, float center, int rmax, const float* __restrict__ prescale,
-                  double* __restrict__ sums, int* __restrict__ counts, unsigned* __restrict__ peaks) {
-    extern __shared__ unsigned char smem[];
-    // one span per wave, one pad word per 8 (stride-1 writes and stride-9 reads hit distinct banks)
-    __shared__ float stage[kRingThreads / 64][kRingSpan + kRingSpan / 8];
-    const int nr = rmax + 1;
-    double* s_sum = reinterpret_cast<double*>(smem);
-    int* s_cnt = reinterpret_cast<int*>(s_sum + nr);
-    unsigned* s_pk = reinterpret_cast<unsigned*>(s_cnt + nr);
-    for (int r = threadIdx.x; r < nr; r += kRingThreads) { s_sum[r] = 0.0; s_cnt[r] = 0; s_pk[r] = 0u; }
-    __syncthreads();
-    const int seg = blockIdx.y;
-    const int64_t npx = int64_t(size) * size;
-    const float* base = img + seg * npx;
-    const float sc = inverse_total(prescale, seg);
-    const bool count = counts != nullptr && seg == 0;
-    const int w = MIRROR ? (size + 1) / 2 : size;  // columns and rows walked
-    const int y0 = blockIdx.x * kRingRowsPerBlock;
-    const int y1 = min(w, y0 + kRingRowsPerBlock);
-    const int spans = (w + kRingSpan - 1) / kRingSpan;  // spans per row
-    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
-    float* st = stage[wave];
-    auto flush = [&](int r, double acc, int n, unsigned pk) {
-        atomicAdd(&s_sum[r], acc);
-        if (count) atomicAdd(&s_cnt[r], n);
-        if (peaks) atomicMax(&s_pk[r], pk);
-    };
-    constexpr int kCopies = MIRROR ? 4 : 1;
-    // tasks (row, span) go to waves round-robin: every lane of a wave has the same trip count
-    for (int task = wave; task < (y1 - y0) * spans; task += kRingThreads / 64) {
-        const int y = y0 + task / spans;
-        const int x0 = (task % spans) * kRingSpan;
-        const int ym = size - 1 - y;
-        // every copy's loads issued together, clamped inside the walked columns
-        float t[kCopies][kRingRun];
-#pragma unroll
-        for (int c = 0; c < kCopies; ++c) {
-            const float* src = base + int64_t((c & 2) ? ym : y) * size;
-#pragma unroll
-            for (int k = 0; k < kRingRun; ++k) {
-                const int x = min(x0 + 64 * k + lane, w - 1);
-                t[c][k] = src[(c & 1) ? size - 1 - x : x];
-            }
-        }
-        float v[kCopies][kRingRun];
-#pragma unroll
-        for (int c = 0; c < kCopies; ++c) {
-#pragma unroll
-            for (int k = 0; k < kRingRun; ++k) {
-                const int i = 64 * k + lane;
-                st[i + i / 8] = t[c][k];
-            }
-            wave_lds_sync();
-#pragma unroll
-            for (int j = 0; j < kRingRun; ++j) {
-                const int i = kRingRun * lane + j;
-                v[c][j] = st[i + i / 8];
-            }
-            wave_lds_sync();
-        }
-        const int xs = x0 + kRingRun * lane;
-        const int n = min(kRingRun, w - xs);  // <= 0 past the row's end
-        if (n <= 0) continue;
-        int rc = ring_of<CEIL>(xs, y, center, rmax);
-        double acc = 0.0;
-        int cnt = 0;
-        unsigned pk = 0u;
-#pragma unroll
-        for (int k = 0; k < kRingRun; ++k) {
-            if (k < n) {
-                const int x = xs + k;
-                const int r = ring_of<CEIL>(x, y, center, rmax);
-                if (r != rc) {
-                    flush(rc, acc, cnt, pk);
-                    rc = r;
-                    acc = 0.0;
-                    cnt = 0;
-                    pk = 0u;
-                }
-#pragma unroll
-                for (int c = 0; c < kCopies; ++c) {
-                    // mirrors that coincide with the pixel (centre row or column) count once
-                    const bool distinct = c == 0 || (((c & 1) == 0 || size - 1 - x != x) && ((c & 2) == 0 || ym != y));
-                    if (distinct) {
-                        const float val = v[c][k] * sc;
-                        acc += double(val);
-                        ++cnt;
-                        pk = max(pk, __float_as_uint(val));  // val >= 0: bit order is value order
-                    }
-                }
-            }
-        }
-        flush(rc, acc, cnt, pk);
-    }
-    __syncthreads();
-    for (int r = threadIdx.x; r < nr; r += kRingThreads) {
-        if (s_cnt[r] == 0 && s_sum[r] == 0.0 && s_pk[r] == 0u) continue;
-        atomicAdd(&sums[int64_t(seg) * nr + r], s_sum[r]);
-        if (count) atomicAdd(&counts[r], s_cnt[r]);
-        if (peaks) atomicMax(&peaks[int64_t(seg) * nr + r], s_pk[r]);
-    }
-}
-
 // The elementwise shaping kernels below run workgroups of kSumThreads threads over blocks of
 // kSeqSumBlock elements of one kernel (grid: seqsum_blocks(K*K) x kernels) and write the FP64
 // sums of the blocks of each array they produce, [array][kernel][block].
@@ -249,13 +119,13 @@ __device__ __forceinline__ void advance_texel(int& x, int& y, int K) {
     }
 }
 
-// Mean of each ring, float(sum) / float(count) (count at least 1), per segment.
-__global__ void ring_means_kernel(const double* __restrict__ sums, const int* __restrict__ counts, int nr,
+// Mean of each ring, sum / float(count) (count at least 1), per segment (:1106-1114).
+__global__ void ring_means_kernel(const float* __restrict__ sums, const int* __restrict__ counts, int nr,
                                   float* __restrict__ means) {
     const int seg = blockIdx.y;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nr) return;
-    means[int64_t(seg) * nr + r] = float(sums[int64_t(seg) * nr + r]) / float(max(1, counts[r]));
+    means[int64_t(seg) * nr + r] = sums[int64_t(seg) * nr + r] / float(max(1, counts[r]));
 }
 
 // The azimuthal-mean image: each texel takes its ring's mean (the ring means precomputed once).
@@ -341,12 +211,12 @@ residual_kernel(const float* __restrict__ shaped, const float* __restrict__ shap
 // the reference's break, is found by a ballot.
 constexpr int kSupportThreads = 256;
 __global__ void __launch_bounds__(kSupportThreads)
-support_radius_kernel(const double* __restrict__ energy, const unsigned* __restrict__ peaks,
+support_radius_kernel(const float* __restrict__ energy, const unsigned* __restrict__ peaks,
                       const float* __restrict__ totals, int rmax, int max_radius, int* __restrict__ radius) {
     extern __shared__ unsigned char smem[];
     const int seg = blockIdx.x;
     const int nr = rmax + 1;
-    double* e = reinterpret_cast<double*>(smem);
+    float* e = reinterpret_cast<float*>(smem);
     unsigned* pk = reinterpret_cast<unsigned*>(e + nr);
     for (int r = threadIdx.x; r < nr; r += kSupportThreads) {
         e[r] = energy[int64_t(seg) * nr + r];
@@ -379,7 +249,7 @@ support_radius_kernel(const double* __restrict__ energy, const unsigned* __restr
         float cum = 0.0f;
         for (int base = 0; base <= rmax; base += 64) {
             const int r = base + lane;
-            const float ev = r <= rmax ? float(e[r]) : 0.0f;
+            const float ev = r <= rmax ? e[r] : 0.0f;
             float mine = 0.0f;
 #pragma unroll
             for (int j = 0; j < 64; ++j) {
@@ -499,37 +369,6 @@ __global__ void fill_zero_kernel(uint32_t* __restrict__ p, int64_t n) {
 
 inline dim3 blocks_grid(int64_t n, int segs) { return dim3(unsigned(seqsum_blocks(n)), unsigned(segs)); }
 
-template <bool CEIL>
-void ring_stats(hipStream_t s, const float* img, int size, float center, int rmax, int segs, const float* prescale,
-                double* sums, int* counts, unsigned* peaks) {
-    const int nr = rmax + 1;
-    // cleared by a kernel, not hipMemsetAsync, so the sequence can be captured into a graph
-    auto clear = [&](void* p, int64_t words) {
-        hipLaunchKernelGGL(fill_zero_kernel, dim3(unsigned(std::min<int64_t>(div_up(words, 256), 1024))), dim3(256),
-                           0, s, static_cast<uint32_t*>(p), words);
-    };
-    clear(sums, int64_t(nr) * segs * 2);
-    if (counts) clear(counts, int64_t(nr));  // segment 0's ring sizes serve every segment
-    if (peaks) clear(peaks, int64_t(nr) * segs);
-    const size_t lds = size_t(nr) * (sizeof(double) + sizeof(int) + sizeof(unsigned));
-    const int64_t npx = int64_t(size) * size;
-    const bool mirror = center == float(size - 1) * 0.5f;
-    using Kernel = void (*)(const float*, int, float, int, const float*, double*, int*, unsigned*);
-    const Kernel both[2] = {&ring_stats_kernel<CEIL, false>, &ring_stats_kernel<CEIL, true>};
-    static bool attr_set = false;
-    if (!attr_set) {
-        for (const Kernel k : both)
-            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      128 * 1024);
-        attr_set = true;
-    }
-    ProfileScope ps("shape-ring-stats", 4.0 * double(npx) * segs);
-    const int walked = mirror ? (size + 1) / 2 : size;
-    hipLaunchKernelGGL(both[mirror ? 1 : 0], dim3(unsigned(div_up(walked, kRingRowsPerBlock)), unsigned(segs)),
-                       dim3(kRingThreads), lds, s, img, size, center, rmax, prescale, sums, counts, peaks);
-    LDH_CHECK_LAUNCH("ring-stats");
-}
-
 }  // namespace
 
 void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_scales, const float* d_raw_totals,
@@ -545,12 +384,25 @@ void base_kernels(hipStream_t s, BankWorkspace& w, const float* raw, int R, cons
                   const float* d_raw_totals, int support, int count, float* out, float* totals) {
     const int K = 2 * support + 1;
     const int64_t kk = int64_t(K) * K;
-    w.reserve_shaping(K, count, support);
+    w.reserve_shaping(s, K, count, support);
     resample_kernels(s, raw, R, d_scales, d_raw_totals, support, count, out, w.block_sums.as<double>());
     w.seqsum.run(s, out, kk, kk, count, totals, w.block_sums.as<double>());
 }
 
-void BankWorkspace::reserve_shaping(int K, int count, int max_radius) {
+const RingPlan& BankWorkspace::ring_plan(hipStream_t s, int size, float center, int rmax, bool ceil) {
+    for (auto& p : ring_plans)
+        if (p->matches(size, center, rmax, ceil)) return *p;
+    constexpr size_t kKeep = 16;  // two per kernel size; a batch holds at most a few sizes
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    LDH_HIP(hipStreamIsCapturing(s, &cap), "ring-plan");
+    if (cap != hipStreamCaptureStatusNone) fail(LDH_ERR_DEVICE, "ring-plan-built-during-capture");
+    if (ring_plans.size() >= kKeep) ring_plans.erase(ring_plans.begin());
+    ring_plans.push_back(std::make_unique<RingPlan>());
+    prepare_ring_plan(s, *ring_plans.back(), size, center, rmax, ceil);
+    return *ring_plans.back();
+}
+
+void BankWorkspace::reserve_shaping(hipStream_t s, int K, int count, int max_radius) {
     const int64_t kk = int64_t(K) * K;
     const int64_t n = kk * count;
     mean.reset(sizeof(float) * n);
@@ -559,12 +411,14 @@ void BankWorkspace::reserve_shaping(int K, int count, int max_radius) {
     // sums: mean, shaped, structure, normalised-shaped run (count each), final (3 * count)
     shape_totals.reset(sizeof(float) * 7 * count);
     const int nr = std::max(K / 2, max_radius) + 1;
-    ring_sum.reset(sizeof(double) * nr * count);
+    ring_sum.reset(sizeof(float) * nr * count);
     block_sums.reset(sizeof(double) * size_t(seqsum_blocks(kk)) * 3 * count);
-    ring_cnt.reset(sizeof(int) * nr * count);
     ring_peak.reset(sizeof(unsigned) * nr * count);
     ring_mean.reset(sizeof(float) * nr * count);
     seqsum.reserve(kk, 3 * count);
+    const float c = float(K - 1) * 0.5f;
+    ring_scratch.reserve(ring_plan(s, K, c, K / 2, false), count);
+    ring_scratch.reserve(ring_plan(s, K, c, std::min(max_radius, K / 2), true), count);
 }
 
 void apply_kernel_scales(hipStream_t s, float* out3, int K, int count, const int* d_radius, const float* scales) {
@@ -579,7 +433,9 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
                    const float* d_gains) {
     const int64_t kk = int64_t(K) * K;
     const int rmax_mean = K / 2;
-    w.reserve_shaping(K, count, max_radius);
+    w.reserve_shaping(s, K, count, max_radius);
+    const float kc = float(K - 1) * 0.5f;
+    const RingPlan& mean_rings = w.ring_plan(s, K, kc, rmax_mean, false);
     float* t_mean = w.shape_totals.as<float>();
     float* t_shaped = t_mean + count;
     float* t_struct = t_shaped + count;
@@ -588,15 +444,14 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
     // Each normalisation below only sums its array; readers apply 1 / total as they load, which
     // gives the values an in-place normalisation would have left.
     // azimuthal mean of the normalised base
-    ring_stats<false>(s, base, K, float(K - 1) * 0.5f, rmax_mean, count, base_totals, w.ring_sum.as<double>(),
-                      w.ring_cnt.as<int>(), nullptr);
+    ring_sums(s, mean_rings, w.ring_scratch, w.seqsum, base, kk, count, base_totals, w.ring_sum.as<float>(), nullptr);
     double* bsum = w.block_sums.as<double>();
     const dim3 grid = blocks_grid(kk, count);
     {
         ProfileScope ps_("shape-ring-mean", 4.0 * double(kk) * count);
         float* means = w.ring_mean.as<float>();
         hipLaunchKernelGGL(ring_means_kernel, dim3(unsigned(div_up(rmax_mean + 1, 256)), unsigned(count)), dim3(256), 0,
-                           s, w.ring_sum.as<double>(), w.ring_cnt.as<int>(), rmax_mean + 1, means);
+                           s, w.ring_sum.as<float>(), mean_rings.count.as<int>(), rmax_mean + 1, means);
         hipLaunchKernelGGL(ring_mean_kernel, grid, dim3(kSumThreads), 0, s, means, K, rmax_mean, w.mean.as<float>(),
                            bsum);
         LDH_CHECK_LAUNCH("ring-mean");
@@ -629,14 +484,14 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
     w.seqsum.run(s, g, kk, 2 * count, t_struct, bsum);  // t_struct, then t_run
     // adaptive support radius of the normalised shaped kernel
     const int rmax_sup = std::min(max_radius, K / 2);
-    ring_stats<true>(s, w.shaped.as<float>(), K, float(K - 1) * 0.5f, rmax_sup, count, t_shaped,
-                     w.ring_sum.as<double>(), nullptr, w.ring_peak.as<unsigned>());
-    const size_t sup_lds = size_t(rmax_sup + 1) * (sizeof(double) + sizeof(unsigned));
-{
-    ProfileScope ps_("shape-support-radius", 0.0);
+    ring_sums(s, w.ring_plan(s, K, kc, rmax_sup, true), w.ring_scratch, w.seqsum, w.shaped.as<float>(), kk, count,
+              t_shaped, w.ring_sum.as<float>(), w.ring_peak.as<unsigned>());
+    const size_t sup_lds = size_t(rmax_sup + 1) * (sizeof(float) + sizeof(unsigned));
+    {
+        ProfileScope ps_("shape-support-radius", 0.0);
         hipLaunchKernelGGL(support_radius_kernel, dim3(unsigned(count)), dim3(kSupportThreads), sup_lds, s,
-                       w.ring_sum.as<double>(), w.ring_peak.as<unsigned>(), t_run, rmax_sup, max_radius, d_radius);
-    LDH_CHECK_LAUNCH("support-radius");
+                           w.ring_sum.as<float>(), w.ring_peak.as<unsigned>(), t_run, rmax_sup, max_radius, d_radius);
+        LDH_CHECK_LAUNCH("support-radius");
     }
     // crop + taper of full (shaped), core (mean), structure, then normalise all three
     TaperSources ts{{w.shaped.as<float>(), w.mean.as<float>(), w.structure.as<float>()}, {t_shaped, t_mean, t_struct}};
@@ -655,17 +510,21 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
 std::vector<float> radial_profile(hipStream_t s, BankWorkspace& w, const float* psf, int R) {
     const int rmax = R / 2;
     const int nr = rmax + 1;
-    w.ring_sum.reset(sizeof(double) * nr);
-    w.ring_cnt.reset(sizeof(int) * nr);
     const float center = (R & 1) == 0 ? float(R) * 0.5f : float(R - 1) * 0.5f;
-    ring_stats<false>(s, psf, R, center, rmax, 1, nullptr, w.ring_sum.as<double>(), w.ring_cnt.as<int>(), nullptr);
-    std::vector<double> sums(static_cast<size_t>(nr));
-    std::vector<int> counts(static_cast<size_t>(nr));
-    LDH_HIP(hipMemcpyAsync(sums.data(), w.ring_sum.get(), sizeof(double) * nr, hipMemcpyDeviceToHost, s), "profile");
-    LDH_HIP(hipMemcpyAsync(counts.data(), w.ring_cnt.get(), sizeof(int) * nr, hipMemcpyDeviceToHost, s), "profile");
+    // one profile per (P, R) and workspace: the plan is not kept
+    RingPlan plan;
+    prepare_ring_plan(s, plan, R, center, rmax, false);
+    RingScratch scratch;
+    DeviceBuffer sums_d(sizeof(float) * nr);
+    ring_sums(s, plan, scratch, w.seqsum, psf, int64_t(R) * R, 1, nullptr, sums_d.as<float>(), nullptr);
+    std::vector<float> sums(static_cast<size_t>(nr));
+    LDH_HIP(hipMemcpyAsync(sums.data(), sums_d.get(), sizeof(float) * nr, hipMemcpyDeviceToHost, s), "profile");
     LDH_HIP(hipStreamSynchronize(s), "profile");
     std::vector<float> prof(static_cast<size_t>(nr));
-    for (int r = 0; r < nr; ++r) prof[size_t(r)] = float(sums[size_t(r)]) / float(std::max(1, counts[size_t(r)]));
+    for (int r = 0; r < nr; ++r) {
+        const int cnt = plan.host_off[size_t(r) + 1] - plan.host_off[size_t(r)];
+        prof[size_t(r)] = sums[size_t(r)] / float(std::max(1, cnt));
+    }
     return prof;
 }
 

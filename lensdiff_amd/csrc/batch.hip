@@ -172,7 +172,7 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
         w.base_totals.reset(sizeof(float) * g.units);
         w.psf.reserve(g.P, g.R, g.units);
         w.seqsum.reserve(int64_t(g.R) * g.R, g.units);
-        w.reserve_shaping(g.K, g.units, g.r);
+        w.reserve_shaping(s, g.K, g.units, g.r);
         tm.psfs += g.units;
     }
     LDH_HIP(hipStreamSynchronize(s), "batch-upload");

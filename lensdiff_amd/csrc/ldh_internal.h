@@ -30,6 +30,10 @@ struct Error {
 
 inline void fail(ldh_status s, const std::string& tag) { throw Error{s, tag}; }
 
+// Test hook (ldh_debug_inject_failure): throws Error{LDH_ERR_DEVICE, "injected-<site>"} while
+// failures are armed for `site`, consuming one each time.
+void debug_failure_point(const char* site);
+
 // ---- small device helpers ------------------------------------------------------------------
 // Complex products may contract to FMA (the library is built with -ffp-contract=off so that the
 // pupil, selection and ring arithmetic keep their float semantics; transforms gain from FMA).

@@ -1,0 +1,51 @@
+// Ring sums as the reference forms them: one float accumulator per ring, the pixels visited in
+// raster order (azimuthalMeanKernel, LensDiffCpuReference.cpp:1094-1104; estimateAdaptiveSupport-
+// Radius, :997-1005; radialMeanProfile, :869-878). The integer support radius compares the running
+// float sum of these ring sums against thresholds, so the sums are reproduced bit for bit.
+#pragma once
+
+#include <vector>
+
+#include "ldh_internal.h"
+#include "seqsum.h"
+
+namespace ldh {
+
+// Pixels of a size x size image grouped by ring about `center`: ring(x, y) = round(|d|)
+// (ceil = false) or ceil(|d|) (ceil = true), clamped to rmax. order[off[r] .. off[r+1]) lists the
+// pixels of ring r in raster order; the last ring (rmax) holds every pixel the clamp sends there.
+struct RingPlan {
+    int size = 0, rmax = -1;
+    float center = 0.0f;
+    bool ceil = false;
+    int64_t pixels = 0;
+    std::vector<int> host_off;  // rmax + 2 entries
+    DeviceBuffer order;         // int32[pixels]
+    DeviceBuffer off;           // int32[rmax + 2]: host_off on the device
+    DeviceBuffer count;         // int32[rmax + 1]: ring sizes
+    bool matches(int sz, float c, int rm, bool cl) const {
+        return size == sz && center == c && rmax == rm && ceil == cl;
+    }
+    int64_t corner_begin() const { return host_off[size_t(rmax)]; }
+    int64_t corner_pixels() const { return pixels - corner_begin(); }
+};
+
+// Builds (host counting sort, uploaded on stream s) unless the plan already matches.
+void prepare_ring_plan(hipStream_t s, RingPlan& plan, int size, float center, int rmax, bool ceil);
+
+// Scratch of ring_sums (corner-ring gather and its sequential sum).
+struct RingScratch {
+    DeviceBuffer corner, corner_total;
+    void reserve(const RingPlan& plan, int segs);
+};
+
+// For every segment g (image g at img + g * seg_stride, each pixel read as v * (1 / prescale[g])
+// when prescale is given and positive, the values an in-place normalisation would leave):
+//   sums[g * (rmax + 1) + r]  = the float sum of ring r's pixels added in raster order,
+//   peaks[g * (rmax + 1) + r] = the bits of their maximum (nullable; values are >= 0).
+// Rings below rmax take one wavefront each; the clamped last ring, which can hold a large part of
+// the image, is gathered and summed by the SeqSum engine. Pixel values must be >= 0 and finite.
+void ring_sums(hipStream_t s, const RingPlan& plan, RingScratch& scratch, SeqSum& seqsum, const float* img,
+               int64_t seg_stride, int segs, const float* prescale, float* sums, unsigned* peaks);
+
+}  // namespace ldh

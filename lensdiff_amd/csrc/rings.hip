@@ -1,0 +1,231 @@
+// Exact ring sums (see rings.h). A float running sum that stays inside one binade [2^e, 2^(e+1))
+// adds each addend v as a multiple of u = 2^(e-23): fl(s + v) = s + rint(v / u) * u, unless v / u
+// is exactly halfway between two integers (the tie then rounds to the even result, which depends
+// on s) or the sum leaves the binade. A wavefront therefore adds its 64 elements of a ring at once
+// as an integer sum of rint(v / u) whenever no lane holds a tie and the total stays below 2^(e+1);
+// otherwise it adds the 64 elements one after the other in float, in order, on every lane. Either
+// way the result is the reference's sequential float sum, bit for bit.
+#include <algorithm>
+#include <cmath>
+
+#include "profiler.h"
+#include "rings.h"
+
+namespace ldh {
+
+namespace {
+
+constexpr int kRingWaves = 4;     // wavefronts (rings) per workgroup
+constexpr int kRingGroups = 8;    // 64-element groups whose loads are issued together
+constexpr int kGatherThreads = 256;
+
+__device__ __forceinline__ float inverse_total(const float* __restrict__ totals, int seg) {
+    if (totals == nullptr) return 1.0f;
+    const float t = totals[seg];
+    return t > 0.0f ? 1.0f / t : 1.0f;
+}
+
+// Sum over the 64 lanes, returned in every lane: DPP row shifts within rows of 16, then the rows'
+// totals broadcast forward (row_bcast:15 / :31), read back from lane 63.
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
+    int x = int(v);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return unsigned(__builtin_amdgcn_readlane(x, 63));
+}
+
+__device__ __forceinline__ float lane_value(float v, int j) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), j));
+}
+
+// s (the same in every lane) followed by the 64 addends v of lanes 0..63, in lane order, with a
+// float accumulator. Addends are >= 0 and finite.
+__device__ __forceinline__ float add_group_sequential(float s, float v) {
+    if (s >= 0x1p-100f) {
+        int ex;
+        (void)frexpf(s, &ex);              // s in [2^(ex-1), 2^ex): u = 2^(ex-24)
+        const int sh = 24 - ex;
+        const float q = ldexpf(v, sh);     // v / u, exact
+        const bool bad = !(q < 0x1p24f) || (q - floorf(q) == 0.5f);
+        const unsigned c = bad ? 0u : unsigned(rintf(q));
+        const unsigned long long any_bad = __ballot(bad);
+        const unsigned tot = wave_sum_u32(c);
+        const unsigned ms = unsigned(ldexpf(s, sh));  // s / u in [2^23, 2^24)
+        if (any_bad == 0ull && ms + tot < (1u << 24)) return ldexpf(float(ms + tot), -sh);
+    }
+#pragma unroll
+    for (int j = 0; j < 64; ++j) s = s + lane_value(v, j);
+    return s;
+}
+
+// One wavefront per (segment, ring r < rings). Workgroups are spread over the eight XCDs in
+// contiguous ranges of items (workgroup b runs on XCD b % 8), so neighbouring rings of one image,
+// whose pixels share cache lines, are summed through the same L2.
+__global__ void __launch_bounds__(64 * kRingWaves)
+ring_wave_sums_kernel(const float* __restrict__ img, int64_t seg_stride, const float* __restrict__ prescale,
+                      const int* __restrict__ order, const int* __restrict__ off, int rings, int items, int per_xcd,
+                      float* __restrict__ sums, unsigned* __restrict__ peaks, int nr) {
+    const int b = int(blockIdx.x);
+    const int logical = (b % 8) * per_xcd + b / 8;
+    const int item = logical * kRingWaves + int(threadIdx.x >> 6);
+    if (item >= items) return;
+    const int lane = int(threadIdx.x) & 63;
+    const int seg = item / rings, r = item % rings;
+    const float* x = img + int64_t(seg) * seg_stride;
+    const float inv = inverse_total(prescale, seg);
+    const int i0 = off[r], i1 = off[r + 1];
+    float s = 0.0f;
+    unsigned pk = 0u;
+    for (int base = i0; base < i1; base += 64 * kRingGroups) {
+        int idx[kRingGroups];
+#pragma unroll
+        for (int g = 0; g < kRingGroups; ++g) idx[g] = order[min(base + 64 * g + lane, i1 - 1)];
+        float v[kRingGroups];
+#pragma unroll
+        for (int g = 0; g < kRingGroups; ++g) {
+            const float t = x[idx[g]] * inv;
+            v[g] = base + 64 * g + lane < i1 ? t : 0.0f;
+        }
+#pragma unroll
+        for (int g = 0; g < kRingGroups; ++g) {
+            if (base + 64 * g >= i1) break;
+            pk = max(pk, __float_as_uint(v[g]));
+            s = add_group_sequential(s, v[g]);
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) pk = max(pk, unsigned(__shfl_xor(int(pk), d)));
+    if (lane == 0) {
+        sums[int64_t(seg) * nr + r] = s;
+        if (peaks) peaks[int64_t(seg) * nr + r] = pk;
+    }
+}
+
+// The last ring's pixels of each segment, gathered in raster order (scaled as above) for the
+// sequential-sum engine; its peak by atomic maximum (bits of values >= 0 order as the values).
+__global__ void __launch_bounds__(kGatherThreads)
+ring_corner_gather_kernel(const float* __restrict__ img, int64_t seg_stride, const float* __restrict__ prescale,
+                          const int* __restrict__ order, int64_t n, float* __restrict__ dst,
+                          unsigned* __restrict__ peak_slot, int nr) {
+    const int seg = blockIdx.y;
+    const float* x = img + int64_t(seg) * seg_stride;
+    const float inv = inverse_total(prescale, seg);
+    unsigned pk = 0u;
+    for (int64_t i = int64_t(blockIdx.x) * kGatherThreads + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * kGatherThreads) {
+        const float v = x[order[i]] * inv;
+        dst[int64_t(seg) * n + i] = v;
+        pk = max(pk, __float_as_uint(v));
+    }
+    if (peak_slot == nullptr) return;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) pk = max(pk, unsigned(__shfl_xor(int(pk), d)));
+    if ((threadIdx.x & 63) == 0 && pk != 0u) atomicMax(&peak_slot[int64_t(seg) * nr], pk);
+}
+
+// Clears the last ring's peak slot of every segment (before the gather's atomic maxima).
+__global__ void ring_corner_clear_kernel(unsigned* __restrict__ peak_slot, int nr, int segs) {
+    const int g = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g < segs) peak_slot[int64_t(g) * nr] = 0u;
+}
+
+// sums[g * nr + rmax] = the last ring's sequential sum (0 for an empty ring).
+__global__ void ring_corner_store_kernel(const float* __restrict__ totals, float* __restrict__ sum_slot, int nr,
+                                         int segs) {
+    const int g = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g < segs) sum_slot[int64_t(g) * nr] = totals != nullptr ? totals[g] : 0.0f;
+}
+
+// The ring of pixel (x, y) in float arithmetic, as the reference evaluates it: dx, dy exact,
+// sqrt correctly rounded, then round (half away from zero) or ceil, clamped to rmax.
+inline int host_ring(int x, int y, float c, int rmax, bool ceil) {
+    const float dx = float(x) - c, dy = float(y) - c;
+    const float d = std::sqrt(dx * dx + dy * dy);
+    const int r = ceil ? int(std::ceil(d)) : int(std::round(d));
+    return std::min(rmax, r);
+}
+
+}  // namespace
+
+void prepare_ring_plan(hipStream_t s, RingPlan& p, int size, float center, int rmax, bool ceil) {
+    if (p.matches(size, center, rmax, ceil)) return;
+    if (size <= 0 || rmax < 0) fail(LDH_ERR_INVALID_ARGUMENT, "ring-plan-invalid");
+    p.size = 0;
+    p.rmax = -1;
+    const int nr = rmax + 1;
+    const int64_t n = int64_t(size) * size;
+    std::vector<int> off(static_cast<size_t>(nr) + 1, 0);
+    for (int y = 0; y < size; ++y)
+        for (int x = 0; x < size; ++x) ++off[size_t(host_ring(x, y, center, rmax, ceil)) + 1];
+    std::vector<int> count(static_cast<size_t>(nr));
+    for (int r = 0; r < nr; ++r) {
+        count[size_t(r)] = off[size_t(r) + 1];
+        off[size_t(r) + 1] += off[size_t(r)];
+    }
+    std::vector<int> order(static_cast<size_t>(n));
+    std::vector<int> next(off.begin(), off.end() - 1);
+    for (int y = 0; y < size; ++y)
+        for (int x = 0; x < size; ++x)
+            order[size_t(next[size_t(host_ring(x, y, center, rmax, ceil))]++)] = y * size + x;
+    p.order.reset(sizeof(int) * size_t(n));
+    p.off.reset(sizeof(int) * (size_t(nr) + 1));
+    p.count.reset(sizeof(int) * size_t(nr));
+    LDH_HIP(hipMemcpyAsync(p.order.get(), order.data(), sizeof(int) * size_t(n), hipMemcpyHostToDevice, s),
+            "ring-plan");
+    LDH_HIP(hipMemcpyAsync(p.off.get(), off.data(), sizeof(int) * (size_t(nr) + 1), hipMemcpyHostToDevice, s),
+            "ring-plan");
+    LDH_HIP(hipMemcpyAsync(p.count.get(), count.data(), sizeof(int) * size_t(nr), hipMemcpyHostToDevice, s),
+            "ring-plan");
+    LDH_HIP(hipStreamSynchronize(s), "ring-plan");  // the host vectors go out of scope
+    p.host_off = std::move(off);
+    p.size = size;
+    p.center = center;
+    p.rmax = rmax;
+    p.ceil = ceil;
+    p.pixels = n;
+}
+
+void RingScratch::reserve(const RingPlan& plan, int segs) {
+    const int64_t n = plan.corner_pixels();
+    corner.reset(sizeof(float) * size_t(std::max<int64_t>(n, 1)) * segs);
+    corner_total.reset(sizeof(float) * size_t(segs));
+}
+
+void ring_sums(hipStream_t s, const RingPlan& plan, RingScratch& scratch, SeqSum& seqsum, const float* img,
+               int64_t seg_stride, int segs, const float* prescale, float* sums, unsigned* peaks) {
+    const int rmax = plan.rmax, nr = rmax + 1;
+    const int64_t corner = plan.corner_pixels();
+    ProfileScope ps("ring-sums", 4.0 * double(plan.pixels) * segs);
+    if (rmax > 0) {
+        const int items = rmax * segs;
+        const int per_xcd = int(div_up(div_up(items, kRingWaves), 8));
+        hipLaunchKernelGGL(ring_wave_sums_kernel, dim3(unsigned(8 * per_xcd)), dim3(64 * kRingWaves), 0, s, img,
+                           seg_stride, prescale, plan.order.as<int>(), plan.off.as<int>(), rmax, items, per_xcd, sums,
+                           peaks, nr);
+        LDH_CHECK_LAUNCH("ring-sums");
+    }
+    const unsigned small = unsigned(div_up(segs, 256));
+    if (corner == 0) {
+        hipLaunchKernelGGL(ring_corner_store_kernel, dim3(small), dim3(256), 0, s, nullptr, sums + rmax, nr, segs);
+        if (peaks) hipLaunchKernelGGL(ring_corner_clear_kernel, dim3(small), dim3(256), 0, s, peaks + rmax, nr, segs);
+        LDH_CHECK_LAUNCH("ring-corner");
+        return;
+    }
+    scratch.reserve(plan, segs);
+    if (peaks) hipLaunchKernelGGL(ring_corner_clear_kernel, dim3(small), dim3(256), 0, s, peaks + rmax, nr, segs);
+    const unsigned gx = unsigned(std::min<int64_t>(div_up(corner, kGatherThreads), 256));
+    hipLaunchKernelGGL(ring_corner_gather_kernel, dim3(gx, unsigned(segs)), dim3(kGatherThreads), 0, s, img,
+                       seg_stride, prescale, plan.order.as<int>() + plan.corner_begin(), corner,
+                       scratch.corner.as<float>(), peaks ? peaks + rmax : nullptr, nr);
+    LDH_CHECK_LAUNCH("ring-corner");
+    seqsum.run(s, scratch.corner.as<float>(), corner, corner, segs, scratch.corner_total.as<float>());
+    hipLaunchKernelGGL(ring_corner_store_kernel, dim3(small), dim3(256), 0, s, scratch.corner_total.as<float>(),
+                       sums + rmax, nr, segs);
+    LDH_CHECK_LAUNCH("ring-corner");
+}
+
+}  // namespace ldh

@@ -47,12 +47,16 @@ struct CacheEntry {
     std::vector<std::shared_ptr<DeviceBank>> zones;
     DeviceBuffer combined;  // precombined natural (+ style) kernels, float[6][K*K]
     bool combined_style = false;
+    // Kernel spectra for one FFT grid. The record is shared with the requests waiting for it, so
+    // a failed build can hand its error to them after it has left the entry.
     struct Spectra {
         SpectraKey key;
-        std::shared_ptr<KernelSpectra> spec;
+        KernelSpectra spec;
         bool ready = false;
+        bool failed = false;
+        std::string error;
     };
-    std::vector<Spectra> spectra;
+    std::vector<std::shared_ptr<Spectra>> spectra;
     bool ready = false;     // bank (and combined kernels) built
     bool failed = false;
     std::string error;

@@ -84,7 +84,7 @@ size_t CacheEntry::bytes() const {
     for (const auto& z : zones)
         if (z) b += z->bytes();
     for (const auto& s : spectra)
-        if (s.spec) b += s.spec->bytes();
+        if (s->ready) b += s->spec.bytes();  // a record in the making is written without the lock
     return b;
 }
 
@@ -275,6 +275,7 @@ void Server::worker(Slot* s) {
             if (!why.empty()) throw Error{LDH_ERR_UNSUPPORTED, why};
             if (builder) {
                 try {
+                    debug_failure_point("served-bank");
                     std::shared_ptr<DeviceBank> bank;
                     std::vector<std::shared_ptr<DeviceBank>> zones;
                     if (has_field_phase(r->params))
@@ -334,7 +335,7 @@ void Server::worker(Slot* s) {
             } else {
                 const DeviceBank& bank = *e->bank;
                 const int nk = cached_kernel_count(r->params);
-                std::shared_ptr<KernelSpectra> spec;
+                std::shared_ptr<CacheEntry::Spectra> spec;
                 if (nk > 0 && (nk == 1 || e->combined_style == with_style(r->params))) {
                     // a resolution-aware frame is convolved at its working size
                     int gw = 0, gh = 0;
@@ -344,39 +345,47 @@ void Server::worker(Slot* s) {
                     bool mine = false;
                     {
                         std::unique_lock<std::mutex> lk(mu_);
-                        CacheEntry::Spectra* found = nullptr;
                         for (auto& sp : e->spectra)
-                            if (sp.key == sk) found = &sp;
-                        if (!found) {
-                            e->spectra.push_back({sk, std::make_shared<KernelSpectra>(), false});
-                            found = &e->spectra.back();
+                            if (sp->key == sk) spec = sp;
+                        if (!spec) {
+                            spec = std::make_shared<CacheEntry::Spectra>();
+                            spec->key = sk;
+                            e->spectra.push_back(spec);
                             mine = true;
                             stats_.spectra_misses += 1;
                         } else {
                             stats_.spectra_hits += 1;
                             r->spectra_hit = true;
+                            cv_.wait(lk, [&] { return spec->ready || spec->failed || stopping_; });
+                            if (!spec->ready)
+                                throw Error{LDH_ERR_DEVICE, spec->failed ? spec->error : "server-stopping"};
                         }
-                        spec = found->spec;
-                        if (!mine) cv_.wait(lk, [&] {
-                            for (auto& sp : e->spectra)
-                                if (sp.key == sk) return sp.ready;
-                            return true;
-                        });
                     }
                     if (mine) {
-                        std::vector<KernelRef> ks;
-                        if (nk == 1) {
-                            ks.push_back({bank.kernel_ptr(LDH_KERNEL_FULL, 0), bank.scale_ptr(LDH_KERNEL_FULL, 0)});
-                        } else {
-                            const size_t kk = size_t(bank.kernel) * bank.kernel;
-                            for (int i = 0; i < nk; ++i) ks.push_back({e->combined.as<float>() + kk * i, nullptr});
+                        try {
+                            debug_failure_point("served-spectra");
+                            std::vector<KernelRef> ks;
+                            if (nk == 1) {
+                                ks.push_back({bank.kernel_ptr(LDH_KERNEL_FULL, 0), bank.scale_ptr(LDH_KERNEL_FULL, 0)});
+                            } else {
+                                const size_t kk = size_t(bank.kernel) * bank.kernel;
+                                for (int i = 0; i < nk; ++i) ks.push_back({e->combined.as<float>() + kk * i, nullptr});
+                            }
+                            build_kernel_spectra(s->stream, fft_, ks, bank.kernel, bank.support, g, sk.fp16 != 0,
+                                                 spec->spec, s->tmp);
+                            LDH_HIP(hipStreamSynchronize(s->stream), "served-spectra");
+                        } catch (const Error& err) {
+                            // the record leaves the entry (a later request builds it again); the
+                            // requests already waiting on it take this error
+                            std::lock_guard<std::mutex> lk(mu_);
+                            spec->failed = true;
+                            spec->error = err.tag;
+                            e->spectra.erase(std::remove(e->spectra.begin(), e->spectra.end(), spec), e->spectra.end());
+                            cv_.notify_all();
+                            throw;
                         }
-                        build_kernel_spectra(s->stream, fft_, ks, bank.kernel, bank.support, g, sk.fp16 != 0, *spec,
-                                             s->tmp);
-                        LDH_HIP(hipStreamSynchronize(s->stream), "served-spectra");
                         std::lock_guard<std::mutex> lk(mu_);
-                        for (auto& sp : e->spectra)
-                            if (sp.key == sk) sp.ready = true;
+                        spec->ready = true;
                         cv_.notify_all();
                     }
                 }
@@ -386,7 +395,8 @@ void Server::worker(Slot* s) {
                     s->fw.combined.reset(0);  // cached spectra carry the combined kernels
                 }
                 render_frame(s->stream, fft_, s->fw, p, bank, r->src, r->dst, r->w, r->h,
-                             spec && nk > 1 ? EffectMode::Precombined : EffectMode::PerBin, spec.get());
+                             spec && nk > 1 ? EffectMode::Precombined : EffectMode::PerBin,
+                             spec ? &spec->spec : nullptr);
             }
             LDH_HIP(hipStreamSynchronize(s->stream), "served-frame");
             r->status = LDH_OK;

@@ -5,7 +5,7 @@ import pytest
 
 from lensdiff_amd import _abi, workloads
 from oracle.oracle import rel_l2
-from tests.kernel_compare import kernel_rel_l2
+from tests.kernel_compare import kernel_rel_l2_ring
 
 pytestmark = pytest.mark.gpu
 
@@ -55,10 +55,11 @@ def test_batched_bank_matches_oracle(ctx, oracle):
             # one crop radius per wavelength sizes its full, core and structure kernels alike
             flips += any(got[name][b].shape != want[name][b].shape for name in ("full", "core", "structure"))
             for name in ("full", "core", "structure"):
-                assert kernel_rel_l2(got[name][b], want[name][b]) < 1e-4, (i, name, b)
-    # a crop radius is an energy threshold on ring sums: one sitting on it moves with the last bit
-    # of the FP32 field (the pupil chirp product), so a few of the radii may differ by one step
-    assert flips <= max(2, kernels // 10), (flips, kernels)
+                assert kernel_rel_l2_ring(got[name][b], want[name][b]) < 1e-4, (i, name, b)
+    # the ring sums are exact, so a flip can only come from a PSF that differs from the oracle's
+    # in its last bits landing on the other side of a threshold: reported, and expected to be rare
+    print("batched banks: %d of %d crop radii differ from the oracle's" % (flips, kernels))
+    assert flips <= 1, (flips, kernels)
 
 
 def test_field_phase_banks_are_the_global_bank(ctx):

@@ -8,7 +8,7 @@ import pytest
 
 from lensdiff_amd import _abi, workloads
 from oracle.oracle import rel_l2
-from tests.kernel_compare import kernel_rel_l2
+from tests.kernel_compare import kernel_rel_l2, kernel_rel_l2_ring
 
 pytestmark = pytest.mark.gpu
 
@@ -155,11 +155,13 @@ def test_raw_psf_every_variant(ctx, oracle, pupil, raw):
         assert rel_l2(g, got[0]) < 1e-5, v
 
 
-def test_first_minimum_radius(ctx, oracle):
-    p = params(pupil_resolution=128, aperture_mode=0, roundness=1.0)
+@pytest.mark.parametrize("pupil,raw", [(128, 512), (96, 1020), (256, 1024)])
+def test_first_minimum_radius(ctx, oracle, pupil, raw):
+    """Same PSF in: the ring sums are the reference's float sums, so the radius is the same float."""
+    p = params(pupil_resolution=pupil, aperture_mode=0, roundness=1.0)
     amp = oracle.pupil_amplitude(p)
-    psf = oracle.raw_psf(amp, None, 512)
-    assert abs(ctx.first_minimum_radius(psf) - oracle.first_minimum_radius(psf)) < 1e-4
+    psf = oracle.raw_psf(amp, None, raw)
+    assert np.float32(ctx.first_minimum_radius(psf)) == np.float32(oracle.first_minimum_radius(psf))
 
 
 @pytest.mark.parametrize("scale,support", [(0.37, 40), (2.5, 30), (1.0, 25)])
@@ -179,8 +181,28 @@ def test_shape_kernels(ctx, oracle, aniso):
     base = oracle.resample_kernel(psf, 0.5, 40)
     got = ctx.shape_kernels(p, base)
     want = oracle.shape_kernels(p, base)
+    # same base in: the same integer crop radius, and kernels equal up to the taper's cosine
     for g, w in zip(got, want[:3]):
-        assert kernel_rel_l2(g, w) < TOL
+        assert kernel_rel_l2(g, w) < 1e-6
+
+
+def test_shape_kernels_of_oracle_bases(ctx, oracle):
+    """The oracle's base kernels of banks at kernel radius 64 and 200 shaped on the device: every
+    bin keeps the oracle's crop radius (ring sums reproduced bit for bit)."""
+    for frac, mode in ((16.0, 3), (50.0, 1)):
+        p = workloads.base_params(400)
+        p.pupil_resolution = 128
+        p.blade_count = 7
+        p.max_kernel_radius_px = frac
+        p.anisotropy_emphasis = 0.25
+        p.spectral_mode = mode
+        p.set_noll({4: 0.2, 11: -0.1, 7: 0.08})
+        want = oracle.bank(p)
+        for b, base in enumerate(want["base"]):
+            got = ctx.shape_kernels(p, base)
+            assert got[0].shape == want["full"][b].shape, (frac, b, got[0].shape, want["full"][b].shape)
+            for g, name in zip(got, ("full", "core", "structure")):
+                assert kernel_rel_l2(g, want[name][b]) < 1e-6, (frac, b, name)
 
 
 BANK_CASES = {
@@ -202,8 +224,12 @@ def test_bank(ctx, oracle, case):
         setattr(p, k, v)
     got = ctx.bank(p)
     want = oracle.bank(p)
-    assert abs(got["support"] - want["support"]) <= 1
     assert np.allclose(got["wavelengths"], want["wavelengths"])
+    # end to end the PSFs differ from the oracle's in their last bits; the radii are expected
+    # equal, and any flip is reported
+    flips = sum(g.shape != w.shape for g, w in zip(got["full"], want["full"]))
+    print("%s: %d of %d crop radii differ" % (case, flips, len(want["full"])))
+    assert flips == 0 and got["support"] == want["support"]
     for name in ("full", "core", "structure"):
         for g, w in zip(got[name], want[name]):
             assert kernel_rel_l2(g, w) < TOL, name
