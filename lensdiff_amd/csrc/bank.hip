@@ -484,8 +484,12 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
     w.seqsum.run(s, g, kk, 2 * count, t_struct, bsum);  // t_struct, then t_run
     // adaptive support radius of the normalised shaped kernel
     const int rmax_sup = std::min(max_radius, K / 2);
+    // The last ring's energy only enters the reference's running sum at r = rmax, whose padded
+    // result is clamped to the maximum radius either way when rmax + 6 reaches it (always, for
+    // K = 2 * max_radius + 1): its sequential sum is formed only when it can matter.
+    const bool last_matters = rmax_sup + 6 < std::max(4, max_radius);
     ring_sums(s, w.ring_plan(s, K, kc, rmax_sup, true), w.ring_scratch, w.seqsum, w.shaped.as<float>(), kk, count,
-              t_shaped, w.ring_sum.as<float>(), w.ring_peak.as<unsigned>());
+              t_shaped, w.ring_sum.as<float>(), w.ring_peak.as<unsigned>(), last_matters);
     const size_t sup_lds = size_t(rmax_sup + 1) * (sizeof(float) + sizeof(unsigned));
     {
         ProfileScope ps_("shape-support-radius", 0.0);

@@ -33,9 +33,9 @@ struct RingPlan {
 // Builds (host counting sort, uploaded on stream s) unless the plan already matches.
 void prepare_ring_plan(hipStream_t s, RingPlan& plan, int size, float center, int rmax, bool ceil);
 
-// Scratch of ring_sums (corner-ring gather and its sequential sum).
+// Scratch of ring_sums: the ring-ordered copy of the segments and the last ring's sums.
 struct RingScratch {
-    DeviceBuffer corner, corner_total;
+    DeviceBuffer ordered, last_total;
     void reserve(const RingPlan& plan, int segs);
 };
 
@@ -43,9 +43,13 @@ struct RingScratch {
 // when prescale is given and positive, the values an in-place normalisation would leave):
 //   sums[g * (rmax + 1) + r]  = the float sum of ring r's pixels added in raster order,
 //   peaks[g * (rmax + 1) + r] = the bits of their maximum (nullable; values are >= 0).
-// Rings below rmax take one wavefront each; the clamped last ring, which can hold a large part of
-// the image, is gathered and summed by the SeqSum engine. Pixel values must be >= 0 and finite.
+// The segments are first copied in ring order (coalesced writes); rings below rmax then take one
+// wavefront each over contiguous data, and the clamped last ring, which can hold a large part of
+// the image, goes to the SeqSum engine. With last_sum false the last ring's sum is not formed
+// (left 0) for callers whose result cannot depend on it; its peak always is. Pixel values must be
+// >= 0 and finite.
 void ring_sums(hipStream_t s, const RingPlan& plan, RingScratch& scratch, SeqSum& seqsum, const float* img,
-               int64_t seg_stride, int segs, const float* prescale, float* sums, unsigned* peaks);
+               int64_t seg_stride, int segs, const float* prescale, float* sums, unsigned* peaks,
+               bool last_sum = true);
 
 }  // namespace ldh

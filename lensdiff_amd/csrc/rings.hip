@@ -62,40 +62,45 @@ __device__ __forceinline__ float add_group_sequential(float s, float v) {
     return s;
 }
 
-// One wavefront per (segment, ring r < rings). Workgroups are spread over the eight XCDs in
-// contiguous ranges of items (workgroup b runs on XCD b % 8), so neighbouring rings of one image,
-// whose pixels share cache lines, are summed through the same L2.
+// One wavefront per (segment, ring r < rings), over the ring-ordered copy (ring r of segment g at
+// ordered[g * pixels + off[r] ..]), reading one batch of groups ahead of the one it adds.
+// Workgroups are spread over the eight XCDs in contiguous ranges of items (workgroup b runs on
+// XCD b % 8), so the rings of one image share an L2.
 __global__ void __launch_bounds__(64 * kRingWaves)
-ring_wave_sums_kernel(const float* __restrict__ img, int64_t seg_stride, const float* __restrict__ prescale,
-                      const int* __restrict__ order, const int* __restrict__ off, int rings, int items, int per_xcd,
-                      float* __restrict__ sums, unsigned* __restrict__ peaks, int nr) {
+ring_wave_sums_kernel(const float* __restrict__ ordered, int64_t pixels, const int* __restrict__ off, int rings,
+                      int items, int per_xcd, float* __restrict__ sums, unsigned* __restrict__ peaks, int nr) {
     const int b = int(blockIdx.x);
     const int logical = (b % 8) * per_xcd + b / 8;
     const int item = logical * kRingWaves + int(threadIdx.x >> 6);
     if (item >= items) return;
     const int lane = int(threadIdx.x) & 63;
     const int seg = item / rings, r = item % rings;
-    const float* x = img + int64_t(seg) * seg_stride;
-    const float inv = inverse_total(prescale, seg);
+    const float* x = ordered + int64_t(seg) * pixels;
     const int i0 = off[r], i1 = off[r + 1];
     float s = 0.0f;
     unsigned pk = 0u;
-    for (int base = i0; base < i1; base += 64 * kRingGroups) {
-        int idx[kRingGroups];
-#pragma unroll
-        for (int g = 0; g < kRingGroups; ++g) idx[g] = order[min(base + 64 * g + lane, i1 - 1)];
-        float v[kRingGroups];
+    constexpr int kStep = 64 * kRingGroups;
+    float cur[kRingGroups];
+    auto load = [&](float (&v)[kRingGroups], int base) {
 #pragma unroll
         for (int g = 0; g < kRingGroups; ++g) {
-            const float t = x[idx[g]] * inv;
-            v[g] = base + 64 * g + lane < i1 ? t : 0.0f;
+            const int i = base + 64 * g + lane;
+            const float t = x[min(i, i1 - 1)];
+            v[g] = i < i1 ? t : 0.0f;
         }
+    };
+    if (i0 < i1) load(cur, i0);
+    for (int base = i0; base < i1; base += kStep) {
+        float nxt[kRingGroups];
+        if (base + kStep < i1) load(nxt, base + kStep);
 #pragma unroll
         for (int g = 0; g < kRingGroups; ++g) {
             if (base + 64 * g >= i1) break;
-            pk = max(pk, __float_as_uint(v[g]));
-            s = add_group_sequential(s, v[g]);
+            pk = max(pk, __float_as_uint(cur[g]));
+            s = add_group_sequential(s, cur[g]);
         }
+#pragma unroll
+        for (int g = 0; g < kRingGroups; ++g) cur[g] = nxt[g];
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) pk = max(pk, unsigned(__shfl_xor(int(pk), d)));
@@ -105,21 +110,38 @@ ring_wave_sums_kernel(const float* __restrict__ img, int64_t seg_stride, const f
     }
 }
 
-// The last ring's pixels of each segment, gathered in raster order (scaled as above) for the
-// sequential-sum engine; its peak by atomic maximum (bits of values >= 0 order as the values).
+// The ring-ordered copy of every segment, dst[g * pixels + i] = img[g][order[i]] * (1 / total),
+// written coalesced; the last ring's peak (i >= last) by atomic maximum (bits of values >= 0 order
+// as the values). Chunks of kGatherChunk positions go to the XCDs in contiguous ranges (chunk c
+// runs on XCD c % 8), so one XCD reads an annulus of rings whose source lines its L2 shares.
+constexpr int kGatherPer = 4;
+constexpr int kGatherChunk = kGatherThreads * kGatherPer;
 __global__ void __launch_bounds__(kGatherThreads)
-ring_corner_gather_kernel(const float* __restrict__ img, int64_t seg_stride, const float* __restrict__ prescale,
-                          const int* __restrict__ order, int64_t n, float* __restrict__ dst,
-                          unsigned* __restrict__ peak_slot, int nr) {
-    const int seg = blockIdx.y;
+ring_gather_kernel(const float* __restrict__ img, int64_t seg_stride, const float* __restrict__ prescale,
+                   const int* __restrict__ order, int64_t pixels, int64_t last, int chunks, int segs, int per_xcd,
+                   float* __restrict__ dst, unsigned* __restrict__ peak_slot, int nr) {
+    const int b = int(blockIdx.x);
+    const int logical = (b % 8) * per_xcd + b / 8;  // seg-major (segment, chunk)
+    if (logical >= chunks * segs) return;
+    const int seg = logical / chunks, chunk = logical % chunks;
     const float* x = img + int64_t(seg) * seg_stride;
+    float* o = dst + int64_t(seg) * pixels;
     const float inv = inverse_total(prescale, seg);
+    const int64_t base = int64_t(chunk) * kGatherChunk + threadIdx.x;
+    int idx[kGatherPer];
+#pragma unroll
+    for (int k = 0; k < kGatherPer; ++k) idx[k] = order[min(base + k * kGatherThreads, pixels - 1)];
+    float v[kGatherPer];
+#pragma unroll
+    for (int k = 0; k < kGatherPer; ++k) v[k] = x[idx[k]] * inv;
     unsigned pk = 0u;
-    for (int64_t i = int64_t(blockIdx.x) * kGatherThreads + threadIdx.x; i < n;
-         i += int64_t(gridDim.x) * kGatherThreads) {
-        const float v = x[order[i]] * inv;
-        dst[int64_t(seg) * n + i] = v;
-        pk = max(pk, __float_as_uint(v));
+#pragma unroll
+    for (int k = 0; k < kGatherPer; ++k) {
+        const int64_t i = base + k * kGatherThreads;
+        if (i < pixels) {
+            o[i] = v[k];
+            if (i >= last) pk = max(pk, __float_as_uint(v[k]));
+        }
     }
     if (peak_slot == nullptr) return;
 #pragma unroll
@@ -190,42 +212,40 @@ void prepare_ring_plan(hipStream_t s, RingPlan& p, int size, float center, int r
 }
 
 void RingScratch::reserve(const RingPlan& plan, int segs) {
-    const int64_t n = plan.corner_pixels();
-    corner.reset(sizeof(float) * size_t(std::max<int64_t>(n, 1)) * segs);
-    corner_total.reset(sizeof(float) * size_t(segs));
+    ordered.reset(sizeof(float) * size_t(plan.pixels) * segs);
+    last_total.reset(sizeof(float) * size_t(segs));
 }
 
 void ring_sums(hipStream_t s, const RingPlan& plan, RingScratch& scratch, SeqSum& seqsum, const float* img,
-               int64_t seg_stride, int segs, const float* prescale, float* sums, unsigned* peaks) {
+               int64_t seg_stride, int segs, const float* prescale, float* sums, unsigned* peaks, bool last_sum) {
     const int rmax = plan.rmax, nr = rmax + 1;
-    const int64_t corner = plan.corner_pixels();
-    ProfileScope ps("ring-sums", 4.0 * double(plan.pixels) * segs);
+    const int64_t n = plan.pixels, last = plan.corner_begin(), nlast = plan.corner_pixels();
+    scratch.reserve(plan, segs);
+    float* ordered = scratch.ordered.as<float>();
+    const unsigned small = unsigned(div_up(segs, 256));
+    {
+        ProfileScope ps("ring-gather", 12.0 * double(n) * segs);
+        if (peaks) hipLaunchKernelGGL(ring_corner_clear_kernel, dim3(small), dim3(256), 0, s, peaks + rmax, nr, segs);
+        const int chunks = int(div_up(n, kGatherChunk));
+        const int per_xcd = int(div_up(int64_t(chunks) * segs, 8));
+        hipLaunchKernelGGL(ring_gather_kernel, dim3(unsigned(8 * per_xcd)), dim3(kGatherThreads), 0, s, img,
+                           seg_stride, prescale, plan.order.as<int>(), n, last, chunks, segs, per_xcd, ordered,
+                           peaks ? peaks + rmax : nullptr, nr);
+        LDH_CHECK_LAUNCH("ring-gather");
+    }
     if (rmax > 0) {
+        ProfileScope ps("ring-sums", 4.0 * double(last) * segs);
         const int items = rmax * segs;
         const int per_xcd = int(div_up(div_up(items, kRingWaves), 8));
-        hipLaunchKernelGGL(ring_wave_sums_kernel, dim3(unsigned(8 * per_xcd)), dim3(64 * kRingWaves), 0, s, img,
-                           seg_stride, prescale, plan.order.as<int>(), plan.off.as<int>(), rmax, items, per_xcd, sums,
-                           peaks, nr);
+        hipLaunchKernelGGL(ring_wave_sums_kernel, dim3(unsigned(8 * per_xcd)), dim3(64 * kRingWaves), 0, s, ordered, n,
+                           plan.off.as<int>(), rmax, items, per_xcd, sums, peaks, nr);
         LDH_CHECK_LAUNCH("ring-sums");
     }
-    const unsigned small = unsigned(div_up(segs, 256));
-    if (corner == 0) {
-        hipLaunchKernelGGL(ring_corner_store_kernel, dim3(small), dim3(256), 0, s, nullptr, sums + rmax, nr, segs);
-        if (peaks) hipLaunchKernelGGL(ring_corner_clear_kernel, dim3(small), dim3(256), 0, s, peaks + rmax, nr, segs);
-        LDH_CHECK_LAUNCH("ring-corner");
-        return;
-    }
-    scratch.reserve(plan, segs);
-    if (peaks) hipLaunchKernelGGL(ring_corner_clear_kernel, dim3(small), dim3(256), 0, s, peaks + rmax, nr, segs);
-    const unsigned gx = unsigned(std::min<int64_t>(div_up(corner, kGatherThreads), 256));
-    hipLaunchKernelGGL(ring_corner_gather_kernel, dim3(gx, unsigned(segs)), dim3(kGatherThreads), 0, s, img,
-                       seg_stride, prescale, plan.order.as<int>() + plan.corner_begin(), corner,
-                       scratch.corner.as<float>(), peaks ? peaks + rmax : nullptr, nr);
-    LDH_CHECK_LAUNCH("ring-corner");
-    seqsum.run(s, scratch.corner.as<float>(), corner, corner, segs, scratch.corner_total.as<float>());
-    hipLaunchKernelGGL(ring_corner_store_kernel, dim3(small), dim3(256), 0, s, scratch.corner_total.as<float>(),
-                       sums + rmax, nr, segs);
-    LDH_CHECK_LAUNCH("ring-corner");
+    const bool sum_last = last_sum && nlast > 0;
+    if (sum_last) seqsum.run(s, ordered + last, nlast, n, segs, scratch.last_total.as<float>());
+    hipLaunchKernelGGL(ring_corner_store_kernel, dim3(small), dim3(256), 0, s,
+                       sum_last ? scratch.last_total.as<float>() : nullptr, sums + rmax, nr, segs);
+    LDH_CHECK_LAUNCH("ring-last");
 }
 
 }  // namespace ldh

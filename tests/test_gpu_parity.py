@@ -173,12 +173,14 @@ def test_resample_kernel(ctx, oracle, scale, support):
     assert rel_l2(got, want) < 1e-5
 
 
-@pytest.mark.parametrize("aniso", [0.0, 0.3])
-def test_shape_kernels(ctx, oracle, aniso):
+# support 40 = the maximum radius (K = 2 r + 1, as in every bank); support 20 < 40, where the
+# clamped last ring's energy can decide the radius
+@pytest.mark.parametrize("aniso,support", [(0.0, 40), (0.3, 40), (0.3, 20)])
+def test_shape_kernels(ctx, oracle, aniso, support):
     p = params(pupil_resolution=64, aperture_mode=1, blade_count=6, frame_short_side_px=400,
                max_kernel_radius_px=10.0, anisotropy_emphasis=aniso)
     psf = oracle.raw_psf(oracle.pupil_amplitude(p), None, 512)
-    base = oracle.resample_kernel(psf, 0.5, 40)
+    base = oracle.resample_kernel(psf, 0.5, support)
     got = ctx.shape_kernels(p, base)
     want = oracle.shape_kernels(p, base)
     # same base in: the same integer crop radius, and kernels equal up to the taper's cosine
