@@ -228,6 +228,8 @@ typedef struct ldh_bank_info {
     int32_t raw_psf_size;
     float wavelengths_nm[LDH_MAX_WAVELENGTHS];
     int32_t kernel_size[LDH_MAX_WAVELENGTHS];   /* cropped side per bin */
+    int32_t raw_window_px;  /* half-width h of the centred (2h+1)^2 raw-PSF block the bins were
+                               resampled from when only that block was synthesised; 0: whole R x R */
 } ldh_bank_info;
 
 /* Build or fetch the bank for `p` from the context's cache (refcounted). */

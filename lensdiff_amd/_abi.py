@@ -161,6 +161,7 @@ class BankInfo(C.Structure):
         ("raw_psf_size", C.c_int32),
         ("wavelengths_nm", C.c_float * LDH_MAX_WAVELENGTHS),
         ("kernel_size", C.c_int32 * LDH_MAX_WAVELENGTHS),
+        ("raw_window_px", C.c_int32),
     ]
 
 

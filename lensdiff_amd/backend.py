@@ -153,6 +153,7 @@ class Context:
         info = _abi.BankInfo()
         self._check(self.lib.ldh_bank_info_get(self.h, b, C.byref(info)))
         out = {"wavelengths": [], "support": info.support_radius_px, "raw_size": info.raw_psf_size,
+               "raw_window": info.raw_window_px,
                "full": [], "core": [], "structure": []}
         for i in range(info.bin_count):
             k = info.kernel_size[i]

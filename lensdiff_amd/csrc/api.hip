@@ -497,6 +497,7 @@ ldh_status ldh_bank_info_get(ldh_context* ctx, const ldh_bank* bank, ldh_bank_in
         out->support_radius_px = b.support;
         out->pupil_size = b.pupil;
         out->raw_psf_size = b.raw;
+        out->raw_window_px = b.raw_window;
         for (int i = 0; i < b.bins && i < LDH_MAX_WAVELENGTHS; ++i) {
             out->wavelengths_nm[i] = b.wavelengths[size_t(i)];
             out->kernel_size[i] = 2 * std::min(std::max(b.radius[size_t(i)], 1), b.kernel / 2) + 1;

@@ -28,6 +28,7 @@ struct DeviceBank {
     int max_radius = 0;   // r, K = 2r+1
     int kernel = 0;       // K
     int support = 4;      // LensDiffPsfBankCache::supportRadiusPx
+    int raw_window = 0;   // half-width of the synthesised raw-PSF block (0: whole R x R)
     std::vector<float> wavelengths;
     std::vector<int> radius;       // cropped radius per bin
     DeviceBuffer kernels;                   // own storage, float[3][bins][K*K]
@@ -80,11 +81,13 @@ float first_minimum_from_profile(const std::vector<float>& profile, int size);
 // Steps of the build, exposed for the parity entry points.
 // d_raw_totals (nullable): per-PSF sums of raw PSFs that are not yet normalised.
 // block_sums (nullable): FP64 sums of the output's kSeqSumBlock-texel blocks, [count][blocks].
+// window > 0: the raw PSFs are held as their centred (2 window + 1)^2 blocks
+// (PsfPlan::window_psfs), else whole (R x R).
 void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_scales, const float* d_raw_totals,
-                      int support, int count, float* out, double* block_sums = nullptr);
+                      int support, int count, float* out, double* block_sums = nullptr, int window = 0);
 // resample_kernels, then the sequential sums of the base kernels into totals[count].
 void base_kernels(hipStream_t s, BankWorkspace& w, const float* raw, int R, const float* d_scales,
-                  const float* d_raw_totals, int support, int count, float* out, float* totals);
+                  const float* d_raw_totals, int support, int count, float* out, float* totals, int window = 0);
 // Shapes `count` base kernels (count x K*K) into full/core/structure (3 x count x K*K) and
 // radius[count] (device). `max_radius` as LensDiffCpuReference.cpp:1633. base_totals (nullable):
 // per-kernel sums of a base that is not yet normalised (read as base / total).

@@ -27,24 +27,29 @@ __device__ __forceinline__ float inverse_total(const float* __restrict__ totals,
     return t > 0.0f ? 1.0f / t : 1.0f;
 }
 
-// Bilinear sample of img * norm: each texel is scaled before interpolation, as when the image
-// was normalised in place first.
-__device__ __forceinline__ float bilinear_sq(const float* __restrict__ img, int size, float x, float y, float norm) {
+// Bilinear sample of img * norm at (x, y) of a size x size image (sampleSquareBilinear,
+// LensDiffCpuReference.cpp:316-336): each texel is scaled before interpolation, as when the image
+// was normalised in place first. The texels are held from (org, org) on, `side` per row (the whole
+// image: org 0, side = size; a centred window of it otherwise).
+__device__ __forceinline__ float bilinear_sq(const float* __restrict__ img, int size, int org, int side, float x,
+                                             float y, float norm) {
     if (x < 0.0f || y < 0.0f || x > float(size - 1) || y > float(size - 1)) return 0.0f;
     const int x0 = int(floorf(x)), y0 = int(floorf(y));
     const int x1 = min(x0 + 1, size - 1), y1 = min(y0 + 1, size - 1);
     const float tx = x - float(x0), ty = y - float(y0);
-    const float v00 = img[size_t(y0) * size + x0] * norm, v10 = img[size_t(y0) * size + x1] * norm;
-    const float v01 = img[size_t(y1) * size + x0] * norm, v11 = img[size_t(y1) * size + x1] * norm;
+    const size_t r0 = size_t(y0 - org) * side, r1 = size_t(y1 - org) * side;
+    const float v00 = img[r0 + (x0 - org)] * norm, v10 = img[r0 + (x1 - org)] * norm;
+    const float v01 = img[r1 + (x0 - org)] * norm, v11 = img[r1 + (x1 - org)] * norm;
     const float a = v00 + (v10 - v00) * tx;
     const float b = v01 + (v11 - v01) * tx;
     return a + (b - a) * ty;
 }
 
 // Workgroups of kSumThreads threads over blocks of kSeqSumBlock texels (seqsum_blocks(K*K) per
-// kernel); block_sums (nullable): their FP64 sums, [lam][block].
+// kernel); block_sums (nullable): their FP64 sums, [lam][block]. The raw PSFs are R x R images
+// held whole (org 0, side R) or as their centred side x side windows from (org, org).
 __global__ void __launch_bounds__(kSumThreads)
-resample_kernel(const float* __restrict__ raw, int R, const float* __restrict__ scales,
+resample_kernel(const float* __restrict__ raw, int R, int org, int side, const float* __restrict__ scales,
                 const float* __restrict__ raw_totals, int support, float* __restrict__ out,
                 double* __restrict__ block_sums) {
     const int lam = blockIdx.y;
@@ -52,7 +57,7 @@ resample_kernel(const float* __restrict__ raw, int R, const float* __restrict__ 
     const float norm = inverse_total(raw_totals, lam);
     const int K = 2 * support + 1;
     const int kk = K * K;
-    const float* img = raw + size_t(lam) * R * R;
+    const float* img = raw + size_t(lam) * side * side;
     const float rc = (R & 1) == 0 ? float(R) * 0.5f : float(R - 1) * 0.5f;
     const float kc = float(K - 1) * 0.5f;
     const float inv = 1.0f / fmaxf(scales[lam], 0.05f);
@@ -66,14 +71,14 @@ resample_kernel(const float* __restrict__ raw, int R, const float* __restrict__ 
         const float sy = rc + (float(y) - kc) * inv;
         float v;
         if (inv <= 1.0f) {
-            v = bilinear_sq(img, R, sx, sy, norm);
+            v = bilinear_sq(img, R, org, side, sx, sy, norm);
         } else {
             float sum = 0.0f;
             for (int ty = 0; ty < taps; ++ty) {
                 const float oy = (float(ty) + 0.5f) * step - inv * 0.5f;
                 for (int tx = 0; tx < taps; ++tx) {
                     const float ox = (float(tx) + 0.5f) * step - inv * 0.5f;
-                    sum += bilinear_sq(img, R, sx + ox, sy + oy, norm);
+                    sum += bilinear_sq(img, R, org, side, sx + ox, sy + oy, norm);
                 }
             }
             v = sum / float(taps * taps);
@@ -372,20 +377,21 @@ inline dim3 blocks_grid(int64_t n, int segs) { return dim3(unsigned(seqsum_block
 }  // namespace
 
 void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_scales, const float* d_raw_totals,
-                      int support, int count, float* out, double* block_sums) {
+                      int support, int count, float* out, double* block_sums, int window) {
     const int K = 2 * support + 1;
+    const int side = window > 0 ? 2 * window + 1 : R, org = window > 0 ? R / 2 - window : 0;
     ProfileScope ps_("resample", 8.0 * double(K) * K * count);
-    hipLaunchKernelGGL(resample_kernel, blocks_grid(int64_t(K) * K, count), dim3(kSumThreads), 0, s, raw, R, d_scales,
-                       d_raw_totals, support, out, block_sums);
+    hipLaunchKernelGGL(resample_kernel, blocks_grid(int64_t(K) * K, count), dim3(kSumThreads), 0, s, raw, R, org,
+                       side, d_scales, d_raw_totals, support, out, block_sums);
     LDH_CHECK_LAUNCH("resample");
 }
 
 void base_kernels(hipStream_t s, BankWorkspace& w, const float* raw, int R, const float* d_scales,
-                  const float* d_raw_totals, int support, int count, float* out, float* totals) {
+                  const float* d_raw_totals, int support, int count, float* out, float* totals, int window) {
     const int K = 2 * support + 1;
     const int64_t kk = int64_t(K) * K;
     w.reserve_shaping(s, K, count, support);
-    resample_kernels(s, raw, R, d_scales, d_raw_totals, support, count, out, w.block_sums.as<double>());
+    resample_kernels(s, raw, R, d_scales, d_raw_totals, support, count, out, w.block_sums.as<double>(), window);
     w.seqsum.run(s, out, kk, kk, count, totals, w.block_sums.as<double>());
 }
 
@@ -657,19 +663,33 @@ std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWo
         pupil_phase(s, w.coefs.as<PhaseCoef>(), P, L, w.phase.as<float>(), true);
     }
 
-    // raw PSFs, then per-wavelength resampled base kernels
+    // raw PSFs, then per-wavelength resampled base kernels. The resample reads each raw PSF only
+    // near its centre and renormalises the kernel, so where that window is small the raw PSFs are
+    // synthesised on it alone and left unnormalised (the unit-sum scale cancels in the kernel's
+    // own normalisation); otherwise whole, with their unit-sum scale applied inside the resample.
     w.plan.prepare(s, P, R);
-    w.raw.reset(sizeof(float) * size_t(R) * R * L);
-    w.totals.reset(sizeof(float) * L);
-    // the unit-sum scale of each raw PSF is applied inside the resample
-    w.plan.raw_psfs(s, t, w.amp.as<float>(), defocus_only ? nullptr : w.phase.as<float>(), L, w.raw.as<float>(),
-                    w.totals.as<float>(), w.seqsum, w.psf, false, nullptr,
-                    defocus_only ? w.coefs.as<PhaseCoef>() : nullptr);
+    const int h = resample_window_half_width(scales, r);
+    const bool windowed = w.plan.prepare_window(s, h);
+    const float* phase = defocus_only ? nullptr : w.phase.as<float>();
+    const PhaseCoef* dcoefs = defocus_only ? w.coefs.as<PhaseCoef>() : nullptr;
     const int64_t kk = int64_t(K) * K;
     w.base.reset(sizeof(float) * kk * L);
     w.base_totals.reset(sizeof(float) * L);
-    base_kernels(s, w, w.raw.as<float>(), R, w.scales.as<float>(), w.totals.as<float>(), r, L, w.base.as<float>(),
-                 w.base_totals.as<float>());
+    bank->raw_window = windowed ? h : 0;
+    if (windowed) {
+        const int n = w.plan.window_side();
+        w.raw.reset(sizeof(float) * size_t(n) * n * L);
+        w.plan.window_psfs(s, t, w.amp.as<float>(), phase, L, w.raw.as<float>(), w.psf, nullptr, dcoefs);
+        base_kernels(s, w, w.raw.as<float>(), R, w.scales.as<float>(), nullptr, r, L, w.base.as<float>(),
+                     w.base_totals.as<float>(), h);
+    } else {
+        w.raw.reset(sizeof(float) * size_t(R) * R * L);
+        w.totals.reset(sizeof(float) * L);
+        w.plan.raw_psfs(s, t, w.amp.as<float>(), phase, L, w.raw.as<float>(), w.totals.as<float>(), w.seqsum, w.psf,
+                        false, nullptr, dcoefs);
+        base_kernels(s, w, w.raw.as<float>(), R, w.scales.as<float>(), w.totals.as<float>(), r, L, w.base.as<float>(),
+                     w.base_totals.as<float>());
+    }
 
     // shaping into full / core / structure (the base is read normalised through its totals)
     bank->kernels.reset(sizeof(float) * kk * L * 3);

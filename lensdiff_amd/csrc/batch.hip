@@ -31,6 +31,7 @@ struct Group {
     PsfPlan plan;
     DeviceBuffer d_geoms, d_amp_index, d_coefs, d_scales, d_gains, d_unit0, amp, phase, radius;
     bool defocus_only = true;  // every request's coefficient sets differ only in defocus
+    int window = 0;            // raw PSFs synthesised on their centred (2 window + 1)^2 block
     std::shared_ptr<DeviceBuffer> store, kernel_scales;  // [which][unit] unit-sum multipliers
 };
 
@@ -45,17 +46,29 @@ void upload(DeviceBuffer& d, const std::vector<T>& h, hipStream_t s) {
 void enqueue_group(hipStream_t s, const FftTables& t, BankWorkspace& w, Group& g) {
     const int P = g.P, R = g.R, U = g.units, K = g.K;
     pupil_amplitudes(s, g.d_geoms.as<PupilGeom>(), P, int(g.requests.size()), g.amp.as<float>(), true);
-    // raw PSFs (unit sums applied inside the resample), base kernels, shaping
-    if (g.defocus_only) {
-        g.plan.raw_psfs(s, t, g.amp.as<float>(), nullptr, U, w.raw.as<float>(), w.totals.as<float>(), w.seqsum, w.psf,
-                        false, nullptr, g.d_coefs.as<PhaseCoef>(), g.d_unit0.as<int>(), int(g.requests.size()));
-    } else {
+    // raw PSFs (their centred windows, or whole with unit sums applied inside the resample), base
+    // kernels, shaping
+    const bool win = g.window > 0;
+    const float* phase = nullptr;
+    const int* amp_index = nullptr;
+    const PhaseCoef* dcoefs = g.d_coefs.as<PhaseCoef>();
+    const int* unit0 = g.d_unit0.as<int>();
+    if (!g.defocus_only) {
         pupil_phase(s, g.d_coefs.as<PhaseCoef>(), P, U, g.phase.as<float>(), true);
-        g.plan.raw_psfs(s, t, g.amp.as<float>(), g.phase.as<float>(), U, w.raw.as<float>(), w.totals.as<float>(),
-                        w.seqsum, w.psf, false, g.d_amp_index.as<int>());
+        phase = g.phase.as<float>();
+        amp_index = g.d_amp_index.as<int>();
+        dcoefs = nullptr;
+        unit0 = nullptr;
     }
-    base_kernels(s, w, w.raw.as<float>(), R, g.d_scales.as<float>(), w.totals.as<float>(), g.r, U, w.base.as<float>(),
-                 w.base_totals.as<float>());
+    const int pupils = g.defocus_only ? int(g.requests.size()) : 1;
+    if (win)
+        g.plan.window_psfs(s, t, g.amp.as<float>(), phase, U, w.raw.as<float>(), w.psf, amp_index, dcoefs, unit0,
+                           pupils);
+    else
+        g.plan.raw_psfs(s, t, g.amp.as<float>(), phase, U, w.raw.as<float>(), w.totals.as<float>(), w.seqsum, w.psf,
+                        false, amp_index, dcoefs, unit0, pupils);
+    base_kernels(s, w, w.raw.as<float>(), R, g.d_scales.as<float>(), win ? nullptr : w.totals.as<float>(), g.r, U,
+                 w.base.as<float>(), w.base_totals.as<float>(), g.window);
     shape_kernels(s, w, w.base.as<float>(), w.base_totals.as<float>(), K, U, 0.0, g.r, g.store->as<float>(),
                   g.radius.as<int>(), g.kernel_scales->as<float>(), g.d_gains.as<float>());
 }
@@ -151,6 +164,8 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
             g.units += int(coefs.size());
         }
         g.plan.prepare(s, g.P, g.R);
+        const int h = resample_window_half_width(g.scales, g.r);
+        g.window = g.plan.prepare_window(s, h) ? h : 0;
         upload(g.d_geoms, g.geoms, s);
         upload(g.d_amp_index, g.amp_index, s);
         upload(g.d_coefs, g.coefs, s);
@@ -166,12 +181,13 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
         g.store = std::make_shared<DeviceBuffer>(sizeof(float) * kk * 3 * g.units);
         g.kernel_scales = std::make_shared<DeviceBuffer>(sizeof(float) * 3 * g.units);
         // the workspace grows to the batch's largest group before any launch
-        w.raw.reset(sizeof(float) * size_t(g.R) * g.R * g.units);
+        const int side = g.window > 0 ? 2 * g.window + 1 : g.R;
+        w.raw.reset(sizeof(float) * size_t(side) * side * g.units);
         w.totals.reset(sizeof(float) * g.units);
         w.base.reset(sizeof(float) * kk * g.units);
         w.base_totals.reset(sizeof(float) * g.units);
-        w.psf.reserve(g.P, g.R, g.units);
-        w.seqsum.reserve(int64_t(g.R) * g.R, g.units);
+        w.psf.reserve(g.P, side, g.units);
+        if (g.window == 0) w.seqsum.reserve(int64_t(g.R) * g.R, g.units);
         w.reserve_shaping(s, g.K, g.units, g.r);
         tm.psfs += g.units;
     }
@@ -235,6 +251,7 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
             bank->raw = g.R;
             bank->max_radius = g.r;
             bank->kernel = g.K;
+            bank->raw_window = g.window;
             bank->wavelengths = wls[size_t(i)];
             bank->radius.assign(radius.begin() + g.unit0[k], radius.begin() + g.unit0[k] + bank->bins);
             bank->support = 4;

@@ -162,35 +162,36 @@ __device__ __forceinline__ void line_transform(float2* lds, const float2* __rest
 // allocation must allow (1: no constraint). The autotuner picks (B, W) for long lines.
 template <int M, int B, int W>
 __global__ void __launch_bounds__((LinePlan<M, B>::NT), W)
-psf_rows_kernel(const float2* __restrict__ field, int P, int R, int off, const float2* __restrict__ chirp,
+psf_rows_kernel(const float2* __restrict__ field, int P, int nout, int off, const float2* __restrict__ chirp,
                 const float2* __restrict__ bhat, int bluestein, float2* __restrict__ G, const float2* __restrict__ tw) {
     extern __shared__ float2 lds[];
     const int y = blockIdx.x, lam = blockIdx.y, tid = threadIdx.x;
     const RowInput in{field + (size_t(lam) * P + y) * P, P, off};
     const float inv_m = 1.0f / float(M);
     const float2 post = bluestein ? make_float2(chirp[y].x * inv_m, chirp[y].y * inv_m) : make_float2(1.0f, 0.0f);
-    const RowOutput out{G + (size_t(lam) * P + y) * R, R, post, bluestein};
+    const RowOutput out{G + (size_t(lam) * P + y) * nout, nout, post, bluestein};
     line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out, off + P);
 }
 
-// Also writes the FP64 sum of each output row to row_sums[lam * R + y] (nullable): the block
+// nout output rows of nout values each, line ky landing in row (ky + shift) mod nout with its
+// values shifted alike (the full form's fftShiftSquare: shift = R - R/2; a window: 0).
+// Also writes the FP64 sum of each output row to row_sums[lam * nout + y] (nullable): the block
 // sums of a sequential sum whose blocks are the rows.
 template <int M, int B, int W>
 __global__ void __launch_bounds__((LinePlan<M, B>::NT), W)
-psf_cols_kernel(const float2* __restrict__ GT, int P, int R, int off, const float2* __restrict__ chirp,
+psf_cols_kernel(const float2* __restrict__ GT, int P, int nout, int shift, int off, const float2* __restrict__ chirp,
                 const float2* __restrict__ bhat, int bluestein, float* __restrict__ IT,
                 const float2* __restrict__ tw, double* __restrict__ row_sums) {
     extern __shared__ float2 lds[];
     constexpr int kWaves = LinePlan<M, B>::NT / 64;
     __shared__ double part[kWaves];
     const int ky = blockIdx.x, lam = blockIdx.y, tid = threadIdx.x;
-    const int shift = R - R / 2;
     int y = ky + shift;
-    if (y >= R) y -= R;
+    if (y >= nout) y -= nout;
     double acc = 0.0;
-    const ColInput in{GT + (size_t(lam) * R + ky) * P, P, off};
+    const ColInput in{GT + (size_t(lam) * nout + ky) * P, P, off};
     const float inv_m = 1.0f / float(M);
-    const ColOutput out{IT + (size_t(lam) * R + y) * R, bluestein ? inv_m * inv_m : 1.0f, R, shift, &acc};
+    const ColOutput out{IT + (size_t(lam) * nout + y) * nout, bluestein ? inv_m * inv_m : 1.0f, nout, shift, &acc};
     line_transform<M, B>(lds, tw, bhat, bluestein != 0, tid, in, out, off + P);
     if (row_sums == nullptr) return;
     acc = wave_total(acc);
@@ -200,7 +201,7 @@ psf_cols_kernel(const float2* __restrict__ GT, int P, int R, int off, const floa
         double t = 0.0;
 #pragma unroll
         for (int i = 0; i < kWaves; ++i) t += part[i];
-        row_sums[size_t(lam) * R + y] = t;
+        row_sums[size_t(lam) * nout + y] = t;
     }
 }
 
@@ -321,18 +322,18 @@ constexpr int kPsfVariants = 5;
 template <int M> constexpr int psf_variants() { return M >= 2048 ? kPsfVariants : 1; }
 
 template <int M, int B, int W>
-void launch_rows_v(hipStream_t s, const float2* field, int P, int R, int off, const float2* chirp, const float2* bhat,
-                   int bluestein, float2* G, const float2* tw, int L) {
+void launch_rows_v(hipStream_t s, const float2* field, int P, int nout, int off, const float2* chirp,
+                   const float2* bhat, int bluestein, float2* G, const float2* tw, int L) {
     allow_large_lds(reinterpret_cast<const void*>(&psf_rows_kernel<M, B, W>), (M + M / 32 + 1) * 8);
     hipLaunchKernelGGL((psf_rows_kernel<M, B, W>), dim3(P, L), dim3(LinePlan<M, B>::NT), (M + M / 32 + 1) * 8, s, field,
-                       P, R, off, chirp, bhat, bluestein, G, tw);
+                       P, nout, off, chirp, bhat, bluestein, G, tw);
 }
 template <int M, int B, int W>
-void launch_cols_v(hipStream_t s, const float2* GT, int P, int R, int off, const float2* chirp, const float2* bhat,
-                   int bluestein, float* IT, const float2* tw, double* row_sums, int L) {
+void launch_cols_v(hipStream_t s, const float2* GT, int P, int nout, int shift, int off, const float2* chirp,
+                   const float2* bhat, int bluestein, float* IT, const float2* tw, double* row_sums, int L) {
     allow_large_lds(reinterpret_cast<const void*>(&psf_cols_kernel<M, B, W>), (M + M / 32 + 1) * 8);
-    hipLaunchKernelGGL((psf_cols_kernel<M, B, W>), dim3(R, L), dim3(LinePlan<M, B>::NT), (M + M / 32 + 1) * 8, s, GT,
-                       P, R, off, chirp, bhat, bluestein, IT, tw, row_sums);
+    hipLaunchKernelGGL((psf_cols_kernel<M, B, W>), dim3(nout, L), dim3(LinePlan<M, B>::NT), (M + M / 32 + 1) * 8, s,
+                       GT, P, nout, shift, off, chirp, bhat, bluestein, IT, tw, row_sums);
 }
 
 std::string tune_key(const char* kind, int M, int L) {
@@ -355,17 +356,21 @@ bool launch_rows(int m, hipStream_t s, const float2* field, int P, int R, int of
     return true;
 }
 template <int M>
-bool launch_cols(int m, hipStream_t s, const float2* GT, int P, int R, int off, const float2* chirp,
+bool launch_cols(int m, hipStream_t s, const float2* GT, int P, int nout, int shift, int off, const float2* chirp,
                  const float2* bhat, int bluestein, float* IT, const float2* tw, double* row_sums, int L) {
     if (m != M) return false;
     Autotuner::instance().run(tune_key("psf-cols", M, L), psf_variants<M>(), s, [&](int v) {
         if constexpr (psf_variants<M>() > 1) {
-            if (v == 1) return launch_cols_v<M, 16, 3>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
-            if (v == 2) return launch_cols_v<M, 16, 4>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
-            if (v == 3) return launch_cols_v<M, 8, 8>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
-            if (v == 4) return launch_cols_v<M, 8, 3>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
+            if (v == 1)
+                return launch_cols_v<M, 16, 3>(s, GT, P, nout, shift, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
+            if (v == 2)
+                return launch_cols_v<M, 16, 4>(s, GT, P, nout, shift, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
+            if (v == 3)
+                return launch_cols_v<M, 8, 8>(s, GT, P, nout, shift, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
+            if (v == 4)
+                return launch_cols_v<M, 8, 3>(s, GT, P, nout, shift, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
         }
-        launch_cols_v<M, 16, 1>(s, GT, P, R, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
+        launch_cols_v<M, 16, 1>(s, GT, P, nout, shift, off, chirp, bhat, bluestein, IT, tw, row_sums, L);
     });
     return true;
 }
@@ -392,6 +397,8 @@ void PsfPlan::prepare(hipStream_t s, int pupil, int raw) {
     if (raw > kMaxRawPsf) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
     pupil_ = pupil;
     raw_ = raw;
+    h_ = -1;  // a window belongs to its (P, R)
+    mw_ = 0;
     bluestein_ = !is_pow2(raw);
     // Bluestein: input at 0 (see above), lags -(P-1) .. R-1 must not wrap
     m_ = bluestein_ ? psf_length_at_least(raw + std::min(pupil, raw) - 1) : raw;
@@ -440,6 +447,24 @@ void pupil_phase(hipStream_t s, const PhaseCoef* d_coefs, int n, int count, floa
     LDH_CHECK_LAUNCH("pupil-phase");
 }
 
+void PsfPlan::fields(hipStream_t s, const float* amp, const float* phase, int count, const float2* chirp,
+                     int bluestein, PsfScratch& scratch, const int* amp_index, const PhaseCoef* defocus_coefs,
+                     const int* unit0, int pupils) {
+    const int P = pupil_, L = count;
+    // amplitude (shared by the wavelengths of a pupil) and phase in, complex field out
+    if (defocus_coefs != nullptr) {
+        ProfileScope ps("psf-field", (4.0 * pupils + 8.0 * L) * P * P);
+        hipLaunchKernelGGL(pupil_field_defocus_kernel, dim3(unsigned(div_up(int64_t(P) * P, 256)), unsigned(pupils)),
+                           dim3(256), 0, s, amp, defocus_coefs, unit0, L, P, chirp, bluestein,
+                           scratch.field.as<float2>());
+    } else {
+        ProfileScope ps("psf-field", 12.0 * P * P * L);
+        hipLaunchKernelGGL(pupil_field_kernel, dim3(unsigned(div_up(int64_t(P) * P, 256)), unsigned(L)), dim3(256), 0,
+                           s, amp, amp_index, phase, P, chirp, bluestein, scratch.field.as<float2>());
+    }
+    LDH_CHECK_LAUNCH("psf-field");
+}
+
 void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, const float* phase, int count,
                        float* out, float* totals, SeqSum& seqsum, PsfScratch& scratch, bool normalize,
                        const int* amp_index, const PhaseCoef* defocus_coefs, const int* unit0, int pupils) {
@@ -455,19 +480,7 @@ void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, cons
     // their sums formed by the column pass
     const bool row_blocks = R >= kSeqSumMinBlock && R <= kSeqSumBlock;
     double* row_sums = row_blocks ? scratch.row_sums.as<double>() : nullptr;
-    {
-        // amplitude (shared by the wavelengths of a pupil) and phase in, complex field out
-        if (defocus_coefs != nullptr) {
-            ProfileScope ps("psf-field", (4.0 * pupils + 8.0 * L) * P * P);
-            hipLaunchKernelGGL(pupil_field_defocus_kernel, dim3(unsigned(div_up(int64_t(P) * P, 256)), unsigned(pupils)),
-                               dim3(256), 0, s, amp, defocus_coefs, unit0, L, P, chirp, blu, scratch.field.as<float2>());
-        } else {
-            ProfileScope ps("psf-field", 12.0 * P * P * L);
-            hipLaunchKernelGGL(pupil_field_kernel, dim3(unsigned(div_up(int64_t(P) * P, 256)), unsigned(L)), dim3(256), 0,
-                               s, amp, amp_index, phase, P, chirp, blu, scratch.field.as<float2>());
-        }
-        LDH_CHECK_LAUNCH("psf-field");
-    }
+    fields(s, amp, phase, L, chirp, blu, scratch, amp_index, defocus_coefs, unit0, pupils);
     {
         // algorithmic bytes: the field in, row transforms out
         ProfileScope ps("psf-rows", (8.0 * P * P + 8.0 * P * R) * L);
@@ -477,15 +490,16 @@ void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, cons
         LDH_CHECK_LAUNCH("psf-rows");
     }
     {
-    ProfileScope ps_t("psf-transpose", 16.0 * P * R * L);
-    hipLaunchKernelGGL(transpose_c64_kernel, dim3(div_up(R, 32), div_up(P, 32), L), dim3(256), 0, s,
-                       g_.as<float2>(), gt_.as<float2>(), P, R);
-    LDH_CHECK_LAUNCH("psf-transpose");
+        ProfileScope ps_t("psf-transpose", 16.0 * P * R * L);
+        hipLaunchKernelGGL(transpose_c64_kernel, dim3(div_up(R, 32), div_up(P, 32), L), dim3(256), 0, s,
+                           g_.as<float2>(), gt_.as<float2>(), P, R);
+        LDH_CHECK_LAUNCH("psf-transpose");
     }
     {
         // algorithmic bytes: the P pass-1 values of each output row in, |F|^2 out
         ProfileScope ps("psf-cols", (8.0 * P * R + 4.0 * R * R) * L);
-#define LDH_PSF_COLS(MM) launch_cols<MM>(m_, s, gt_.as<float2>(), P, R, offset_, chirp, bhat, blu, out, tw, row_sums, L)
+#define LDH_PSF_COLS(MM) \
+    launch_cols<MM>(m_, s, gt_.as<float2>(), P, R, R - R / 2, offset_, chirp, bhat, blu, out, tw, row_sums, L)
         if (!(LDH_PSF_LENGTHS(LDH_PSF_COLS))) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
 #undef LDH_PSF_COLS
         LDH_CHECK_LAUNCH("psf-cols");
@@ -496,6 +510,94 @@ void PsfPlan::raw_psfs(hipStream_t s, const FftTables& t, const float* amp, cons
     g.stride = int64_t(R) * R;
     seqsum.run(s, g, int64_t(R) * R, L, totals, row_sums, row_blocks ? R : kSeqSumBlock);
     if (normalize) seqsum.scale(s, out, int64_t(R) * R, int64_t(R) * R, L, totals);
+}
+
+int resample_window_half_width(const std::vector<float>& scales, int support) {
+    const float kc = float(support);  // (K - 1) / 2
+    int h = 0;
+    for (float sc : scales) {
+        // the resample's own float arithmetic (resample_kernel): sample positions reach
+        // centre +- kc * inv, the box filter half a footprint further, bilinear one texel more
+        const float inv = 1.0f / std::max(sc, 0.05f);
+        const float extent = kc * inv + (inv > 1.0f ? inv * 0.5f : 0.0f);
+        h = std::max(h, int(std::ceil(extent)) + 2);
+    }
+    return h;
+}
+
+bool PsfPlan::prepare_window(hipStream_t s, int h) {
+    const int P = pupil_, R = raw_;
+    const int n = 2 * h + 1;
+    // the window must stay inside the PSF (R is even: centre R/2) and its lines must be cheaper
+    // than the full form's P + R lines of length m_
+    if (h < 1 || R / 2 - h < 1 || R / 2 + h > R - 2) return false;
+    const int mw = psf_length_at_least(P + n - 1);
+    if (mw == 0 || double(P + n) * mw >= 0.8 * double(P + R) * m_) return false;
+    if (h == h_) return true;
+    // chirp-z onto k = k0 + m, m in [0, n), k0 = -h, with W = exp(-2 i pi_f / R):
+    //   X[k] = W^(k^2/2) sum_j (x_j W^(j^2/2)) W^(-(k-j)^2/2)
+    // pre-chirp W^(j^2/2) = exp(-i pi_f j^2 / R) for j < P; filter g[d] = exp(+i pi_f (k0+d)^2 / R)
+    // for lags d in [-(P-1), n-1], laid out circularly over mw >= P + n - 1 points and transformed
+    // in double; the post-chirp W^(k^2/2) has unit modulus and |F|^2 drops it.
+    const double pi_f = double(kPiF);
+    std::vector<float2> chirp(static_cast<size_t>(P));
+    for (int j = 0; j < P; ++j) {
+        const double ph = -pi_f * double(j) * double(j) / double(R);
+        chirp[size_t(j)] = make_float2(float(std::cos(ph)), float(std::sin(ph)));
+    }
+    std::vector<double> br(static_cast<size_t>(mw), 0.0), bi(static_cast<size_t>(mw), 0.0);
+    for (int d = -(P - 1); d <= n - 1; ++d) {
+        const double k = double(d - h);
+        const double ph = pi_f * k * k / double(R);
+        const size_t at = size_t(d >= 0 ? d : mw + d);
+        // float, as the full form's table entries, before the double transform
+        br[at] = float(std::cos(ph));
+        bi[at] = float(std::sin(ph));
+    }
+    host_dft(br, bi);
+    std::vector<float2> bhat(static_cast<size_t>(mw));
+    for (int k = 0; k < mw; ++k) bhat[size_t(k)] = make_float2(float(br[size_t(k)]), float(bi[size_t(k)]));
+    wchirp_.reset(sizeof(float2) * P);
+    wbhat_.reset(sizeof(float2) * mw);
+    LDH_HIP(hipMemcpyAsync(wchirp_.get(), chirp.data(), sizeof(float2) * P, hipMemcpyHostToDevice, s), "psf-window");
+    LDH_HIP(hipMemcpyAsync(wbhat_.get(), bhat.data(), sizeof(float2) * mw, hipMemcpyHostToDevice, s), "psf-window");
+    LDH_HIP(hipStreamSynchronize(s), "psf-window");
+    h_ = h;
+    mw_ = mw;
+    return true;
+}
+
+void PsfPlan::window_psfs(hipStream_t s, const FftTables& t, const float* amp, const float* phase, int count,
+                          float* out, PsfScratch& scratch, const int* amp_index, const PhaseCoef* defocus_coefs,
+                          const int* unit0, int pupils) {
+    if (h_ < 1) fail(LDH_ERR_DEVICE, "psf-window-not-prepared");
+    const int P = pupil_, L = count, n = 2 * h_ + 1;
+    scratch.reserve(P, n, L);
+    const float2* tw = t.twiddle.as<float2>();
+    const float2* chirp = wchirp_.as<float2>();
+    const float2* bhat = wbhat_.as<float2>();
+    fields(s, amp, phase, L, chirp, 1, scratch, amp_index, defocus_coefs, unit0, pupils);
+    {
+        ProfileScope ps("psf-rows", (8.0 * P * P + 8.0 * P * n) * L);
+#define LDH_PSF_ROWS(MM) launch_rows<MM>(mw_, s, scratch.field.as<float2>(), P, n, 0, chirp, bhat, 1, scratch.g.as<float2>(), tw, L)
+        if (!(LDH_PSF_LENGTHS(LDH_PSF_ROWS))) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
+#undef LDH_PSF_ROWS
+        LDH_CHECK_LAUNCH("psf-rows");
+    }
+    {
+        ProfileScope ps_t("psf-transpose", 16.0 * P * n * L);
+        hipLaunchKernelGGL(transpose_c64_kernel, dim3(div_up(n, 32), div_up(P, 32), L), dim3(256), 0, s,
+                           scratch.g.as<float2>(), scratch.gt.as<float2>(), P, n);
+        LDH_CHECK_LAUNCH("psf-transpose");
+    }
+    {
+        ProfileScope ps("psf-cols", (8.0 * P * n + 4.0 * n * n) * L);
+#define LDH_PSF_COLS(MM) \
+    launch_cols<MM>(mw_, s, scratch.gt.as<float2>(), P, n, 0, 0, chirp, bhat, 1, out, tw, nullptr, L)
+        if (!(LDH_PSF_LENGTHS(LDH_PSF_COLS))) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
+#undef LDH_PSF_COLS
+        LDH_CHECK_LAUNCH("psf-cols");
+    }
 }
 
 }  // namespace ldh

@@ -212,6 +212,8 @@ BANK_CASES = {
     "tristimulus-phase": dict(spectral_mode=1, chromatic_focus=0.3, chromatic_spread=0.1),
     "spectral5-star": dict(spectral_mode=2, aperture_mode=2, anisotropy_emphasis=0.2),
     "spectral9": dict(spectral_mode=3),
+    # a diffraction scale this small samples most of the raw PSF: it is synthesised whole
+    "whole-raw-tristimulus": dict(spectral_mode=1, diffraction_scale_px=0.3),
 }
 
 
@@ -230,7 +232,10 @@ def test_bank(ctx, oracle, case):
     # end to end the PSFs differ from the oracle's in their last bits; the radii are expected
     # equal, and any flip is reported
     flips = sum(g.shape != w.shape for g, w in zip(got["full"], want["full"]))
-    print("%s: %d of %d crop radii differ" % (case, flips, len(want["full"])))
+    print("%s: raw window %d, %d of %d crop radii differ" % (case, got["raw_window"], flips, len(want["full"])))
+    # the bins resample a centred window of the raw PSF, synthesised alone, unless it reaches
+    # the PSF's edge
+    assert (got["raw_window"] == 0) == case.startswith("whole-raw"), got["raw_window"]
     assert flips == 0 and got["support"] == want["support"]
     for name in ("full", "core", "structure"):
         for g, w in zip(got[name], want[name]):
