@@ -233,6 +233,7 @@ ldh_status ldh_context_create(int32_t device, uint64_t budget, ldh_context** out
     ctx->budget = budget ? budget : (uint64_t(64) << 30);
     ldh_status st = guarded(ctx, [&] {
         LDH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream-create");
+        ldh::pool_register_stream(ctx->device, ctx->stream);
         ctx->prof.stream = ctx->stream;
         ctx->fft.init(ctx->stream);
     });
@@ -255,8 +256,13 @@ ldh_status ldh_context_destroy(ldh_context* ctx) {
         ctx->pinned = nullptr;
     }
     hipStream_t s = ctx->stream;
-    delete ctx;  // device buffers free here
-    if (s) (void)hipStreamDestroy(s);
+    const int device = ctx->device;
+    delete ctx;  // device buffers go back to the pool here
+    if (s) {
+        ldh::pool_unregister_stream(device, s);
+        (void)hipStreamDestroy(s);
+    }
+    ldh::pool_trim(device);
     return LDH_OK;
 }
 

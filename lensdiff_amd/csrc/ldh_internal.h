@@ -66,7 +66,21 @@ inline int pow2_at_least(int v) {
 inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ---- device memory ---------------------------------------------------------------------------
-// A device allocation owned by RAII. Allocation failures become LDH_ERR_OUT_OF_MEMORY.
+// Pooled device allocations (pool.hip). pool_alloc returns a block of at least `bytes` on the
+// current device (its size and device in granted / device), or null when the device is full;
+// pool_release hands it back, reusable once the work queued before the release on every
+// registered stream of that device has completed. Streams that run library work are registered
+// while they live (unregister after synchronising them, before destroying them).
+void* pool_alloc(size_t bytes, size_t* granted, int* device);
+void pool_release(void* p, size_t granted, int device);
+void pool_register_stream(int device, hipStream_t s);
+void pool_unregister_stream(int device, hipStream_t s);
+// Returns the device's cached blocks to the driver (after a device synchronisation).
+void pool_trim(int device);
+size_t pool_cached_bytes(int device);
+
+// A device allocation owned by RAII, from the pool. Allocation failures become
+// LDH_ERR_OUT_OF_MEMORY. bytes() is the granted size (at least the requested one).
 class DeviceBuffer {
 public:
     DeviceBuffer() = default;
@@ -74,24 +88,34 @@ public:
     ~DeviceBuffer() { release(); }
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) { o.ptr_ = nullptr; o.bytes_ = 0; }
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_), dev_(o.dev_) {
+        o.ptr_ = nullptr;
+        o.bytes_ = 0;
+    }
     DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
-        if (this != &o) { release(); ptr_ = o.ptr_; bytes_ = o.bytes_; o.ptr_ = nullptr; o.bytes_ = 0; }
+        if (this != &o) {
+            release();
+            ptr_ = o.ptr_;
+            bytes_ = o.bytes_;
+            dev_ = o.dev_;
+            o.ptr_ = nullptr;
+            o.bytes_ = 0;
+        }
         return *this;
     }
     void reset(size_t bytes) {
         if (bytes <= bytes_ && ptr_) return;
         release();
         if (bytes == 0) return;
-        if (hipMalloc(&ptr_, bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            ptr_ = nullptr;
-            fail(LDH_ERR_OUT_OF_MEMORY, "device-allocation-failed");
-        }
-        bytes_ = bytes;
+        size_t granted = 0;
+        int dev = 0;
+        ptr_ = pool_alloc(bytes, &granted, &dev);
+        if (ptr_ == nullptr) fail(LDH_ERR_OUT_OF_MEMORY, "device-allocation-failed");
+        bytes_ = granted;
+        dev_ = dev;
     }
     void release() {
-        if (ptr_) (void)hipFree(ptr_);
+        if (ptr_) pool_release(ptr_, bytes_, dev_);
         ptr_ = nullptr;
         bytes_ = 0;
     }
@@ -102,6 +126,7 @@ public:
 private:
     void* ptr_ = nullptr;
     size_t bytes_ = 0;
+    int dev_ = 0;
 };
 
 // ---- timing of kernel families with HIP events (on the launch stream) -----------------------

@@ -129,6 +129,7 @@ Server::Server(const FftTables& fft, int device, int slots, uint64_t budget)
         auto s = std::make_unique<Slot>();
         s->index = i;
         LDH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "server-stream");
+        pool_register_stream(device, s->stream);
         s->prof.stream = s->stream;
         slots_.push_back(std::move(s));
     }
@@ -148,8 +149,12 @@ Server::~Server() {
         if (s->thread.joinable()) s->thread.join();
         (void)hipStreamSynchronize(s->stream);
     }
+    cache_.clear();
     for (auto& s : slots_) {
         s->fw = FrameWork();
+        s->bw = BankWorkspace();
+        s->tmp.release();
+        pool_unregister_stream(device_, s->stream);
         (void)hipStreamDestroy(s->stream);
     }
 }
