@@ -71,6 +71,13 @@ enum {
     LDH_CONV_PER_BIN = 0,      /* one convolution per bin, then the per-pixel spectral map */
     LDH_CONV_PRECOMBINED = 1   /* bins pre-combined into natural/style RGB kernels */
 };
+/* Raw PSFs of a bank build. AUTO: where the resample reads only a centred block of each raw PSF
+ * (makeResampledKernel, LensDiffCpuReference.cpp:1060-1086) and that block is cheaper, only the
+ * block is synthesised and its unit-sum scale, which the kernel's own normalisation divides out,
+ * is skipped; the kernels then follow the reference's to the size of its FP32 sum rounding
+ * (up to about 5e-4 relative L2 at R = 4092). WHOLE: every raw PSF is synthesised R x R and
+ * normalised by its sequential FP32 sum, as BuildLensDiffShiftedRawPsf (:831-859) does. */
+enum { LDH_RAW_PSF_AUTO = 0, LDH_RAW_PSF_WHOLE = 1 };
 
 /* ---- image views (LensDiffImageRect / LensDiffImageView, LensDiffTypes.h:93-111) ---- */
 typedef struct ldh_rect {
@@ -136,10 +143,8 @@ typedef struct ldh_params {
     /* Zernike terms the reference basis lacks (LensDiffPhase.cpp:182-197), in waves:
      * Noll 2 (tilt x), 3 (tilt y), 18, 19, 20, 21. */
     double zernike_extra[6];
-    /* Reserved, must be 0. The convolution grid is the smallest 2^a*3^b (b <= 2) size
-     * >= (W+k-1) x (H+k-1); the reference pads to powers of two
-     * (LensDiffCpuReference.cpp:1168-1169), the cropped result is the same convolution. */
-    int32_t fft_tile;
+    /* LDH_RAW_PSF_*: how bank builds form the raw PSFs they resample. */
+    int32_t raw_psf_form;
     /* 1 = store cached kernel spectra as FP16 (FP32 accumulation). */
     int32_t spectra_fp16;
 } ldh_params;
@@ -179,6 +184,14 @@ ldh_status ldh_pupil_phase(ldh_context* ctx, const ldh_params* p, float* out);
  * `phase_waves` may be NULL (flat phase). */
 ldh_status ldh_raw_psf(ldh_context* ctx, const float* amplitude, const float* phase_waves,
                        int32_t pupil_size, int32_t raw_size, float* out);
+/* The centred (2h+1) x (2h+1) block of the raw PSF alone, as bank builds synthesise it when the
+ * resample reads no more (no counterpart in the reference, whose bank path computes the whole
+ * R x R PSF, BuildLensDiffShiftedRawPsf, and samples it near its centre, :1060-1086): element
+ * (i, j) is raw pixel (R/2 - h + j, R/2 - h + i), |F|^2 without the unit-sum normalisation (any
+ * common factor). LDH_ERR_UNSUPPORTED ("raw-psf-window-not-applicable") where the block would
+ * reach the PSF's edge or would not be cheaper than the whole PSF. */
+ldh_status ldh_raw_psf_window(ldh_context* ctx, const float* amplitude, const float* phase_waves,
+                              int32_t pupil_size, int32_t raw_size, int32_t half_width, float* out);
 /* EstimateLensDiffFirstMinimumRadius (LensDiffCpuReference.cpp:897-955) of a raw PSF. */
 ldh_status ldh_first_minimum_radius(ldh_context* ctx, const float* shifted_raw_psf,
                                     int32_t raw_size, float* out_radius);

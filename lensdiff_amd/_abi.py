@@ -36,6 +36,7 @@ TRANSFER_LINEAR, TRANSFER_DAVINCI_INTERMEDIATE = 0, 1
 PHASE_OFF, PHASE_ENABLED = 0, 1
 KERNEL_FULL, KERNEL_CORE, KERNEL_STRUCTURE = range(3)
 CONV_PER_BIN, CONV_PRECOMBINED = 0, 1
+RAW_PSF_AUTO, RAW_PSF_WHOLE = 0, 1
 
 
 class Rect(C.Structure):
@@ -98,7 +99,7 @@ _PARAM_FIELDS = [
     ("wavelength_count", _I), ("conv_mode", _I),
     ("wavelengths_nm", C.c_float * LDH_MAX_WAVELENGTHS),
     ("zernike_extra", _D * 6),
-    ("fft_tile", _I), ("spectra_fp16", _I),
+    ("raw_psf_form", _I), ("spectra_fp16", _I),
 ]
 
 

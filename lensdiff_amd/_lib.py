@@ -28,6 +28,7 @@ SIGNATURES = {
     "ldh_pupil_amplitude": (C.c_int32, [_CTX, _P, _F]),
     "ldh_pupil_phase": (C.c_int32, [_CTX, _P, _F]),
     "ldh_raw_psf": (C.c_int32, [_CTX, _F, _F, C.c_int32, C.c_int32, _F]),
+    "ldh_raw_psf_window": (C.c_int32, [_CTX, _F, _F, C.c_int32, C.c_int32, C.c_int32, _F]),
     "ldh_first_minimum_radius": (C.c_int32, [_CTX, _F, C.c_int32, _F]),
     "ldh_resample_kernel": (C.c_int32, [_CTX, _F, C.c_int32, C.c_float, C.c_int32, _F]),
     "ldh_shape_kernels": (C.c_int32, [_CTX, _P, _F, C.c_int32, _F, _F, _F, C.POINTER(C.c_int32)]),

@@ -84,6 +84,17 @@ class Context:
                                          amp.shape[0], raw_size, _fptr(out)))
         return out
 
+    def raw_psf_window(self, amp, phase, raw_size, half_width) -> np.ndarray:
+        """Centred (2h+1)^2 block of the raw PSF synthesised alone (unnormalised)."""
+        amp = np.ascontiguousarray(amp, np.float32)
+        p = amp.shape[0]
+        n = 2 * half_width + 1
+        out = np.zeros((n, n), np.float32)
+        ph = None if phase is None else np.ascontiguousarray(phase, np.float32)
+        self._check(self.lib.ldh_raw_psf_window(self.h, _fptr(amp), None if ph is None else _fptr(ph), p, raw_size,
+                                                half_width, _fptr(out)))
+        return out
+
     def first_minimum_radius(self, psf) -> float:
         psf = np.ascontiguousarray(psf, np.float32)
         r = C.c_float()

@@ -338,6 +338,36 @@ ldh_status ldh_raw_psf(ldh_context* ctx, const float* amp, const float* phase, i
     });
 }
 
+ldh_status ldh_raw_psf_window(ldh_context* ctx, const float* amp, const float* phase, int32_t P, int32_t R,
+                              int32_t h, float* out) {
+    return guarded(ctx, [&] {
+        require(amp && out && P > 0 && R >= P && R >= 16 && h > 0, LDH_ERR_INVALID_ARGUMENT, "invalid-psf-arguments");
+        require(R <= ldh::kMaxRawPsf, LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
+        ldh::DeviceBuffer damp(sizeof(float) * P * P), dph(sizeof(float) * P * P);
+        auto transposed = [P](const float* m) {
+            std::vector<float> t(size_t(P) * P);
+            for (int y = 0; y < P; ++y)
+                for (int x = 0; x < P; ++x) t[size_t(x) * P + y] = m[size_t(y) * P + x];
+            return t;
+        };
+        const std::vector<float> amp_t = transposed(amp);
+        const std::vector<float> phase_t = phase ? transposed(phase) : std::vector<float>();
+        h2d(ctx, damp.get(), amp_t.data(), sizeof(float) * P * P);
+        if (phase) {
+            h2d(ctx, dph.get(), phase_t.data(), sizeof(float) * P * P);
+        } else {
+            LDH_HIP(hipMemsetAsync(dph.get(), 0, sizeof(float) * P * P, ctx->stream), "raw-psf-window");
+        }
+        ldh::PsfPlan& plan = ctx->bw.plan;
+        plan.prepare(ctx->stream, P, R);
+        require(plan.prepare_window(ctx->stream, h), LDH_ERR_UNSUPPORTED, "raw-psf-window-not-applicable");
+        const size_t n = size_t(2 * h + 1);
+        ldh::DeviceBuffer dout(sizeof(float) * n * n);
+        plan.window_psfs(ctx->stream, ctx->fft, damp.as<float>(), dph.as<float>(), 1, dout.as<float>(), ctx->bw.psf);
+        d2h(ctx, out, dout.get(), sizeof(float) * n * n);
+    });
+}
+
 ldh_status ldh_first_minimum_radius(ldh_context* ctx, const float* psf, int32_t R, float* out) {
     return guarded(ctx, [&] {
         require(psf && out && R >= 4, LDH_ERR_INVALID_ARGUMENT, "invalid-argument");

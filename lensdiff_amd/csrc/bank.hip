@@ -396,9 +396,15 @@ void base_kernels(hipStream_t s, BankWorkspace& w, const float* raw, int R, cons
 }
 
 const RingPlan& BankWorkspace::ring_plan(hipStream_t s, int size, float center, int rmax, bool ceil) {
-    for (auto& p : ring_plans)
-        if (p->matches(size, center, rmax, ceil)) return *p;
-    constexpr size_t kKeep = 16;  // two per kernel size; a batch holds at most a few sizes
+    // least recently used last out: a batch's set-up touches every plan its capture will use
+    for (size_t i = 0; i < ring_plans.size(); ++i)
+        if (ring_plans[i]->matches(size, center, rmax, ceil)) {
+            std::unique_ptr<RingPlan> hit = std::move(ring_plans[i]);
+            ring_plans.erase(ring_plans.begin() + std::ptrdiff_t(i));
+            ring_plans.push_back(std::move(hit));
+            return *ring_plans.back();
+        }
+    constexpr size_t kKeep = 32;  // two per kernel size
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     LDH_HIP(hipStreamIsCapturing(s, &cap), "ring-plan");
     if (cap != hipStreamCaptureStatusNone) fail(LDH_ERR_DEVICE, "ring-plan-built-during-capture");
@@ -669,7 +675,7 @@ std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWo
     // own normalisation); otherwise whole, with their unit-sum scale applied inside the resample.
     w.plan.prepare(s, P, R);
     const int h = resample_window_half_width(scales, r);
-    const bool windowed = w.plan.prepare_window(s, h);
+    const bool windowed = p.raw_psf_form == LDH_RAW_PSF_AUTO && w.plan.prepare_window(s, h);
     const float* phase = defocus_only ? nullptr : w.phase.as<float>();
     const PhaseCoef* dcoefs = defocus_only ? w.coefs.as<PhaseCoef>() : nullptr;
     const int64_t kk = int64_t(K) * K;

@@ -32,6 +32,7 @@ struct Group {
     DeviceBuffer d_geoms, d_amp_index, d_coefs, d_scales, d_gains, d_unit0, amp, phase, radius;
     bool defocus_only = true;  // every request's coefficient sets differ only in defocus
     int window = 0;            // raw PSFs synthesised on their centred (2 window + 1)^2 block
+    int form = LDH_RAW_PSF_AUTO;
     std::shared_ptr<DeviceBuffer> store, kernel_scales;  // [which][unit] unit-sum multipliers
 };
 
@@ -115,7 +116,7 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
     }
 
     // ---- plan: groups by (P, R, r), units in request order ----
-    std::map<std::tuple<int, int, int>, int> index;
+    std::map<std::tuple<int, int, int, int>, int> index;
     std::vector<std::unique_ptr<Group>> groups;
     std::vector<int> group_of(static_cast<size_t>(n)), slot_of(static_cast<size_t>(n));
     std::vector<std::vector<float>> wls(static_cast<size_t>(n));
@@ -124,7 +125,7 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
         const Sizes sz = resolve_sizes(p);
         if (sz.max_radius > kMaxKernelRadius) fail(LDH_ERR_TOO_LARGE, "kernel-radius-unsupported");
         if (sz.raw > kMaxRawPsf) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
-        const auto key = std::make_tuple(sz.pupil, sz.raw, sz.max_radius);
+        const auto key = std::make_tuple(sz.pupil, sz.raw, sz.max_radius, p.raw_psf_form);
         auto it = index.find(key);
         if (it == index.end()) {
             it = index.emplace(key, int(groups.size())).first;
@@ -133,6 +134,7 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
             g->R = sz.raw;
             g->r = sz.max_radius;
             g->K = 2 * sz.max_radius + 1;
+            g->form = p.raw_psf_form;
             groups.push_back(std::move(g));
         }
         Group& g = *groups[size_t(it->second)];
@@ -165,7 +167,7 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
         }
         g.plan.prepare(s, g.P, g.R);
         const int h = resample_window_half_width(g.scales, g.r);
-        g.window = g.plan.prepare_window(s, h) ? h : 0;
+        g.window = g.form == LDH_RAW_PSF_AUTO && g.plan.prepare_window(s, h) ? h : 0;
         upload(g.d_geoms, g.geoms, s);
         upload(g.d_amp_index, g.amp_index, s);
         upload(g.d_coefs, g.coefs, s);

@@ -119,6 +119,7 @@ BankKey make_bank_key(const ldh_params& p) {
     const std::vector<float> wl = wavelength_list(p);
     add(double(wl.size()));
     for (float v : wl) add(double(v));
+    add(p.raw_psf_form);  // the two forms round differently
     return k;
 }
 
@@ -186,6 +187,7 @@ std::string unsupported_reason(const ldh_params& p, bool allow_field_zones) {
     if (p.aperture_mode < 0 || p.aperture_mode > LDH_APERTURE_SNOWFLAKE) return "invalid-aperture-mode";
     if (!allow_field_zones && has_field_phase(p)) return "field-zones-not-supported";
     if (p.wavelength_count < 0 || p.wavelength_count > LDH_MAX_WAVELENGTHS) return "invalid-wavelength-count";
+    if (p.raw_psf_form != LDH_RAW_PSF_AUTO && p.raw_psf_form != LDH_RAW_PSF_WHOLE) return "invalid-raw-psf-form";
     switch (p.debug_view) {
         case LDH_VIEW_FINAL: case LDH_VIEW_SELECTION: case LDH_VIEW_EFFECT: case LDH_VIEW_CORE:
         case LDH_VIEW_STRUCTURE: case LDH_VIEW_CREATIVE_FRINGE: case LDH_VIEW_SCATTER: case LDH_VIEW_PUPIL:

@@ -464,20 +464,26 @@ template <typename T> void fft_any(std::vector<std::complex<T>>& x, bool inverse
     }
 }
 
-// Rows, then columns (LensDiffCpuReference.cpp:277-301).
+// Rows, then columns (LensDiffCpuReference.cpp:277-301). Every line is transformed on its own
+// with the same arithmetic whichever thread takes it, so spreading the lines over OpenMP threads
+// leaves the result bit for bit the reference's sequential one.
 template <typename T> void fft_2d(std::vector<std::complex<T>>& d, int w, int h, bool inverse) {
-    std::vector<std::complex<T>> line(size_t(std::max(w, h)));
-    for (int y = 0; y < h; ++y) {
-        line.resize(size_t(w));
-        for (int x = 0; x < w; ++x) line[size_t(x)] = d[size_t(y) * w + x];
-        fft_any(line, inverse);
-        for (int x = 0; x < w; ++x) d[size_t(y) * w + x] = line[size_t(x)];
-    }
-    for (int x = 0; x < w; ++x) {
+#pragma omp parallel
+    {
+        std::vector<std::complex<T>> line(static_cast<size_t>(w));
+#pragma omp for schedule(static)
+        for (int y = 0; y < h; ++y) {
+            for (int x = 0; x < w; ++x) line[size_t(x)] = d[size_t(y) * w + x];
+            fft_any(line, inverse);
+            for (int x = 0; x < w; ++x) d[size_t(y) * w + x] = line[size_t(x)];
+        }
         line.resize(size_t(h));
-        for (int y = 0; y < h; ++y) line[size_t(y)] = d[size_t(y) * w + x];
-        fft_any(line, inverse);
-        for (int y = 0; y < h; ++y) d[size_t(y) * w + x] = line[size_t(y)];
+#pragma omp for schedule(static)
+        for (int x = 0; x < w; ++x) {
+            for (int y = 0; y < h; ++y) line[size_t(y)] = d[size_t(y) * w + x];
+            fft_any(line, inverse);
+            for (int y = 0; y < h; ++y) d[size_t(y) * w + x] = line[size_t(y)];
+        }
     }
 }
 
@@ -820,12 +826,15 @@ template <typename T> Bank<T>* build_bank(const ldh_params& p) {
         Kernel<T> k{max_r * 2 + 1, resample_kernel<T>(rp.data(), raw, scale, max_r)};
         bank->base.push_back(k);
     }
+    // the bins are shaped independently of one another (populatePsfBankCacheFromBaseKernels,
+    // :1616-1656, one bin per loop iteration); threads take whole bins
     const std::vector<float> wls = wavelength_list(p);
-    for (size_t i = 0; i < wls.size(); ++i) {
-        int r = 0;
-        bank->bins.push_back(shape_bin<T>(p, bank->base[i], wls[i], &r));
-        bank->support = std::max(bank->support, r);
-    }
+    const int nb = int(wls.size());
+    bank->bins.resize(size_t(nb));
+    std::vector<int> radii(size_t(nb), 0);
+#pragma omp parallel for schedule(dynamic)
+    for (int i = 0; i < nb; ++i) bank->bins[size_t(i)] = shape_bin<T>(p, bank->base[size_t(i)], wls[size_t(i)], &radii[size_t(i)]);
+    for (int r : radii) bank->support = std::max(bank->support, r);
     return bank;
 }
 
