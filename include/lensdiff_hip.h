@@ -218,6 +218,16 @@ ldh_status ldh_sequential_sums(ldh_context* ctx, const float* x, int64_t n, int3
  * powers of two (width >= 16, height >= 8), 3*2^a from 48 and 9*2^a from 144, up to 16384;
  * the reference takes powers of two only. */
 ldh_status ldh_fft2d(ldh_context* ctx, float* data, int32_t width, int32_t height, int32_t inverse);
+/* ldh_sequential_sums without allocating (no counterpart in the reference; for callers that cannot
+ * allocate device memory: code under HIP graph capture, serving loops, harnesses that forbid it).
+ * x: `count` device arrays of n floats at x + i * n, each >= 0 and finite; totals: device float[count];
+ * stream: a hipStream_t, or NULL for the context's stream. The work is only enqueued (synchronise
+ * the stream before reading totals). The block tables live in a 1.75 MiB static device array of the
+ * library, one per GPU, held until the process ends: count * ceil(n / 4096) must not exceed 65536
+ * (LDH_ERR_TOO_LARGE, "noalloc-scratch-too-small"). Calls that overlap on one GPU share that array
+ * and race; each call overwrites whatever an earlier call left in it. */
+ldh_status ldh_sequential_sums_noalloc(ldh_context* ctx, const float* x, int64_t n, int32_t count, float* totals,
+                                       void* stream);
 
 /* ---- PSF banks (EnsureLensDiffPsfBank, LensDiffCpuReference.cpp:2068-2071) ------------- */
 /* Batched bank synthesis (no counterpart in the reference, which builds one bank per render). */

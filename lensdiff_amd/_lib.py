@@ -35,6 +35,7 @@ SIGNATURES = {
     "ldh_sequential_sum": (C.c_int32, [_CTX, _F, C.c_int64, _F]),
     "ldh_sequential_sums": (C.c_int32, [_CTX, _F, C.c_int64, C.c_int32, C.c_int32, _F]),
     "ldh_fft2d": (C.c_int32, [_CTX, _F, C.c_int32, C.c_int32, C.c_int32]),
+    "ldh_sequential_sums_noalloc": (C.c_int32, [_CTX, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]),
     "ldh_bank_acquire": (C.c_int32, [_CTX, _P, C.POINTER(C.c_void_p)]),
     "ldh_bank_release": (C.c_int32, [_CTX, _BANK]),
     "ldh_bank_build_batch": (C.c_int32, [_CTX, _P, C.c_int32, C.c_int32, C.POINTER(C.c_void_p),

@@ -234,6 +234,7 @@ ldh_status ldh_context_create(int32_t device, uint64_t budget, ldh_context** out
     ldh_status st = guarded(ctx, [&] {
         LDH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream-create");
         ldh::pool_register_stream(ctx->device, ctx->stream);
+        ldh::noalloc_scratch_init(ctx->device);
         ctx->prof.stream = ctx->stream;
         ctx->fft.init(ctx->stream);
     });
@@ -447,6 +448,15 @@ ldh_status ldh_sequential_sums(ldh_context* ctx, const float* x, int64_t n, int3
         g.stride = n;
         ctx->bw.seqsum.run(ctx->stream, g, n, count, t.as<float>(), nullptr, block == 0 ? ldh::kSeqSumBlock : block);
         d2h(ctx, out, t.get(), sizeof(float) * count);
+    });
+}
+
+ldh_status ldh_sequential_sums_noalloc(ldh_context* ctx, const float* x, int64_t n, int32_t count, float* totals,
+                                       void* stream) {
+    return guarded(ctx, [&] {
+        require(x && totals && n >= 0 && count > 0, LDH_ERR_INVALID_ARGUMENT, "invalid-argument");
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+        ldh::sequential_sums_noalloc(s, ctx->device, x, n, count, totals);
     });
 }
 

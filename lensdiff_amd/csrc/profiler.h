@@ -93,7 +93,9 @@ struct ProfileScope {
     hipEvent_t start = nullptr;
     std::string name;
     double bytes = 0.0;
+    // An empty name records nothing (launch sequences that must not create events).
     ProfileScope(const std::string& n, double b) {
+        if (n.empty()) return;
         Profiler* ap = active_profiler();
         if (!ap || !ap->enabled || !ap->stream) return;
         p = ap;

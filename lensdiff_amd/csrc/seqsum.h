@@ -110,4 +110,12 @@ private:
     DeviceBuffer bsum_, bins_, ebase_;
 };
 
+// Allocation-free sums (ldh_sequential_sums_noalloc): the block tables live in a static device
+// array of the library, one per device. noalloc_scratch_init looks up its address on `device` (once
+// per device, at context creation); sequential_sums_noalloc then sums `segments` device arrays of
+// n floats at x + g * n into totals[g] on stream s without allocating, for at most 65536
+// (segment, 4096-element block) pairs. Calls that overlap on one device share the tables and race.
+void noalloc_scratch_init(int device);
+void sequential_sums_noalloc(hipStream_t s, int device, const float* x, int64_t n, int segments, float* totals);
+
 }  // namespace ldh

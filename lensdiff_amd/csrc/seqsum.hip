@@ -24,6 +24,8 @@
 // non-negative finite input.
 #include <climits>
 #include <cstdint>
+#include <map>
+#include <mutex>
 
 #include "profiler.h"
 #include "seqsum.h"
@@ -397,34 +399,107 @@ void SeqSum::reserve(int64_t n, int segments) {
     ebase_.reset(sizeof(int) * size_t(nb) * segments);
 }
 
-void SeqSum::run(hipStream_t s, const SeqSegments& g, int64_t n, int segments, float* totals, const double* block_sums,
-                 int block) {
-    if (n <= 0 || segments <= 0) return;
-    if (segments > g.per_array * SeqSegments::kMaxArrays) fail(LDH_ERR_INVALID_ARGUMENT, "seqsum-segments");
-    if (block < kSeqSumMinBlock || block > kSeqSumBlock) fail(LDH_ERR_INVALID_ARGUMENT, "seqsum-block");
+namespace {
+
+// The block tables of one run: FP64 block sums, the per-block base binade, and each block's sums
+// for its two binades. Every entry a kernel reads was written earlier in the same run (blocks
+// whose sum is 0 or whose binade is unknown are never looked up), so the tables need no clearing.
+struct Tables {
+    double* bsum;
+    double* bins;
+    int* ebase;
+};
+
+void launch_run(hipStream_t s, const SeqSegments& g, int64_t n, int segments, float* totals,
+                const double* block_sums, int block, const Tables& t, bool profiled) {
     const int nb = int(div_up(n, block));
-    reserve(n, segments);
     const double data_bytes = 4.0 * double(n) * segments;  // one read of the summed data
-    const double* bsum = block_sums != nullptr ? block_sums : bsum_.as<double>();
+    const double* bsum = block_sums != nullptr ? block_sums : t.bsum;
     {
-        ProfileScope ps("seqsum-block-sums", block_sums != nullptr ? 0.0 : data_bytes);
+        ProfileScope ps(profiled ? "seqsum-block-sums" : "", block_sums != nullptr ? 0.0 : data_bytes);
         if (block_sums == nullptr)
-            hipLaunchKernelGGL(block_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, g, n, block, nb,
-                               bsum_.as<double>());
-        hipLaunchKernelGGL(prefix_kernel, dim3(segments), dim3(kThreads), 0, s, bsum, nb, ebase_.as<int>());
+            hipLaunchKernelGGL(block_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, g, n, block, nb, t.bsum);
+        hipLaunchKernelGGL(prefix_kernel, dim3(segments), dim3(kThreads), 0, s, bsum, nb, t.ebase);
     }
     {
-        ProfileScope ps("seqsum-binade-sums", data_bytes);
+        ProfileScope ps(profiled ? "seqsum-binade-sums" : "", data_bytes);
         hipLaunchKernelGGL(binade_sums_kernel, dim3(nb, segments), dim3(kThreads), 0, s, g, n, block, nb, bsum,
-                           ebase_.as<int>(), bins_.as<double>());
+                           t.ebase, t.bins);
     }
     {
         // reads the block tables and replays a few blocks: latency-bound, no byte model
-        ProfileScope ps("seqsum-walk", 0.0);
-        hipLaunchKernelGGL(walk_kernel, dim3(segments), dim3(kWalkThreads), 0, s, g, n, block, nb, bsum,
-                           ebase_.as<int>(), bins_.as<double>(), totals);
+        ProfileScope ps(profiled ? "seqsum-walk" : "", 0.0);
+        hipLaunchKernelGGL(walk_kernel, dim3(segments), dim3(kWalkThreads), 0, s, g, n, block, nb, bsum, t.ebase,
+                           t.bins, totals);
     }
     LDH_CHECK_LAUNCH("seqsum-launch");
+}
+
+void check_run(const SeqSegments& g, int segments, int block) {
+    if (segments > g.per_array * SeqSegments::kMaxArrays) fail(LDH_ERR_INVALID_ARGUMENT, "seqsum-segments");
+    if (block < kSeqSumMinBlock || block > kSeqSumBlock) fail(LDH_ERR_INVALID_ARGUMENT, "seqsum-block");
+}
+
+// Scratch of the allocation-free sums (sequential_sums_noalloc): the block tables of at most
+// kNoAllocBlocks (segment, block) pairs, 1.75 MiB per device, held until the process ends. Host
+// code never uses this name as an address: it takes the device address that hipGetSymbolAddress
+// returns for the current device (noalloc_scratch).
+constexpr int64_t kNoAllocBlocks = int64_t(1) << 16;
+constexpr size_t kNoAllocBytes = size_t(kNoAllocBlocks) * (2 * sizeof(double) + sizeof(double) + sizeof(int));
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;  // device -> address of its copy of `scratch`
+
+}  // namespace
+
+void SeqSum::run(hipStream_t s, const SeqSegments& g, int64_t n, int segments, float* totals, const double* block_sums,
+                 int block) {
+    if (n <= 0 || segments <= 0) return;
+    check_run(g, segments, block);
+    reserve(n, segments);
+    launch_run(s, g, n, segments, totals, block_sums, block,
+               Tables{bsum_.as<double>(), bins_.as<double>(), ebase_.as<int>()}, true);
+}
+
+void noalloc_scratch_init(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.count(device)) return;
+    int prev = -1;
+    LDH_HIP(hipGetDevice(&prev), "noalloc-scratch");
+    if (prev != device) LDH_HIP(hipSetDevice(device), "noalloc-scratch");
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (prev != device) (void)hipSetDevice(prev);
+    LDH_HIP(e, "noalloc-scratch-address");
+    if (ptr == nullptr) fail(LDH_ERR_DEVICE, "noalloc-scratch-address");
+    g_scratch_ptr[device] = ptr;
+}
+
+void sequential_sums_noalloc(hipStream_t s, int device, const float* x, int64_t n, int segments, float* totals) {
+    if (n <= 0 || segments <= 0) return;
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        auto it = g_scratch_ptr.find(device);
+        if (it == g_scratch_ptr.end()) fail(LDH_ERR_DEVICE, "noalloc-scratch-not-initialised");
+        base = it->second;
+    }
+    const int64_t pairs = div_up(n, kSeqSumBlock) * segments;
+    if (pairs > kNoAllocBlocks) fail(LDH_ERR_TOO_LARGE, "noalloc-scratch-too-small");
+    SeqSegments g;
+    g.array[0] = x;
+    g.per_array = segments;
+    g.stride = n;
+    check_run(g, segments, kSeqSumBlock);
+    // [bsum: pairs doubles][bins: 2 pairs doubles][ebase: pairs ints], each table 256-byte aligned
+    auto align = [](size_t b) { return (b + 255) / 256 * 256; };
+    unsigned char* p = static_cast<unsigned char*>(base);
+    Tables t;
+    t.bsum = reinterpret_cast<double*>(p);
+    t.bins = reinterpret_cast<double*>(p + align(sizeof(double) * size_t(pairs)));
+    t.ebase = reinterpret_cast<int*>(p + align(sizeof(double) * size_t(pairs)) + align(2 * sizeof(double) * size_t(pairs)));
+    launch_run(s, g, n, segments, totals, nullptr, kSeqSumBlock, t, false);
 }
 
 void SeqSum::scale(hipStream_t s, float* x, int64_t n, int64_t stride, int segments, const float* totals) {
