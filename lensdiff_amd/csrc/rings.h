@@ -11,16 +11,23 @@
 
 namespace ldh {
 
+// Raster pixels per band of the ring-order copy.
+constexpr int kRingBand = 8192;
+
 // Pixels of a size x size image grouped by ring about `center`: ring(x, y) = round(|d|)
-// (ceil = false) or ceil(|d|) (ceil = true), clamped to rmax. order[off[r] .. off[r+1]) lists the
-// pixels of ring r in raster order; the last ring (rmax) holds every pixel the clamp sends there.
+// (ceil = false) or ceil(|d|) (ceil = true), clamped to rmax. The ring order lists the pixels of
+// ring 0, then ring 1, ..., each ring's pixels in raster order, ring r at positions
+// [off[r], off[r+1]); the last ring (rmax) holds every pixel the clamp sends there. On the device
+// the order is kept per band of kRingBand raster pixels: band b's pixels, sorted by ring-order
+// position, at [b * kRingBand, ...) of band_local (index within the band) and band_out (position).
 struct RingPlan {
     int size = 0, rmax = -1;
     float center = 0.0f;
     bool ceil = false;
     int64_t pixels = 0;
     std::vector<int> host_off;  // rmax + 2 entries
-    DeviceBuffer order;         // int32[pixels]
+    DeviceBuffer band_local;    // uint16[pixels]
+    DeviceBuffer band_out;      // int32[pixels]
     DeviceBuffer off;           // int32[rmax + 2]: host_off on the device
     DeviceBuffer count;         // int32[rmax + 1]: ring sizes
     bool matches(int sz, float c, int rm, bool cl) const {

@@ -17,7 +17,6 @@ namespace {
 
 constexpr int kRingWaves = 4;     // wavefronts (rings) per workgroup
 constexpr int kRingGroups = 8;    // 64-element groups whose loads are issued together
-constexpr int kGatherThreads = 256;
 
 __device__ __forceinline__ float inverse_total(const float* __restrict__ totals, int seg) {
     if (totals == nullptr) return 1.0f;
@@ -111,42 +110,68 @@ ring_wave_sums_kernel(const float* __restrict__ ordered, int64_t pixels, const i
 }
 
 // The ring-ordered copy of every segment, dst[g * pixels + i] = img[g][order[i]] * (1 / total),
-// written coalesced; the last ring's peak (i >= last) by atomic maximum (bits of values >= 0 order
-// as the values). Chunks of kGatherChunk positions go to the XCDs in contiguous ranges (chunk c
-// runs on XCD c % 8), so one XCD reads an annulus of rings whose source lines its L2 shares.
-constexpr int kGatherPer = 4;
-constexpr int kGatherChunk = kGatherThreads * kGatherPer;
-__global__ void __launch_bounds__(kGatherThreads)
-ring_gather_kernel(const float* __restrict__ img, int64_t seg_stride, const float* __restrict__ prescale,
-                   const int* __restrict__ order, int64_t pixels, int64_t last, int chunks, int segs, int per_xcd,
-                   float* __restrict__ dst, unsigned* __restrict__ peak_slot, int nr) {
+// and the last ring's peak (i >= last) by atomic maximum (bits of values >= 0 order as the
+// values). A workgroup takes one band of kRingBand raster pixels and kBandSegs segments: each
+// segment's band is read coalesced into LDS, and its pixels are stored to their ring-ordered
+// positions in ascending order (the band's pixels of one ring are consecutive there), so a wave's
+// stores fall in a few contiguous runs. The band's position lists are read once for all its
+// segments; the next segment's band is loaded while the current one is stored.
+constexpr int kBandThreads = 256;
+constexpr int kBandPer = kRingBand / kBandThreads;
+constexpr int kBandSegs = 8;
+__global__ void __launch_bounds__(kBandThreads)
+ring_band_gather_kernel(const float* __restrict__ img, int64_t seg_stride, const float* __restrict__ prescale,
+                        const unsigned short* __restrict__ band_local, const int* __restrict__ band_out,
+                        int64_t pixels, int64_t last, int bands, int segs, int per_xcd, float* __restrict__ dst,
+                        unsigned* __restrict__ peak_slot, int nr) {
     const int b = int(blockIdx.x);
-    const int logical = (b % 8) * per_xcd + b / 8;  // seg-major (segment, chunk)
-    if (logical >= chunks * segs) return;
-    const int seg = logical / chunks, chunk = logical % chunks;
-    const float* x = img + int64_t(seg) * seg_stride;
-    float* o = dst + int64_t(seg) * pixels;
-    const float inv = inverse_total(prescale, seg);
-    const int64_t base = int64_t(chunk) * kGatherChunk + threadIdx.x;
-    int idx[kGatherPer];
+    const int logical = (b % 8) * per_xcd + b / 8;  // band-major (band, segment group)
+    const int groups = (segs + kBandSegs - 1) / kBandSegs;
+    if (logical >= bands * groups) return;
+    const int band = logical / groups, g0 = (logical % groups) * kBandSegs;
+    const int g1 = min(segs, g0 + kBandSegs);
+    const int64_t p0 = int64_t(band) * kRingBand;
+    const int len = int(min(int64_t(kRingBand), pixels - p0));
+    const int t = int(threadIdx.x);
+    __shared__ float tile[kRingBand];
+    int loc[kBandPer], out[kBandPer];
 #pragma unroll
-    for (int k = 0; k < kGatherPer; ++k) idx[k] = order[min(base + k * kGatherThreads, pixels - 1)];
-    float v[kGatherPer];
+    for (int k = 0; k < kBandPer; ++k) {
+        const int e = t + k * kBandThreads;
+        const int c = min(e, len - 1);
+        loc[k] = int(band_local[p0 + c]);
+        out[k] = e < len ? band_out[p0 + c] : -1;
+    }
+    float cur[kBandPer];
+    auto load = [&](int g, float (&v)[kBandPer]) {
+        const float* x = img + int64_t(g) * seg_stride + p0;
 #pragma unroll
-    for (int k = 0; k < kGatherPer; ++k) v[k] = x[idx[k]] * inv;
-    unsigned pk = 0u;
+        for (int k = 0; k < kBandPer; ++k) v[k] = x[min(t + k * kBandThreads, len - 1)];
+    };
+    load(g0, cur);
+    for (int g = g0; g < g1; ++g) {
+        __syncthreads();  // the previous segment's band has been read
 #pragma unroll
-    for (int k = 0; k < kGatherPer; ++k) {
-        const int64_t i = base + k * kGatherThreads;
-        if (i < pixels) {
-            o[i] = v[k];
-            if (i >= last) pk = max(pk, __float_as_uint(v[k]));
+        for (int k = 0; k < kBandPer; ++k) tile[t + k * kBandThreads] = cur[k];
+        __syncthreads();
+        if (g + 1 < g1) load(g + 1, cur);
+        const float inv = inverse_total(prescale, g);
+        float* o = dst + int64_t(g) * pixels;
+        unsigned pk = 0u;
+#pragma unroll
+        for (int k = 0; k < kBandPer; ++k) {
+            if (out[k] >= 0) {
+                const float v = tile[loc[k]] * inv;
+                o[out[k]] = v;
+                if (out[k] >= last) pk = max(pk, __float_as_uint(v));
+            }
+        }
+        if (peak_slot != nullptr) {
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) pk = max(pk, unsigned(__shfl_xor(int(pk), d)));
+            if ((t & 63) == 0 && pk != 0u) atomicMax(&peak_slot[int64_t(g) * nr], pk);
         }
     }
-    if (peak_slot == nullptr) return;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) pk = max(pk, unsigned(__shfl_xor(int(pk), d)));
-    if ((threadIdx.x & 63) == 0 && pk != 0u) atomicMax(&peak_slot[int64_t(seg) * nr], pk);
 }
 
 // Clears the last ring's peak slot of every segment (before the gather's atomic maxima).
@@ -193,10 +218,27 @@ void prepare_ring_plan(hipStream_t s, RingPlan& p, int size, float center, int r
     for (int y = 0; y < size; ++y)
         for (int x = 0; x < size; ++x)
             order[size_t(next[size_t(host_ring(x, y, center, rmax, ceil))]++)] = y * size + x;
-    p.order.reset(sizeof(int) * size_t(n));
+    // per band of kRingBand raster pixels, its pixels in ring order: band-local index and
+    // ring-ordered position (walking the ring order appends to each band in ascending position)
+    const int64_t bands = div_up(n, int64_t(kRingBand));
+    std::vector<unsigned short> band_local(static_cast<size_t>(n));
+    std::vector<int> band_out(static_cast<size_t>(n));
+    std::vector<int> fill(static_cast<size_t>(bands), 0);
+    for (int64_t i = 0; i < n; ++i) {
+        const int px = order[size_t(i)];
+        const int band = px / kRingBand;
+        const size_t k = size_t(band) * kRingBand + size_t(fill[size_t(band)]++);
+        band_local[k] = static_cast<unsigned short>(px - band * kRingBand);
+        band_out[k] = int(i);
+    }
+    p.band_local.reset(sizeof(unsigned short) * size_t(n));
+    p.band_out.reset(sizeof(int) * size_t(n));
     p.off.reset(sizeof(int) * (size_t(nr) + 1));
     p.count.reset(sizeof(int) * size_t(nr));
-    LDH_HIP(hipMemcpyAsync(p.order.get(), order.data(), sizeof(int) * size_t(n), hipMemcpyHostToDevice, s),
+    LDH_HIP(hipMemcpyAsync(p.band_local.get(), band_local.data(), sizeof(unsigned short) * size_t(n),
+                           hipMemcpyHostToDevice, s),
+            "ring-plan");
+    LDH_HIP(hipMemcpyAsync(p.band_out.get(), band_out.data(), sizeof(int) * size_t(n), hipMemcpyHostToDevice, s),
             "ring-plan");
     LDH_HIP(hipMemcpyAsync(p.off.get(), off.data(), sizeof(int) * (size_t(nr) + 1), hipMemcpyHostToDevice, s),
             "ring-plan");
@@ -224,13 +266,15 @@ void ring_sums(hipStream_t s, const RingPlan& plan, RingScratch& scratch, SeqSum
     float* ordered = scratch.ordered.as<float>();
     const unsigned small = unsigned(div_up(segs, 256));
     {
-        ProfileScope ps("ring-gather", 12.0 * double(n) * segs);
+        const int bands = int(div_up(n, int64_t(kRingBand)));
+        const int groups = int(div_up(segs, kBandSegs));
+        // each segment read and written once, the band lists (6 bytes a pixel) once per group
+        ProfileScope ps("ring-gather", 8.0 * double(n) * segs + 6.0 * double(n) * groups);
         if (peaks) hipLaunchKernelGGL(ring_corner_clear_kernel, dim3(small), dim3(256), 0, s, peaks + rmax, nr, segs);
-        const int chunks = int(div_up(n, kGatherChunk));
-        const int per_xcd = int(div_up(int64_t(chunks) * segs, 8));
-        hipLaunchKernelGGL(ring_gather_kernel, dim3(unsigned(8 * per_xcd)), dim3(kGatherThreads), 0, s, img,
-                           seg_stride, prescale, plan.order.as<int>(), n, last, chunks, segs, per_xcd, ordered,
-                           peaks ? peaks + rmax : nullptr, nr);
+        const int per_xcd = int(div_up(int64_t(bands) * groups, 8));
+        hipLaunchKernelGGL(ring_band_gather_kernel, dim3(unsigned(8 * per_xcd)), dim3(kBandThreads), 0, s, img,
+                           seg_stride, prescale, plan.band_local.as<unsigned short>(), plan.band_out.as<int>(), n, last,
+                           bands, segs, per_xcd, ordered, peaks ? peaks + rmax : nullptr, nr);
         LDH_CHECK_LAUNCH("ring-gather");
     }
     if (rmax > 0) {
