@@ -167,10 +167,12 @@ def batched_form(ctx, progress):
         res[name] = {"ms": round(st.exec_ms, 2), "psfs_per_s": round(st.psfs / (st.exec_ms / 1e3), 1),
                      "groups": st.groups}
         if mode == _abi.LDH_BATCH_GRAPH:
-            res[name].update({"capture_ms": round(st.capture_ms, 2), "graph_nodes": st.graph_nodes})
+            res[name].update({"capture_ms": round(st.capture_ms, 2), "graph_nodes": st.graph_nodes,
+                              "branches": st.branches})
         res["psfs"] = st.psfs
         progress("batched %s: %.1f ms, %.0f PSFs/s" % (name, st.exec_ms, res[name]["psfs_per_s"]))
     res["graph_vs_per_request"] = round(res["per_request"]["ms"] / res["graph"]["ms"], 2)
+    res["graph_vs_grouped"] = round(res["grouped"]["ms"] / res["graph"]["ms"], 2)
     return res
 
 

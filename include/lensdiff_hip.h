@@ -234,7 +234,8 @@ ldh_status ldh_sequential_sums_noalloc(ldh_context* ctx, const float* x, int64_t
 enum {
     LDH_BATCH_PER_REQUEST = 0,  /* one bank build after another (the single-bank path) */
     LDH_BATCH_GROUPED = 1,      /* requests with equal pupil/raw/kernel sizes share every launch */
-    LDH_BATCH_GRAPH = 2         /* LDH_BATCH_GROUPED captured into one HIP graph, launched once */
+    LDH_BATCH_GRAPH = 2         /* the groups captured into one HIP graph as up to three parallel
+                                   branches (each with its own workspace), launched once */
 };
 typedef struct ldh_batch_stats {
     double exec_ms;        /* device time of the build (HIP events around its launches) */
@@ -243,6 +244,7 @@ typedef struct ldh_batch_stats {
     int32_t psfs;          /* (request, wavelength) PSFs built */
     int32_t graph_nodes;   /* nodes of the captured graph (0 without one) */
     int32_t mode;
+    int32_t branches;      /* parallel branches of the graph (1 without one) */
 } ldh_batch_stats;
 typedef struct ldh_bank_info {
     int32_t bin_count;

@@ -177,6 +177,7 @@ class BatchStats(C.Structure):
         ("psfs", C.c_int32),
         ("graph_nodes", C.c_int32),
         ("mode", C.c_int32),
+        ("branches", C.c_int32),
     ]
 
 

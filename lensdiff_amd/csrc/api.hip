@@ -46,6 +46,7 @@ struct ldh_context {
     uint64_t budget = 0;
     ldh::FftTables fft;
     ldh::BankWorkspace bw;
+    ldh::BatchLanes lanes;  // extra branches of graph batches
     ldh::FrameWork fw;
     ldh::Profiler prof;
     // single-entry cache of the plain path
@@ -510,7 +511,7 @@ ldh_status ldh_bank_build_batch(ldh_context* ctx, const ldh_params* params, int3
         // a capture must not record timing events: bind a disabled profiler
         ldh::Profiler none;
         ldh::ProfilerBinding unbind(none);
-        auto built = ldh::build_banks_batched(ctx->stream, ctx->fft, ctx->bw, reqs, mode, &tm);
+        auto built = ldh::build_banks_batched(ctx->stream, ctx->fft, ctx->bw, reqs, mode, &tm, &ctx->lanes);
         for (int32_t i = 0; i < count; ++i) {
             auto* b = new ldh_bank();
             b->bank = built[size_t(i)];
@@ -523,6 +524,7 @@ ldh_status ldh_bank_build_batch(ldh_context* ctx, const ldh_params* params, int3
             stats->psfs = tm.psfs;
             stats->graph_nodes = tm.graph_nodes;
             stats->mode = tm.mode;
+            stats->branches = std::max(1, tm.branches);
         }
     });
 }

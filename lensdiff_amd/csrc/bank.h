@@ -120,10 +120,26 @@ bool defocus_only_coefs(const std::vector<PhaseCoef>& coefs);
 // Batched build of many banks (batch.hip). mode: LDH_BATCH_* (include/lensdiff_hip.h).
 struct BatchTiming {
     double exec_ms = 0.0, capture_ms = 0.0;
-    int groups = 0, psfs = 0, graph_nodes = 0, mode = 0;
+    int groups = 0, psfs = 0, graph_nodes = 0, mode = 0, branches = 0;
 };
+// Workspaces and capture streams of the extra branches of a graph batch, kept with a context so
+// that their buffers and ring plans are reused from batch to batch. Branch 0 is the caller's
+// workspace and stream; the streams here only take part in captures (the graph runs on the
+// caller's stream).
+struct BatchLanes {
+    std::vector<std::unique_ptr<BankWorkspace>> work;
+    std::vector<hipStream_t> streams;
+    void ensure(int extra);
+    BatchLanes() = default;
+    BatchLanes(const BatchLanes&) = delete;
+    BatchLanes& operator=(const BatchLanes&) = delete;
+    ~BatchLanes();
+};
+constexpr int kGraphBranches = 3;
+// lanes (nullable): with LDH_BATCH_GRAPH, the groups are spread over up to kGraphBranches
+// parallel branches of the graph, each with its own workspace.
 std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, const FftTables& t, BankWorkspace& w,
                                                              const std::vector<ldh_params>& requests, int mode,
-                                                             BatchTiming* timing);
+                                                             BatchTiming* timing, BatchLanes* lanes = nullptr);
 
 }  // namespace ldh

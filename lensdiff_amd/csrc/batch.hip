@@ -6,6 +6,9 @@
 // coefficients, resample scale and anisotropy gain. Everything that synchronises with the host
 // (clear-circle first minima, Bluestein filter tables, parameter uploads, buffer sizing) happens
 // before the launches, so the whole schedule can be captured into one HIP graph and launched once.
+// In the graph the groups form up to kGraphBranches parallel branches (balanced by their texel
+// counts, each branch with its own workspace), so that one group's latency-bound steps (the
+// sequential-sum walks, one wavefront per sum) overlap another's bandwidth-bound ones.
 // The banks of a group share one kernel store, [which][unit][K*K].
 #include <algorithm>
 #include <chrono>
@@ -34,6 +37,8 @@ struct Group {
     int window = 0;            // raw PSFs synthesised on their centred (2 window + 1)^2 block
     int form = LDH_RAW_PSF_AUTO;
     std::shared_ptr<DeviceBuffer> store, kernel_scales;  // [which][unit] unit-sum multipliers
+    int branch = 0;                                       // graph branch (its workspace and stream)
+    double cost = 0.0;                                    // texels synthesised and shaped
 };
 
 template <class T>
@@ -76,9 +81,23 @@ void enqueue_group(hipStream_t s, const FftTables& t, BankWorkspace& w, Group& g
 
 }  // namespace
 
+void BatchLanes::ensure(int extra) {
+    while (int(streams.size()) < extra) {
+        hipStream_t st = nullptr;
+        LDH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "batch-stream");
+        streams.push_back(st);
+        work.push_back(std::make_unique<BankWorkspace>());
+    }
+}
+
+BatchLanes::~BatchLanes() {
+    work.clear();
+    for (hipStream_t st : streams) (void)hipStreamDestroy(st);
+}
+
 std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, const FftTables& t, BankWorkspace& w,
                                                              const std::vector<ldh_params>& requests, int mode,
-                                                             BatchTiming* timing) {
+                                                             BatchTiming* timing, BatchLanes* lanes) {
     const int n = int(requests.size());
     std::vector<std::shared_ptr<DeviceBank>> banks(static_cast<size_t>(n));
     BatchTiming tm;
@@ -118,7 +137,6 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
     // ---- plan: groups by (P, R, r), units in request order ----
     std::map<std::tuple<int, int, int, int>, int> index;
     std::vector<std::unique_ptr<Group>> groups;
-    std::vector<int> group_of(static_cast<size_t>(n)), slot_of(static_cast<size_t>(n));
     std::vector<std::vector<float>> wls(static_cast<size_t>(n));
     for (int i = 0; i < n; ++i) {
         const ldh_params& p = requests[size_t(i)];
@@ -138,11 +156,16 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
             groups.push_back(std::move(g));
         }
         Group& g = *groups[size_t(it->second)];
-        group_of[size_t(i)] = it->second;
-        slot_of[size_t(i)] = int(g.requests.size());
         g.requests.push_back(i);
         wls[size_t(i)] = wavelength_list(p);
     }
+    // ---- graph branches: groups dealt to the least loaded branch, largest first ----
+    int branches = 1;
+    if (mode == LDH_BATCH_GRAPH && lanes != nullptr && groups.size() > 1) {
+        branches = std::min(kGraphBranches, int(groups.size()));
+        lanes->ensure(branches - 1);
+    }
+    auto work_of = [&](const Group& g) -> BankWorkspace& { return g.branch == 0 ? w : *lanes->work[size_t(g.branch - 1)]; };
     // ---- host-side set-up: first minima, plans, per-unit parameters, buffers ----
     for (auto& gp : groups) {
         Group& g = *gp;
@@ -168,6 +191,23 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
         g.plan.prepare(s, g.P, g.R);
         const int h = resample_window_half_width(g.scales, g.r);
         g.window = g.form == LDH_RAW_PSF_AUTO && g.plan.prepare_window(s, h) ? h : 0;
+        const double side = g.window > 0 ? 2.0 * g.window + 1.0 : double(g.R);
+        g.cost = double(g.units) * (side * side + double(g.K) * g.K);
+    }
+    if (branches > 1) {
+        std::vector<Group*> order;
+        for (auto& gp : groups) order.push_back(gp.get());
+        std::stable_sort(order.begin(), order.end(), [](const Group* a, const Group* b) { return a->cost > b->cost; });
+        std::vector<double> load(static_cast<size_t>(branches), 0.0);
+        for (Group* g : order) {
+            const size_t k = size_t(std::min_element(load.begin(), load.end()) - load.begin());
+            g->branch = int(k);
+            load[k] += g->cost;
+        }
+    }
+    for (auto& gp : groups) {
+        Group& g = *gp;
+        BankWorkspace& gw = work_of(g);
         upload(g.d_geoms, g.geoms, s);
         upload(g.d_amp_index, g.amp_index, s);
         upload(g.d_coefs, g.coefs, s);
@@ -182,17 +222,18 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
         g.radius.reset(sizeof(int) * g.units);
         g.store = std::make_shared<DeviceBuffer>(sizeof(float) * kk * 3 * g.units);
         g.kernel_scales = std::make_shared<DeviceBuffer>(sizeof(float) * 3 * g.units);
-        // the workspace grows to the batch's largest group before any launch
+        // each workspace grows to its branch's largest group before any launch
         const int side = g.window > 0 ? 2 * g.window + 1 : g.R;
-        w.raw.reset(sizeof(float) * size_t(side) * side * g.units);
-        w.totals.reset(sizeof(float) * g.units);
-        w.base.reset(sizeof(float) * kk * g.units);
-        w.base_totals.reset(sizeof(float) * g.units);
-        w.psf.reserve(g.P, side, g.units);
-        if (g.window == 0) w.seqsum.reserve(int64_t(g.R) * g.R, g.units);
-        w.reserve_shaping(s, g.K, g.units, g.r);
+        gw.raw.reset(sizeof(float) * size_t(side) * side * g.units);
+        gw.totals.reset(sizeof(float) * g.units);
+        gw.base.reset(sizeof(float) * kk * g.units);
+        gw.base_totals.reset(sizeof(float) * g.units);
+        gw.psf.reserve(g.P, side, g.units);
+        if (g.window == 0) gw.seqsum.reserve(int64_t(g.R) * g.R, g.units);
+        gw.reserve_shaping(s, g.K, g.units, g.r);
         tm.psfs += g.units;
     }
+    tm.branches = branches;
     LDH_HIP(hipStreamSynchronize(s), "batch-upload");
     tm.groups = int(groups.size());
 
@@ -201,9 +242,29 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
     if (mode == LDH_BATCH_GRAPH) {
         const auto c0 = std::chrono::steady_clock::now();
         hipGraph_t graph = nullptr;
+        // branch b > 0 joins the capture on its own stream after a fork event on s and is joined
+        // back before the capture ends
+        std::vector<hipEvent_t> marks(static_cast<size_t>(branches), nullptr);
+        struct Marks {
+            std::vector<hipEvent_t>& v;
+            ~Marks() {
+                for (hipEvent_t e : v)
+                    if (e) (void)hipEventDestroy(e);
+            }
+        } marks_guard{marks};
+        for (auto& e : marks) LDH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), "batch-event");
+        auto stream_of = [&](int b) { return b == 0 ? s : lanes->streams[size_t(b - 1)]; };
         LDH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "batch-capture");
         try {
-            for (auto& g : groups) enqueue_group(s, t, w, *g);
+            if (branches > 1) {
+                LDH_HIP(hipEventRecord(marks[0], s), "batch-fork");
+                for (int b = 1; b < branches; ++b) LDH_HIP(hipStreamWaitEvent(stream_of(b), marks[0], 0), "batch-fork");
+            }
+            for (auto& g : groups) enqueue_group(stream_of(g->branch), t, work_of(*g), *g);
+            for (int b = 1; b < branches; ++b) {
+                LDH_HIP(hipEventRecord(marks[size_t(b)], stream_of(b)), "batch-join");
+                LDH_HIP(hipStreamWaitEvent(s, marks[size_t(b)], 0), "batch-join");
+            }
         } catch (...) {
             hipGraph_t broken = nullptr;
             (void)hipStreamEndCapture(s, &broken);
@@ -271,8 +332,6 @@ std::vector<std::shared_ptr<DeviceBank>> build_banks_batched(hipStream_t s, cons
             banks[size_t(i)] = bank;
         }
     }
-    (void)group_of;
-    (void)slot_of;
     if (timing) *timing = tm;
     return banks;
 }

@@ -31,8 +31,10 @@ def test_batch_modes_build_identical_banks(ctx):
     graph, st2 = build(ctx, reqs, _abi.LDH_BATCH_GRAPH)
     assert st0.psfs == st1.psfs == st2.psfs == sum(len(b["wavelengths"]) for b in per)
     assert st1.groups == len({(p.pupil_resolution, ) for p in reqs}) and st2.graph_nodes > 0
-    print("per-request %.2f ms, grouped %.2f ms, graph %.2f ms (+%.1f ms capture, %d nodes), %d PSFs"
-          % (st0.exec_ms, st1.exec_ms, st2.exec_ms, st2.capture_ms, st2.graph_nodes, st2.psfs))
+    # the graph's groups run as parallel branches, each with its own workspace
+    assert st1.branches == 1 and st2.branches == min(3, st1.groups)
+    print("per-request %.2f ms, grouped %.2f ms, graph %.2f ms (%d branches, +%.1f ms capture, %d nodes), %d PSFs"
+          % (st0.exec_ms, st1.exec_ms, st2.exec_ms, st2.branches, st2.capture_ms, st2.graph_nodes, st2.psfs))
     for a, b, c in zip(per, grouped, graph):
         assert a["support"] == b["support"] == c["support"]
         for name in ("full", "core", "structure"):
