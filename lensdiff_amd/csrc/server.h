@@ -29,6 +29,7 @@ struct ServedRequest {
     double t_submit = 0, t_start = 0, t_done = 0;
     bool done = false;
     bool bank_hit = false, spectra_hit = false;
+    int passed = 0;  // times a later request was admitted ahead of this one at the queue's head
     ldh_status status = LDH_OK;
     std::string error;
 };

@@ -215,51 +215,95 @@ bool Server::evict_locked(size_t need) {
     return resident + need <= budget_;
 }
 
-// Takes the oldest queued request once its cache needs fit the budget. Pins its cache entry
-// (creating it, with the caller as builder, when absent).
+constexpr int kAdmitWindow = 8;  // queued requests a free slot considers
+constexpr int kMaxPassed = 16;   // admissions ahead of the oldest request before it goes first
+
+// Picks the next request for slot s among the oldest queued ones (kAdmitWindow of them; only the
+// oldest once later ones have gone ahead of it kMaxPassed times). A request whose bank or spectra
+// another slot is still building is left queued: a slot taking it would only wait. Of the others,
+// the oldest one whose cache needs fit the budget as it stands goes first; failing that, the
+// oldest one that fits after evicting unpinned entries (or any, when nothing is in flight). Pins
+// the request's cache entry, creating it, with the caller as builder, when absent.
 ServedRequest* Server::admit(Slot* s, std::shared_ptr<CacheEntry>* entry, bool* builder) {
     std::unique_lock<std::mutex> lk(mu_);
     while (true) {
         if (stopping_) return nullptr;
-        if (!queue_.empty()) {
-            ServedRequest* r = queue_.front();
+        bool in_flight = false;
+        for (const auto& c : cache_) in_flight = in_flight || c->pins > 0;
+        const size_t resident = resident_bytes_locked();
+        const int window = (!queue_.empty() && queue_.front()->passed >= kMaxPassed) ? 1 : kAdmitWindow;
+        struct Candidate {
+            std::deque<ServedRequest*>::iterator it;
+            std::shared_ptr<CacheEntry> e;
+            size_t need;
+        };
+        std::vector<Candidate> cands;
+        int scanned = 0;
+        for (auto q = queue_.begin(); q != queue_.end() && scanned < window; ++q, ++scanned) {
+            ServedRequest* r = *q;
             const BankKey key = served_key(r->params);
             std::shared_ptr<CacheEntry> e;
-            for (auto it = cache_.begin(); it != cache_.end(); ++it)
-                if ((*it)->key == key) {
-                    e = *it;
-                    cache_.splice(cache_.begin(), cache_, it);  // most recently used first
+            for (const auto& c : cache_)
+                if (c->key == key) {
+                    e = c;
                     break;
                 }
+            if (e && !e->ready) continue;  // its bank is being built
             size_t need = e ? 0 : bank_bytes_estimate(r->params);
             const int nk = cached_kernel_count(r->params);
-            if (nk > 0) {
-                const int support = e && e->ready ? e->bank->support : resolve_sizes(r->params).max_radius;
+            if (nk > 0 && (!e || nk == 1 || e->combined_style == with_style(r->params))) {
+                const int support = e ? e->bank->support : resolve_sizes(r->params).max_radius;
                 int gw = 0, gh = 0;
                 served_frame_size(r->params, r->w, r->h, gw, gh);
-                need += spectra_bytes(make_grid(gw, gh, support), nk, r->params.spectra_fp16 != 0);
+                const Grid g = make_grid(gw, gh, support);
+                const SpectraKey sk{g.pw, g.ph, nk, r->params.spectra_fp16 != 0 ? 1 : 0};
+                std::shared_ptr<CacheEntry::Spectra> spec;
+                if (e)
+                    for (const auto& sp : e->spectra)
+                        if (sp->key == sk) spec = sp;
+                if (spec && !spec->ready) continue;  // its spectra are being built
+                if (!spec) need += spectra_bytes(g, nk, r->params.spectra_fp16 != 0);
             }
-            bool in_flight = false;
-            for (const auto& c : cache_) in_flight = in_flight || c->pins > 0;
-            if (evict_locked(need) || !in_flight) {
-                queue_.pop_front();
-                *builder = !e;
-                if (!e) {
-                    e = std::make_shared<CacheEntry>();
-                    e->key = key;
-                    cache_.push_front(e);
-                    stats_.bank_misses += 1;
-                } else {
-                    stats_.bank_hits += 1;
-                    r->bank_hit = true;
+            cands.push_back({q, e, need});
+        }
+        const Candidate* take = nullptr;
+        for (const auto& c : cands)
+            if (c.need == 0 || resident + c.need <= budget_) {
+                take = &c;
+                break;
+            }
+        if (!take)
+            for (const auto& c : cands)
+                if (evict_locked(c.need) || !in_flight) {
+                    take = &c;
+                    break;
                 }
-                e->pins += 1;
-                e->last_use = ++clock_;
-                *entry = e;
-                r->t_start = now_ms();
-                (void)s;
-                return r;
+        if (take) {
+            ServedRequest* r = *take->it;
+            std::shared_ptr<CacheEntry> e = take->e;
+            if (take->it != queue_.begin()) queue_.front()->passed += 1;
+            queue_.erase(take->it);
+            *builder = !e;
+            if (!e) {
+                e = std::make_shared<CacheEntry>();
+                e->key = served_key(r->params);
+                cache_.push_front(e);
+                stats_.bank_misses += 1;
+            } else {
+                for (auto it = cache_.begin(); it != cache_.end(); ++it)
+                    if (*it == e) {
+                        cache_.splice(cache_.begin(), cache_, it);  // most recently used first
+                        break;
+                    }
+                stats_.bank_hits += 1;
+                r->bank_hit = true;
             }
+            e->pins += 1;
+            e->last_use = ++clock_;
+            *entry = e;
+            r->t_start = now_ms();
+            (void)s;
+            return r;
         }
         cv_.wait(lk);
     }
