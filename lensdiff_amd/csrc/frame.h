@@ -71,9 +71,14 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
 // LensDiffCpuReference.cpp:2160-2170): banks[zy * 3 + zx] is the bank of zone (zx, zy); each
 // zone's effect, core and structure planes are blended into the frame's with the zone's tent
 // weights (accumulateWeightedImage, :1452-1475) before the Preserve rescale and composite.
+// Every zone is convolved on zone_grid(w, h, banks), so the frame's spectra are formed once.
+// cached (nullable, kFieldZones entries): each zone's kernel spectra on that grid, as for
+// render_frame.
 void render_frame_zones(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p,
                         const std::vector<const DeviceBank*>& banks, const float4* src, float4* dst, int w, int h,
-                        EffectMode mode);
+                        EffectMode mode, const std::vector<const KernelSpectra*>* cached = nullptr);
+// The convolution grid of a field-phase frame: the largest support of its zone banks.
+Grid zone_grid(int w, int h, const std::vector<const DeviceBank*>& banks);
 
 // The static debug views (BuildLensDiffStaticDebugRgba, LensDiffCpuReference.cpp:1909-1994)
 // at w x h into dst: pupil, phase and phase edge from the pupil of `p`, PSF and field PSF from the

@@ -797,12 +797,15 @@ void select_stage(hipStream_t s, FrameWork& fw, const ldh_params& p, const float
 }
 
 // Steps 2-4 for one bank: the effect planes [effect, core, structure] (3 each; only the effect
-// planes unless the look is split) into effect9 (npx * 9 floats).
+// planes unless the look is split) into effect9 (npx * 9 floats). shared_grid (nullable): the
+// grid to convolve on instead of the bank's own (the zones of a field-dependent phase share one);
+// image_ready: fw.image_spec already holds the frame's spectra on that grid.
 void effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
-                  const float4* src, int w, int h, EffectMode mode, const KernelSpectra* cached, float* effect9) {
+                  const float4* src, int w, int h, EffectMode mode, const KernelSpectra* cached, float* effect9,
+                  const Grid* shared_grid = nullptr, bool image_ready = false) {
     const Plan pl = plan_of(p);
     const int64_t npx = int64_t(w) * h;
-    const Grid g = make_grid(w, h, bank.support);
+    const Grid g = shared_grid != nullptr ? *shared_grid : make_grid(w, h, bank.support);
     const size_t cells = g.cells();
     const float2* tw = t.twiddle.as<float2>();
     const int n = bank.bins;
@@ -816,7 +819,7 @@ void effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
     // 2. image spectra: mono -> (R + iG), (B); spectral -> driver
     const int nimg = pl.spectral ? 1 : 2;
     fw.image_spec.reset(sizeof(float2) * cells * nimg);
-    for (int j = 0; j < nimg; ++j) {
+    for (int j = 0; j < nimg && !image_ready; ++j) {
         float2* spec = fw.image_spec.as<float2>() + cells * j;
         FrameRows op{src, fw.mask.as<float>(), spec, w, h, g.pw, p.input_transfer, p.extraction_mode,
                      pl.spectral ? 1 : 0, j == 0 ? 0 : 2, j == 0 ? 1 : -1, redist_scale};
@@ -1144,7 +1147,7 @@ namespace {
 // with zone banks, of every zone blended into fw.effect.
 void effect_planes(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank* bank,
                    const std::vector<const DeviceBank*>* zones, const float4* src, int w, int h, EffectMode mode,
-                   const KernelSpectra* cached) {
+                   const KernelSpectra* cached, const std::vector<const KernelSpectra*>* zone_cached) {
     const int64_t npx = int64_t(w) * h;
     select_stage(s, fw, p, src, npx);
     fw.effect.reset(sizeof(float) * npx * 9);
@@ -1158,8 +1161,11 @@ void effect_planes(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_p
     hipLaunchKernelGGL(zero_kernel, dim3(unsigned(std::min<int64_t>(div_up(npx * planes, 256), 4096))), dim3(256), 0, s,
                        fw.effect.as<float>(), npx * planes);
     LDH_CHECK_LAUNCH("zone-clear");
+    // one grid for every zone (the largest zone support): the frame's spectra are formed once
+    const Grid gz = zone_grid(w, h, *zones);
     for (int z = 0; z < kFieldZones; ++z) {
-        effect_stage(s, t, fw, p, *(*zones)[size_t(z)], src, w, h, mode, nullptr, fw.zone.as<float>());
+        const KernelSpectra* zc = zone_cached != nullptr ? (*zone_cached)[size_t(z)] : nullptr;
+        effect_stage(s, t, fw, p, *(*zones)[size_t(z)], src, w, h, mode, zc, fw.zone.as<float>(), &gz, z > 0);
         ProfileScope ps_("frame-zone-blend", 12.0 * planes * double(npx));
         hipLaunchKernelGGL(zone_accumulate_kernel, grid1(npx), dim3(256), 0, s, fw.zone.as<float>(),
                            fw.effect.as<float>(), planes, w, h, z % 3, z / 3);
@@ -1172,10 +1178,11 @@ void effect_planes(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_p
 // working size and composites its resampled images at the frame's size.
 void render_impl(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank* bank,
                  const std::vector<const DeviceBank*>* zones, const float4* src, float4* dst, int w, int h,
-                 EffectMode mode, const KernelSpectra* cached) {
+                 EffectMode mode, const KernelSpectra* cached,
+                 const std::vector<const KernelSpectra*>* zone_cached = nullptr) {
     const double ws = working_scale(p);
     if (!resolution_aware_active(p)) {
-        effect_planes(s, t, fw, p, bank, zones, src, w, h, mode, cached);
+        effect_planes(s, t, fw, p, bank, zones, src, w, h, mode, cached, zone_cached);
         finish_stage(s, t, fw, p, src, dst, w, h, nullptr);
         return;
     }
@@ -1187,7 +1194,7 @@ void render_impl(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_par
     LDH_CHECK_LAUNCH("working-resample");
     ldh_params lin = p;  // the working frame is already decoded
     lin.input_transfer = LDH_TRANSFER_LINEAR;
-    effect_planes(s, t, fw, lin, bank, zones, fw.work.as<float4>(), ww, wh, mode, cached);
+    effect_planes(s, t, fw, lin, bank, zones, fw.work.as<float4>(), ww, wh, mode, cached, zone_cached);
     const Display d{src, w, h, p.input_transfer};
     finish_stage(s, t, fw, lin, fw.work.as<float4>(), dst, ww, wh, &d);
 }
@@ -1201,9 +1208,16 @@ void render_frame(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
 
 void render_frame_zones(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p,
                         const std::vector<const DeviceBank*>& banks, const float4* src, float4* dst, int w, int h,
-                        EffectMode mode) {
+                        EffectMode mode, const std::vector<const KernelSpectra*>* cached) {
     if (banks.size() != size_t(kFieldZones)) fail(LDH_ERR_INVALID_ARGUMENT, "field-zone-bank-count");
-    render_impl(s, t, fw, p, nullptr, &banks, src, dst, w, h, mode, nullptr);
+    if (cached != nullptr && cached->size() != size_t(kFieldZones)) fail(LDH_ERR_INVALID_ARGUMENT, "field-zone-spectra-count");
+    render_impl(s, t, fw, p, nullptr, &banks, src, dst, w, h, mode, nullptr, cached);
+}
+
+Grid zone_grid(int w, int h, const std::vector<const DeviceBank*>& banks) {
+    int support = 0;
+    for (const DeviceBank* b : banks) support = std::max(support, b->support);
+    return make_grid(w, h, support);
 }
 
 bool is_static_view(int view) {

@@ -46,13 +46,16 @@ struct CacheEntry {
     std::shared_ptr<DeviceBank> bank;
     // the nine zone banks of a field-dependent phase, zone (zx, zy) at zy * 3 + zx (bank unset)
     std::vector<std::shared_ptr<DeviceBank>> zones;
-    DeviceBuffer combined;  // precombined natural (+ style) kernels, float[6][K*K]
+    // precombined natural (+ style) kernels, float[6][K*K] (per zone, zone z at z * 6 * K * K)
+    DeviceBuffer combined;
+    bool has_combined = false;
     bool combined_style = false;
     // Kernel spectra for one FFT grid. The record is shared with the requests waiting for it, so
     // a failed build can hand its error to them after it has left the entry.
     struct Spectra {
         SpectraKey key;
         KernelSpectra spec;
+        std::vector<KernelSpectra> zones;  // per zone, for a field-dependent phase (spec unused)
         bool ready = false;
         bool failed = false;
         std::string error;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <functional>
 
 #include "server.h"
 
@@ -24,10 +25,10 @@ bool spectral_fullkernel(const ldh_params& p) {
 bool with_style(const ldh_params& p) {
     return p.spectrum_style != LDH_STYLE_NATURAL && std::max(0.0, std::min(p.spectrum_force, 1.0)) > 0.0;
 }
-// Real kernels whose spectra serve this request: precombined RGB (3, or 6 with style) for a
-// spectral bank, the single full kernel for a mono bank; 0 = not cacheable (split look).
+// Real kernels whose spectra serve this request (per zone with a field-dependent phase):
+// precombined RGB (3, or 6 with style) for a spectral bank, the single full kernel for a mono
+// bank; 0 = not cacheable (split look).
 int cached_kernel_count(const ldh_params& p) {
-    if (has_field_phase(p)) return 0;  // nine zone convolutions, per bin
     if (spectral_fullkernel(p)) return with_style(p) ? 6 : 3;
     if (!is_spectral(p) && p.look_mode == LDH_LOOK_PHYSICAL && p.debug_view != LDH_VIEW_CORE &&
         p.debug_view != LDH_VIEW_STRUCTURE)
@@ -77,6 +78,17 @@ void served_frame_size(const ldh_params& p, int w, int h, int& gw, int& gh) {
 size_t spectra_bytes(const Grid& g, int kernels, bool fp16) {
     return g.cells() * size_t((kernels + 1) / 2) * (fp16 ? 4 : 8);
 }
+// Kernel sets whose spectra a request keeps: one per zone with a field-dependent phase.
+int spectra_sets(const ldh_params& p) { return has_field_phase(p) ? kFieldZones : 1; }
+// The support a request's grid is sized for: its bank's, its zones' largest, or before they are
+// built the kernel radius bound.
+int entry_support(const CacheEntry* e, const ldh_params& p) {
+    if (e == nullptr || !e->ready) return resolve_sizes(p).max_radius;
+    if (e->bank) return e->bank->support;
+    int support = 0;
+    for (const auto& z : e->zones) support = std::max(support, z->support);
+    return support;
+}
 }  // namespace
 
 size_t CacheEntry::bytes() const {
@@ -84,7 +96,10 @@ size_t CacheEntry::bytes() const {
     for (const auto& z : zones)
         if (z) b += z->bytes();
     for (const auto& s : spectra)
-        if (s->ready) b += s->spec.bytes();  // a record in the making is written without the lock
+        if (s->ready) {  // a record in the making is written without the lock
+            b += s->spec.bytes();
+            for (const auto& z : s->zones) b += z.bytes();
+        }
     return b;
 }
 
@@ -251,8 +266,8 @@ ServedRequest* Server::admit(Slot* s, std::shared_ptr<CacheEntry>* entry, bool* 
             if (e && !e->ready) continue;  // its bank is being built
             size_t need = e ? 0 : bank_bytes_estimate(r->params);
             const int nk = cached_kernel_count(r->params);
-            if (nk > 0 && (!e || nk == 1 || e->combined_style == with_style(r->params))) {
-                const int support = e ? e->bank->support : resolve_sizes(r->params).max_radius;
+            if (nk > 0 && (!e || nk == 1 || (e->has_combined && e->combined_style == with_style(r->params)))) {
+                const int support = entry_support(e.get(), r->params);
                 int gw = 0, gh = 0;
                 served_frame_size(r->params, r->w, r->h, gw, gh);
                 const Grid g = make_grid(gw, gh, support);
@@ -262,7 +277,7 @@ ServedRequest* Server::admit(Slot* s, std::shared_ptr<CacheEntry>* entry, bool* 
                     for (const auto& sp : e->spectra)
                         if (sp->key == sk) spec = sp;
                 if (spec && !spec->ready) continue;  // its spectra are being built
-                if (!spec) need += spectra_bytes(g, nk, r->params.spectra_fp16 != 0);
+                if (!spec) need += spectra_bytes(g, nk, r->params.spectra_fp16 != 0) * size_t(spectra_sets(r->params));
             }
             cands.push_back({q, e, need});
         }
@@ -332,14 +347,23 @@ void Server::worker(Slot* s) {
                                                     nullptr);
                     else
                         bank = std::shared_ptr<DeviceBank>(build_bank(s->stream, fft_, s->bw, r->params).release());
-                    if (bank && spectral_fullkernel(r->params)) {
+                    if (spectral_fullkernel(r->params)) {
+                        // the precombined RGB kernels of the bank, or of each zone (zone z at
+                        // z * 6 * K * K; the zones share their kernel size)
+                        std::vector<const DeviceBank*> sets;
+                        if (bank) sets.push_back(bank.get());
+                        for (const auto& z : zones) sets.push_back(z.get());
                         const bool sty = with_style(r->params);
-                        const SpectrumConfig cfg = build_spectrum_config(r->params, bank->wavelengths);
-                        e->combined.reset(sizeof(float) * size_t(bank->kernel) * bank->kernel * 6);
-                        combine_kernels(s->stream, bank->kernel_ptr(LDH_KERNEL_FULL, 0),
-                                        bank->scale_ptr(LDH_KERNEL_FULL, 0), bank->bins, bank->kernel, cfg, sty,
-                                        e->combined.as<float>(), s->tmp);
+                        const size_t kk6 = size_t(sets.front()->kernel) * sets.front()->kernel * 6;
+                        e->combined.reset(sizeof(float) * kk6 * sets.size());
+                        for (size_t k = 0; k < sets.size(); ++k) {
+                            const DeviceBank& b = *sets[k];
+                            const SpectrumConfig cfg = build_spectrum_config(r->params, b.wavelengths);
+                            combine_kernels(s->stream, b.kernel_ptr(LDH_KERNEL_FULL, 0), b.scale_ptr(LDH_KERNEL_FULL, 0),
+                                            b.bins, b.kernel, cfg, sty, e->combined.as<float>() + kk6 * k, s->tmp);
+                        }
                         e->combined_style = sty;
+                        e->has_combined = true;
                     }
                     LDH_HIP(hipStreamSynchronize(s->stream), "served-bank");
                     std::lock_guard<std::mutex> lk(mu_);
@@ -360,6 +384,58 @@ void Server::worker(Slot* s) {
                 cv_.wait(lk, [&] { return e->ready || e->failed || stopping_; });
                 if (!e->ready) throw Error{LDH_ERR_DEVICE, e->failed ? e->error : "server-stopping"};
             }
+            // The spectra record of this entry for key sk: found (waited for until it is ready) or
+            // created and built here by `build`.
+            auto spectra_for = [&](const SpectraKey& sk,
+                                   const std::function<void(CacheEntry::Spectra&)>& build) {
+                std::shared_ptr<CacheEntry::Spectra> spec;
+                bool mine = false;
+                {
+                    std::unique_lock<std::mutex> lk(mu_);
+                    for (auto& sp : e->spectra)
+                        if (sp->key == sk) spec = sp;
+                    if (!spec) {
+                        spec = std::make_shared<CacheEntry::Spectra>();
+                        spec->key = sk;
+                        e->spectra.push_back(spec);
+                        mine = true;
+                        stats_.spectra_misses += 1;
+                    } else {
+                        stats_.spectra_hits += 1;
+                        r->spectra_hit = true;
+                        cv_.wait(lk, [&] { return spec->ready || spec->failed || stopping_; });
+                        if (!spec->ready) throw Error{LDH_ERR_DEVICE, spec->failed ? spec->error : "server-stopping"};
+                    }
+                }
+                if (mine) {
+                    try {
+                        debug_failure_point("served-spectra");
+                        build(*spec);
+                        LDH_HIP(hipStreamSynchronize(s->stream), "served-spectra");
+                    } catch (const Error& err) {
+                        // the record leaves the entry (a later request builds it again); the
+                        // requests already waiting on it take this error
+                        std::lock_guard<std::mutex> lk(mu_);
+                        spec->failed = true;
+                        spec->error = err.tag;
+                        e->spectra.erase(std::remove(e->spectra.begin(), e->spectra.end(), spec), e->spectra.end());
+                        cv_.notify_all();
+                        throw;
+                    }
+                    std::lock_guard<std::mutex> lk(mu_);
+                    spec->ready = true;
+                    cv_.notify_all();
+                }
+                return spec;
+            };
+            const int nk = cached_kernel_count(r->params);
+            // precombined spectra need the entry's combined kernels, which exist when the request
+            // that built it had a spectral full-kernel look with the same style
+            const bool cacheable = nk > 0 && (nk == 1 || (e->has_combined && e->combined_style == with_style(r->params)));
+            const bool fp16 = r->params.spectra_fp16 != 0;
+            // a resolution-aware frame is convolved at its working size
+            int gw = 0, gh = 0;
+            served_frame_size(r->params, r->w, r->h, gw, gh);
             if (!e->zones.empty()) {
                 if (is_static_view(r->params.debug_view)) {
                     // zone (1, 1) holds the global terms; the edge views show zone (2, 2)
@@ -375,68 +451,50 @@ void Server::worker(Slot* s) {
                     ldh_params p = r->params;
                     const bool pre = spectral_fullkernel(p);
                     p.conv_mode = pre ? LDH_CONV_PRECOMBINED : LDH_CONV_PER_BIN;
+                    std::shared_ptr<CacheEntry::Spectra> spec;
+                    std::vector<const KernelSpectra*> zc;
+                    if (cacheable) {
+                        // every zone's kernel spectra on the zones' common grid
+                        const Grid g = zone_grid(gw, gh, zb);
+                        spec = spectra_for(SpectraKey{g.pw, g.ph, nk, fp16 ? 1 : 0}, [&](CacheEntry::Spectra& sp) {
+                            sp.zones.resize(size_t(kFieldZones));
+                            for (int z = 0; z < kFieldZones; ++z) {
+                                const DeviceBank& zbank = *zb[size_t(z)];
+                                const size_t kk = size_t(zbank.kernel) * zbank.kernel;
+                                std::vector<KernelRef> ks;
+                                if (nk == 1) {
+                                    ks.push_back({zbank.kernel_ptr(LDH_KERNEL_FULL, 0), zbank.scale_ptr(LDH_KERNEL_FULL, 0)});
+                                } else {
+                                    const float* comb = e->combined.as<float>() + kk * 6 * size_t(z);
+                                    for (int i = 0; i < nk; ++i) ks.push_back({comb + kk * i, nullptr});
+                                }
+                                build_kernel_spectra(s->stream, fft_, ks, zbank.kernel, zbank.support, g, fp16,
+                                                     sp.zones[size_t(z)], s->tmp);
+                            }
+                        });
+                        for (const auto& z : spec->zones) zc.push_back(&z);
+                    }
                     render_frame_zones(s->stream, fft_, s->fw, p, zb, r->src, r->dst, r->w, r->h,
-                                       pre ? EffectMode::Precombined : EffectMode::PerBin);
+                                       pre ? EffectMode::Precombined : EffectMode::PerBin, spec ? &zc : nullptr);
                 }
             } else if (is_static_view(r->params.debug_view)) {
                 // drawn from the resident bank or the pupil
                 render_static_view(s->stream, s->fw, r->params, r->params.debug_view, *e->bank, r->dst, r->w, r->h);
             } else {
                 const DeviceBank& bank = *e->bank;
-                const int nk = cached_kernel_count(r->params);
                 std::shared_ptr<CacheEntry::Spectra> spec;
-                if (nk > 0 && (nk == 1 || e->combined_style == with_style(r->params))) {
-                    // a resolution-aware frame is convolved at its working size
-                    int gw = 0, gh = 0;
-                    served_frame_size(r->params, r->w, r->h, gw, gh);
+                if (cacheable) {
                     const Grid g = make_grid(gw, gh, bank.support);
-                    const SpectraKey sk{g.pw, g.ph, nk, r->params.spectra_fp16 != 0 ? 1 : 0};
-                    bool mine = false;
-                    {
-                        std::unique_lock<std::mutex> lk(mu_);
-                        for (auto& sp : e->spectra)
-                            if (sp->key == sk) spec = sp;
-                        if (!spec) {
-                            spec = std::make_shared<CacheEntry::Spectra>();
-                            spec->key = sk;
-                            e->spectra.push_back(spec);
-                            mine = true;
-                            stats_.spectra_misses += 1;
+                    spec = spectra_for(SpectraKey{g.pw, g.ph, nk, fp16 ? 1 : 0}, [&](CacheEntry::Spectra& sp) {
+                        std::vector<KernelRef> ks;
+                        if (nk == 1) {
+                            ks.push_back({bank.kernel_ptr(LDH_KERNEL_FULL, 0), bank.scale_ptr(LDH_KERNEL_FULL, 0)});
                         } else {
-                            stats_.spectra_hits += 1;
-                            r->spectra_hit = true;
-                            cv_.wait(lk, [&] { return spec->ready || spec->failed || stopping_; });
-                            if (!spec->ready)
-                                throw Error{LDH_ERR_DEVICE, spec->failed ? spec->error : "server-stopping"};
+                            const size_t kk = size_t(bank.kernel) * bank.kernel;
+                            for (int i = 0; i < nk; ++i) ks.push_back({e->combined.as<float>() + kk * i, nullptr});
                         }
-                    }
-                    if (mine) {
-                        try {
-                            debug_failure_point("served-spectra");
-                            std::vector<KernelRef> ks;
-                            if (nk == 1) {
-                                ks.push_back({bank.kernel_ptr(LDH_KERNEL_FULL, 0), bank.scale_ptr(LDH_KERNEL_FULL, 0)});
-                            } else {
-                                const size_t kk = size_t(bank.kernel) * bank.kernel;
-                                for (int i = 0; i < nk; ++i) ks.push_back({e->combined.as<float>() + kk * i, nullptr});
-                            }
-                            build_kernel_spectra(s->stream, fft_, ks, bank.kernel, bank.support, g, sk.fp16 != 0,
-                                                 spec->spec, s->tmp);
-                            LDH_HIP(hipStreamSynchronize(s->stream), "served-spectra");
-                        } catch (const Error& err) {
-                            // the record leaves the entry (a later request builds it again); the
-                            // requests already waiting on it take this error
-                            std::lock_guard<std::mutex> lk(mu_);
-                            spec->failed = true;
-                            spec->error = err.tag;
-                            e->spectra.erase(std::remove(e->spectra.begin(), e->spectra.end(), spec), e->spectra.end());
-                            cv_.notify_all();
-                            throw;
-                        }
-                        std::lock_guard<std::mutex> lk(mu_);
-                        spec->ready = true;
-                        cv_.notify_all();
-                    }
+                        build_kernel_spectra(s->stream, fft_, ks, bank.kernel, bank.support, g, fp16, sp.spec, s->tmp);
+                    });
                 }
                 ldh_params p = r->params;
                 if (spec && nk > 1) {

@@ -164,3 +164,38 @@ def test_served_field_zones_match_plain(ctx):
         ctx.server_stop()
         ctx.free(s)
         ctx.free(d)
+
+
+def test_served_field_zone_spectra_cached(ctx):
+    """The nine zones' kernel spectra of a field-dependent phase are kept with the entry (one set per
+    grid): a repeated request reuses them, and every result equals the plain render."""
+    img = workloads.frame(96, 64, 0.01, seed=9)
+    lenses = []
+    for spectral in (1, 0):
+        p = workloads.base_params(64)
+        p.pupil_resolution = 64
+        p.max_kernel_radius_px = 20.0
+        p.spectral_mode = spectral
+        p.set_noll({4: 0.2, 8: -0.1})
+        p.phase_field_strength, p.phase_field_edge_bias = 1.0, 0.3
+        p.phase_field_coma_radial, p.phase_field_defocus = 0.3, 0.15
+        lenses.append(p)
+    plain = [ctx.render_image(p, img) for p in lenses]
+    s = ctx.alloc(img.nbytes)
+    d = ctx.alloc(img.nbytes)
+    ctx.upload(s, img)
+    ctx.server_start(1, 1 << 30)
+    try:
+        for p, want in zip(lenses, plain):
+            for k in range(3):
+                st = ctx.wait(ctx.submit(p, s, d, 96, 64))
+                assert st.spectra_hit == (k > 0)
+                got = np.empty_like(img)
+                ctx.download(d, got)
+                assert rel_l2(got, want) <= 1e-5, (p.spectral_mode, k)
+        stats = ctx.server_stats()
+        assert stats.spectra_misses == 2 and stats.spectra_hits == 4
+    finally:
+        ctx.server_stop()
+        ctx.free(s)
+        ctx.free(d)
