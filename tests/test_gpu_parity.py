@@ -133,6 +133,24 @@ def test_raw_psf(ctx, oracle, pupil, raw):
     assert abs(gs - ws) / ws < 1e-5, (gs, ws)
 
 
+# The window form: only the centred (2h+1)^2 block, each line a chirp-z transform onto the 2h+1
+# frequencies about DC with the same pi-in-float kernel; unnormalised, so compared after scaling
+# both to unit sum over the block. h = 208 is the c2 bank's window.
+@pytest.mark.parametrize("pupil,raw,h", [(64, 256, 20), (96, 1020, 60), (300, 2200, 90), (1024, 4092, 208)])
+def test_raw_psf_window(ctx, oracle, pupil, raw, h):
+    p = params(pupil_resolution=pupil, aperture_mode=1, blade_count=7, roundness=0.15)
+    p.set_noll({4: 0.25, 6: -0.1, 11: 0.15})
+    amp = oracle.pupil_amplitude(p)
+    ph = oracle.pupil_phase(p)
+    got = ctx.raw_psf_window(amp, ph, raw, h).astype(np.float64)
+    full = oracle.raw_psf(amp, ph, raw)
+    c = raw // 2
+    want = full[c - h:c + h + 1, c - h:c + h + 1].astype(np.float64)
+    err = rel_l2(got / got.sum(), want / want.sum())
+    print("window P=%d R=%d h=%d: rel L2 %.3g" % (pupil, raw, h, err))
+    assert err < TOL
+
+
 # every autotuner variant of the line kernels (radix 16 or 8 after the first stage, with the
 # first stage pruned to the pupil's extent) on lines long enough to be tuned
 @pytest.mark.parametrize("pupil,raw", [(300, 2200), (1024, 4092)])
