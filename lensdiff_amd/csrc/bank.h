@@ -68,7 +68,6 @@ struct BankWorkspace {
     // The plan for (size, center, rmax, ceil), built on first use (a host step: call
     // reserve_shaping before a graph capture).
     const RingPlan& ring_plan(hipStream_t s, int size, float center, int rmax, bool ceil);
-    DeviceBuffer block_sums;  // producers' block sums handed to the sequential sums, [3][count][blocks]
     // first-zero radius of the clear-circle PSF per (P, R) (GetLensDiffReferenceRawPsfCached)
     std::vector<std::pair<std::pair<int, int>, float>> first_zero;
     // Grows the shaping buffers for `count` kernels of side K, and builds their ring plans, ahead
@@ -80,11 +79,12 @@ float first_minimum_from_profile(const std::vector<float>& profile, int size);
 
 // Steps of the build, exposed for the parity entry points.
 // d_raw_totals (nullable): per-PSF sums of raw PSFs that are not yet normalised.
-// block_sums (nullable): FP64 sums of the output's kSeqSumBlock-texel blocks, [count][blocks].
+// place (nullable): the output's blocks are placed for a sequential sum (SeqSum::place), whose
+// walk the caller then runs.
 // window > 0: the raw PSFs are held as their centred (2 window + 1)^2 blocks
 // (PsfPlan::window_psfs), else whole (R x R).
 void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_scales, const float* d_raw_totals,
-                      int support, int count, float* out, double* block_sums = nullptr, int window = 0);
+                      int support, int count, float* out, const SeqPlace* place = nullptr, int window = 0);
 // resample_kernels, then the sequential sums of the base kernels into totals[count].
 void base_kernels(hipStream_t s, BankWorkspace& w, const float* raw, int R, const float* d_scales,
                   const float* d_raw_totals, int support, int count, float* out, float* totals, int window = 0);

@@ -45,27 +45,39 @@ __device__ __forceinline__ float bilinear_sq(const float* __restrict__ img, int 
     return a + (b - a) * ty;
 }
 
+// Elements per thread of a block; each thread's element j is block element j * kSumThreads + tid.
+// Kernels that gather (ring means) or read several arrays per texel (crop + taper) issue all of
+// a thread's loads before using them (K*K < 2^31).
+constexpr int kPerSum = kSeqSumBlock / kSumThreads;
+
 // Workgroups of kSumThreads threads over blocks of kSeqSumBlock texels (seqsum_blocks(K*K) per
-// kernel); block_sums (nullable): their FP64 sums, [lam][block]. The raw PSFs are R x R images
-// held whole (org 0, side R) or as their centred side x side windows from (org, org).
+// kernel); with an active placement (pl.bsum set) each workgroup places its block for the
+// sequential sum of the output (segment lam). The raw PSFs are R x R images held whole (org 0,
+// side R) or as their centred side x side windows from (org, org).
 __global__ void __launch_bounds__(kSumThreads)
 resample_kernel(const float* __restrict__ raw, int R, int org, int side, const float* __restrict__ scales,
-                const float* __restrict__ raw_totals, int support, float* __restrict__ out,
-                double* __restrict__ block_sums) {
-    const int lam = blockIdx.y;
-    // raw_totals: the raw PSFs are not yet normalised; texels are scaled by 1/total
-    const float norm = inverse_total(raw_totals, lam);
+                const float* __restrict__ raw_totals, int support, int count, float* __restrict__ out, SeqPlace pl) {
     const int K = 2 * support + 1;
     const int kk = K * K;
+    SegBlock sb;
+    if (!place_block_of(seqsum_blocks(kk), count, sb)) return;
+    const int lam = sb.seg;
+    // raw_totals: the raw PSFs are not yet normalised; texels are scaled by 1/total
+    const float norm = inverse_total(raw_totals, lam);
     const float* img = raw + size_t(lam) * side * side;
     const float rc = (R & 1) == 0 ? float(R) * 0.5f : float(R - 1) * 0.5f;
     const float kc = float(K - 1) * 0.5f;
     const float inv = 1.0f / fmaxf(scales[lam], 0.05f);
     const int taps = min(max(int(ceilf(inv)), 2), 6);
     const float step = inv / float(taps);
-    const int start = blockIdx.x * kSeqSumBlock, end = min(kk, start + kSeqSumBlock);
+    const int start = sb.block * kSeqSumBlock, end = min(kk, start + kSeqSumBlock);
     double share[1] = {0.0};
-    for (int i = start + threadIdx.x; i < end; i += kSumThreads) {
+    float vals[1][kPerSum];
+#pragma unroll
+    for (int j = 0; j < kPerSum; ++j) {
+        const int i = start + int(threadIdx.x) + j * kSumThreads;
+        vals[0][j] = 0.0f;
+        if (i >= end) continue;
         const int x = i % K, y = i / K;
         const float sx = rc + (float(x) - kc) * inv;
         const float sy = rc + (float(y) - kc) * inv;
@@ -84,11 +96,11 @@ resample_kernel(const float* __restrict__ raw, int R, int org, int side, const f
             v = sum / float(taps * taps);
         }
         out[size_t(lam) * kk + i] = v;
+        vals[0][j] = v;
         share[0] += double(v);
     }
-    if (block_sums == nullptr) return;
-    double* const dst[1] = {block_sums + size_t(lam) * gridDim.x + blockIdx.x};
-    store_block_sums<1>(share, dst);
+    if (pl.bsum == nullptr) return;
+    place_blocks<1, kPerSum>(pl, lam, 0, sb.block, vals, share);
 }
 
 // Ring index of pixel (x, y) about centre c: round(|d|) (azimuthal mean, radial profile) or
@@ -103,17 +115,8 @@ __device__ __forceinline__ int ring_of(int x, int y, float c, int rmax) {
 }
 
 // The elementwise shaping kernels below run workgroups of kSumThreads threads over blocks of
-// kSeqSumBlock elements of one kernel (grid: seqsum_blocks(K*K) x kernels) and write the FP64
-// sums of the blocks of each array they produce, [array][kernel][block].
-__device__ __forceinline__ int64_t block_start() { return int64_t(blockIdx.x) * kSeqSumBlock; }
-__device__ __forceinline__ double* block_sum_slot(double* sums, int array, int kernels) {
-    return sums + (size_t(array) * kernels + blockIdx.y) * gridDim.x + blockIdx.x;
-}
-
-// Elements per thread of a block; each thread's element j is block element j * kSumThreads + tid.
-// Kernels that gather (ring means) or read several arrays per texel (crop + taper) issue all of
-// a thread's loads before using them (K*K < 2^31).
-constexpr int kPerSum = kSeqSumBlock / kSumThreads;
+// kSeqSumBlock elements of one kernel (grid: place_grid(seqsum_blocks(K*K), kernels)) and place
+// the blocks of each array they produce for its sequential sum.
 
 // Advances (x, y) by kSumThreads texels along the rows of a K-wide image.
 __device__ __forceinline__ void advance_texel(int& x, int& y, int K) {
@@ -135,12 +138,14 @@ __global__ void ring_means_kernel(const float* __restrict__ sums, const int* __r
 
 // The azimuthal-mean image: each texel takes its ring's mean (the ring means precomputed once).
 __global__ void __launch_bounds__(kSumThreads)
-ring_mean_kernel(const float* __restrict__ means, int K, int rmax, float* __restrict__ out,
-                 double* __restrict__ block_sums) {
-    const int seg = blockIdx.y;
+ring_mean_kernel(const float* __restrict__ means, int K, int rmax, int count, float* __restrict__ out, SeqPlace pl) {
+    const int kk = K * K;
+    SegBlock sb;
+    if (!place_block_of(seqsum_blocks(kk), count, sb)) return;
+    const int seg = sb.seg;
     const float c = float(K - 1) * 0.5f;
     const int nr = rmax + 1;
-    const int kk = K * K, start = int(block_start()), end = min(kk, start + kSeqSumBlock);
+    const int start = sb.block * kSeqSumBlock, end = min(kk, start + kSeqSumBlock);
     const int i0 = start + int(threadIdx.x);
     int x = i0 % K, y = i0 / K;
     float mv[kPerSum];
@@ -150,60 +155,76 @@ ring_mean_kernel(const float* __restrict__ means, int K, int rmax, float* __rest
         advance_texel(x, y, K);
     }
     double share[1] = {0.0};
+    float vals[1][kPerSum];
 #pragma unroll
     for (int j = 0; j < kPerSum; ++j) {
         const int i = i0 + j * kSumThreads;
-        if (i >= end) break;
+        vals[0][j] = i < end ? mv[j] : 0.0f;
+        if (i >= end) continue;
         out[int64_t(seg) * kk + i] = mv[j];
         share[0] += double(mv[j]);
     }
-    double* const dst[1] = {block_sum_slot(block_sums, 0, gridDim.y)};
-    store_block_sums<1>(share, dst);
+    place_blocks<1, kPerSum>(pl, seg, 0, sb.block, vals, share);
 }
 
 // reshapeAnisotropy on the normalised base and mean (each read through its totals).
 __global__ void __launch_bounds__(kSumThreads)
 reshape_kernel(const float* __restrict__ base, const float* __restrict__ base_tot, const float* __restrict__ mean,
                const float* __restrict__ mean_tot, float gain0, const float* __restrict__ gains, int64_t kk,
-               float* __restrict__ shaped, double* __restrict__ block_sums) {
-    const int seg = blockIdx.y;
+               int count, float* __restrict__ shaped, SeqPlace pl) {
+    SegBlock sb;
+    if (!place_block_of(seqsum_blocks(kk), count, sb)) return;
+    const int seg = sb.seg;
     const float gain = gains != nullptr ? gains[seg] : gain0;
     const float inv_m = inverse_total(mean_tot, seg), inv_b = inverse_total(base_tot, seg);
-    const int64_t start = block_start(), end = min(kk, start + kSeqSumBlock);
+    const int64_t start = int64_t(sb.block) * kSeqSumBlock, end = min(kk, start + kSeqSumBlock);
     double share[1] = {0.0};
-    for (int64_t i = start + threadIdx.x; i < end; i += kSumThreads) {
+    float vals[1][kPerSum];
+#pragma unroll
+    for (int j = 0; j < kPerSum; ++j) {
+        const int64_t i = start + threadIdx.x + int64_t(j) * kSumThreads;
+        vals[0][j] = 0.0f;
+        if (i >= end) continue;
         const int64_t o = int64_t(seg) * kk + i;
         const float m = mean[o] * inv_m;
         const float b = base[o] * inv_b;
         const float v = fmaxf(0.0f, m + (b - m) * gain);
         shaped[o] = v;
+        vals[0][j] = v;
         share[0] += double(v);
     }
-    double* const dst[1] = {block_sum_slot(block_sums, 0, gridDim.y)};
-    store_block_sums<1>(share, dst);
+    place_blocks<1, kPerSum>(pl, seg, 0, sb.block, vals, share);
 }
 
 // positiveResidualKernel of the normalised shaped and mean kernels.
-// Also sums the normalised shaped kernel (array 1 of the block sums), the support estimate's
-// total.
+// Also places the normalised shaped kernel (segment count + seg of the sum), the support
+// estimate's total.
 __global__ void __launch_bounds__(kSumThreads)
 residual_kernel(const float* __restrict__ shaped, const float* __restrict__ shaped_tot, const float* __restrict__ mean,
-                const float* __restrict__ mean_tot, int64_t kk, float* __restrict__ out,
-                double* __restrict__ block_sums) {
-    const int seg = blockIdx.y;
+                const float* __restrict__ mean_tot, int64_t kk, int count, float* __restrict__ out, SeqPlace pl) {
+    SegBlock sb;
+    if (!place_block_of(seqsum_blocks(kk), count, sb)) return;
+    const int seg = sb.seg;
     const float inv_s = inverse_total(shaped_tot, seg), inv_m = inverse_total(mean_tot, seg);
-    const int64_t start = block_start(), end = min(kk, start + kSeqSumBlock);
+    const int64_t start = int64_t(sb.block) * kSeqSumBlock, end = min(kk, start + kSeqSumBlock);
     double share[2] = {0.0, 0.0};
-    for (int64_t i = start + threadIdx.x; i < end; i += kSumThreads) {
+    float vals[2][kPerSum];
+#pragma unroll
+    for (int j = 0; j < kPerSum; ++j) {
+        const int64_t i = start + threadIdx.x + int64_t(j) * kSumThreads;
+        vals[0][j] = 0.0f;
+        vals[1][j] = 0.0f;
+        if (i >= end) continue;
         const int64_t o = int64_t(seg) * kk + i;
         const float sn = shaped[o] * inv_s;
         const float v = fmaxf(0.0f, sn - mean[o] * inv_m);
         out[o] = v;
+        vals[0][j] = v;
+        vals[1][j] = sn;
         share[0] += double(v);
         share[1] += double(sn);
     }
-    double* const dst[2] = {block_sum_slot(block_sums, 0, gridDim.y), block_sum_slot(block_sums, 1, gridDim.y)};
-    store_block_sums<2>(share, dst);
+    place_blocks<2, kPerSum>(pl, seg, count, sb.block, vals, share);
 }
 
 // estimateAdaptiveSupportRadius + paddedAdaptiveSupportRadius (LensDiffCpuReference.cpp:984-1036),
@@ -285,12 +306,15 @@ struct TaperSources {
     const float* tot[3];
 };
 __global__ void __launch_bounds__(kSumThreads)
-crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, float* __restrict__ dst,
-                  double* __restrict__ block_sums) {
-    const int bin = blockIdx.y, count = gridDim.y;
+crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, int count, float* __restrict__ dst,
+                  SeqPlace pl) {
+    const int64_t kk = int64_t(K) * K;
+    SegBlock sb;
+    if (!place_block_of(seqsum_blocks(kk), count, sb)) return;
+    const int bin = sb.seg;
     const int r = min(max(radius[bin], 1), K / 2);
     const int c = K / 2;
-    const int64_t kk = int64_t(K) * K, start = block_start(), end = min(kk, start + kSeqSumBlock);
+    const int64_t start = int64_t(sb.block) * kSeqSumBlock, end = min(kk, start + kSeqSumBlock);
     float inv[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) inv[k] = inverse_total(ts.tot[k], bin);
@@ -317,11 +341,14 @@ crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, float*
         }
     }
     double share[3] = {0.0, 0.0, 0.0};
+    float vals[3][kPerSum];
     int x = i0 % K, y = i0 / K;
 #pragma unroll
     for (int j = 0; j < kPerSum; ++j) {
         const int i = i0 + j * kSumThreads;
-        if (i >= end) break;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vals[k][j] = 0.0f;
+        if (i >= end) continue;
         const bool inside = abs(x - c) <= r && abs(y - c) <= r;
         float w = 0.0f;
         if (inside) {
@@ -337,13 +364,12 @@ crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, float*
         for (int k = 0; k < 3; ++k) {
             const float v = inside ? sv[k][j] * inv[k] * w : 0.0f;
             dst[(int64_t(k) * count + bin) * kk + i] = v;
+            vals[k][j] = v;
             share[k] += double(v);
         }
         advance_texel(x, y, K);
     }
-    double* const slots[3] = {block_sum_slot(block_sums, 0, count), block_sum_slot(block_sums, 1, count),
-                              block_sum_slot(block_sums, 2, count)};
-    store_block_sums<3>(share, slots);
+    place_blocks<3, kPerSum>(pl, bin, count, sb.block, vals, share);
 }
 
 // x[seg] *= scales[seg] (segments [3][count] of K*K) over the crop square of the segment's bin
@@ -372,17 +398,27 @@ __global__ void fill_zero_kernel(uint32_t* __restrict__ p, int64_t n) {
         p[i] = 0u;
 }
 
-inline dim3 blocks_grid(int64_t n, int segs) { return dim3(unsigned(seqsum_blocks(n)), unsigned(segs)); }
+inline dim3 place_grid_of(int64_t n, int segs) { return dim3(place_grid(seqsum_blocks(n), segs)); }
+
+// Segments [count] of stride kk in one array, scaled by 1 / prescale[seg] when prescale is given.
+inline SeqSegments one_array(const float* x, int count, int64_t kk, const float* prescale = nullptr) {
+    SeqSegments g;
+    g.array[0] = x;
+    g.prescale[0] = prescale;
+    g.per_array = count;
+    g.stride = kk;
+    return g;
+}
 
 }  // namespace
 
 void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_scales, const float* d_raw_totals,
-                      int support, int count, float* out, double* block_sums, int window) {
+                      int support, int count, float* out, const SeqPlace* place, int window) {
     const int K = 2 * support + 1;
     const int side = window > 0 ? 2 * window + 1 : R, org = window > 0 ? R / 2 - window : 0;
     ProfileScope ps_("resample", 8.0 * double(K) * K * count);
-    hipLaunchKernelGGL(resample_kernel, blocks_grid(int64_t(K) * K, count), dim3(kSumThreads), 0, s, raw, R, org,
-                       side, d_scales, d_raw_totals, support, out, block_sums);
+    hipLaunchKernelGGL(resample_kernel, place_grid_of(int64_t(K) * K, count), dim3(kSumThreads), 0, s, raw, R, org,
+                       side, d_scales, d_raw_totals, support, count, out, place != nullptr ? *place : SeqPlace{});
     LDH_CHECK_LAUNCH("resample");
 }
 
@@ -391,8 +427,9 @@ void base_kernels(hipStream_t s, BankWorkspace& w, const float* raw, int R, cons
     const int K = 2 * support + 1;
     const int64_t kk = int64_t(K) * K;
     w.reserve_shaping(s, K, count, support);
-    resample_kernels(s, raw, R, d_scales, d_raw_totals, support, count, out, w.block_sums.as<double>(), window);
-    w.seqsum.run(s, out, kk, kk, count, totals, w.block_sums.as<double>());
+    const SeqPlace pl = w.seqsum.place(kk, count);
+    resample_kernels(s, raw, R, d_scales, d_raw_totals, support, count, out, &pl, window);
+    w.seqsum.walk(s, one_array(out, count, kk), kk, count, totals);
 }
 
 const RingPlan& BankWorkspace::ring_plan(hipStream_t s, int size, float center, int rmax, bool ceil) {
@@ -424,7 +461,6 @@ void BankWorkspace::reserve_shaping(hipStream_t s, int K, int count, int max_rad
     shape_totals.reset(sizeof(float) * 7 * count);
     const int nr = std::max(K / 2, max_radius) + 1;
     ring_sum.reset(sizeof(float) * nr * count);
-    block_sums.reset(sizeof(double) * size_t(seqsum_blocks(kk)) * 3 * count);
     ring_peak.reset(sizeof(unsigned) * nr * count);
     ring_mean.reset(sizeof(float) * nr * count);
     seqsum.reserve(kk, 3 * count);
@@ -457,43 +493,44 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
     // gives the values an in-place normalisation would have left.
     // azimuthal mean of the normalised base
     ring_sums(s, mean_rings, w.ring_scratch, w.seqsum, base, kk, count, base_totals, w.ring_sum.as<float>(), nullptr);
-    double* bsum = w.block_sums.as<double>();
-    const dim3 grid = blocks_grid(kk, count);
+    // Every producer below places the blocks of the array it writes for the sequential sum that
+    // follows it (SeqSum::place / walk), so the sums read no data except in the walk's replays.
+    const dim3 grid = place_grid_of(kk, count);
     {
         ProfileScope ps_("shape-ring-mean", 4.0 * double(kk) * count);
         float* means = w.ring_mean.as<float>();
         hipLaunchKernelGGL(ring_means_kernel, dim3(unsigned(div_up(rmax_mean + 1, 256)), unsigned(count)), dim3(256), 0,
                            s, w.ring_sum.as<float>(), mean_rings.count.as<int>(), rmax_mean + 1, means);
-        hipLaunchKernelGGL(ring_mean_kernel, grid, dim3(kSumThreads), 0, s, means, K, rmax_mean, w.mean.as<float>(),
-                           bsum);
+        hipLaunchKernelGGL(ring_mean_kernel, grid, dim3(kSumThreads), 0, s, means, K, rmax_mean, count,
+                           w.mean.as<float>(), w.seqsum.place(kk, count));
         LDH_CHECK_LAUNCH("ring-mean");
     }
-    w.seqsum.run(s, w.mean.as<float>(), kk, kk, count, t_mean, bsum);
+    w.seqsum.walk(s, one_array(w.mean.as<float>(), count, kk), kk, count, t_mean);
     // anisotropy reshape
     const float gain = 1.0f + std::max(0.0f, float(anisotropy)) * 4.0f;
     {
         ProfileScope ps_("shape-reshape", 12.0 * double(kk) * count);
         hipLaunchKernelGGL(reshape_kernel, grid, dim3(kSumThreads), 0, s, base, base_totals, w.mean.as<float>(),
-                           t_mean, gain, d_gains, kk, w.shaped.as<float>(), bsum);
+                           t_mean, gain, d_gains, kk, count, w.shaped.as<float>(), w.seqsum.place(kk, count));
         LDH_CHECK_LAUNCH("reshape");
     }
-    w.seqsum.run(s, w.shaped.as<float>(), kk, kk, count, t_shaped, bsum);
-    // positive residual
+    w.seqsum.walk(s, one_array(w.shaped.as<float>(), count, kk), kk, count, t_shaped);
+    // positive residual; it also places the normalised shaped kernels (the support estimate's
+    // total), so one walk sums both: t_struct, then t_run
     {
         ProfileScope ps_("shape-residual", 12.0 * double(kk) * count);
         hipLaunchKernelGGL(residual_kernel, grid, dim3(kSumThreads), 0, s, w.shaped.as<float>(), t_shaped,
-                           w.mean.as<float>(), t_mean, kk, w.structure.as<float>(), bsum);
+                           w.mean.as<float>(), t_mean, kk, count, w.structure.as<float>(),
+                           w.seqsum.place(kk, 2 * count));
         LDH_CHECK_LAUNCH("residual");
     }
-    // one launch sums the structure kernels and the normalised shaped kernels (the support
-    // estimate's total)
     SeqSegments g;
     g.array[0] = w.structure.as<float>();
     g.array[1] = w.shaped.as<float>();
     g.prescale[1] = t_shaped;
     g.per_array = count;
     g.stride = kk;
-    w.seqsum.run(s, g, kk, 2 * count, t_struct, bsum);  // t_struct, then t_run
+    w.seqsum.walk(s, g, kk, 2 * count, t_struct);
     // adaptive support radius of the normalised shaped kernel
     const int rmax_sup = std::min(max_radius, K / 2);
     // The last ring's energy only enters the reference's running sum at r = rmax, whose padded
@@ -513,10 +550,11 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
     TaperSources ts{{w.shaped.as<float>(), w.mean.as<float>(), w.structure.as<float>()}, {t_shaped, t_mean, t_struct}};
     {
         ProfileScope ps_("shape-crop-taper", 24.0 * double(kk) * count);
-        hipLaunchKernelGGL(crop_taper_kernel, grid, dim3(kSumThreads), 0, s, ts, d_radius, K, out3, bsum);
+        hipLaunchKernelGGL(crop_taper_kernel, grid, dim3(kSumThreads), 0, s, ts, d_radius, K, count, out3,
+                           w.seqsum.place(kk, 3 * count));
         LDH_CHECK_LAUNCH("crop-taper");
     }
-    w.seqsum.run(s, out3, kk, kk, 3 * count, t_final, bsum);
+    w.seqsum.walk(s, one_array(out3, 3 * count, kk), kk, 3 * count, t_final);
     // the unit-sum scales travel with the kernels; readers multiply as they load
     hipLaunchKernelGGL(inverse_totals_kernel, dim3(unsigned(div_up(3 * count, 256))), dim3(256), 0, s, t_final,
                        3 * count, out_scales);

@@ -1,6 +1,8 @@
 // Sequential-FP32-sum reproduction and unit-sum normalisation on the device.
 #pragma once
 
+#include <climits>
+
 #include "ldh_internal.h"
 
 namespace ldh {
@@ -64,7 +66,7 @@ __device__ __forceinline__ double wave_total(double v) {
 // segment, each thread adds the values it writes into FP64 shares, and every thread then calls
 // store_block_sums, which writes the workgroup's totals to dst[0..N).
 constexpr int kSumThreads = 256;
-inline int seqsum_blocks(int64_t n) { return int((n + kSeqSumBlock - 1) / kSeqSumBlock); }
+__host__ __device__ inline int seqsum_blocks(int64_t n) { return int((n + kSeqSumBlock - 1) / kSeqSumBlock); }
 
 template <int N>
 __device__ __forceinline__ void store_block_sums(double (&share)[N], double* const (&dst)[N]) {
@@ -81,6 +83,180 @@ __device__ __forceinline__ void store_block_sums(double (&share)[N], double* con
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) t += part[threadIdx.x][w];
         *dst[threadIdx.x] = t;
+    }
+}
+
+// ---- single-pass block placement ---------------------------------------------------------------
+// A producer whose output is summed next can place its blocks itself, in the pass that writes
+// them: each workgroup (one kSeqSumBlock block of up to four segments) publishes its block total,
+// finds the exclusive prefix of the segment by a decoupled look-back over the totals and prefixes
+// the earlier workgroups publish, and forms the block's two binade sums from the values still in
+// its registers. SeqSum::walk then finishes the sum with no prefix pass and no second read of the
+// data. The prefix only places blocks (an FP32 estimate, as the FP64 one was): a look-back that
+// has waited kPlaceMaxPolls polls without an answer leaves its block unplaced, which the walk
+// replays, and words of other runs carry other tags, so the estimate can cost time but never
+// change the value. Workgroups publish their totals before they wait and are dispatched in
+// order, so the earliest waiting workgroup always has its predecessor running or done.
+constexpr int kSeqNoBase = INT_MIN;   // running sum placed at zero (or unplaced): the walk replays
+constexpr double kSeqTied = -1.0;     // a binade sum holding a rounding tie
+constexpr int kSeqBins = 2;           // binades per block (the running sum starts in eb or eb+1)
+constexpr int kPlaceMaxPolls = 4096;
+
+struct SeqPlace {
+    double* bsum = nullptr;                // [segment][nb] FP64 block sums
+    double* bins = nullptr;                // [segment][nb][kSeqBins] binade sums
+    int* ebase = nullptr;                  // [segment][nb] base binades
+    unsigned long long* status = nullptr;  // [segment][nb] look-back words
+    unsigned tag = 0;                      // this run's tag (< 2^21)
+    int nb = 0;                            // blocks per segment
+};
+
+// Look-back word: high half 0x7FC00000 | tag << 1 | (1: inclusive prefix, 0: block total only),
+// a float NaN pattern no summed data holds; low half the float value.
+__device__ __forceinline__ unsigned place_word_hi(unsigned tag, bool prefix) {
+    return 0x7FC00000u | (tag << 1) | (prefix ? 1u : 0u);
+}
+__device__ __forceinline__ void place_publish(unsigned long long* p, unsigned hi, float v) {
+    const unsigned long long w = (static_cast<unsigned long long>(hi) << 32) | __float_as_uint(v);
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive prefix of block b of one segment (st: the segment's words), run by one whole wave:
+// lane k reads the word of block j - k for a window of 64 predecessors; the closest one with a
+// prefix ends the look-back, with every block between it and b showing at least its total.
+// Returns -1 when the wait exceeds kPlaceMaxPolls polls.
+__device__ __forceinline__ float place_lookback(unsigned long long* st, unsigned tag, int b, float total) {
+    const int lane = threadIdx.x & 63;
+    const unsigned hi_total = place_word_hi(tag, false), hi_prefix = place_word_hi(tag, true);
+    if (b == 0) {
+        if (lane == 0) place_publish(st, hi_prefix, total);
+        return 0.0f;
+    }
+    if (lane == 0) place_publish(st + b, hi_total, total);
+    float excl = 0.0f;
+    int j = b - 1;
+    int polls = 0;
+    for (;;) {
+        const int idx = j - lane;
+        const unsigned long long w =
+            idx >= 0 ? __hip_atomic_load(st + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+        const unsigned hi = unsigned(w >> 32);
+        const bool has_prefix = idx < 0 || hi == hi_prefix;
+        const bool ready = has_prefix || hi == hi_total;
+        const unsigned long long pmask = __ballot(has_prefix);
+        const int first = pmask != 0ull ? __ffsll(static_cast<long long>(pmask)) - 1 : 64;
+        const unsigned long long upto = first >= 63 ? ~0ull : (2ull << first) - 1ull;  // lanes 0..first
+        if ((__ballot(!ready) & upto) != 0ull) {
+            if (++polls > kPlaceMaxPolls) return -1.0f;
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        float v = (lane <= first && idx >= 0) ? __uint_as_float(unsigned(w)) : 0.0f;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+        excl += v;
+        if (first < 64) return excl;
+        j -= 64;
+    }
+}
+
+// Producers that place their blocks run a 1-D grid of place_grid(nb, segments) workgroups of
+// one block each, block-major: workgroup p takes block p / segments of segment p % segments, so
+// a block's predecessor in its segment was dispatched `segments` workgroups before it and has
+// usually published its total, often its prefix, by the time the block looks back.
+inline unsigned place_grid(int nb, int segments) { return unsigned(nb) * unsigned(segments); }
+struct SegBlock {
+    int seg, block;
+};
+__device__ __forceinline__ bool place_block_of(int nb, int segments, SegBlock& sb) {
+    const int p = int(blockIdx.x);
+    sb.seg = p % segments;
+    sb.block = p / segments;
+    return sb.block < nb;
+}
+
+// Places block b of N segments (seg0 + k * seg_stride, k < N) from the producer's
+// values: v[k][j] is the j-th value the thread wrote into segment k's block (0 for positions
+// past the end), share[k] the thread's FP64 sum of them. Writes the block's FP64 sum, base binade
+// and binade sums into pl. Every thread of the kSumThreads workgroup calls it.
+template <int N, int PER>
+__device__ void place_blocks(const SeqPlace& pl, int seg0, int seg_stride, int b, const float (&v)[N][PER],
+                             const double (&share)[N]) {
+    constexpr int kWaves = kSumThreads / 64;
+    static_assert(N <= kWaves, "one look-back wave per segment");
+    __shared__ double part[N][kWaves];
+    __shared__ double total_sh[N];
+    __shared__ int eb_sh[N];
+    __shared__ double red[N][kSeqBins][kWaves];
+    __shared__ int tied[N][kSeqBins][kWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double t = wave_total(share[k]);
+        if (lane == 0) part[k][wave] = t;
+    }
+    __syncthreads();
+    if (wave < N) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) t += part[wave][w];
+        const size_t row = size_t(seg0 + wave * seg_stride) * size_t(pl.nb);
+        unsigned long long* st = pl.status + row;
+        const float excl = place_lookback(st, pl.tag, b, float(t));
+        int eb = kSeqNoBase;
+        if (excl >= 0.0f) {
+            if (lane == 0) place_publish(st + b, place_word_hi(pl.tag, true), excl + float(t));
+            const double lo = double(excl) * (1.0 - 1.0 / 64.0);
+            eb = lo > 0.0 ? ilogb(lo) : kSeqNoBase;
+        }
+        if (lane == 0) {
+            pl.bsum[row + b] = t;
+            pl.ebase[row + b] = eb;
+            total_sh[wave] = t;
+            eb_sh[wave] = eb;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const int eb = eb_sh[k];
+        if (eb == kSeqNoBase || total_sh[k] == 0.0) continue;  // the walk does not look these up
+#pragma unroll
+        for (int q = 0; q < kSeqBins; ++q) {
+            // contributions counted in units of u = 2^(e-23): rint(x * 2^(23-e))
+            const int shift = 23 - (eb + q);
+            double acc = 0.0;
+            bool tie = false;
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const float t = ldexpf(v[k][j], shift);
+                const float r = rintf(t);
+                tie |= fabsf(t - r) == 0.5f;
+                acc += double(r);
+            }
+            acc = wave_total(acc);
+            const bool any_tie = __ballot(tie) != 0ull;
+            if (lane == 0) {
+                red[k][q][wave] = acc;
+                tied[k][q][wave] = any_tie ? 1 : 0;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < N * kSeqBins) {
+        const int k = threadIdx.x / kSeqBins, q = threadIdx.x % kSeqBins;
+        const int eb = eb_sh[k];
+        if (eb != kSeqNoBase && total_sh[k] != 0.0) {
+            double t = 0.0;
+            int tie = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                t += red[k][q][w];
+                tie |= tied[k][q][w];
+            }
+            const size_t row = size_t(seg0 + k * seg_stride) * size_t(pl.nb);
+            pl.bins[(row + b) * kSeqBins + q] = tie ? kSeqTied : ldexp(t, (eb + q) - 23);
+        }
     }
 }
 
@@ -105,9 +281,17 @@ public:
     void normalize(hipStream_t s, float* x, int64_t n, int64_t stride, int segments, float* totals);
     // Grows the block tables for sums of `segments` arrays of n ahead of a graph capture.
     void reserve(int64_t n, int segments);
+    // Single-pass placement: place() hands out the tables of a run of `segments` sums over n
+    // elements in kSeqSumBlock blocks, with a fresh tag, for a producer that calls place_blocks
+    // in every block of every segment; walk() then finishes that run on the same stream
+    // (totals[seg] as run() would give). g describes where the producer wrote the data.
+    SeqPlace place(int64_t n, int segments);
+    void walk(hipStream_t s, const SeqSegments& g, int64_t n, int segments, float* totals);
 
 private:
-    DeviceBuffer bsum_, bins_, ebase_;
+    DeviceBuffer bsum_, bins_, ebase_, status_;
+    int64_t placed_n_ = -1;
+    int placed_segments_ = 0;
 };
 
 // Allocation-free sums (ldh_sequential_sums_noalloc): the block tables live in a static device

@@ -22,6 +22,7 @@
 // marked and always go the element-by-element way. The estimate only decides which blocks are
 // replayed, never the value, so the result is the sequential FP32 sum, bit for bit, for any
 // non-negative finite input.
+#include <atomic>
 #include <climits>
 #include <cstdint>
 #include <map>
@@ -36,9 +37,9 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kPerThread = kSeqSumBlock / kThreads;
-constexpr int kBins = 2;                   // binades per block (the running sum starts in eb or eb+1)
-constexpr int kNoBase = INT_MIN;           // running sum estimated at zero: replay the block
-constexpr double kTied = -1.0;             // a binade sum holding a rounding tie
+constexpr int kBins = kSeqBins;            // binades per block (the running sum starts in eb or eb+1)
+constexpr int kNoBase = kSeqNoBase;        // running sum estimated at zero: replay the block
+constexpr double kTied = kSeqTied;         // a binade sum holding a rounding tie
 
 
 // Blocks are `bl` consecutive elements (bl <= kSeqSumBlock); block b covers
@@ -397,7 +398,9 @@ void SeqSum::reserve(int64_t n, int segments) {
     bsum_.reset(sizeof(double) * size_t(nb) * segments);
     bins_.reset(sizeof(double) * size_t(nb) * segments * kBins);
     ebase_.reset(sizeof(int) * size_t(nb) * segments);
+    status_.reset(sizeof(unsigned long long) * size_t(seqsum_blocks(n)) * segments);
 }
+
 
 namespace {
 
@@ -500,6 +503,38 @@ void sequential_sums_noalloc(hipStream_t s, int device, const float* x, int64_t 
     t.bins = reinterpret_cast<double*>(p + align(sizeof(double) * size_t(pairs)));
     t.ebase = reinterpret_cast<int*>(p + align(sizeof(double) * size_t(pairs)) + align(2 * sizeof(double) * size_t(pairs)));
     launch_run(s, g, n, segments, totals, nullptr, kSeqSumBlock, t, false);
+}
+
+namespace {
+// Tags of placement runs, process-wide so that a status buffer handed from one workspace to
+// another by the pool cannot hold words of the same tag.
+std::atomic<unsigned> g_place_tag{0};
+}  // namespace
+
+SeqPlace SeqSum::place(int64_t n, int segments) {
+    if (n <= 0 || segments <= 0) fail(LDH_ERR_INVALID_ARGUMENT, "seqsum-place-size");
+    reserve(n, segments);
+    SeqPlace pl;
+    pl.bsum = bsum_.as<double>();
+    pl.bins = bins_.as<double>();
+    pl.ebase = ebase_.as<int>();
+    pl.status = status_.as<unsigned long long>();
+    pl.tag = (g_place_tag.fetch_add(1, std::memory_order_relaxed) & ((1u << 21) - 1u));
+    pl.nb = seqsum_blocks(n);
+    placed_n_ = n;
+    placed_segments_ = segments;
+    return pl;
+}
+
+void SeqSum::walk(hipStream_t s, const SeqSegments& g, int64_t n, int segments, float* totals) {
+    if (n != placed_n_ || segments != placed_segments_) fail(LDH_ERR_INVALID_ARGUMENT, "seqsum-walk-without-placement");
+    check_run(g, segments, kSeqSumBlock);
+    placed_n_ = -1;
+    const int nb = seqsum_blocks(n);
+    ProfileScope ps("seqsum-walk", 0.0);
+    hipLaunchKernelGGL(walk_kernel, dim3(segments), dim3(kWalkThreads), 0, s, g, n, kSeqSumBlock, nb,
+                       bsum_.as<double>(), ebase_.as<int>(), bins_.as<double>(), totals);
+    LDH_CHECK_LAUNCH("seqsum-walk");
 }
 
 void SeqSum::scale(hipStream_t s, float* x, int64_t n, int64_t stride, int segments, const float* totals) {
