@@ -187,11 +187,12 @@ void check_params(const ldh_params* p, bool allow_field_zones = false) {
     if (!why.empty()) throw ldh::Error{LDH_ERR_UNSUPPORTED, why};
 }
 
-__global__ void permute_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, int w, int h, int to_storage) {
+__global__ void permute_rows_kernel(const float2* __restrict__ in, float2* __restrict__ out, int w, int h,
+                                    ldh::ColSplit sp, int to_storage) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= int64_t(w) * h) return;
     const int x = int(i % w), k = int(i / w);
-    const int sr = ldh::col_storage_row(h, k);
+    const int sr = ldh::col_storage_row(sp, k);
     if (to_storage) out[int64_t(sr) * w + x] = in[i];
     else out[i] = in[int64_t(sr) * w + x];
 }
@@ -469,11 +470,14 @@ ldh_status ldh_fft2d(ldh_context* ctx, float* data, int32_t w, int32_t h, int32_
         ldh::DeviceBuffer a(sizeof(float2) * n), b(sizeof(float2) * n);
         h2d(ctx, a.get(), data, sizeof(float2) * n);
         const dim3 grid(unsigned(ldh::div_up(n, 256)));
+        const ldh::ColSplit sp = ldh::col_plan(ctx->stream, ctx->fft.twiddle.as<float2>(), w, h);
         if (!inverse) {
             ldh::fft2d_forward(ctx->stream, ctx->fft, a.as<float2>(), w, h);
-            hipLaunchKernelGGL(permute_rows_kernel, grid, dim3(256), 0, ctx->stream, a.as<float2>(), b.as<float2>(), w, h, 0);
+            hipLaunchKernelGGL(permute_rows_kernel, grid, dim3(256), 0, ctx->stream, a.as<float2>(), b.as<float2>(), w, h,
+                               sp, 0);
         } else {
-            hipLaunchKernelGGL(permute_rows_kernel, grid, dim3(256), 0, ctx->stream, a.as<float2>(), b.as<float2>(), w, h, 1);
+            hipLaunchKernelGGL(permute_rows_kernel, grid, dim3(256), 0, ctx->stream, a.as<float2>(), b.as<float2>(), w, h,
+                               sp, 1);
             ldh::fft2d_inverse(ctx->stream, ctx->fft, b.as<float2>(), w, h);
             hipLaunchKernelGGL(scale_c64_kernel, grid, dim3(256), 0, ctx->stream, b.as<float2>(), n, 1.0f / float(n));
         }
@@ -743,7 +747,7 @@ ldh_status ldh_profiling_reset(ldh_context* ctx) {
 }
 
 int32_t ldh_tuning_report(char* buf, int32_t capacity) {
-    const std::string r = ldh::Autotuner::instance().report();
+    const std::string r = ldh::Autotuner::instance().report() + ldh::col_plan_report();
     if (buf != nullptr && capacity > 0) {
         const size_t n = std::min(r.size(), size_t(capacity - 1));
         std::memcpy(buf, r.data(), n);

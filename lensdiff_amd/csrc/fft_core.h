@@ -476,8 +476,9 @@ __host__ __device__ constexpr bool col_tile_length_ok(int n) {
            n == 96 || n == 128 || n == 144;
 }
 
-// Split of a column length into N1 x N2 (both column tile lengths) for the two-pass column
-// transform, as balanced as possible; n1 = 0 when the height takes a single pass.
+// The balanced split of a column length into N1 x N2 (both column tile lengths, N1 <= N2) for
+// the two-pass column transform, n1 = 0 when the height takes a single pass: the split a height
+// keeps when it is not tuned (col_plan).
 __host__ __device__ inline void col_split(int n, int* n1, int* n2) {
     *n1 = 0;
     *n2 = n;
@@ -491,14 +492,6 @@ __host__ __device__ inline void col_split(int n, int* n1, int* n2) {
     }
     *n1 = best;
     *n2 = best ? n / best : n;
-}
-
-// Storage row of spectrum row k after the forward column transform of length `height`.
-__host__ __device__ inline int col_storage_row(int height, int k) {
-    int n1, n2;
-    col_split(height, &n1, &n2);
-    if (n1 == 0) return k;
-    return n2 * (k % n1) + k / n1;
 }
 
 }  // namespace ldh

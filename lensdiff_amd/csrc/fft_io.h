@@ -14,6 +14,8 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <string>
+#include <vector>
 
 #include "fft_core.h"
 #include "profiler.h"
@@ -165,15 +167,32 @@ inline void check_cols(int width, int height) {
     if (!grid_length_ok(height)) fail(LDH_ERR_INVALID_ARGUMENT, "fft-height-unsupported");
 }
 
-// Forward column transform (in place). `op` applies to the loads of the first pass, which also
-// reads the rows [zero_lo, zero_hi) of `first` as zero.
+// ---- column split plans ---------------------------------------------------------------------------
+// A height that is not a column tile length is transformed in two passes N = N1 x N2; every pair
+// of tile lengths whose product is the height (in either order) computes the same transform, with
+// different tile shapes, occupancy and twiddle traffic. The split also fixes the row order of the
+// spectrum (col_storage_row), so it is chosen once per height and device for the life of the
+// process: the first column transform of a height times every split (forward and inverse passes
+// over scratch memory of the caller's width, HIP events on its stream) and keeps the fastest. A
+// height first met while the stream is capturing a graph keeps the balanced split of col_split.
+struct ColSplit {
+    int n1 = 0, n2 = 0;  // n1 == 0: one pass of length n2
+};
+std::vector<ColSplit> col_splits(int height);
+ColSplit col_plan(hipStream_t s, const float2* tw, int width, int height);
+// One line per planned height: "fft-cols/<height>[n1xn2,...]@dev<d>: v<chosen> <ms per split> ms".
+std::string col_plan_report();
+__host__ __device__ inline int col_storage_row(ColSplit sp, int k) {
+    return sp.n1 == 0 ? k : sp.n2 * (k % sp.n1) + k / sp.n1;
+}
+
+// Forward column transform (in place) with split sp. `op` applies to the loads of the first
+// pass, which also reads the rows [zero_lo, zero_hi) of `first` as zero.
 template <class Load>
-void cols_forward(hipStream_t s, const float2* tw, float2* data, int width, int height, const Load& op,
-                  ColRows first = ColRows{}) {
-    check_cols(width, height);
+void cols_forward_split(hipStream_t s, const float2* tw, float2* data, int width, int height, const Load& op,
+                        ColRows first, ColSplit sp) {
     first.store_rows = 1 << 30;
-    int n1, n2;
-    col_split(height, &n1, &n2);
+    const int n1 = sp.n1, n2 = sp.n2;
     if (n1 == 0) {
         col_pass<0>(s, tw, data, data, width, height, 1, 0, 1, height, op, first, false);
         return;
@@ -182,23 +201,35 @@ void cols_forward(hipStream_t s, const float2* tw, float2* data, int width, int 
     col_pass<0>(s, tw, data, data, width, n2, n1, n2, 1, height, PlainLoad{}, ColRows{}, false);
 }
 
-// Inverse column transform (unscaled). The first pass loads from `src` through `op` and every
-// pass stores into `data` (src == data for in place); the last pass stores only rows
-// < store_rows.
+// Inverse column transform (unscaled) with split sp. The first pass loads from `src` through
+// `op` and every pass stores into `data` (src == data for in place); the last pass stores only
+// rows < store_rows.
 template <class Load>
-void cols_inverse(hipStream_t s, const float2* tw, const float2* src, float2* data, int width, int height,
-                  const Load& op, int store_rows = 1 << 30) {
-    check_cols(width, height);
+void cols_inverse_split(hipStream_t s, const float2* tw, const float2* src, float2* data, int width, int height,
+                        const Load& op, int store_rows, ColSplit sp) {
     ColRows last;
     last.store_rows = store_rows;
-    int n1, n2;
-    col_split(height, &n1, &n2);
+    const int n1 = sp.n1, n2 = sp.n2;
     if (n1 == 0) {
         col_pass<0>(s, tw, src, data, width, height, 1, 0, 1, height, op, last, true);
         return;
     }
     col_pass<-1>(s, tw, src, data, width, n2, n1, n2, 1, height, op, ColRows{}, true);
     col_pass<0>(s, tw, data, data, width, n1, n2, 1, n2, height, PlainLoad{}, last, true);
+}
+
+// The same with the height's planned split (col_plan).
+template <class Load>
+void cols_forward(hipStream_t s, const float2* tw, float2* data, int width, int height, const Load& op,
+                  ColRows first = ColRows{}) {
+    check_cols(width, height);
+    cols_forward_split(s, tw, data, width, height, op, first, col_plan(s, tw, width, height));
+}
+template <class Load>
+void cols_inverse(hipStream_t s, const float2* tw, const float2* src, float2* data, int width, int height,
+                  const Load& op, int store_rows = 1 << 30) {
+    check_cols(width, height);
+    cols_inverse_split(s, tw, src, data, width, height, op, store_rows, col_plan(s, tw, width, height));
 }
 
 // ---- row passes ---------------------------------------------------------------------------------

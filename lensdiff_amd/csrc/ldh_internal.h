@@ -199,7 +199,8 @@ void fft2d_inverse(hipStream_t s, const FftTables& t, float2* data, int width, i
 void fft_rows(hipStream_t s, const FftTables& t, float2* data, int width, int rows, bool inverse);
 void fft_cols_forward(hipStream_t s, const FftTables& t, float2* data, int width, int height);
 void fft_cols_inverse(hipStream_t s, const FftTables& t, float2* data, int width, int height);
-// Map a natural frequency row index to the storage row used by the split column transform.
-int fft_col_storage_row(int height, int k);
+// Map a natural frequency row index to the storage row used by the split column transform of
+// a width x height grid (the height's planned split, tuned on first use on stream s).
+int fft_col_storage_row(hipStream_t s, const FftTables& t, int width, int height, int k);
 
 }  // namespace ldh
