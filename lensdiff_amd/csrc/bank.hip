@@ -354,7 +354,9 @@ crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, int co
         if (inside) {
             const float dx = float(x - c), dy = float(y - c);
             const float rr = sqrtf(dx * dx + dy * dy);
-            if (rr < extent) {
+            if (rr <= ramp) {
+                w = 1.0f;  // t = 0 up to the ramp: cos(0) = 1 exactly
+            } else if (rr < extent) {
                 const float t = fminf(fmaxf((rr - ramp) / fmaxf(fade, 1e-6f), 0.0f), 1.0f);
                 const float sm = t * t * (3.0f - 2.0f * t);
                 w = fmaxf(0.0f, cosf(sm * 1.57079637f));

@@ -8,6 +8,8 @@
 // memory) use (1, NL).
 #pragma once
 
+#include <type_traits>
+
 #include "ldh_internal.h"
 
 namespace ldh {
@@ -276,6 +278,14 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
     return (bid & 7) * (nblocks >> 3) + (bid >> 3);
 }
 
+// An IO may store a tile's finished lines itself, reading any positions of the transform from
+// LDS (IO::kStoreLds and IO::store_lds<N, NL, NT>(tile, tid, lds); element i of line l is at
+// lds[lds_pad(l * N + i)] for row tiles).
+template <class T, class = void>
+struct io_stores_lds : std::false_type {};
+template <class T>
+struct io_stores_lds<T, std::void_t<decltype(T::kStoreLds)>> : std::integral_constant<bool, T::kStoreLds> {};
+
 // The tile kernel. IO supplies:
 //   __device__ float2 load(int tile, int line, int i) const;
 //   __device__ void store(int tile, int line, int i, float2 v) const;
@@ -396,7 +406,9 @@ fft_tile_kernel(IO io, const float2* __restrict__ tw, int ntiles) {
         }
     }
     fft_stages<N, 1, NL, NT, INV, LS, IS, S::TWL>(lds, tw, twl, tid);
-    if constexpr (kPairs) {
+    if constexpr (io_stores_lds<IO>::value) {
+        io.template store_lds<N, NL, NT>(tile, tid, lds);
+    } else if constexpr (kPairs) {
         const int lp = (tid % NP) * 2, i0 = tid / NP;
 #pragma unroll
         for (int c = 0; c < IP; ++c) {

@@ -270,6 +270,20 @@ struct RowIO {
         const int l = tile * lines_per_tile + line;
         if (l < nrows) op.store(row_of(tile, line), i, v);
     }
+    // Row ops with kStoreLds store by symmetry: for k = 0 .. op.hw - 1 of each line,
+    // op.store_sym(row, k, Z[k], Z[(N - k) % N], N) (k < N; hw <= N).
+    static constexpr bool kStoreLds = io_stores_lds<RowOp>::value;
+    template <int N, int NL, int NT>
+    __device__ void store_lds(int tile, int tid, const float2* lds) const {
+        const int hw = op.hw;
+        for (int e = tid; e < NL * hw; e += NT) {
+            const int line = e / hw, k = e - line * hw;
+            if (tile * lines_per_tile + line >= nrows) continue;
+            const float2 zk = lds[lds_pad(line * N + k)];
+            const float2 zm = lds[lds_pad(line * N + (N - k) % N)];
+            op.store_sym(row_of(tile, line), k, zk, zm, N);
+        }
+    }
 };
 
 template <class RowOp>

@@ -21,27 +21,37 @@ SpectrumConfig build_spectrum_config(const ldh_params& p, const std::vector<floa
 
 // FFT grid of a frame convolved with kernels of support radius `support`: the reference's
 // nextPow2(W + n - 1) x nextPow2(H + n - 1), n = 2*support+1 (LensDiffCpuReference.cpp:1168-1169).
+// A half spectrum (the transform of a real plane) keeps columns 0 .. pw/2 of each row, the rest
+// following by symmetry (X[-k] = conj X[k]), at a row pitch of hw columns (pw/2 + 1 rounded up
+// to the column tile width; the extra columns hold zeros).
 struct Grid {
     int w = 0, h = 0;     // frame
     int pw = 0, ph = 0;   // FFT grid
+    int hw = 0;           // half-spectrum row pitch
     size_t cells() const { return size_t(pw) * ph; }
+    size_t half_cells() const { return size_t(hw) * ph; }
 };
 Grid make_grid(int w, int h, int support);
 
-// Kernel spectra of a set of real K x K kernels, packed two per complex spectrum
-// (k_a + i k_b), in the split-column layout of the forward transform.
+// Kernel spectra of a set of real K x K kernels in the split-column layout of the forward
+// transform: packed two per complex spectrum (k_a + i k_b), or with `half`, one half spectrum
+// per kernel.
 struct KernelSpectra {
     Grid grid;
     int kernels = 0;      // real kernels
-    int pairs = 0;        // complex spectra
+    int pairs = 0;        // complex spectra (half: one per kernel)
     bool fp16 = false;
-    DeviceBuffer data;    // float2 or __half2 [pairs][ph][pw]
+    bool half = false;
+    DeviceBuffer data;    // float2 or __half2: [pairs][ph][pw], or with half [kernels][ph][hw]
+    size_t stride() const { return half ? grid.half_cells() : grid.cells(); }
     size_t bytes() const { return data.bytes(); }
 };
-// kernels: K x K each (texels times their scale), packed in order (0,1), (2,3), ...; `radius`
-// bounds the non-zero support of every kernel.
+// kernels: K x K each (texels times their scale), packed in order (0,1), (2,3), ... (each pair
+// transformed together and, with half, separated into two half spectra); `radius` bounds the
+// non-zero support of every kernel.
 void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<KernelRef>& kernels, int K,
-                          int radius, const Grid& g, bool fp16, KernelSpectra& out, DeviceBuffer& scratch);
+                          int radius, const Grid& g, bool fp16, KernelSpectra& out, DeviceBuffer& scratch,
+                          bool half = false);
 // Combined RGB kernels of a spectral bank: out[c] = sum_b M[c][b] (k_b * scales[b]) for the
 // natural (c=0..2) and, if with_style, style (c=3..5) matrices. out: device float[6][K*K],
 // unit scale.

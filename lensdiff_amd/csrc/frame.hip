@@ -64,6 +64,21 @@ __global__ void selection_kernel(const float4* __restrict__ src, int64_t npx, Se
 }
 
 // Row pass of an image spectrum: reads the frame, writes the grid.
+// The scalar driver of a pixel (applyMaskScalarDriver, LensDiffCpuReference.cpp:1266-1279):
+// signal (max RGB or Rec.709 luma of the decoded pixel) times mask times scale.
+__device__ __forceinline__ float driver_value(float4 raw, float m, int tf, int extraction, float scale) {
+    const float r = decode_channel(raw.x, tf), g = decode_channel(raw.y, tf), b = decode_channel(raw.z, tf);
+    const float sig = extraction == LDH_EXTRACT_LUMA ? luma709(r, g, b) : max3(r, g, b);
+    return sig * m * scale;
+}
+
+// Separation of the transform Z of z = a + i b (a, b real) into half spectra:
+// A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = (Z[k] - conj Z[N-k]) / 2i.
+__device__ __forceinline__ void split_real_pair(float2 zk, float2 zm, float2& a, float2& b) {
+    a = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+    b = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
+}
+
 struct FrameRows {
     const float4* src;
     const float* mask;
@@ -88,10 +103,7 @@ struct FrameRows {
         if (!t.in) return make_float2(0.0f, 0.0f);
         const float4 q = make_float4(decode_channel(t.q.x, tf), decode_channel(t.q.y, tf), decode_channel(t.q.z, tf), t.q.w);
         const float m = t.m;
-        if (driver) {
-            const float sig = extraction == LDH_EXTRACT_LUMA ? luma709(q.x, q.y, q.z) : max3(q.x, q.y, q.z);
-            return make_float2(sig * m * scale, 0.0f);
-        }
+        if (driver) return make_float2(driver_value(t.q, m, tf, extraction, scale), 0.0f);
         const float k = m * scale;
         const float c[3] = {q.x, q.y, q.z};
         return make_float2(c[ca] * k, cb >= 0 ? c[cb] * k : 0.0f);
@@ -128,6 +140,85 @@ struct KernelRows {
         return make_float2(a, b);
     }
     __device__ void store(int row, int x, float2 v) const { out[size_t(row) * pw + x] = v; }
+};
+
+// Row pass of the scalar driver into a half spectrum: driver rows 2p and 2p + 1 transformed
+// together as a + i b and separated into rows 2p, 2p + 1 (pitch hw; columns past pw/2 zero).
+// Rows from h on are zero and not stored.
+struct DriverRowsHalf {
+    const float4* src;
+    const float* mask;
+    float2* out;
+    int w, h, hw, tf, extraction;
+    float scale;
+    static constexpr bool kStoreLds = true;
+    struct Raw {
+        float4 qa, qb;
+        float ma, mb;
+        bool ina, inb;
+    };
+    __device__ Raw fetch(int p, int x) const {
+        const int ra = 2 * p, rb = 2 * p + 1;
+        const bool ina = ra < h && x < w, inb = rb < h && x < w;
+        const size_t ia = ina ? size_t(ra) * w + x : 0, ib = inb ? size_t(rb) * w + x : 0;
+        return Raw{src[ia], src[ib], mask[ia], mask[ib], ina, inb};
+    }
+    __device__ float2 finish(const Raw& t) const {
+        return make_float2(t.ina ? driver_value(t.qa, t.ma, tf, extraction, scale) : 0.0f,
+                           t.inb ? driver_value(t.qb, t.mb, tf, extraction, scale) : 0.0f);
+    }
+    __device__ void store(int, int, float2) const {}
+    __device__ void store_sym(int p, int k, float2 zk, float2 zm, int n) const {
+        float2 a = make_float2(0.0f, 0.0f), b = a;
+        if (k <= n / 2) split_real_pair(zk, zm, a, b);
+        out[size_t(2 * p) * hw + k] = a;
+        if (2 * p + 1 < h) out[size_t(2 * p + 1) * hw + k] = b;
+    }
+};
+
+// Kernel rows of two real kernels transformed together (KernelRows' loads), separated into the
+// half spectra of each (outa, outb: pitch hw).
+struct KernelRowsHalf : KernelRows {
+    float2* outb;
+    int hw;
+    static constexpr bool kStoreLds = true;
+    __device__ void store_sym(int row, int k, float2 zk, float2 zm, int n) const {
+        float2 a = make_float2(0.0f, 0.0f), b = a;
+        if (k <= n / 2) split_real_pair(zk, zm, a, b);
+        out[size_t(row) * hw + k] = a;
+        if (outb) outb[size_t(row) * hw + k] = b;
+    }
+};
+
+// Last inverse pass from half spectra: the rows of two real planes' half spectra a, b (pitch hw,
+// b nullable) extended by symmetry to Z = A + i B, so one inverse row transform gives plane a in
+// its real and plane b in its imaginary part; cropped, scaled and clamped as ExtractRows.
+struct ExtractRowsHalf {
+    const float2* a;
+    const float2* b;
+    float* oa;
+    float* ob;
+    int w, h, hw, n;
+    float scale;
+    struct Raw {
+        float2 xa, xb;
+        bool mirror;
+    };
+    __device__ Raw fetch(int row, int x) const {
+        const bool mirror = x > n / 2;
+        const size_t i = size_t(row) * hw + (mirror ? n - x : x);
+        return Raw{a[i], b ? b[i] : make_float2(0.0f, 0.0f), mirror};
+    }
+    __device__ float2 finish(const Raw& t) const {
+        const float s = t.mirror ? -1.0f : 1.0f;  // conjugates past pw/2
+        return make_float2(t.xa.x - s * t.xb.y, s * t.xa.y + t.xb.x);
+    }
+    __device__ void store(int row, int x, float2 v) const {
+        if (row >= h || x >= w) return;
+        const size_t i = size_t(row) * w + x;
+        oa[i] = fmaxf(0.0f, v.x * scale);
+        if (ob) ob[i] = fmaxf(0.0f, v.y * scale);
+    }
 };
 
 // Last inverse pass: crop to the frame, scale by 1/(pw*ph) (exact), clamp at zero.
@@ -669,20 +760,55 @@ Grid make_grid(int w, int h, int support) {
     g.pw = grid_length_at_least(std::max(16, w + n - 1));
     g.ph = grid_length_at_least(std::max(16, h + n - 1));
     if (g.pw == 0 || g.ph == 0) fail(LDH_ERR_TOO_LARGE, "frame-too-large-for-fft-grid");
+    g.hw = (g.pw / 2 + 1 + kColWidth - 1) / kColWidth * kColWidth;
     return g;
 }
 
 void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<KernelRef>& kernels, int K,
-                          int radius, const Grid& g, bool fp16, KernelSpectra& out, DeviceBuffer& scratch) {
+                          int radius, const Grid& g, bool fp16, KernelSpectra& out, DeviceBuffer& scratch, bool half) {
     const int n = int(kernels.size());
     const int pairs = (n + 1) / 2;
     out.grid = g;
     out.kernels = n;
-    out.pairs = pairs;
+    out.pairs = half ? n : pairs;
     out.fp16 = fp16;
+    out.half = half;
     const size_t cells = g.cells();
-    out.data.reset((fp16 ? sizeof(__half2) : sizeof(float2)) * cells * pairs);
     const float2* tw = t.twiddle.as<float2>();
+    if (half) {
+        // each pair transformed together along the rows, separated into two half spectra
+        const size_t hc = g.half_cells();
+        out.data.reset((fp16 ? sizeof(__half2) : sizeof(float2)) * hc * n);
+        if (fp16) scratch.reset(sizeof(float2) * hc * 2);
+        for (int p = 0; p < pairs; ++p) {
+            const bool two = 2 * p + 1 < n;
+            float2* da = fp16 ? scratch.as<float2>() : out.data.as<float2>() + hc * (2 * p);
+            float2* db = !two ? nullptr : fp16 ? scratch.as<float2>() + hc : out.data.as<float2>() + hc * (2 * p + 1);
+            const KernelRef ka = kernels[size_t(2 * p)];
+            const KernelRef kb = two ? kernels[size_t(2 * p + 1)] : KernelRef{};
+            KernelRowsHalf op;
+            static_cast<KernelRows&>(op) = KernelRows{ka.data, kb.data, ka.scale, kb.scale, da, K, K / 2, radius, g.pw,
+                                                      g.ph};
+            op.outb = db;
+            op.hw = g.hw;
+            const int band = 2 * radius + 1;
+            rows_pass(s, tw, g.pw, band, op, false, radius + 1, g.ph - band);
+            ColRows zero;
+            zero.zero_lo = radius + 1;
+            zero.zero_hi = g.ph - radius;
+            for (int j = 0; j < (two ? 2 : 1); ++j) {
+                float2* d = j == 0 ? da : db;
+                cols_forward(s, tw, d, g.hw, g.ph, PlainLoad{}, zero);
+                if (fp16) {
+                    hipLaunchKernelGGL(to_half_kernel, grid1(int64_t(hc)), dim3(256), 0, s, d,
+                                       out.data.as<__half2>() + hc * (2 * p + j), int64_t(hc));
+                    LDH_CHECK_LAUNCH("spectra-to-half");
+                }
+            }
+        }
+        return;
+    }
+    out.data.reset((fp16 ? sizeof(__half2) : sizeof(float2)) * cells * pairs);
     if (fp16) scratch.reset(sizeof(float2) * cells);
     for (int p = 0; p < pairs; ++p) {
         float2* dst = fp16 ? scratch.as<float2>() : out.data.as<float2>() + cells * p;
@@ -796,6 +922,71 @@ void select_stage(hipStream_t s, FrameWork& fw, const ldh_params& p, const float
     }
 }
 
+// Steps 2-4 for the spectral full-kernel effect with pre-combined RGB kernels (natural, and style
+// when styled), on half spectra: the driver is real, so its transform is stored for columns
+// 0 .. pw/2 only (DriverRowsHalf, two rows per row transform); each combined kernel likewise;
+// each channel's product is inverted over the half-width columns, and channels are paired for
+// the last row pass (ExtractRowsHalf), which yields two real planes per row transform. The
+// planes then go through the pre-combined spectral map into effect9[0..3).
+void effect_precombined(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
+                        const float4* src, int w, int h, const Grid& g, const SpectrumConfig& cfg, bool with_style,
+                        float force, float redist_scale, const KernelSpectra* cached, float* effect9,
+                        bool image_ready) {
+    const int64_t npx = int64_t(w) * h;
+    const size_t hc = g.half_cells();
+    const float2* tw = t.twiddle.as<float2>();
+    const int nk = with_style ? 6 : 3;
+    // 2. the driver's half spectrum
+    fw.image_spec.reset(sizeof(float2) * hc);
+    float2* ispec = fw.image_spec.as<float2>();
+    if (!image_ready) {
+        DriverRowsHalf op{src, fw.mask.as<float>(), ispec, w, h, g.hw, p.input_transfer, p.extraction_mode,
+                          redist_scale};
+        rows_pass(s, tw, g.pw, (h + 1) / 2, op, false);
+        ColRows zero;
+        zero.zero_lo = h;
+        zero.zero_hi = g.ph;
+        cols_forward(s, tw, ispec, g.hw, g.ph, PlainLoad{}, zero);
+    }
+    // 3. half spectra of the combined kernels (cached with the lens, or built for this frame)
+    const KernelSpectra* ks = cached;
+    if (ks == nullptr) {
+        const int64_t kk = int64_t(bank.kernel) * bank.kernel;
+        fw.combined.reset(sizeof(float) * kk * 6);
+        combine_kernels(s, bank.kernel_ptr(LDH_KERNEL_FULL, 0), bank.scale_ptr(LDH_KERNEL_FULL, 0), bank.bins,
+                        bank.kernel, cfg, with_style, fw.combined.as<float>(), fw.kspec_tmp);
+        std::vector<KernelRef> refs;
+        for (int r = 0; r < nk; ++r) refs.push_back({fw.combined.as<float>() + r * kk, nullptr});
+        build_kernel_spectra(s, t, refs, bank.kernel, bank.support, g, false, fw.kspec, fw.kspec_tmp, true);
+        ks = &fw.kspec;
+    }
+    if (!ks->half || ks->grid.pw != g.pw || ks->grid.ph != g.ph || ks->kernels != nk)
+        fail(LDH_ERR_INVALID_ARGUMENT, "cached-spectra-mismatch");
+    // 4. per channel: product and inverse columns; per channel pair: one row pass to two planes
+    fw.out_spec.reset(sizeof(float2) * hc * 2);
+    fw.planes.reset(sizeof(float) * npx * 6);
+    float* planes = fw.planes.as<float>();
+    const float inv_cells = 1.0f / float(g.cells());
+    for (int c = 0; c < nk; c += 2) {
+        const int m = std::min(2, nk - c);
+        for (int j = 0; j < m; ++j) {
+            float2* o = fw.out_spec.as<float2>() + hc * j;
+            // only rows < h reach the crop
+            if (ks->fp16)
+                cols_inverse(s, tw, ispec, o, g.hw, g.ph, MulSpectrumHalfLoad{ks->data.as<__half2>() + hc * (c + j)}, h);
+            else
+                cols_inverse(s, tw, ispec, o, g.hw, g.ph, MulSpectrumLoad{ks->data.as<float2>() + hc * (c + j)}, h);
+        }
+        ExtractRowsHalf ex{fw.out_spec.as<float2>(), m == 2 ? fw.out_spec.as<float2>() + hc : nullptr,
+                           planes + npx * c, m == 2 ? planes + npx * (c + 1) : nullptr, w, h, g.hw, g.pw, inv_cells};
+        rows_pass(s, tw, g.pw, h, ex, true);
+    }
+    ProfileScope ps_map("frame-spectral-map", (4.0 * 6 + 12.0) * double(npx));
+    hipLaunchKernelGGL(precombined_map_kernel, grid1(npx), dim3(256), 0, s, planes, with_style ? 1 : 0, npx, force,
+                       float(std::max(0.0, p.spectrum_saturation)), p.chromatic_affects_luma, effect9);
+    LDH_CHECK_LAUNCH("effect-map");
+}
+
 // Steps 2-4 for one bank: the effect planes [effect, core, structure] (3 each; only the effect
 // planes unless the look is split) into effect9 (npx * 9 floats). shared_grid (nullable): the
 // grid to convolve on instead of the bank's own (the zones of a field-dependent phase share one);
@@ -815,6 +1006,12 @@ void effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
     if (mode == EffectMode::Precombined && (!pl.spectral || pl.split || pl.sets.front() != kFull))
         mode = EffectMode::PerBin;  // pre-combination applies to the spectral full-kernel effect
     const float redist_scale = 1.0f - float(std::max(0.0, std::min(p.core_preserve, 1.0)));
+
+    if (mode == EffectMode::Precombined) {
+        effect_precombined(s, t, fw, p, bank, src, w, h, g, cfg, with_style, force, redist_scale, cached, effect9,
+                           image_ready);
+        return;
+    }
 
     // 2. image spectra: mono -> (R + iG), (B); spectral -> driver
     const int nimg = pl.spectral ? 1 : 2;

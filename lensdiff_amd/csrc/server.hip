@@ -468,8 +468,9 @@ void Server::worker(Slot* s) {
                                     const float* comb = e->combined.as<float>() + kk * 6 * size_t(z);
                                     for (int i = 0; i < nk; ++i) ks.push_back({comb + kk * i, nullptr});
                                 }
+                                // pre-combined kernels as half spectra (the real-input path)
                                 build_kernel_spectra(s->stream, fft_, ks, zbank.kernel, zbank.support, g, fp16,
-                                                     sp.zones[size_t(z)], s->tmp);
+                                                     sp.zones[size_t(z)], s->tmp, nk > 1);
                             }
                         });
                         for (const auto& z : spec->zones) zc.push_back(&z);
@@ -493,7 +494,8 @@ void Server::worker(Slot* s) {
                             const size_t kk = size_t(bank.kernel) * bank.kernel;
                             for (int i = 0; i < nk; ++i) ks.push_back({e->combined.as<float>() + kk * i, nullptr});
                         }
-                        build_kernel_spectra(s->stream, fft_, ks, bank.kernel, bank.support, g, fp16, sp.spec, s->tmp);
+                        build_kernel_spectra(s->stream, fft_, ks, bank.kernel, bank.support, g, fp16, sp.spec, s->tmp,
+                                             nk > 1);
                     });
                 }
                 ldh_params p = r->params;
