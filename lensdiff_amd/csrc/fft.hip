@@ -93,7 +93,7 @@ void tune_splits(hipStream_t s, const float2* tw, int width, int height, SplitPl
             LDH_HIP(hipEventElapsedTime(&t, a, b), "fft-split-tune");
             ms.push_back(t * 0.5f);
         }
-        // the balanced split unless another is faster by more than 3% (timing noise would
+        // the balanced split unless another is faster by more than 5% (timing noise would
         // otherwise pick among equals from run to run)
         size_t best = 0, balanced = 0;
         for (size_t i = 0; i < ms.size(); ++i) {
@@ -101,7 +101,7 @@ void tune_splits(hipStream_t s, const float2* tw, int width, int height, SplitPl
             if (e.cand[i].n1 == e.chosen.n1) balanced = i;
         }
         e.ms = ms;
-        if (ms[best] < 0.97f * ms[balanced]) e.chosen = e.cand[best];
+        if (ms[best] < 0.95f * ms[balanced]) e.chosen = e.cand[best];
     } catch (const Error&) {
         // out of memory or a failed call: the balanced split, untimed
         e.ms.clear();
