@@ -345,6 +345,30 @@ def test_precombined_matches_per_bin(ctx):
     assert rel_l2(b, a) < 1e-5
 
 
+@pytest.mark.parametrize("w,h,style", [(97, 63, 0), (120, 81, 1), (64, 17, 1)],
+                         ids=["odd-97x63-natural", "odd-120x81-styled", "odd-64x17-styled"])
+def test_precombined_half_spectra_odd_frames(ctx, oracle, w, h, style):
+    """The pre-combined path runs on half spectra: the driver's rows are transformed in pairs (an
+    odd height leaves the last row unpaired) and channels share the last row transform in pairs
+    (three natural channels leave one unpaired). Odd widths and heights, natural and styled,
+    against the per-bin path and the oracle."""
+    p = workloads.base_params(min(w, h))
+    p.pupil_resolution = 64
+    p.max_kernel_radius_px = 20.0
+    p.set_wavelengths(np.linspace(400, 700, 9))
+    if style:
+        p.spectrum_style = _abi.STYLE_WARM_COOL
+        p.spectrum_force = 0.5
+    img = small_frame(w, h, seed=5)
+    a = ctx.render_image(p, img)
+    p.conv_mode = _abi.CONV_PRECOMBINED
+    b = ctx.render_image(p, img)
+    assert rel_l2(b, a) < 1e-5
+    p.conv_mode = _abi.CONV_PER_BIN
+    want = oracle.render_image(p, img)
+    assert rel_l2(b, want) < TOL
+
+
 def test_c0_frame_matches_oracle(ctx, oracle):
     """The reference's own CPU case at full size (512x512, P=256, R=1024, K=513)."""
     p = workloads.c0_params()
