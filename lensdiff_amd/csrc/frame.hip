@@ -493,12 +493,22 @@ __global__ void phase_view_kernel(SquareSrc phase, SquareSrc pupil, float4* __re
 // OTF view (computeOtfMagnitude, LensDiffCpuReference.cpp:1403-1414): the n x n DFT of the kernel
 // wrapped to the origin, as the reference's FFT computes it for odd n (Bluestein with its chirp
 // angle pi i^2 / n taken with pi rounded to float, which is the DFT with kernel
-// exp(-2 i pi_f x k / n)). Evaluated directly, one output per thread: the twiddle w^x advances by
-// one complex multiplication per term and the sums run in double. Rows first: thread kx of
-// workgroup (kx block, wrapped row py) sums row (py + c) % n of the normalised kernel, staged in
-// LDS in wrapped order.
+// exp(-2 i pi_f x k / n)). Evaluated directly in FP32, one output per thread. That kernel is not
+// periodic in x k mod n (pi_f != pi), so each term's twiddle is the table value
+// exp(-2 i pi (x k mod n) / n) (computed in double on the host, rounded once) times
+// exp(-2 i (pi_f - pi) x k / n), whose angle stays below 8e-4 for n <= 4097 and is formed from its
+// series (1 - a^2 / 2, -a; the next terms are below 1e-11). Rows first: thread kx of workgroup
+// (kx block, wrapped row py) sums row (py + c) % n of the normalised kernel, staged in LDS in
+// wrapped order.
+__device__ __forceinline__ float2 otf_twiddle(const float2* __restrict__ tw, int idx, int m, float eps) {
+    const float2 t = tw[idx];
+    const float a = eps * float(m);
+    const float cr = 1.0f - 0.5f * a * a;
+    return make_float2(t.x * cr + t.y * a, t.y * cr - t.x * a);
+}
+
 __global__ void otf_rows_kernel(const float* __restrict__ k, int pitch, int n, const float* __restrict__ scale,
-                                float2* __restrict__ out) {
+                                const float2* __restrict__ tw, float eps, float2* __restrict__ out) {
     extern __shared__ float otf_row[];
     const int py = blockIdx.y, c = n / 2;
     const float sc = scale ? *scale : 1.0f;
@@ -507,45 +517,45 @@ __global__ void otf_rows_kernel(const float* __restrict__ k, int pitch, int n, c
     __syncthreads();
     const int kx = blockIdx.x * blockDim.x + threadIdx.x;
     if (kx >= n) return;
-    const double theta = -2.0 * double(float(M_PI)) * double(kx) / double(n);
-    double sn, cs;
-    sincos(theta, &sn, &cs);
-    double wr = 1.0, wi = 0.0, ar = 0.0, ai = 0.0;
+    float ar = 0.0f, ai = 0.0f;
+    int idx = 0, m = 0;  // m = kx px, idx = m mod n
     for (int px = 0; px < n; ++px) {
-        const double v = otf_row[px];
-        ar = fma(v, wr, ar);
-        ai = fma(v, wi, ai);
-        const double t = wr * cs - wi * sn;
-        wi = wr * sn + wi * cs;
-        wr = t;
+        const float2 w = otf_twiddle(tw, idx, m, eps);
+        const float v = otf_row[px];
+        ar += v * w.x;
+        ai += v * w.y;
+        m += kx;
+        idx += kx;
+        if (idx >= n) idx -= n;
     }
-    out[int64_t(py) * n + kx] = make_float2(float(ar), float(ai));
+    out[int64_t(py) * n + kx] = make_float2(ar, ai);
 }
 
 // Columns: thread ky of workgroup (ky block, column kx) sums column kx of the row transforms
-// (staged in LDS) against w^py, and writes |X| at its fftShiftSquare position (:303-314).
-__global__ void otf_cols_kernel(const float2* __restrict__ rows, int n, float* __restrict__ mag) {
+// (staged in LDS) against the twiddles of ky py, and writes |X| at its fftShiftSquare position
+// (:303-314).
+__global__ void otf_cols_kernel(const float2* __restrict__ rows, int n, const float2* __restrict__ tw, float eps,
+                                float* __restrict__ mag) {
     extern __shared__ float2 otf_col[];
     const int kx = blockIdx.y;
     for (int py = threadIdx.x; py < n; py += blockDim.x) otf_col[py] = rows[int64_t(py) * n + kx];
     __syncthreads();
     const int ky = blockIdx.x * blockDim.x + threadIdx.x;
     if (ky >= n) return;
-    const double theta = -2.0 * double(float(M_PI)) * double(ky) / double(n);
-    double sn, cs;
-    sincos(theta, &sn, &cs);
-    double wr = 1.0, wi = 0.0, ar = 0.0, ai = 0.0;
+    float ar = 0.0f, ai = 0.0f;
+    int idx = 0, m = 0;
     for (int py = 0; py < n; ++py) {
-        const double vr = otf_col[py].x, vi = otf_col[py].y;
-        ar = fma(vr, wr, fma(-vi, wi, ar));
-        ai = fma(vr, wi, fma(vi, wr, ai));
-        const double t = wr * cs - wi * sn;
-        wi = wr * sn + wi * cs;
-        wr = t;
+        const float2 w = otf_twiddle(tw, idx, m, eps);
+        const float2 v = otf_col[py];
+        ar += v.x * w.x - v.y * w.y;
+        ai += v.x * w.y + v.y * w.x;
+        m += ky;
+        idx += ky;
+        if (idx >= n) idx -= n;
     }
     const int half = n / 2;
     const int ys = (ky - half + n) % n, xs = (kx - half + n) % n;
-    mag[int64_t(ys) * n + xs] = float(sqrt(ar * ar + ai * ai));
+    mag[int64_t(ys) * n + xs] = sqrtf(ar * ar + ai * ai);
 }
 
 // Bilinear sample at (x, y) in the form of sampleScalarBilinear / sampleRgbaBilinearImage
@@ -1469,9 +1479,22 @@ void render_static_view(hipStream_t s, FrameWork& fw, const ldh_params& p, int v
         float2* rows = fw.view_pupil.as<float2>();
         float* mag = reinterpret_cast<float*>(rows + size_t(n) * n);
         const unsigned nb = unsigned(div_up(n, 256));
+        if (n > 4097) fail(LDH_ERR_TOO_LARGE, "otf-view-kernel-too-large");
+        // exp(-2 i pi j / n) in double, rounded once; eps = 2 (pi_f - pi) / n
+        std::vector<float2> tw(static_cast<size_t>(n));
+        for (int j = 0; j < n; ++j) {
+            const double a = -2.0 * M_PI * double(j) / double(n);
+            tw[size_t(j)] = make_float2(float(std::cos(a)), float(std::sin(a)));
+        }
+        const float eps = float(2.0 * (double(float(M_PI)) - M_PI) / double(n));
+        fw.view_coef.reset(sizeof(float2) * size_t(n));
+        LDH_HIP(hipMemcpyAsync(fw.view_coef.get(), tw.data(), sizeof(float2) * size_t(n), hipMemcpyHostToDevice, s),
+                "view-otf");
         hipLaunchKernelGGL(otf_rows_kernel, dim3(nb, unsigned(n)), dim3(256), sizeof(float) * n, s, k.data, k.pitch, n,
-                           k.scale, rows);
-        hipLaunchKernelGGL(otf_cols_kernel, dim3(nb, unsigned(n)), dim3(256), sizeof(float2) * n, s, rows, n, mag);
+                           k.scale, fw.view_coef.as<float2>(), eps, rows);
+        hipLaunchKernelGGL(otf_cols_kernel, dim3(nb, unsigned(n)), dim3(256), sizeof(float2) * n, s, rows, n,
+                           fw.view_coef.as<float2>(), eps, mag);
+        LDH_HIP(hipStreamSynchronize(s), "view-otf");  // the host table goes out of scope
         hipLaunchKernelGGL(max_kernel, dim3(256), dim3(256), 0, s, mag, int64_t(n) * n, mx + 1);
         const SquareSrc m{mag, n, n, nullptr, mx + 1};
         hipLaunchKernelGGL(square_view_kernel, grid1(npx), dim3(256), 0, s, m, m, m, 1, dst, w, h);
