@@ -61,30 +61,10 @@ __device__ __forceinline__ double wave_total(double v) {
     return __hiloint2double(hi, lo);
 }
 
-// Kernels that produce a summed array can hand the sum its block sums: a workgroup of
-// kSumThreads threads covers block blockIdx.x (kSeqSumBlock consecutive elements) of one
-// segment, each thread adds the values it writes into FP64 shares, and every thread then calls
-// store_block_sums, which writes the workgroup's totals to dst[0..N).
+// Producers of summed arrays run workgroups of kSumThreads threads over blocks of kSeqSumBlock
+// elements and place them for the sum in the pass that writes them (place_blocks below).
 constexpr int kSumThreads = 256;
 __host__ __device__ inline int seqsum_blocks(int64_t n) { return int((n + kSeqSumBlock - 1) / kSeqSumBlock); }
-
-template <int N>
-__device__ __forceinline__ void store_block_sums(double (&share)[N], double* const (&dst)[N]) {
-    constexpr int kWaves = kSumThreads / 64;
-    __shared__ double part[N][kWaves];
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const double v = wave_total(share[j]);
-        if ((threadIdx.x & 63) == 0) part[j][threadIdx.x >> 6] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < N) {
-        double t = 0.0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) t += part[threadIdx.x][w];
-        *dst[threadIdx.x] = t;
-    }
-}
 
 // ---- single-pass block placement ---------------------------------------------------------------
 // A producer whose output is summed next can place its blocks itself, in the pass that writes
