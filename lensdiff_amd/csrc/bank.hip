@@ -6,7 +6,9 @@
 //
 // Ring sums (azimuthal means, support-radius energies, the radial profile) are the reference's
 // float sums in raster order, reproduced exactly by rings.hip. Every unit-sum normalisation goes
-// through SeqSum so the scale matches the reference's sequential FP32 accumulation.
+// through SeqSum so the scale matches the reference's sequential FP32 accumulation; the kernels
+// that write a normalised array (resample, ring mean, reshape, residual, crop + taper) place its
+// blocks for that sum as they write them.
 #include <algorithm>
 #include <cmath>
 #include <cstring>

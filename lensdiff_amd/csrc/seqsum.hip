@@ -22,6 +22,10 @@
 // marked and always go the element-by-element way. The estimate only decides which blocks are
 // replayed, never the value, so the result is the sequential FP32 sum, bit for bit, for any
 // non-negative finite input.
+// Steps 1-3 run here for sums whose data no kernel of ours is producing (SeqSum::run). A kernel
+// that writes a summed array does them itself in the same pass (place_blocks in seqsum.h: the
+// block's total, its prefix by a decoupled look-back, its two binade sums from registers), and
+// SeqSum::walk runs step 4 alone.
 #include <atomic>
 #include <climits>
 #include <cstdint>
