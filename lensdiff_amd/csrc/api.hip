@@ -118,13 +118,14 @@ float* pinned(ldh_context* ctx, size_t bytes) {
     return ctx->pinned;
 }
 
-std::shared_ptr<ldh::DeviceBank> ensure_bank(ldh_context* ctx, const ldh_params& p) {
+// The bank of p holding at least its first `sets` kernel sets: the cached one, or built in its place.
+std::shared_ptr<ldh::DeviceBank> ensure_bank(ldh_context* ctx, const ldh_params& p, int sets) {
     const ldh::BankKey key = ldh::make_bank_key(p);
-    if (ctx->cached && ctx->cached_key == key) return ctx->cached_bank;
+    if (ctx->cached && ctx->cached_key == key && ctx->cached_bank->sets >= sets) return ctx->cached_bank;
     ldh::ProfileScope ps("bank-build", 0.0);
     ctx->cached = false;
     ctx->cached_bank.reset();
-    std::shared_ptr<ldh::DeviceBank> b(ldh::build_bank(ctx->stream, ctx->fft, ctx->bw, p).release());
+    std::shared_ptr<ldh::DeviceBank> b(ldh::build_bank(ctx->stream, ctx->fft, ctx->bw, p, sets).release());
     ctx->cached_bank = b;
     ctx->cached_key = key;
     ctx->cached = true;
@@ -166,7 +167,7 @@ void render_with_banks(ldh_context* ctx, const ldh_params& p, const float4* d_sr
                           (p.debug_view == LDH_VIEW_PHASE_EDGE || p.debug_view == LDH_VIEW_FIELD_PSF);
         ldh_params q = edge ? ldh::field_zone_params(p, 2, 2) : p;
         q.phase_field_strength = 0.0;
-        auto bank = ensure_bank(ctx, q);
+        auto bank = ensure_bank(ctx, q, 1);  // the static views read the full kernels only
         ldh::render_static_view(ctx->stream, ctx->fw, q, p.debug_view, *bank, d_dst, w, h);
         return;
     }
@@ -176,7 +177,7 @@ void render_with_banks(ldh_context* ctx, const ldh_params& p, const float4* d_sr
         ldh::render_frame_zones(ctx->stream, ctx->fft, ctx->fw, p, banks, d_src, d_dst, w, h, mode);
         return;
     }
-    auto bank = ensure_bank(ctx, p);
+    auto bank = ensure_bank(ctx, p, ldh::kernel_sets_needed(p));
     ldh::ProfileScope ps("frame", 0.0);
     ldh::render_frame(ctx->stream, ctx->fft, ctx->fw, p, *bank, d_src, d_dst, w, h, mode, nullptr);
 }
@@ -495,7 +496,7 @@ ldh_status ldh_bank_acquire(ldh_context* ctx, const ldh_params* p, ldh_bank** ou
         ldh_params q = *p;
         q.phase_field_strength = 0.0;
         auto* b = new ldh_bank();
-        b->bank = ensure_bank(ctx, q);
+        b->bank = ensure_bank(ctx, q, 3);
         b->key = ctx->cached_key;
         *out = b;
     });

@@ -23,6 +23,9 @@ constexpr int kMaxKernelRadius = 2048;
 // values are t * scale (scale = 1 / its sequential FP32 sum, or 1 for a non-positive sum, the
 // values LensDiffCpuReference.cpp:440-449 leaves), applied by every reader.
 struct DeviceBank {
+    // kernel sets held: the first `sets` of (full, core, structure). A render that convolves only
+    // the full kernels builds its bank with that set alone; the others are then not formed.
+    int sets = 3;
     int bins = 0;
     int pupil = 0, raw = 0;
     int max_radius = 0;   // r, K = 2r+1
@@ -41,12 +44,22 @@ struct DeviceBank {
     float* scales = nullptr;
     int64_t scales_which_stride = 0;
     std::vector<float> host_scales;  // [which][bin]
-    size_t bytes() const { return size_t(3) * bins * size_t(kernel) * kernel * sizeof(float); }
+    size_t bytes() const { return size_t(sets) * bins * size_t(kernel) * kernel * sizeof(float); }
+    void check_set(int which) const {
+        if (which < 0 || which >= sets) fail(LDH_ERR_INVALID_ARGUMENT, "kernel-set-not-built");
+    }
     const float* kernel_ptr(int which, int bin) const {
+        check_set(which);
         return store + which * which_stride + int64_t(bin) * kernel * kernel;
     }
-    const float* scale_ptr(int which, int bin) const { return scales + which * scales_which_stride + bin; }
-    float host_scale(int which, int bin) const { return host_scales[size_t(which) * bins + size_t(bin)]; }
+    const float* scale_ptr(int which, int bin) const {
+        check_set(which);
+        return scales + which * scales_which_stride + bin;
+    }
+    float host_scale(int which, int bin) const {
+        check_set(which);
+        return host_scales[size_t(which) * bins + size_t(bin)];
+    }
 };
 
 // A kernel as its readers see it: texels `data`, each multiplied by *scale when scale is set.
@@ -88,21 +101,24 @@ void resample_kernels(hipStream_t s, const float* raw, int R, const float* d_sca
 // resample_kernels, then the sequential sums of the base kernels into totals[count].
 void base_kernels(hipStream_t s, BankWorkspace& w, const float* raw, int R, const float* d_scales,
                   const float* d_raw_totals, int support, int count, float* out, float* totals, int window = 0);
-// Shapes `count` base kernels (count x K*K) into full/core/structure (3 x count x K*K) and
-// radius[count] (device). `max_radius` as LensDiffCpuReference.cpp:1633. base_totals (nullable):
-// per-kernel sums of a base that is not yet normalised (read as base / total).
-// d_gains (nullable): per-kernel anisotropy gain 1 + 4 * max(0, emphasis), else from `anisotropy`.
-// out3 is left unnormalised; out_scales (device [3][count]) receives the unit-sum multipliers.
+// Shapes `count` base kernels (count x K*K) into full/core/structure (sets x count x K*K, the
+// first `sets` of the three) and radius[count] (device). `max_radius` as
+// LensDiffCpuReference.cpp:1633. base_totals (nullable): per-kernel sums of a base that is not yet
+// normalised (read as base / total). d_gains (nullable): per-kernel anisotropy gain
+// 1 + 4 * max(0, emphasis), else from `anisotropy`. out3 is left unnormalised; out_scales (device
+// [sets][count]) receives the unit-sum multipliers.
 void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const float* base_totals, int K, int count,
                    double anisotropy, int max_radius, float* out3, int* d_radius, float* out_scales,
-                   const float* d_gains = nullptr);
+                   const float* d_gains = nullptr, int sets = 3);
 // out3 *= out_scales in place (normalised kernels for a caller that copies them out).
 void apply_kernel_scales(hipStream_t s, float* out3, int K, int count, const int* d_radius, const float* scales);
 // Radial mean profile of an R x R PSF about its centre (radialMeanProfile, :865-887), computed
 // on the device and returned to the host.
 std::vector<float> radial_profile(hipStream_t s, BankWorkspace& w, const float* psf, int R);
 
-std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWorkspace& w, const ldh_params& p);
+// The bank of p with its first `sets` kernel sets (1: full; 3: full, core, structure).
+std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWorkspace& w, const ldh_params& p,
+                                       int sets = 3);
 
 // First minimum of the clear-circle PSF for (P, R) (GetLensDiffReferenceRawPsfCached +
 // estimateFirstMinimumRadius, LensDiffCpuReference.cpp:2018-2038): built once per workspace and

@@ -198,35 +198,42 @@ reshape_kernel(const float* __restrict__ base, const float* __restrict__ base_to
     place_blocks<1, kPerSum>(pl, seg, 0, sb.block, vals, share);
 }
 
-// positiveResidualKernel of the normalised shaped and mean kernels.
-// Also places the normalised shaped kernel (segment count + seg of the sum), the support
-// estimate's total.
+// positiveResidualKernel of the normalised shaped and mean kernels (STRUCT), placed for its sum
+// (segment seg); also places the normalised shaped kernel, the support estimate's total (segment
+// count + seg with STRUCT, else seg).
+template <bool STRUCT>
 __global__ void __launch_bounds__(kSumThreads)
 residual_kernel(const float* __restrict__ shaped, const float* __restrict__ shaped_tot, const float* __restrict__ mean,
                 const float* __restrict__ mean_tot, int64_t kk, int count, float* __restrict__ out, SeqPlace pl) {
+    constexpr int N = STRUCT ? 2 : 1;
     SegBlock sb;
     if (!place_block_of(seqsum_blocks(kk), count, sb)) return;
     const int seg = sb.seg;
-    const float inv_s = inverse_total(shaped_tot, seg), inv_m = inverse_total(mean_tot, seg);
+    const float inv_s = inverse_total(shaped_tot, seg);
+    const float inv_m = STRUCT ? inverse_total(mean_tot, seg) : 0.0f;
     const int64_t start = int64_t(sb.block) * kSeqSumBlock, end = min(kk, start + kSeqSumBlock);
-    double share[2] = {0.0, 0.0};
-    float vals[2][kPerSum];
+    double share[N];
+    float vals[N][kPerSum];
+#pragma unroll
+    for (int k = 0; k < N; ++k) share[k] = 0.0;
 #pragma unroll
     for (int j = 0; j < kPerSum; ++j) {
         const int64_t i = start + threadIdx.x + int64_t(j) * kSumThreads;
-        vals[0][j] = 0.0f;
-        vals[1][j] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < N; ++k) vals[k][j] = 0.0f;
         if (i >= end) continue;
         const int64_t o = int64_t(seg) * kk + i;
         const float sn = shaped[o] * inv_s;
-        const float v = fmaxf(0.0f, sn - mean[o] * inv_m);
-        out[o] = v;
-        vals[0][j] = v;
-        vals[1][j] = sn;
-        share[0] += double(v);
-        share[1] += double(sn);
+        if constexpr (STRUCT) {
+            const float v = fmaxf(0.0f, sn - mean[o] * inv_m);
+            out[o] = v;
+            vals[0][j] = v;
+            share[0] += double(v);
+        }
+        vals[N - 1][j] = sn;
+        share[N - 1] += double(sn);
     }
-    place_blocks<2, kPerSum>(pl, seg, count, sb.block, vals, share);
+    place_blocks<N, kPerSum>(pl, seg, count, sb.block, vals, share);
 }
 
 // estimateAdaptiveSupportRadius + paddedAdaptiveSupportRadius (LensDiffCpuReference.cpp:984-1036),
@@ -300,13 +307,15 @@ support_radius_kernel(const float* __restrict__ energy, const unsigned* __restri
 }
 
 // cropKernelToRadius + applySupportBoundaryTaper on the K x K grid, for the full (shaped), core
-// (mean) and structure kernels of a bin at once (they share the radius and so the taper weight):
-// keeps the centred (2r+1)^2 square, multiplies by the cosine taper, zeroes everything else.
-// Sources are read through their totals (normalised values); dst is [3][count][K*K].
+// (mean) and structure kernels of a bin at once, the first NS of them (they share the radius and
+// so the taper weight): keeps the centred (2r+1)^2 square, multiplies by the cosine taper, zeroes
+// everything else. Sources are read through their totals (normalised values); dst is
+// [NS][count][K*K].
 struct TaperSources {
     const float* src[3];
     const float* tot[3];
 };
+template <int NS>
 __global__ void __launch_bounds__(kSumThreads)
 crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, int count, float* __restrict__ dst,
                   SeqPlace pl) {
@@ -317,9 +326,9 @@ crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, int co
     const int r = min(max(radius[bin], 1), K / 2);
     const int c = K / 2;
     const int64_t start = int64_t(sb.block) * kSeqSumBlock, end = min(kk, start + kSeqSumBlock);
-    float inv[3];
+    float inv[NS];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) inv[k] = inverse_total(ts.tot[k], bin);
+    for (int k = 0; k < NS; ++k) inv[k] = inverse_total(ts.tot[k], bin);
     const float extent = fmaxf(1.0f, float(r));
     const float fade = fmaxf(6.0f, fminf(24.0f, extent * 0.04f));
     const float ramp = fmaxf(0.0f, extent - fade);
@@ -329,7 +338,7 @@ crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, int co
     const int i0 = int(start) + int(threadIdx.x);
     const int64_t o0 = int64_t(bin) * kk;
     const float far2 = (extent + 1.0f) * (extent + 1.0f);
-    float sv[3][kPerSum];
+    float sv[NS][kPerSum];
     {
         int x = i0 % K, y = i0 / K;
 #pragma unroll
@@ -338,18 +347,20 @@ crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, int co
             const bool inside = i0 + j * kSumThreads < end && abs(x - c) <= r && abs(y - c) <= r &&
                                 fx * fx + fy * fy < far2;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) sv[k][j] = inside ? ts.src[k][o0 + i0 + j * kSumThreads] : 0.0f;
+            for (int k = 0; k < NS; ++k) sv[k][j] = inside ? ts.src[k][o0 + i0 + j * kSumThreads] : 0.0f;
             advance_texel(x, y, K);
         }
     }
-    double share[3] = {0.0, 0.0, 0.0};
-    float vals[3][kPerSum];
+    double share[NS];
+    float vals[NS][kPerSum];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) share[k] = 0.0;
     int x = i0 % K, y = i0 / K;
 #pragma unroll
     for (int j = 0; j < kPerSum; ++j) {
         const int i = i0 + j * kSumThreads;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) vals[k][j] = 0.0f;
+        for (int k = 0; k < NS; ++k) vals[k][j] = 0.0f;
         if (i >= end) continue;
         const bool inside = abs(x - c) <= r && abs(y - c) <= r;
         float w = 0.0f;
@@ -365,7 +376,7 @@ crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, int co
             }
         }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < NS; ++k) {
             const float v = inside ? sv[k][j] * inv[k] * w : 0.0f;
             dst[(int64_t(k) * count + bin) * kk + i] = v;
             vals[k][j] = v;
@@ -373,7 +384,7 @@ crop_taper_kernel(TaperSources ts, const int* __restrict__ radius, int K, int co
         }
         advance_texel(x, y, K);
     }
-    place_blocks<3, kPerSum>(pl, bin, count, sb.block, vals, share);
+    place_blocks<NS, kPerSum>(pl, bin, count, sb.block, vals, share);
 }
 
 // x[seg] *= scales[seg] (segments [3][count] of K*K) over the crop square of the segment's bin
@@ -482,7 +493,8 @@ void apply_kernel_scales(hipStream_t s, float* out3, int K, int count, const int
 
 void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const float* base_totals, int K, int count,
                    double anisotropy, int max_radius, float* out3, int* d_radius, float* out_scales,
-                   const float* d_gains) {
+                   const float* d_gains, int sets) {
+    if (sets < 1 || sets > 3) fail(LDH_ERR_INVALID_ARGUMENT, "kernel-sets");
     const int64_t kk = int64_t(K) * K;
     const int rmax_mean = K / 2;
     w.reserve_shaping(s, K, count, max_radius);
@@ -519,22 +531,35 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
         LDH_CHECK_LAUNCH("reshape");
     }
     w.seqsum.walk(s, one_array(w.shaped.as<float>(), count, kk), kk, count, t_shaped);
-    // positive residual; it also places the normalised shaped kernels (the support estimate's
-    // total), so one walk sums both: t_struct, then t_run
+    // positive residual (only when the structure kernels are formed); the pass also places the
+    // normalised shaped kernels (the support estimate's total), so one walk sums both: t_struct,
+    // then t_run
+    const bool with_structure = sets == 3;
     {
-        ProfileScope ps_("shape-residual", 12.0 * double(kk) * count);
-        hipLaunchKernelGGL(residual_kernel, grid, dim3(kSumThreads), 0, s, w.shaped.as<float>(), t_shaped,
-                           w.mean.as<float>(), t_mean, kk, count, w.structure.as<float>(),
-                           w.seqsum.place(kk, 2 * count));
+        ProfileScope ps_("shape-residual", (with_structure ? 12.0 : 4.0) * double(kk) * count);
+        if (with_structure)
+            hipLaunchKernelGGL(residual_kernel<true>, grid, dim3(kSumThreads), 0, s, w.shaped.as<float>(), t_shaped,
+                               w.mean.as<float>(), t_mean, kk, count, w.structure.as<float>(),
+                               w.seqsum.place(kk, 2 * count));
+        else
+            hipLaunchKernelGGL(residual_kernel<false>, grid, dim3(kSumThreads), 0, s, w.shaped.as<float>(), t_shaped,
+                               static_cast<const float*>(nullptr), static_cast<const float*>(nullptr), kk, count,
+                               static_cast<float*>(nullptr), w.seqsum.place(kk, count));
         LDH_CHECK_LAUNCH("residual");
     }
     SeqSegments g;
-    g.array[0] = w.structure.as<float>();
-    g.array[1] = w.shaped.as<float>();
-    g.prescale[1] = t_shaped;
     g.per_array = count;
     g.stride = kk;
-    w.seqsum.walk(s, g, kk, 2 * count, t_struct);
+    if (with_structure) {
+        g.array[0] = w.structure.as<float>();
+        g.array[1] = w.shaped.as<float>();
+        g.prescale[1] = t_shaped;
+        w.seqsum.walk(s, g, kk, 2 * count, t_struct);
+    } else {
+        g.array[0] = w.shaped.as<float>();
+        g.prescale[0] = t_shaped;
+        w.seqsum.walk(s, g, kk, count, t_run);
+    }
     // adaptive support radius of the normalised shaped kernel
     const int rmax_sup = std::min(max_radius, K / 2);
     // The last ring's energy only enters the reference's running sum at r = rmax, whose padded
@@ -550,18 +575,19 @@ void shape_kernels(hipStream_t s, BankWorkspace& w, const float* base, const flo
                            w.ring_sum.as<float>(), w.ring_peak.as<unsigned>(), t_run, rmax_sup, max_radius, d_radius);
         LDH_CHECK_LAUNCH("support-radius");
     }
-    // crop + taper of full (shaped), core (mean), structure, then normalise all three
+    // crop + taper of full (shaped), core (mean), structure (the first `sets`), then normalise them
     TaperSources ts{{w.shaped.as<float>(), w.mean.as<float>(), w.structure.as<float>()}, {t_shaped, t_mean, t_struct}};
     {
-        ProfileScope ps_("shape-crop-taper", 24.0 * double(kk) * count);
-        hipLaunchKernelGGL(crop_taper_kernel, grid, dim3(kSumThreads), 0, s, ts, d_radius, K, count, out3,
-                           w.seqsum.place(kk, 3 * count));
+        ProfileScope ps_("shape-crop-taper", 8.0 * sets * double(kk) * count);
+        auto crop = sets == 3 ? crop_taper_kernel<3> : sets == 2 ? crop_taper_kernel<2> : crop_taper_kernel<1>;
+        hipLaunchKernelGGL(crop, grid, dim3(kSumThreads), 0, s, ts, d_radius, K, count, out3,
+                           w.seqsum.place(kk, sets * count));
         LDH_CHECK_LAUNCH("crop-taper");
     }
-    w.seqsum.walk(s, one_array(out3, 3 * count, kk), kk, 3 * count, t_final);
+    w.seqsum.walk(s, one_array(out3, sets * count, kk), kk, sets * count, t_final);
     // the unit-sum scales travel with the kernels; readers multiply as they load
-    hipLaunchKernelGGL(inverse_totals_kernel, dim3(unsigned(div_up(3 * count, 256))), dim3(256), 0, s, t_final,
-                       3 * count, out_scales);
+    hipLaunchKernelGGL(inverse_totals_kernel, dim3(unsigned(div_up(sets * count, 256))), dim3(256), 0, s, t_final,
+                       sets * count, out_scales);
     LDH_CHECK_LAUNCH("kernel-scales");
 }
 
@@ -676,7 +702,9 @@ void bank_wavelength_params(const ldh_params& p, const Sizes& sz, const std::vec
     }
 }
 
-std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWorkspace& w, const ldh_params& p) {
+std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWorkspace& w, const ldh_params& p,
+                                       int sets) {
+    if (sets < 1 || sets > 3) fail(LDH_ERR_INVALID_ARGUMENT, "kernel-sets");
     const Sizes sz = resolve_sizes(p);
     if (sz.max_radius > kMaxKernelRadius) fail(LDH_ERR_TOO_LARGE, "kernel-radius-unsupported");
     if (sz.raw > kMaxRawPsf) fail(LDH_ERR_TOO_LARGE, "raw-psf-size-unsupported");
@@ -739,20 +767,22 @@ std::unique_ptr<DeviceBank> build_bank(hipStream_t s, const FftTables& t, BankWo
                      w.base_totals.as<float>());
     }
 
-    // shaping into full / core / structure (the base is read normalised through its totals)
-    bank->kernels.reset(sizeof(float) * kk * L * 3);
+    // shaping into full / core / structure, the first `sets` (the base is read normalised through
+    // its totals)
+    bank->sets = sets;
+    bank->kernels.reset(sizeof(float) * kk * L * sets);
     bank->store = bank->kernels.as<float>();
     bank->which_stride = kk * L;
     w.radius.reset(sizeof(int) * L);
-    bank->own_scales.reset(sizeof(float) * 3 * L);
+    bank->own_scales.reset(sizeof(float) * sets * L);
     bank->scales = bank->own_scales.as<float>();
     bank->scales_which_stride = L;
     shape_kernels(s, w, w.base.as<float>(), w.base_totals.as<float>(), K, L, p.anisotropy_emphasis, r,
-                  bank->kernels.as<float>(), w.radius.as<int>(), bank->scales);
+                  bank->kernels.as<float>(), w.radius.as<int>(), bank->scales, nullptr, sets);
     bank->radius.resize(size_t(L));
-    bank->host_scales.resize(size_t(3) * L);
+    bank->host_scales.resize(size_t(sets) * L);
     LDH_HIP(hipMemcpyAsync(bank->radius.data(), w.radius.get(), sizeof(int) * L, hipMemcpyDeviceToHost, s), "bank-radius");
-    LDH_HIP(hipMemcpyAsync(bank->host_scales.data(), bank->scales, sizeof(float) * 3 * L, hipMemcpyDeviceToHost, s),
+    LDH_HIP(hipMemcpyAsync(bank->host_scales.data(), bank->scales, sizeof(float) * sets * L, hipMemcpyDeviceToHost, s),
             "bank-scales");
     LDH_HIP(hipStreamSynchronize(s), "bank-radius");
     bank->support = 4;

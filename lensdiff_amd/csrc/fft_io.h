@@ -286,13 +286,16 @@ struct RowIO {
     }
 };
 
+// The pass is timed as "fft-row-pass-inv" / "-fwd" with 16 bytes per point, or, given `family`,
+// under that name with `bytes` (a pass whose loads and stores carry other data).
 template <class RowOp>
 void rows_pass(hipStream_t s, const float2* tw, int width, int nrows, const RowOp& op, bool inverse,
-               int split = 1 << 30, int gap = 0) {
+               int split = 1 << 30, int gap = 0, const char* family = nullptr, double bytes = 0.0) {
     const int nl = row_tile_lines(width);
     RowIO<RowOp> io{op, nl, nrows, split, gap};
     const int ntiles = int(div_up(nrows, nl));
-    ProfileScope ps(std::string("fft-row-pass") + (inverse ? "-inv" : "-fwd"), 16.0 * double(width) * double(nrows));
+    ProfileScope ps(family ? std::string(family) : std::string("fft-row-pass") + (inverse ? "-inv" : "-fwd"),
+                    family ? bytes : 16.0 * double(width) * double(nrows));
     if (inverse) launch_tiles<true, true>(s, width, io, tw, ntiles);
     else launch_tiles<true, false>(s, width, io, tw, ntiles);
 }

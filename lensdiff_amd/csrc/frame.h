@@ -71,6 +71,10 @@ struct FrameWork {
 // Effect-producing convolution modes.
 enum class EffectMode { PerBin, Precombined };
 
+// The kernel sets (DeviceBank::sets) a render of p convolves with: 1 when only the full kernels
+// are read (a physical look outside the core and structure views), 2 for the core view, else 3.
+int kernel_sets_needed(const ldh_params& p);
+
 // Renders one frame with the bank. `src`/`dst`: device RGBA float32, tightly packed, w x h.
 // `cached`: kernel spectra matching the bank and grid (full kernels, bins or combined), or null
 // to compute them in `fw.kspec` for this frame.

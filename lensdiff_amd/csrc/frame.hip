@@ -299,9 +299,29 @@ __global__ void spectral_map_kernel(const float* __restrict__ planes, int n, int
     for (int c = 0; c < 3; ++c) eff[int64_t(c) * npx + i] = o[c];
 }
 
-// The same map when the bins were pre-combined: planes hold natural R,G,B (and style R,G,B).
-__global__ void precombined_map_kernel(const float* __restrict__ planes, int with_style, int64_t npx, float force,
-                                       float saturation, int affects_luma, float* __restrict__ eff) {
+// The map of one pixel when the bins were pre-combined: natural R,G,B and style R,G,B in, the
+// effect R,G,B out (the blend, saturation and luma steps of spectral_map_kernel).
+struct PrecombinedMap {
+    float force, saturation;
+    int affects_luma;
+    __device__ void apply(const float nat[3], const float sty[3], float o[3]) const {
+        for (int c = 0; c < 3; ++c) o[c] = fmaxf(0.0f, nat[c] * (1.0f - force) + sty[c] * force);
+        const float gray = 0.2126f * o[0] + 0.7152f * o[1] + 0.0722f * o[2];
+        for (int c = 0; c < 3; ++c) o[c] = gray + (o[c] - gray) * saturation;
+        if (!affects_luma) {
+            const float target = 0.2126f * nat[0] + 0.7152f * nat[1] + 0.0722f * nat[2];
+            const float cur = 0.2126f * o[0] + 0.7152f * o[1] + 0.0722f * o[2];
+            if (cur > 1e-6f) {
+                const float k = target / cur;
+                for (int c = 0; c < 3; ++c) o[c] *= k;
+            }
+        }
+    }
+};
+
+// The same map over planes holding natural R,G,B (and style R,G,B).
+__global__ void precombined_map_kernel(const float* __restrict__ planes, int with_style, int64_t npx,
+                                       PrecombinedMap map, float* __restrict__ eff) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= npx) return;
     float nat[3], sty[3];
@@ -310,17 +330,7 @@ __global__ void precombined_map_kernel(const float* __restrict__ planes, int wit
         sty[c] = with_style ? planes[int64_t(3 + c) * npx + i] : nat[c];
     }
     float o[3];
-    for (int c = 0; c < 3; ++c) o[c] = fmaxf(0.0f, nat[c] * (1.0f - force) + sty[c] * force);
-    const float gray = 0.2126f * o[0] + 0.7152f * o[1] + 0.0722f * o[2];
-    for (int c = 0; c < 3; ++c) o[c] = gray + (o[c] - gray) * saturation;
-    if (!affects_luma) {
-        const float target = 0.2126f * nat[0] + 0.7152f * nat[1] + 0.0722f * nat[2];
-        const float cur = 0.2126f * o[0] + 0.7152f * o[1] + 0.0722f * o[2];
-        if (cur > 1e-6f) {
-            const float k = target / cur;
-            for (int c = 0; c < 3; ++c) o[c] *= k;
-        }
-    }
+    map.apply(nat, sty, o);
     for (int c = 0; c < 3; ++c) eff[int64_t(c) * npx + i] = o[c];
 }
 
@@ -650,6 +660,28 @@ struct CompositeParams {
     float max_sub;      // (1 - protect) / scale
 };
 
+// The redistributed image of one pixel: the decoded frame s times mask * (1 - corePreserve).
+__device__ __forceinline__ void redistributed(float4 s, float mask, const CompositeParams& cp, float r[3]) {
+    const float m = mask * cp.scale;
+    r[0] = s.x * m;
+    r[1] = s.y * m;
+    r[2] = s.z * m;
+}
+
+// The final image of one pixel from the decoded frame s, its redistributed image r and the effect
+// e: max(s - comp * r + gain * e, s - r * max_sub), clamped at zero and encoded; alpha from s.
+__device__ __forceinline__ float4 composite_final(float4 s, const float r[3], const float e[3],
+                                                  const CompositeParams& cp) {
+    const float sv[3] = {s.x, s.y, s.z};
+    float o[3];
+    for (int c = 0; c < 3; ++c) {
+        const float raw = fmaxf(0.0f, sv[c] - cp.comp * r[c] + cp.gain * e[c]);
+        const float flo = fmaxf(0.0f, sv[c] - r[c] * cp.max_sub);
+        o[c] = encode_channel(fmaxf(raw, flo), cp.tf);
+    }
+    return make_float4(o[0], o[1], o[2], s.w);
+}
+
 // eff *= factor[0] where the Preserve rescale applies (factor[1] > 0): scaleImage on the effect
 // before the post passes read it (LensDiffCpuReference.cpp:2172-2178).
 __global__ void scale_planes_kernel(float* __restrict__ eff, int64_t n, const float* __restrict__ factor) {
@@ -725,16 +757,13 @@ __global__ void composite_kernel(const float4* __restrict__ src, const float* __
         dst[i] = make_float4(e[0], e[1], e[2], 1.0f);
         return;
     }
-    const float m = redist ? 0.0f : mask[i] * cp.scale;
-    const float sv[3] = {s.x, s.y, s.z};
-    float o[3];
-    for (int c = 0; c < 3; ++c) {
-        const float r = redist ? redist[int64_t(c) * npx + i] : sv[c] * m;
-        const float raw = fmaxf(0.0f, sv[c] - cp.comp * r + cp.gain * e[c]);
-        const float flo = fmaxf(0.0f, sv[c] - r * cp.max_sub);
-        o[c] = encode_channel(fmaxf(raw, flo), cp.tf);
+    float r[3];
+    if (redist) {
+        for (int c = 0; c < 3; ++c) r[c] = redist[int64_t(c) * npx + i];
+    } else {
+        redistributed(s, mask[i], cp, r);
     }
-    dst[i] = make_float4(o[0], o[1], o[2], s.w);
+    dst[i] = composite_final(s, r, e, cp);
 }
 
 // out[r] = sum_b m[r][b] * (kernels[b] * scales[b]) (rows <= 6), accumulated over b in order for
@@ -895,6 +924,12 @@ Plan plan_of(const ldh_params& p) {
 
 }  // namespace
 
+int kernel_sets_needed(const ldh_params& p) {
+    int sets = 1;
+    for (int s : plan_of(p).sets) sets = std::max(sets, s + 1);
+    return sets;
+}
+
 std::vector<KernelRef> effect_kernels(const ldh_params& p, const DeviceBank& bank, EffectMode mode,
                                       const float* combined, int* which_out) {
     const Plan pl = plan_of(p);
@@ -932,16 +967,56 @@ void select_stage(hipStream_t s, FrameWork& fw, const ldh_params& p, const float
     }
 }
 
+// The final composite folded into the effect's last pass: the output frame and the composite's
+// parameters (for a frame whose effect goes to the output unchanged: no Preserve rescale, no
+// scatter or fringe pass, no display resampling, the final view).
+struct FusedFinal {
+    float4* dst;
+    CompositeParams cp;
+};
+
+// The last row pass of the pre-combined effect with the map and the composite folded in: the
+// transform yields channels NK - 2 and NK - 1 (ExtractRowsHalf's pair), the channels before them
+// are read from `planes` (channel c at planes + c * npx); each pixel then goes through
+// PrecombinedMap and composite_final exactly as precombined_map_kernel and composite_kernel take
+// it, and only the output pixel is stored. NK = 3: natural R, G, B; NK = 6: then style R, G, B.
+template <int NK>
+struct FinalRowsHalf : ExtractRowsHalf {
+    const float* planes;
+    const float4* src;
+    const float* mask;
+    float4* dst;
+    int64_t npx;
+    PrecombinedMap map;
+    CompositeParams cp;
+    __device__ void store(int row, int x, float2 v) const {
+        if (row >= h || x >= w) return;
+        const size_t i = size_t(row) * w + x;
+        const float4 q = decoded(src, i, cp.tf);
+        const float mk = mask[i];
+        float ch[NK];
+#pragma unroll
+        for (int c = 0; c < NK - 2; ++c) ch[c] = planes[int64_t(c) * npx + i];
+        ch[NK - 2] = fmaxf(0.0f, v.x * scale);
+        ch[NK - 1] = fmaxf(0.0f, v.y * scale);
+        float e[3], r[3];
+        map.apply(ch, NK == 6 ? ch + 3 : ch, e);
+        redistributed(q, mk, cp, r);
+        dst[i] = composite_final(q, r, e, cp);
+    }
+};
+
 // Steps 2-4 for the spectral full-kernel effect with pre-combined RGB kernels (natural, and style
 // when styled), on half spectra: the driver is real, so its transform is stored for columns
 // 0 .. pw/2 only (DriverRowsHalf, two rows per row transform); each combined kernel likewise;
-// each channel's product is inverted over the half-width columns, and channels are paired for
-// the last row pass (ExtractRowsHalf), which yields two real planes per row transform. The
-// planes then go through the pre-combined spectral map into effect9[0..3).
+// each channel's product is inverted over the half-width columns, and channels are grouped in
+// pairs (an odd count starts with one alone) for the last row pass (ExtractRowsHalf), which yields
+// two real planes per row transform. The planes then go through the pre-combined spectral map
+// into effect9[0..3) or, with `fin`, the last group's pass maps and composites into fin->dst.
 void effect_precombined(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
                         const float4* src, int w, int h, const Grid& g, const SpectrumConfig& cfg, bool with_style,
                         float force, float redist_scale, const KernelSpectra* cached, float* effect9,
-                        bool image_ready) {
+                        bool image_ready, const FusedFinal* fin) {
     const int64_t npx = int64_t(w) * h;
     const size_t hc = g.half_cells();
     const float2* tw = t.twiddle.as<float2>();
@@ -972,13 +1047,14 @@ void effect_precombined(hipStream_t s, const FftTables& t, FrameWork& fw, const 
     }
     if (!ks->half || ks->grid.pw != g.pw || ks->grid.ph != g.ph || ks->kernels != nk)
         fail(LDH_ERR_INVALID_ARGUMENT, "cached-spectra-mismatch");
-    // 4. per channel: product and inverse columns; per channel pair: one row pass to two planes
+    // 4. per channel: product and inverse columns; per group: one row pass to one or two planes
     fw.out_spec.reset(sizeof(float2) * hc * 2);
     fw.planes.reset(sizeof(float) * npx * 6);
     float* planes = fw.planes.as<float>();
     const float inv_cells = 1.0f / float(g.cells());
-    for (int c = 0; c < nk; c += 2) {
-        const int m = std::min(2, nk - c);
+    const PrecombinedMap map{force, float(std::max(0.0, p.spectrum_saturation)), p.chromatic_affects_luma};
+    for (int c = 0; c < nk;) {
+        const int m = c == 0 && nk % 2 == 1 ? 1 : 2;
         for (int j = 0; j < m; ++j) {
             float2* o = fw.out_spec.as<float2>() + hc * j;
             // only rows < h reach the crop
@@ -987,23 +1063,39 @@ void effect_precombined(hipStream_t s, const FftTables& t, FrameWork& fw, const 
             else
                 cols_inverse(s, tw, ispec, o, g.hw, g.ph, MulSpectrumLoad{ks->data.as<float2>() + hc * (c + j)}, h);
         }
-        ExtractRowsHalf ex{fw.out_spec.as<float2>(), m == 2 ? fw.out_spec.as<float2>() + hc : nullptr,
-                           planes + npx * c, m == 2 ? planes + npx * (c + 1) : nullptr, w, h, g.hw, g.pw, inv_cells};
-        rows_pass(s, tw, g.pw, h, ex, true);
+        const ExtractRowsHalf ex{fw.out_spec.as<float2>(), m == 2 ? fw.out_spec.as<float2>() + hc : nullptr,
+                                 planes + npx * c, m == 2 ? planes + npx * (c + 1) : nullptr, w, h, g.hw, g.pw,
+                                 inv_cells};
+        c += m;
+        if (c < nk || !fin) {
+            rows_pass(s, tw, g.pw, h, ex, true);
+            continue;
+        }
+        // spectra in, then per pixel the earlier planes, frame and mask in and the output pixel out
+        const double bytes = 16.0 * double(g.hw) * h + (4.0 * (nk - 2) + 36.0) * double(npx);
+        if (nk == 6)
+            rows_pass(s, tw, g.pw, h, FinalRowsHalf<6>{ex, planes, src, fw.mask.as<float>(), fin->dst, npx, map, fin->cp},
+                      true, 1 << 30, 0, "fft-row-pass-final", bytes);
+        else
+            rows_pass(s, tw, g.pw, h, FinalRowsHalf<3>{ex, planes, src, fw.mask.as<float>(), fin->dst, npx, map, fin->cp},
+                      true, 1 << 30, 0, "fft-row-pass-final", bytes);
+        return;
     }
-    ProfileScope ps_map("frame-spectral-map", (4.0 * 6 + 12.0) * double(npx));
-    hipLaunchKernelGGL(precombined_map_kernel, grid1(npx), dim3(256), 0, s, planes, with_style ? 1 : 0, npx, force,
-                       float(std::max(0.0, p.spectrum_saturation)), p.chromatic_affects_luma, effect9);
+    ProfileScope ps_map("frame-spectral-map", (4.0 * nk + 12.0) * double(npx));
+    hipLaunchKernelGGL(precombined_map_kernel, grid1(npx), dim3(256), 0, s, planes, with_style ? 1 : 0, npx, map,
+                       effect9);
     LDH_CHECK_LAUNCH("effect-map");
 }
 
 // Steps 2-4 for one bank: the effect planes [effect, core, structure] (3 each; only the effect
 // planes unless the look is split) into effect9 (npx * 9 floats). shared_grid (nullable): the
 // grid to convolve on instead of the bank's own (the zones of a field-dependent phase share one);
-// image_ready: fw.image_spec already holds the frame's spectra on that grid.
-void effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
+// image_ready: fw.image_spec already holds the frame's spectra on that grid. fin (nullable): the
+// final composite to fold into the last pass where the effect takes the pre-combined path; returns
+// whether it was (fin->dst then holds the output frame and effect9 is untouched).
+bool effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
                   const float4* src, int w, int h, EffectMode mode, const KernelSpectra* cached, float* effect9,
-                  const Grid* shared_grid = nullptr, bool image_ready = false) {
+                  const Grid* shared_grid = nullptr, bool image_ready = false, const FusedFinal* fin = nullptr) {
     const Plan pl = plan_of(p);
     const int64_t npx = int64_t(w) * h;
     const Grid g = shared_grid != nullptr ? *shared_grid : make_grid(w, h, bank.support);
@@ -1019,8 +1111,8 @@ void effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
 
     if (mode == EffectMode::Precombined) {
         effect_precombined(s, t, fw, p, bank, src, w, h, g, cfg, with_style, force, redist_scale, cached, effect9,
-                           image_ready);
-        return;
+                           image_ready, fin);
+        return fin != nullptr;
     }
 
     // 2. image spectra: mono -> (R + iG), (B); spectral -> driver
@@ -1040,7 +1132,7 @@ void effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
 
     // 3-4. per kernel set: spectra x image spectra -> planes -> 3 effect planes
     fw.out_spec.reset(sizeof(float2) * cells);
-    const int nplanes = pl.spectral ? (mode == EffectMode::Precombined ? 6 : n) : 3;
+    const int nplanes = pl.spectral ? n : 3;
     fw.planes.reset(sizeof(float) * npx * std::max(nplanes, 3));
     fw.matrices.reset(sizeof(float) * 6 * n);
     LDH_HIP(hipMemcpyAsync(fw.matrices.as<float>(), cfg.natural.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, s),
@@ -1054,21 +1146,14 @@ void effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
         const int set = pl.sets[si];
         // kernels of this set
         std::vector<KernelRef> ks;
-        const int64_t kk = int64_t(bank.kernel) * bank.kernel;
         const bool use_cache = cached != nullptr && si == 0 && pl.sets.size() == 1;
         if (use_cache) {
             // the cached spectra stand for the kernels; only their count matters here
             ks.assign(size_t(cached->kernels), KernelRef{});
         } else if (!pl.spectral) {
             ks.push_back({bank.kernel_ptr(set, 0), bank.scale_ptr(set, 0)});
-        } else if (mode == EffectMode::PerBin) {
-            for (int b = 0; b < n; ++b) ks.push_back({bank.kernel_ptr(set, b), bank.scale_ptr(set, b)});
         } else {
-            fw.combined.reset(sizeof(float) * kk * 6);
-            combine_kernels(s, bank.kernel_ptr(set, 0), bank.scale_ptr(set, 0), n, bank.kernel, cfg, with_style,
-                            fw.combined.as<float>(), fw.kspec_tmp);
-            const int nk = with_style ? 6 : 3;
-            for (int r = 0; r < nk; ++r) ks.push_back({fw.combined.as<float>() + r * kk, nullptr});
+            for (int b = 0; b < n; ++b) ks.push_back({bank.kernel_ptr(set, b), bank.scale_ptr(set, b)});
         }
         const int pairs = pl.spectral ? int((ks.size() + 1) / 2) : 1;
         if (use_cache && (cached->grid.pw != g.pw || cached->grid.ph != g.ph || cached->pairs != pairs))
@@ -1113,15 +1198,11 @@ void effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
         float* eff = effect9 + npx * 3 * (pl.split ? 1 + int(si) : 0);
         if (!pl.spectral) {
             LDH_HIP(hipMemcpyAsync(eff, planes, sizeof(float) * npx * 3, hipMemcpyDeviceToDevice, s), "effect-copy");
-        } else if (mode == EffectMode::PerBin) {
+        } else {
             ProfileScope ps_map("frame-spectral-map", (4.0 * n + 12.0) * double(npx));
             hipLaunchKernelGGL(spectral_map_kernel, grid1(npx), dim3(256), 0, s, planes, n, npx, fw.matrices.as<float>(),
                                fw.matrices.as<float>() + 3 * n, force, float(std::max(0.0, p.spectrum_saturation)),
                                p.chromatic_affects_luma, eff);
-        } else {
-            ProfileScope ps_map("frame-spectral-map", (4.0 * 6 + 12.0) * double(npx));
-            hipLaunchKernelGGL(precombined_map_kernel, grid1(npx), dim3(256), 0, s, planes, with_style ? 1 : 0, npx,
-                               force, float(std::max(0.0, p.spectrum_saturation)), p.chromatic_affects_luma, eff);
         }
         LDH_CHECK_LAUNCH("effect-map");
     }
@@ -1133,9 +1214,9 @@ void effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
                            float(std::max(0.0, p.structure_gain)), core_or_structure_view(p) ? 1 : 0, effect9);
         LDH_CHECK_LAUNCH("split-combine");
     }
+    return false;
 }
 
-// Steps 5-6 on the effect planes in fw.effect: Preserve rescale and composite.
 // buildGaussianKernel (LensDiffCpuReference.cpp:1680-1700) on the host: unnormalised weights
 // exp(-(x^2+y^2) / (2 sigma^2)) summed in raster order in float, then each divided by the sum.
 std::vector<float> gaussian_kernel(float radius_px, int& size) {
@@ -1205,6 +1286,26 @@ struct Display {
     int w, h, tf;
 };
 
+bool scatter_on(const ldh_params& p) { return p.scatter_amount > 1e-6 && scatter_radius_px(p) > 0.25; }
+float fringe_px(const ldh_params& p) { return float(std::max(0.0, creative_fringe_px(p))); }
+
+// The composite's parameters (LensDiffCpuReference.cpp:2096-2109, used by the final mix at
+// :2203-2217): Preserve caps the gain at 1 and compensates by it; protect = max(0.2, corePreserve).
+CompositeParams composite_params(const ldh_params& p) {
+    const bool preserve = p.energy_mode == LDH_ENERGY_PRESERVE;
+    const float redist_scale = 1.0f - float(std::max(0.0, std::min(p.core_preserve, 1.0)));
+    const float gain = preserve ? float(std::max(0.0, std::min(p.effect_gain, 1.0))) : float(std::max(0.0, p.effect_gain));
+    const float protect = std::max(0.2f, float(std::max(0.0, std::min(p.core_preserve, 1.0))));
+    CompositeParams cp;
+    cp.tf = p.input_transfer;
+    cp.view = p.debug_view;
+    cp.scale = redist_scale;
+    cp.comp = preserve ? gain : float(std::max(0.0, p.core_compensation));
+    cp.gain = gain;
+    cp.max_sub = redist_scale > 1e-6f ? (1.0f - protect) / redist_scale : 0.0f;
+    return cp;
+}
+
 // Steps 5-6 on the effect planes in fw.effect of the (working) frame src, w x h. With `disp`, the
 // effect (or view) planes, the redistributed image and the mask are resampled to the display
 // frame and the composite runs there (LensDiffCpuReference.cpp:2192-2220).
@@ -1221,8 +1322,6 @@ void finish_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
 
     // 5. Preserve rescale
     const bool preserve = p.energy_mode == LDH_ENERGY_PRESERVE;
-    const float gain = preserve ? float(std::max(0.0, std::min(p.effect_gain, 1.0))) : float(std::max(0.0, p.effect_gain));
-    const float comp = preserve ? gain : float(std::max(0.0, p.core_compensation));
     const float* factor = nullptr;
     if (preserve) {
         fw.lum.reset(sizeof(float) * npx * 2);
@@ -1238,8 +1337,8 @@ void finish_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
 
     // 5b. scatter and creative fringe on the rescaled effect (:2179-2190); they change only the
     // final image and the effect, scatter and fringe views
-    const bool scatter = p.scatter_amount > 1e-6 && scatter_radius_px(p) > 0.25;
-    const float fringe = float(std::max(0.0, creative_fringe_px(p)));
+    const bool scatter = scatter_on(p);
+    const float fringe = fringe_px(p);
     const bool post_view = p.debug_view == LDH_VIEW_SCATTER || p.debug_view == LDH_VIEW_CREATIVE_FRINGE;
     const bool post_out = p.debug_view == LDH_VIEW_FINAL || p.debug_view == LDH_VIEW_EFFECT || post_view;
     const float* post_effect = fw.effect.as<float>();  // the effect after the post passes
@@ -1287,14 +1386,7 @@ void finish_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
         hipLaunchKernelGGL(max_kernel, dim3(256), dim3(256), 0, s, fw.mask.as<float>(), npx, fw.max_buf.as<unsigned>());
         mask_max = fw.max_buf.as<unsigned>();
     }
-    const float protect = std::max(0.2f, float(std::max(0.0, std::min(p.core_preserve, 1.0))));
-    CompositeParams cp;
-    cp.tf = p.input_transfer;
-    cp.view = p.debug_view;
-    cp.scale = redist_scale;
-    cp.comp = comp;
-    cp.gain = gain;
-    cp.max_sub = redist_scale > 1e-6f ? (1.0f - protect) / redist_scale : 0.0f;
+    CompositeParams cp = composite_params(p);
     const float* eff_out = (p.debug_view == LDH_VIEW_FINAL || p.debug_view == LDH_VIEW_EFFECT || !pl.split)
                                ? post_effect : effect;
     if (p.debug_view == LDH_VIEW_CORE || p.debug_view == LDH_VIEW_STRUCTURE) {
@@ -1351,17 +1443,16 @@ void finish_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
 namespace {
 
 // Steps 1-4 on the frame src (w x h): the selection mask, then the effect planes of the bank or,
-// with zone banks, of every zone blended into fw.effect.
-void effect_planes(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank* bank,
+// with zone banks, of every zone blended into fw.effect. With one bank, fin (nullable) is passed
+// on to effect_stage; returns whether the final composite was folded in there.
+bool effect_planes(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank* bank,
                    const std::vector<const DeviceBank*>* zones, const float4* src, int w, int h, EffectMode mode,
-                   const KernelSpectra* cached, const std::vector<const KernelSpectra*>* zone_cached) {
+                   const KernelSpectra* cached, const std::vector<const KernelSpectra*>* zone_cached,
+                   const FusedFinal* fin = nullptr) {
     const int64_t npx = int64_t(w) * h;
     select_stage(s, fw, p, src, npx);
     fw.effect.reset(sizeof(float) * npx * 9);
-    if (!zones) {
-        effect_stage(s, t, fw, p, *bank, src, w, h, mode, cached, fw.effect.as<float>());
-        return;
-    }
+    if (!zones) return effect_stage(s, t, fw, p, *bank, src, w, h, mode, cached, fw.effect.as<float>(), nullptr, false, fin);
     // the core and structure planes of a split look reach the output only in their views
     const int planes = plan_of(p).split && core_or_structure_view(p) ? 9 : 3;
     fw.zone.reset(sizeof(float) * npx * 9);
@@ -1378,6 +1469,7 @@ void effect_planes(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_p
                            fw.effect.as<float>(), planes, w, h, z % 3, z / 3);
         LDH_CHECK_LAUNCH("zone-accumulate");
     }
+    return false;
 }
 
 // The whole frame; a resolution-aware render (working scale optics / frame short side away from
@@ -1389,7 +1481,13 @@ void render_impl(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_par
                  const std::vector<const KernelSpectra*>* zone_cached = nullptr) {
     const double ws = working_scale(p);
     if (!resolution_aware_active(p)) {
-        effect_planes(s, t, fw, p, bank, zones, src, w, h, mode, cached, zone_cached);
+        // a final image with nothing between the effect and the composite: the composite runs in
+        // the effect's last pass where the effect takes the pre-combined path
+        const bool direct = !zones && p.debug_view == LDH_VIEW_FINAL && p.energy_mode != LDH_ENERGY_PRESERVE &&
+                            !scatter_on(p) && !(fringe_px(p) > 1e-6f);
+        const FusedFinal fin{dst, composite_params(p)};
+        if (effect_planes(s, t, fw, p, bank, zones, src, w, h, mode, cached, zone_cached, direct ? &fin : nullptr))
+            return;
         finish_stage(s, t, fw, p, src, dst, w, h, nullptr);
         return;
     }

@@ -36,10 +36,17 @@ int cached_kernel_count(const ldh_params& p) {
     return 0;
 }
 
+// The kernel sets of a request's bank: the zone banks of a field-dependent phase hold all three;
+// a single bank the sets its renders convolve with (a static view reads the full kernels only).
+int served_sets(const ldh_params& p) {
+    if (has_field_phase(p)) return 3;
+    return is_static_view(p.debug_view) ? 1 : kernel_sets_needed(p);
+}
+
 size_t bank_bytes_estimate(const ldh_params& p) {
     const Sizes s = resolve_sizes(p);
     const size_t kk = size_t(2 * s.max_radius + 1) * size_t(2 * s.max_radius + 1);
-    return kk * 4 * (3 * wavelength_list(p).size() + 6) * (has_field_phase(p) ? kFieldZones : 1);
+    return kk * 4 * (size_t(served_sets(p)) * wavelength_list(p).size() + 6) * (has_field_phase(p) ? kFieldZones : 1);
 }
 
 // The parameters of the nine zone banks of a field-dependent phase, each built from its resolved
@@ -53,9 +60,13 @@ std::vector<ldh_params> zone_requests(const ldh_params& p) {
     return zones;
 }
 
-// Cache identity of a request: its bank key, or its zones' keys in order.
+// Cache identity of a request: its bank key and kernel sets, or its zones' keys in order.
 BankKey served_key(const ldh_params& p) {
-    if (!has_field_phase(p)) return make_bank_key(p);
+    if (!has_field_phase(p)) {
+        BankKey k = make_bank_key(p);
+        k.values.push_back(double(served_sets(p)));
+        return k;
+    }
     BankKey k;
     k.values.push_back(double(kFieldZones));
     for (const ldh_params& z : zone_requests(p)) {
@@ -346,7 +357,8 @@ void Server::worker(Slot* s) {
                         zones = build_banks_batched(s->stream, fft_, s->bw, zone_requests(r->params), LDH_BATCH_GROUPED,
                                                     nullptr);
                     else
-                        bank = std::shared_ptr<DeviceBank>(build_bank(s->stream, fft_, s->bw, r->params).release());
+                        bank = std::shared_ptr<DeviceBank>(
+                            build_bank(s->stream, fft_, s->bw, r->params, served_sets(r->params)).release());
                     if (spectral_fullkernel(r->params)) {
                         // the precombined RGB kernels of the bank, or of each zone (zone z at
                         // z * 6 * K * K; the zones share their kernel size)

@@ -345,17 +345,23 @@ def test_precombined_matches_per_bin(ctx):
     assert rel_l2(b, a) < 1e-5
 
 
-@pytest.mark.parametrize("w,h,style", [(97, 63, 0), (120, 81, 1), (64, 17, 1)],
-                         ids=["odd-97x63-natural", "odd-120x81-styled", "odd-64x17-styled"])
-def test_precombined_half_spectra_odd_frames(ctx, oracle, w, h, style):
+@pytest.mark.parametrize("w,h,style,energy", [(97, 63, 0, "augment"), (120, 81, 1, "augment"), (64, 17, 1, "augment"),
+                                               (97, 63, 0, "preserve"), (120, 81, 1, "preserve")],
+                         ids=["odd-97x63-natural", "odd-120x81-styled", "odd-64x17-styled",
+                              "odd-97x63-natural-preserve", "odd-120x81-styled-preserve"])
+def test_precombined_half_spectra_odd_frames(ctx, oracle, w, h, style, energy):
     """The pre-combined path runs on half spectra: the driver's rows are transformed in pairs (an
     odd height leaves the last row unpaired) and channels share the last row transform in pairs
-    (three natural channels leave one unpaired). Odd widths and heights, natural and styled,
-    against the per-bin path and the oracle."""
+    (three natural channels leave one unpaired). With Augment and the final view the map and the
+    composite run in the last row pass; with Preserve they run as passes of their own after the
+    rescale. Odd widths and heights, natural and styled, against the per-bin path and the oracle."""
     p = workloads.base_params(min(w, h))
     p.pupil_resolution = 64
     p.max_kernel_radius_px = 20.0
     p.set_wavelengths(np.linspace(400, 700, 9))
+    if energy == "preserve":
+        p.energy_mode = _abi.ENERGY_PRESERVE
+        p.core_preserve = 0.3
     if style:
         p.spectrum_style = _abi.STYLE_WARM_COOL
         p.spectrum_force = 0.5

@@ -76,6 +76,45 @@ def test_served_mix_matches_alone_and_plain(ctx):
     assert st.bank_misses == 3 and st.completed == 20
 
 
+def test_bank_kernel_sets_follow_the_look(ctx):
+    """A render that convolves only the full kernels (a physical look) builds its bank with that
+    set alone; the bank interface and a split look get all three. The full kernels are the same
+    bit for bit either way (every sum is the sequential FP32 sum, whatever else is summed beside
+    it), so a frame is identical whether its bank holds one set or three. Served, a physical and a
+    split request of one lens are two cache entries."""
+    p = lens(5)
+    q = type(p).from_buffer_copy(p)
+    q.look_mode = _abi.LOOK_SPLIT
+    img = workloads.frame(160, 96, 0.004, seed=7)
+    h, w = img.shape[:2]
+    one_set = ctx.render_image(p, img)      # bank with the full kernels alone
+    sets3 = ctx.bank(p)                      # the bank interface: all three sets, cached for p
+    three_sets = ctx.render_image(p, img)   # the cached three-set bank
+    assert np.array_equal(one_set, three_sets)
+    assert all(np.abs(k).sum() > 0 for k in sets3["core"] + sets3["structure"])
+    split = ctx.render_image(q, img)
+    s = ctx.alloc(img.nbytes)
+    d = ctx.alloc(img.nbytes)
+    ctx.upload(s, img)
+    ctx.server_start(2, 4 << 30)
+    try:
+        served = []
+        for r in (p, q, p):
+            ctx.wait(ctx.submit(r, s, d, w, h))
+            out = np.empty_like(img)
+            ctx.download(d, out)
+            served.append(out)
+        st = ctx.server_stats()
+    finally:
+        ctx.server_stop()
+        ctx.free(s)
+        ctx.free(d)
+    assert rel_l2(served[0], one_set) <= 1e-5   # pre-combined spectra vs per-bin convolution
+    assert rel_l2(served[1], split) <= 1e-6
+    assert np.array_equal(served[2], served[0])
+    assert st.bank_misses == 2 and st.bank_hits == 1
+
+
 def test_served_rejects_unsupported_and_reports(ctx):
     p = lens(4)
     p.aperture_mode = _abi.APERTURE_CUSTOM

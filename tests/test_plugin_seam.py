@@ -9,6 +9,7 @@ built anywhere else.
 The tags follow the plugin's backends: RunLensDiffCuda's early exits
 (LensDiffCuda.cu:4642-4657) and the error-string convention of LensDiff.cpp:3830-3838.
 """
+import fcntl
 import os
 import subprocess
 
@@ -26,7 +27,11 @@ pytestmark = pytest.mark.skipif(
 
 @pytest.fixture(scope="module")
 def seam_check():
-    subprocess.run(["make", "-s", "-j8", "-C", SEAM], check=True, capture_output=True, timeout=600)
+    # parallel test workers each run this fixture: one make at a time into the shared _build/
+    os.makedirs(os.path.join(SEAM, "_build"), exist_ok=True)
+    with open(os.path.join(SEAM, "_build", ".lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        subprocess.run(["make", "-s", "-j8", "-C", SEAM], check=True, capture_output=True, timeout=600)
     return os.path.join(SEAM, "_build", "seam_check")
 
 
