@@ -54,9 +54,9 @@ void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<K
                           bool half = false);
 // Combined RGB kernels of a spectral bank: out[c] = sum_b M[c][b] (k_b * scales[b]) for the
 // natural (c=0..2) and, if with_style, style (c=3..5) matrices. out: device float[6][K*K],
-// unit scale.
+// unit scale. bins <= LDH_MAX_WAVELENGTHS.
 void combine_kernels(hipStream_t s, const float* kernels, const float* scales, int bins, int K,
-                     const SpectrumConfig& cfg, bool with_style, float* out, DeviceBuffer& scratch);
+                     const SpectrumConfig& cfg, bool with_style, float* out);
 
 // Scratch and outputs of one frame on one stream.
 struct FrameWork {

@@ -766,22 +766,41 @@ __global__ void composite_kernel(const float4* __restrict__ src, const float* __
     dst[i] = composite_final(s, r, e, cp);
 }
 
-// out[r] = sum_b m[r][b] * (kernels[b] * scales[b]) (rows <= 6), accumulated over b in order for
-// every r; each kernel texel is read once.
-__global__ void combine_kernel(const float* __restrict__ kernels, const float* __restrict__ scales, int bins,
-                               int64_t kk, const float* __restrict__ m, int rows, float* __restrict__ out) {
+// The rows of a combine: m[r * bins + b] for r < ROWS, passed by value.
+struct CombineMatrix {
+    float m[6 * LDH_MAX_WAVELENGTHS];
+};
+
+// out[r] = sum_b m[r][b] * (kernels[b] * scales[b]) (ROWS = 3 or 6), accumulated over b in order
+// for every r; each kernel texel is read once, eight bins' texels loaded before they are used.
+template <int ROWS>
+__global__ void __launch_bounds__(256)
+combine_kernel(const float* __restrict__ kernels, const float* __restrict__ scales, int bins, int64_t kk,
+               CombineMatrix cm, float* __restrict__ out) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= kk) return;
-    float acc[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    for (int b = 0; b < bins; ++b) {
-        const float k = kernels[int64_t(b) * kk + i] * scales[b];
+    float acc[ROWS];
 #pragma unroll
-        for (int r = 0; r < 6; ++r)
-            if (r < rows) acc[r] += m[r * bins + b] * k;
+    for (int r = 0; r < ROWS; ++r) acc[r] = 0.0f;
+    int b = 0;
+    for (; b + 8 <= bins; b += 8) {
+        float k[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) k[j] = kernels[int64_t(b + j) * kk + i];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float v = k[j] * scales[b + j];
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) acc[r] += cm.m[r * bins + b + j] * v;
+        }
+    }
+    for (; b < bins; ++b) {
+        const float v = kernels[int64_t(b) * kk + i] * scales[b];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) acc[r] += cm.m[r * bins + b] * v;
     }
 #pragma unroll
-    for (int r = 0; r < 6; ++r)
-        if (r < rows) out[int64_t(r) * kk + i] = acc[r];
+    for (int r = 0; r < ROWS; ++r) out[int64_t(r) * kk + i] = acc[r];
 }
 
 inline dim3 grid1(int64_t n) { return dim3(unsigned(div_up(n, 256))); }
@@ -870,24 +889,22 @@ void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<K
 }
 
 void combine_kernels(hipStream_t s, const float* kernels, const float* scales, int bins, int K,
-                     const SpectrumConfig& cfg, bool with_style, float* out, DeviceBuffer& scratch) {
+                     const SpectrumConfig& cfg, bool with_style, float* out) {
+    if (bins < 1 || bins > LDH_MAX_WAVELENGTHS) fail(LDH_ERR_INVALID_ARGUMENT, "combine-bins");
     const int rows = with_style ? 6 : 3;
-    std::vector<float> m(static_cast<size_t>(rows * bins));
+    CombineMatrix cm{};
     for (int r = 0; r < 3; ++r)
         for (int b = 0; b < bins; ++b) {
-            m[size_t(r * bins + b)] = cfg.natural[size_t(r * bins + b)];
-            if (with_style) m[size_t((3 + r) * bins + b)] = cfg.style[size_t(r * bins + b)];
+            cm.m[r * bins + b] = cfg.natural[size_t(r * bins + b)];
+            if (with_style) cm.m[(3 + r) * bins + b] = cfg.style[size_t(r * bins + b)];
         }
-    scratch.reset(sizeof(float) * m.size());
-    LDH_HIP(hipMemcpyAsync(scratch.get(), m.data(), sizeof(float) * m.size(), hipMemcpyHostToDevice, s), "combine");
     const int64_t kk = int64_t(K) * K;
-{
     ProfileScope ps_("spectra-combine", 4.0 * double(kk) * (bins + rows));
-        hipLaunchKernelGGL(combine_kernel, grid1(kk), dim3(256), 0, s, kernels, scales, bins, kk, scratch.as<float>(),
-                           rows, out);
+    if (with_style)
+        hipLaunchKernelGGL(combine_kernel<6>, grid1(kk), dim3(256), 0, s, kernels, scales, bins, kk, cm, out);
+    else
+        hipLaunchKernelGGL(combine_kernel<3>, grid1(kk), dim3(256), 0, s, kernels, scales, bins, kk, cm, out);
     LDH_CHECK_LAUNCH("combine-kernels");
-    }
-    LDH_HIP(hipStreamSynchronize(s), "combine");
 }
 
 namespace {
@@ -1039,7 +1056,7 @@ void effect_precombined(hipStream_t s, const FftTables& t, FrameWork& fw, const 
         const int64_t kk = int64_t(bank.kernel) * bank.kernel;
         fw.combined.reset(sizeof(float) * kk * 6);
         combine_kernels(s, bank.kernel_ptr(LDH_KERNEL_FULL, 0), bank.scale_ptr(LDH_KERNEL_FULL, 0), bank.bins,
-                        bank.kernel, cfg, with_style, fw.combined.as<float>(), fw.kspec_tmp);
+                        bank.kernel, cfg, with_style, fw.combined.as<float>());
         std::vector<KernelRef> refs;
         for (int r = 0; r < nk; ++r) refs.push_back({fw.combined.as<float>() + r * kk, nullptr});
         build_kernel_spectra(s, t, refs, bank.kernel, bank.support, g, false, fw.kspec, fw.kspec_tmp, true);

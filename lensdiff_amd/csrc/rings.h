@@ -11,8 +11,10 @@
 
 namespace ldh {
 
-// Raster pixels per band of the ring-order copy.
-constexpr int kRingBand = 8192;
+// Raster pixels per band of the ring-order copy (64 KiB of LDS a workgroup). A band's pixels of
+// one ring are stored as one run, so the run length grows with the band: at K = 2047 a band of
+// 16384 covers 8 rows and a ring's run at its sides 16 pixels.
+constexpr int kRingBand = 16384;
 
 // Pixels of a size x size image grouped by ring about `center`: ring(x, y) = round(|d|)
 // (ceil = false) or ceil(|d|) (ceil = true), clamped to rmax. The ring order lists the pixels of

@@ -372,7 +372,7 @@ void Server::worker(Slot* s) {
                             const DeviceBank& b = *sets[k];
                             const SpectrumConfig cfg = build_spectrum_config(r->params, b.wavelengths);
                             combine_kernels(s->stream, b.kernel_ptr(LDH_KERNEL_FULL, 0), b.scale_ptr(LDH_KERNEL_FULL, 0),
-                                            b.bins, b.kernel, cfg, sty, e->combined.as<float>() + kk6 * k, s->tmp);
+                                            b.bins, b.kernel, cfg, sty, e->combined.as<float>() + kk6 * k);
                         }
                         e->combined_style = sty;
                         e->has_combined = true;
