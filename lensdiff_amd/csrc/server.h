@@ -46,7 +46,8 @@ struct CacheEntry {
     std::shared_ptr<DeviceBank> bank;
     // the nine zone banks of a field-dependent phase, zone (zx, zy) at zy * 3 + zx (bank unset)
     std::vector<std::shared_ptr<DeviceBank>> zones;
-    // precombined natural (+ style) kernels, float[6][K*K] (per zone, zone z at z * 6 * K * K)
+    // precombined natural (+ style) kernels, float[n][K*K] with n = 3 (6 with style), per zone
+    // (zone z at z * n * K * K)
     DeviceBuffer combined;
     bool has_combined = false;
     bool combined_style = false;
@@ -65,6 +66,7 @@ struct CacheEntry {
     bool failed = false;
     std::string error;
     int pins = 0;
+    int uses = 0;  // requests admitted with this entry
     uint64_t last_use = 0;
     size_t bytes() const;
 };

@@ -46,7 +46,9 @@ int served_sets(const ldh_params& p) {
 size_t bank_bytes_estimate(const ldh_params& p) {
     const Sizes s = resolve_sizes(p);
     const size_t kk = size_t(2 * s.max_radius + 1) * size_t(2 * s.max_radius + 1);
-    return kk * 4 * (size_t(served_sets(p)) * wavelength_list(p).size() + 6) * (has_field_phase(p) ? kFieldZones : 1);
+    const size_t combined = spectral_fullkernel(p) ? (with_style(p) ? 6 : 3) : 0;
+    return kk * 4 * (size_t(served_sets(p)) * wavelength_list(p).size() + combined) *
+           (has_field_phase(p) ? kFieldZones : 1);
 }
 
 // The parameters of the nine zone banks of a field-dependent phase, each built from its resolved
@@ -228,16 +230,20 @@ size_t Server::resident_bytes_locked() const {
     return b;
 }
 
+// Segmented LRU: unpinned entries that one request has used so far go first, least recently
+// used first, then the others in the same order. A stream of lenses seen once then cycles through
+// the first segment and leaves the lenses that recur resident.
 bool Server::evict_locked(size_t need) {
     size_t resident = resident_bytes_locked();
-    for (auto it = cache_.end(); resident + need > budget_ && it != cache_.begin();) {
-        --it;
-        if ((*it)->pins == 0) {
-            resident -= (*it)->bytes();
-            it = cache_.erase(it);
-            stats_.evictions += 1;
+    for (int pass = 0; pass < 2; ++pass)
+        for (auto it = cache_.end(); resident + need > budget_ && it != cache_.begin();) {
+            --it;
+            if ((*it)->pins == 0 && (pass == 1 || (*it)->uses <= 1)) {
+                resident -= (*it)->bytes();
+                it = cache_.erase(it);
+                stats_.evictions += 1;
+            }
         }
-    }
     return resident + need <= budget_;
 }
 
@@ -325,6 +331,7 @@ ServedRequest* Server::admit(Slot* s, std::shared_ptr<CacheEntry>* entry, bool* 
                 r->bank_hit = true;
             }
             e->pins += 1;
+            e->uses += 1;
             e->last_use = ++clock_;
             *entry = e;
             r->t_start = now_ms();
@@ -366,13 +373,13 @@ void Server::worker(Slot* s) {
                         if (bank) sets.push_back(bank.get());
                         for (const auto& z : zones) sets.push_back(z.get());
                         const bool sty = with_style(r->params);
-                        const size_t kk6 = size_t(sets.front()->kernel) * sets.front()->kernel * 6;
-                        e->combined.reset(sizeof(float) * kk6 * sets.size());
+                        const size_t per = size_t(sets.front()->kernel) * sets.front()->kernel * (sty ? 6 : 3);
+                        e->combined.reset(sizeof(float) * per * sets.size());
                         for (size_t k = 0; k < sets.size(); ++k) {
                             const DeviceBank& b = *sets[k];
                             const SpectrumConfig cfg = build_spectrum_config(r->params, b.wavelengths);
                             combine_kernels(s->stream, b.kernel_ptr(LDH_KERNEL_FULL, 0), b.scale_ptr(LDH_KERNEL_FULL, 0),
-                                            b.bins, b.kernel, cfg, sty, e->combined.as<float>() + kk6 * k);
+                                            b.bins, b.kernel, cfg, sty, e->combined.as<float>() + per * k);
                         }
                         e->combined_style = sty;
                         e->has_combined = true;
@@ -477,7 +484,7 @@ void Server::worker(Slot* s) {
                                 if (nk == 1) {
                                     ks.push_back({zbank.kernel_ptr(LDH_KERNEL_FULL, 0), zbank.scale_ptr(LDH_KERNEL_FULL, 0)});
                                 } else {
-                                    const float* comb = e->combined.as<float>() + kk * 6 * size_t(z);
+                                    const float* comb = e->combined.as<float>() + kk * nk * size_t(z);
                                     for (int i = 0; i < nk; ++i) ks.push_back({comb + kk * i, nullptr});
                                 }
                                 // pre-combined kernels as half spectra (the real-input path)

@@ -127,6 +127,32 @@ def test_evictions_and_admission_waits_keep_results(ctx):
     assert all(s.status == 0 for s in stats)
 
 
+def test_lenses_seen_once_leave_recurring_ones_resident(ctx):
+    """Segmented eviction: with room for three entries, a lens used twice stays resident while
+    five lenses used once each pass through (they evict each other); plain LRU would have evicted
+    it when the third of them arrived."""
+    f = Frames(ctx, [workloads.frame(160, 96, 0.004, seed=500)])
+    recurring = lens(20)
+    once = [lens(21 + i) for i in range(5)]
+    try:
+        ctx.server_start(1, 1 << 30)
+        try:
+            ctx.wait(served(ctx, f, 0, recurring))
+            entry = ctx.server_stats().resident_bytes
+        finally:
+            ctx.server_stop()
+        ctx.server_start(1, int(entry * 3.5))
+        try:
+            stats = [ctx.wait(served(ctx, f, 0, p)) for p in [recurring, recurring] + once + [recurring]]
+            st = ctx.server_stats()
+        finally:
+            ctx.server_stop()
+    finally:
+        f.free()
+    assert [s.bank_hit for s in stats] == [0, 1, 0, 0, 0, 0, 0, 1]
+    assert st.evictions == 3 and st.bank_misses == 6
+
+
 def test_queued_hits_pass_a_build_in_flight(ctx):
     """A free slot does not take a request whose bank another slot is still building (it would only
     wait): queued requests for a resident lens go ahead of it, and every result is unchanged."""
