@@ -45,10 +45,10 @@ struct SelectParams {
     float threshold_stops, softness, threshold_linear, point_emphasis;
 };
 
-__global__ void selection_kernel(const float4* __restrict__ src, int64_t npx, SelectParams sp, float* __restrict__ mask) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= npx) return;
-    const float4 q = decoded(src, size_t(i), sp.tf);
+// The selection mask of one decoded pixel (buildSelectionMask, LensDiffCpuReference.cpp): a
+// smoothstep gate on the signal's stops above the threshold, times the point-emphasis boost,
+// clamped to [0, 1]. A function of the pixel alone, so passes that read the pixel evaluate it.
+__device__ __forceinline__ float select_decoded(float4 q, const SelectParams& sp) {
     const float mx = max3(q.x, q.y, q.z);
     const float sig = sp.extraction == LDH_EXTRACT_LUMA ? luma709(q.x, q.y, q.z) : mx;
     float gate = 0.0f;
@@ -60,16 +60,25 @@ __global__ void selection_kernel(const float4* __restrict__ src, int64_t npx, Se
         gate = t * t * (3.0f - 2.0f * t);
     }
     const float boost = 1.0f + sp.point_emphasis * fmaxf(0.0f, mx / fmaxf(sp.threshold_linear, 1e-4f) - 1.0f);
-    mask[i] = fminf(fmaxf(gate * boost, 0.0f), 1.0f);
+    return fminf(fmaxf(gate * boost, 0.0f), 1.0f);
+}
+
+__global__ void selection_kernel(const float4* __restrict__ src, int64_t npx, SelectParams sp, float* __restrict__ mask) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    mask[i] = select_decoded(decoded(src, size_t(i), sp.tf), sp);
 }
 
 // Row pass of an image spectrum: reads the frame, writes the grid.
 // The scalar driver of a pixel (applyMaskScalarDriver, LensDiffCpuReference.cpp:1266-1279):
 // signal (max RGB or Rec.709 luma of the decoded pixel) times mask times scale.
-__device__ __forceinline__ float driver_value(float4 raw, float m, int tf, int extraction, float scale) {
-    const float r = decode_channel(raw.x, tf), g = decode_channel(raw.y, tf), b = decode_channel(raw.z, tf);
-    const float sig = extraction == LDH_EXTRACT_LUMA ? luma709(r, g, b) : max3(r, g, b);
+__device__ __forceinline__ float driver_decoded(float4 q, float m, int extraction, float scale) {
+    const float sig = extraction == LDH_EXTRACT_LUMA ? luma709(q.x, q.y, q.z) : max3(q.x, q.y, q.z);
     return sig * m * scale;
+}
+__device__ __forceinline__ float driver_value(float4 raw, float m, int tf, int extraction, float scale) {
+    const float4 q = make_float4(decode_channel(raw.x, tf), decode_channel(raw.y, tf), decode_channel(raw.z, tf), raw.w);
+    return driver_decoded(q, m, extraction, scale);
 }
 
 // Separation of the transform Z of z = a + i b (a, b real) into half spectra:
@@ -144,28 +153,31 @@ struct KernelRows {
 
 // Row pass of the scalar driver into a half spectrum: driver rows 2p and 2p + 1 transformed
 // together as a + i b and separated into rows 2p, 2p + 1 (pitch hw; columns past pw/2 zero).
-// Rows from h on are zero and not stored.
+// Rows from h on are zero and not stored. The mask is evaluated from the pixel (select_decoded).
 struct DriverRowsHalf {
     const float4* src;
-    const float* mask;
+    SelectParams sp;
     float2* out;
-    int w, h, hw, tf, extraction;
+    int w, h, hw, extraction;
     float scale;
     static constexpr bool kStoreLds = true;
     struct Raw {
         float4 qa, qb;
-        float ma, mb;
         bool ina, inb;
     };
     __device__ Raw fetch(int p, int x) const {
         const int ra = 2 * p, rb = 2 * p + 1;
         const bool ina = ra < h && x < w, inb = rb < h && x < w;
         const size_t ia = ina ? size_t(ra) * w + x : 0, ib = inb ? size_t(rb) * w + x : 0;
-        return Raw{src[ia], src[ib], mask[ia], mask[ib], ina, inb};
+        return Raw{src[ia], src[ib], ina, inb};
+    }
+    __device__ float value(float4 raw) const {
+        const float4 q = make_float4(decode_channel(raw.x, sp.tf), decode_channel(raw.y, sp.tf),
+                                     decode_channel(raw.z, sp.tf), raw.w);
+        return driver_decoded(q, select_decoded(q, sp), extraction, scale);
     }
     __device__ float2 finish(const Raw& t) const {
-        return make_float2(t.ina ? driver_value(t.qa, t.ma, tf, extraction, scale) : 0.0f,
-                           t.inb ? driver_value(t.qb, t.mb, tf, extraction, scale) : 0.0f);
+        return make_float2(t.ina ? value(t.qa) : 0.0f, t.inb ? value(t.qb) : 0.0f);
     }
     __device__ void store(int, int, float2) const {}
     __device__ void store_sym(int p, int k, float2 zk, float2 zm, int n) const {
@@ -967,9 +979,14 @@ std::vector<KernelRef> effect_kernels(const ldh_params& p, const DeviceBank& ban
 namespace {
 
 // Step 1: the selection mask of the frame into fw.mask.
-void select_stage(hipStream_t s, FrameWork& fw, const ldh_params& p, const float4* src, int64_t npx) {
-    // 1. selection mask
-    fw.mask.reset(sizeof(float) * npx);
+// Whether effect_stage takes the pre-combined half-spectrum path (effect_precombined): the
+// spectral full-kernel effect in EffectMode::Precombined.
+bool precombined_path(const ldh_params& p, EffectMode mode) {
+    const Plan pl = plan_of(p);
+    return mode == EffectMode::Precombined && pl.spectral && !pl.split && pl.sets.front() == kFull;
+}
+
+SelectParams select_params(const ldh_params& p) {
     SelectParams sp;
     sp.tf = p.input_transfer;
     sp.extraction = p.extraction_mode;
@@ -977,6 +994,13 @@ void select_stage(hipStream_t s, FrameWork& fw, const ldh_params& p, const float
     sp.threshold_stops = float(p.threshold);
     sp.threshold_linear = 0.18f * std::pow(2.0f, sp.threshold_stops);
     sp.point_emphasis = float(p.point_emphasis);
+    return sp;
+}
+
+void select_stage(hipStream_t s, FrameWork& fw, const ldh_params& p, const float4* src, int64_t npx) {
+    // 1. selection mask
+    fw.mask.reset(sizeof(float) * npx);
+    const SelectParams sp = select_params(p);
 {
     ProfileScope ps_("frame-select", 20.0 * double(npx));
         hipLaunchKernelGGL(selection_kernel, grid1(npx), dim3(256), 0, s, src, npx, sp, fw.mask.as<float>());
@@ -996,12 +1020,13 @@ struct FusedFinal {
 // transform yields channels NK - 2 and NK - 1 (ExtractRowsHalf's pair), the channels before them
 // are read from `planes` (channel c at planes + c * npx); each pixel then goes through
 // PrecombinedMap and composite_final exactly as precombined_map_kernel and composite_kernel take
-// it, and only the output pixel is stored. NK = 3: natural R, G, B; NK = 6: then style R, G, B.
+// it, with its selection mask evaluated from the pixel, and only the output pixel is stored.
+// NK = 3: natural R, G, B; NK = 6: then style R, G, B.
 template <int NK>
 struct FinalRowsHalf : ExtractRowsHalf {
     const float* planes;
     const float4* src;
-    const float* mask;
+    SelectParams sp;  // the mask is evaluated from the pixel (select_decoded)
     float4* dst;
     int64_t npx;
     PrecombinedMap map;
@@ -1010,7 +1035,7 @@ struct FinalRowsHalf : ExtractRowsHalf {
         if (row >= h || x >= w) return;
         const size_t i = size_t(row) * w + x;
         const float4 q = decoded(src, i, cp.tf);
-        const float mk = mask[i];
+        const float mk = select_decoded(q, sp);
         float ch[NK];
 #pragma unroll
         for (int c = 0; c < NK - 2; ++c) ch[c] = planes[int64_t(c) * npx + i];
@@ -1042,8 +1067,7 @@ void effect_precombined(hipStream_t s, const FftTables& t, FrameWork& fw, const 
     fw.image_spec.reset(sizeof(float2) * hc);
     float2* ispec = fw.image_spec.as<float2>();
     if (!image_ready) {
-        DriverRowsHalf op{src, fw.mask.as<float>(), ispec, w, h, g.hw, p.input_transfer, p.extraction_mode,
-                          redist_scale};
+        DriverRowsHalf op{src, select_params(p), ispec, w, h, g.hw, p.extraction_mode, redist_scale};
         rows_pass(s, tw, g.pw, (h + 1) / 2, op, false);
         ColRows zero;
         zero.zero_lo = h;
@@ -1089,12 +1113,12 @@ void effect_precombined(hipStream_t s, const FftTables& t, FrameWork& fw, const 
             continue;
         }
         // spectra in, then per pixel the earlier planes, frame and mask in and the output pixel out
-        const double bytes = 16.0 * double(g.hw) * h + (4.0 * (nk - 2) + 36.0) * double(npx);
+        const double bytes = 16.0 * double(g.hw) * h + (4.0 * (nk - 2) + 32.0) * double(npx);
         if (nk == 6)
-            rows_pass(s, tw, g.pw, h, FinalRowsHalf<6>{ex, planes, src, fw.mask.as<float>(), fin->dst, npx, map, fin->cp},
+            rows_pass(s, tw, g.pw, h, FinalRowsHalf<6>{ex, planes, src, select_params(p), fin->dst, npx, map, fin->cp},
                       true, 1 << 30, 0, "fft-row-pass-final", bytes);
         else
-            rows_pass(s, tw, g.pw, h, FinalRowsHalf<3>{ex, planes, src, fw.mask.as<float>(), fin->dst, npx, map, fin->cp},
+            rows_pass(s, tw, g.pw, h, FinalRowsHalf<3>{ex, planes, src, select_params(p), fin->dst, npx, map, fin->cp},
                       true, 1 << 30, 0, "fft-row-pass-final", bytes);
         return;
     }
@@ -1122,8 +1146,7 @@ bool effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
     const SpectrumConfig cfg = build_spectrum_config(p, bank.wavelengths);
     const float force = std::max(0.0f, std::min(float(p.spectrum_force), 1.0f));
     const bool with_style = p.spectrum_style != LDH_STYLE_NATURAL && force > 0.0f;
-    if (mode == EffectMode::Precombined && (!pl.spectral || pl.split || pl.sets.front() != kFull))
-        mode = EffectMode::PerBin;  // pre-combination applies to the spectral full-kernel effect
+    if (!precombined_path(p, mode)) mode = EffectMode::PerBin;  // pre-combination: the spectral full-kernel effect
     const float redist_scale = 1.0f - float(std::max(0.0, std::min(p.core_preserve, 1.0)));
 
     if (mode == EffectMode::Precombined) {
@@ -1467,7 +1490,8 @@ bool effect_planes(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_p
                    const KernelSpectra* cached, const std::vector<const KernelSpectra*>* zone_cached,
                    const FusedFinal* fin = nullptr) {
     const int64_t npx = int64_t(w) * h;
-    select_stage(s, fw, p, src, npx);
+    // the folded pre-combined path evaluates the mask wherever it reads a pixel
+    if (!(fin && !zones && precombined_path(p, mode))) select_stage(s, fw, p, src, npx);
     fw.effect.reset(sizeof(float) * npx * 9);
     if (!zones) return effect_stage(s, t, fw, p, *bank, src, w, h, mode, cached, fw.effect.as<float>(), nullptr, false, fin);
     // the core and structure planes of a split look reach the output only in their views
