@@ -1,10 +1,11 @@
 // Exact ring sums (see rings.h). A float running sum that stays inside one binade [2^e, 2^(e+1))
 // adds each addend v as a multiple of u = 2^(e-23): fl(s + v) = s + rint(v / u) * u, unless v / u
 // is exactly halfway between two integers (the tie then rounds to the even result, which depends
-// on s) or the sum leaves the binade. A wavefront therefore adds its 64 elements of a ring at once
-// as an integer sum of rint(v / u) whenever no lane holds a tie and the total stays below 2^(e+1);
-// otherwise it adds the 64 elements one after the other in float, in order, on every lane. Either
-// way the result is the reference's sequential float sum, bit for bit.
+// on s) or the sum leaves the binade. A wavefront therefore adds 512 elements of a ring at once as
+// an integer sum of rint(v / u) whenever none is a tie and the total stays below 2^(e+1), else
+// each 64 of them at once under the same conditions, else those 64 one after the other in float,
+// in order, on every lane. Either way the result is the reference's sequential float sum, bit for
+// bit.
 #include <algorithm>
 #include <cmath>
 
@@ -61,6 +62,34 @@ __device__ __forceinline__ float add_group_sequential(float s, float v) {
     return s;
 }
 
+// The kRingGroups groups v (64 lanes each, in group then lane order) added to s at once: one
+// integer sum of every rint(v / u) in s's binade when no element is a tie or too large and the
+// total stays in the binade (the addends are >= 0, so every partial sum then does too, and each
+// addition is exact as in add_group_sequential). Returns false, s unchanged, otherwise.
+__device__ __forceinline__ bool add_batch(float& s, const float (&v)[kRingGroups]) {
+    if (!(s >= 0x1p-100f)) return false;
+    int ex;
+    (void)frexpf(s, &ex);
+    const int sh = 24 - ex;
+    unsigned c = 0u;
+    bool bad = false;
+#pragma unroll
+    for (int g = 0; g < kRingGroups; ++g) {
+        const float q = ldexpf(v[g], sh);
+        const bool b = !(q < 0x1p24f) || (q - floorf(q) == 0.5f);
+        c += b ? 0u : unsigned(rintf(q));
+        bad = bad || b;
+    }
+    // a lane's count below 2^24 keeps the wave's total below 2^30
+    bad = bad || c >= (1u << 24);
+    if (__ballot(bad) != 0ull) return false;
+    const unsigned tot = wave_sum_u32(c);
+    const unsigned ms = unsigned(ldexpf(s, sh));
+    if (ms + tot >= (1u << 24)) return false;
+    s = ldexpf(float(ms + tot), -sh);
+    return true;
+}
+
 // One wavefront per (segment, ring r < rings), over the ring-ordered copy (ring r of segment g at
 // ordered[g * pixels + off[r] ..]), reading one batch of groups ahead of the one it adds.
 // Workgroups are spread over the eight XCDs in contiguous ranges of items (workgroup b runs on
@@ -92,11 +121,15 @@ ring_wave_sums_kernel(const float* __restrict__ ordered, int64_t pixels, const i
     for (int base = i0; base < i1; base += kStep) {
         float nxt[kRingGroups];
         if (base + kStep < i1) load(nxt, base + kStep);
+        // groups past the ring's end read 0: they change neither the peak nor a batch's sum
 #pragma unroll
-        for (int g = 0; g < kRingGroups; ++g) {
-            if (base + 64 * g >= i1) break;
-            pk = max(pk, __float_as_uint(cur[g]));
-            s = add_group_sequential(s, cur[g]);
+        for (int g = 0; g < kRingGroups; ++g) pk = max(pk, __float_as_uint(cur[g]));
+        if (!add_batch(s, cur)) {
+#pragma unroll
+            for (int g = 0; g < kRingGroups; ++g) {
+                if (base + 64 * g >= i1) break;
+                s = add_group_sequential(s, cur[g]);
+            }
         }
 #pragma unroll
         for (int g = 0; g < kRingGroups; ++g) cur[g] = nxt[g];
