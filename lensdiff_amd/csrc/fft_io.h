@@ -25,15 +25,20 @@ namespace ldh {
 // ---- element ops ---------------------------------------------------------------------------
 // extra_bytes: bytes an op reads per element besides the element itself (byte model).
 // pair(d, idx): elements idx and idx + 1 (idx even: one 16-byte load per array).
+// at(c): the op for channel c of a multi-channel pass (its own arrays at c times a stride).
 struct PlainLoad {
     static constexpr double extra_bytes = 0.0;
+    __device__ PlainLoad at(int) const { return *this; }
     __device__ float2 operator()(const float2* d, size_t idx) const { return d[idx]; }
     __device__ float4 pair(const float2* d, size_t idx) const { return *reinterpret_cast<const float4*>(d + idx); }
 };
-// Spectrum multiply fused into the first inverse pass: v = d[idx] * k[idx].
+// Spectrum multiply fused into the first inverse pass: v = d[idx] * k[idx]; channel c multiplies
+// by the spectrum at k + c * chan.
 struct MulSpectrumLoad {
     static constexpr double extra_bytes = 8.0;
     const float2* k;
+    size_t chan = 0;
+    __device__ MulSpectrumLoad at(int c) const { return MulSpectrumLoad{k + size_t(c) * chan, chan}; }
     __device__ float2 operator()(const float2* d, size_t idx) const { return cmul(d[idx], k[idx]); }
     __device__ float4 pair(const float2* d, size_t idx) const {
         const float4 a = *reinterpret_cast<const float4*>(d + idx);
@@ -47,6 +52,8 @@ struct MulSpectrumLoad {
 struct MulSpectrumHalfLoad {
     static constexpr double extra_bytes = 4.0;
     const __half2* k;
+    size_t chan = 0;
+    __device__ MulSpectrumHalfLoad at(int c) const { return MulSpectrumHalfLoad{k + size_t(c) * chan, chan}; }
     __device__ float2 operator()(const float2* d, size_t idx) const {
         const float2 kk = __half22float2(k[idx]);
         return cmul(d[idx], kk);
@@ -70,21 +77,28 @@ struct ColRows {
 };
 
 // row(g, i) = g * g_row_mul + i * i_row_mul; the twiddle exponent of an output point is g * i.
+// A pass over nchan channels (arrays of equal shape) treats them as one grid nchan times as wide:
+// column group cg of channel c is tile column nchan * cg + c, so a column group's channels are
+// neighbouring tiles and share, through L2, what they read in common (src_chan = 0).
 template <class Load, int TW>
 struct ColIO {
-    const float2* src;   // loads (may equal data)
-    float2* data;        // stores
+    const float2* src;   // loads (may equal data); channel c at src + c * src_chan
+    float2* data;        // stores; channel c at data + c * dst_chan
     const float2* tw;
     int pitch;
-    int ncolgroups;   // width / kColWidth
+    int ncolgroups;   // nchan * width / kColWidth
     int groups;       // line groups (g values) per tile
     int gcount;       // line groups in the pass: tiles past it (a partial last tile) do nothing
     int g_row_mul, i_row_mul;
     int tw_stride;    // table length / N for the twiddles W_N^(g i) of the split
     ColRows rows;
     Load op;
-    __device__ int row_of(int tile, int line, int i, int* g_out, int* col) const {
-        const int cg = tile % ncolgroups;
+    int nchan = 1;
+    size_t src_chan = 0, dst_chan = 0;
+    __device__ int row_of(int tile, int line, int i, int* g_out, int* col, int* ch) const {
+        const int t = tile % ncolgroups;
+        const int cg = t / nchan;
+        *ch = t - cg * nchan;
         const int g = (tile / ncolgroups) * groups + line / kColWidth;
         *g_out = g;
         *col = cg * kColWidth + line % kColWidth;
@@ -96,68 +110,76 @@ struct ColIO {
     using Raw = float2;
     __device__ float2 finish(float2 v) const { return v; }
     __device__ float2 fetch(int tile, int line, int i) const {
-        int g, col;
-        const int row = row_of(tile, line, i, &g, &col);
+        int g, col, ch;
+        const int row = row_of(tile, line, i, &g, &col, &ch);
         const bool zero = g >= gcount || (row >= rows.zero_lo && row < rows.zero_hi);
-        const float2 v = op(src, zero ? size_t(0) : size_t(row) * size_t(pitch) + size_t(col));
+        const float2 v = op.at(ch)(src + size_t(ch) * src_chan, zero ? size_t(0) : size_t(row) * size_t(pitch) + size_t(col));
         return zero ? make_float2(0.0f, 0.0f) : v;
     }
     // The same for the two adjacent columns line, line + 1 (line even): one 16-byte access.
     static constexpr bool kPairs = true;
     __device__ float4 fetch_pair(int tile, int line, int i) const {
-        int g, col;
-        const int row = row_of(tile, line, i, &g, &col);
+        int g, col, ch;
+        const int row = row_of(tile, line, i, &g, &col, &ch);
         const bool zero = g >= gcount || (row >= rows.zero_lo && row < rows.zero_hi);
-        const float4 v = op.pair(src, zero ? size_t(0) : size_t(row) * size_t(pitch) + size_t(col));
+        const float4 v = op.at(ch).pair(src + size_t(ch) * src_chan,
+                                        zero ? size_t(0) : size_t(row) * size_t(pitch) + size_t(col));
         return zero ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : v;
     }
     __device__ void store_pair(int tile, int line, int i, float2 a, float2 b, float2 w) const {
-        int g, col;
-        const int row = row_of(tile, line, i, &g, &col);
+        int g, col, ch;
+        const int row = row_of(tile, line, i, &g, &col, &ch);
         if (g >= gcount || row >= rows.store_rows) return;
         if constexpr (TW != 0) {
             a = cmul(a, w);
             b = cmul(b, w);
         }
-        *reinterpret_cast<float4*>(data + size_t(row) * size_t(pitch) + size_t(col)) = make_float4(a.x, a.y, b.x, b.y);
+        *reinterpret_cast<float4*>(data + size_t(ch) * dst_chan + size_t(row) * size_t(pitch) + size_t(col)) =
+            make_float4(a.x, a.y, b.x, b.y);
     }
     // The split's twiddle W_N^(g i) of a stored point (conjugated for TW < 0), loaded by the tile
     // kernel before its stages; anything for a point that is not stored.
     static constexpr bool kStoreTwiddle = TW != 0;
     __device__ float2 twiddle(int tile, int line, int i) const {
-        int g, col;
-        (void)row_of(tile, line, i, &g, &col);
+        int g, col, ch;
+        (void)row_of(tile, line, i, &g, &col, &ch);
         float2 w = tw[(g < gcount ? g : 0) * i * tw_stride];
         if (TW < 0) w.y = -w.y;
         return w;
     }
     __device__ void store(int tile, int line, int i, float2 v, float2 w) const {
-        int g, col;
-        const int row = row_of(tile, line, i, &g, &col);
+        int g, col, ch;
+        const int row = row_of(tile, line, i, &g, &col, &ch);
         if (g >= gcount || row >= rows.store_rows) return;
         if constexpr (TW != 0) v = cmul(v, w);
-        data[size_t(row) * size_t(pitch) + size_t(col)] = v;
+        data[size_t(ch) * dst_chan + size_t(row) * size_t(pitch) + size_t(col)] = v;
     }
 };
 
+// One column pass over nchan channels (ColIO): channel c reads src + c * src_chan through
+// op.at(c) and writes data + c * dst_chan. Strides keep 16-byte alignment.
 template <int TW, class Load>
 void col_pass(hipStream_t s, const float2* tw, const float2* src, float2* data, int width, int npts, int gcount,
-              int g_row_mul, int i_row_mul, int height, const Load& op, ColRows rows, bool inverse) {
+              int g_row_mul, int i_row_mul, int height, const Load& op, ColRows rows, bool inverse, int nchan = 1,
+              size_t src_chan = 0, size_t dst_chan = 0) {
     const int nl = col_tile_lines(npts);
     const int groups = nl / kColWidth;
     // column tiles move 16-byte pairs of adjacent elements
-    if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(data)) & 15) != 0)
+    if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(data)) & 15) != 0 || (src_chan | dst_chan) % 2)
         fail(LDH_ERR_INVALID_ARGUMENT, "fft-column-data-not-16-byte-aligned");
-    ColIO<Load, TW> io{src, data, tw + tw_offset(height), width, width / kColWidth, groups, gcount, g_row_mul,
-                       i_row_mul, tw_table(height) / height, rows, op};
-    const int ntiles = (width / kColWidth) * int(div_up(gcount, groups));
-    // algorithmic bytes: elements read (outside the zero band) and written (below store_rows)
+    if (nchan < 1) fail(LDH_ERR_INVALID_ARGUMENT, "fft-column-channels");
+    ColIO<Load, TW> io{src, data, tw + tw_offset(height), width, nchan * (width / kColWidth), groups, gcount,
+                       g_row_mul, i_row_mul, tw_table(height) / height, rows, op, nchan, src_chan, dst_chan};
+    const int ntiles = nchan * (width / kColWidth) * int(div_up(gcount, groups));
+    // algorithmic bytes: elements read (outside the zero band; a source the channels share once)
+    // and written (below store_rows)
     const double all = double(width) * double(height);
     const double zero = double(std::max(0, std::min(rows.zero_hi, height) - std::max(rows.zero_lo, 0)));
     const double stored = double(std::min(rows.store_rows, height));
     const double loaded = all - zero * double(width);
+    const double src_reads = src_chan != 0 ? double(nchan) : 1.0;
     ProfileScope ps(std::string("fft-col-pass") + (inverse ? "-inv" : "-fwd"),
-                    (8.0 + Load::extra_bytes) * loaded + 8.0 * stored * double(width));
+                    8.0 * loaded * src_reads + Load::extra_bytes * loaded * nchan + 8.0 * stored * double(width) * nchan);
     if (inverse) launch_tiles<false, true>(s, npts, io, tw, ntiles);
     else launch_tiles<false, false>(s, npts, io, tw, ntiles);
 }
@@ -230,6 +252,26 @@ void cols_inverse(hipStream_t s, const float2* tw, const float2* src, float2* da
                   const Load& op, int store_rows = 1 << 30) {
     check_cols(width, height);
     cols_inverse_split(s, tw, src, data, width, height, op, store_rows, col_plan(s, tw, width, height));
+}
+
+// The inverse column transforms of nchan channels in the same launches, all from one source:
+// channel c is the transform of src through op.at(c), stored at data + c * dst_chan (only rows
+// < store_rows of the last pass). Neighbouring tiles carry a column group's channels, so the
+// source is read from memory about once.
+template <class Load>
+void cols_inverse_channels(hipStream_t s, const float2* tw, const float2* src, float2* data, size_t dst_chan,
+                           int nchan, int width, int height, const Load& op, int store_rows) {
+    check_cols(width, height);
+    const ColSplit sp = col_plan(s, tw, width, height);
+    ColRows last;
+    last.store_rows = store_rows;
+    const int n1 = sp.n1, n2 = sp.n2;
+    if (n1 == 0) {
+        col_pass<0>(s, tw, src, data, width, height, 1, 0, 1, height, op, last, true, nchan, 0, dst_chan);
+        return;
+    }
+    col_pass<-1>(s, tw, src, data, width, n2, n1, n2, 1, height, op, ColRows{}, true, nchan, 0, dst_chan);
+    col_pass<0>(s, tw, data, data, width, n1, n2, 1, n2, height, PlainLoad{}, last, true, nchan, dst_chan, dst_chan);
 }
 
 // ---- row passes ---------------------------------------------------------------------------------

@@ -1051,10 +1051,11 @@ struct FinalRowsHalf : ExtractRowsHalf {
 // Steps 2-4 for the spectral full-kernel effect with pre-combined RGB kernels (natural, and style
 // when styled), on half spectra: the driver is real, so its transform is stored for columns
 // 0 .. pw/2 only (DriverRowsHalf, two rows per row transform); each combined kernel likewise;
-// each channel's product is inverted over the half-width columns, and channels are grouped in
-// pairs (an odd count starts with one alone) for the last row pass (ExtractRowsHalf), which yields
-// two real planes per row transform. The planes then go through the pre-combined spectral map
-// into effect9[0..3) or, with `fin`, the last group's pass maps and composites into fin->dst.
+// every channel's product is inverted over the half-width columns (one set of launches for all
+// channels), and channels are grouped in pairs (an odd count starts with one alone) for the last
+// row pass (ExtractRowsHalf), which yields two real planes per row transform. The planes then go
+// through the pre-combined spectral map into effect9[0..3) or, with `fin`, the last group's pass
+// maps and composites into fin->dst.
 void effect_precombined(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
                         const float4* src, int w, int h, const Grid& g, const SpectrumConfig& cfg, bool with_style,
                         float force, float redist_scale, const KernelSpectra* cached, float* effect9,
@@ -1088,25 +1089,23 @@ void effect_precombined(hipStream_t s, const FftTables& t, FrameWork& fw, const 
     }
     if (!ks->half || ks->grid.pw != g.pw || ks->grid.ph != g.ph || ks->kernels != nk)
         fail(LDH_ERR_INVALID_ARGUMENT, "cached-spectra-mismatch");
-    // 4. per channel: product and inverse columns; per group: one row pass to one or two planes
-    fw.out_spec.reset(sizeof(float2) * hc * 2);
+    // 4. every channel's product and inverse columns in the same launches (the driver's spectrum
+    // read about once for all); per group: one row pass to one or two planes
+    fw.out_spec.reset(sizeof(float2) * hc * nk);
+    float2* outs = fw.out_spec.as<float2>();
+    // only rows < h reach the crop
+    if (ks->fp16)
+        cols_inverse_channels(s, tw, ispec, outs, hc, nk, g.hw, g.ph, MulSpectrumHalfLoad{ks->data.as<__half2>(), hc}, h);
+    else
+        cols_inverse_channels(s, tw, ispec, outs, hc, nk, g.hw, g.ph, MulSpectrumLoad{ks->data.as<float2>(), hc}, h);
     fw.planes.reset(sizeof(float) * npx * 6);
     float* planes = fw.planes.as<float>();
     const float inv_cells = 1.0f / float(g.cells());
     const PrecombinedMap map{force, float(std::max(0.0, p.spectrum_saturation)), p.chromatic_affects_luma};
     for (int c = 0; c < nk;) {
         const int m = c == 0 && nk % 2 == 1 ? 1 : 2;
-        for (int j = 0; j < m; ++j) {
-            float2* o = fw.out_spec.as<float2>() + hc * j;
-            // only rows < h reach the crop
-            if (ks->fp16)
-                cols_inverse(s, tw, ispec, o, g.hw, g.ph, MulSpectrumHalfLoad{ks->data.as<__half2>() + hc * (c + j)}, h);
-            else
-                cols_inverse(s, tw, ispec, o, g.hw, g.ph, MulSpectrumLoad{ks->data.as<float2>() + hc * (c + j)}, h);
-        }
-        const ExtractRowsHalf ex{fw.out_spec.as<float2>(), m == 2 ? fw.out_spec.as<float2>() + hc : nullptr,
-                                 planes + npx * c, m == 2 ? planes + npx * (c + 1) : nullptr, w, h, g.hw, g.pw,
-                                 inv_cells};
+        const ExtractRowsHalf ex{outs + hc * c, m == 2 ? outs + hc * (c + 1) : nullptr, planes + npx * c,
+                                 m == 2 ? planes + npx * (c + 1) : nullptr, w, h, g.hw, g.pw, inv_cells};
         c += m;
         if (c < nk || !fin) {
             rows_pass(s, tw, g.pw, h, ex, true);
