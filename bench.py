@@ -20,7 +20,9 @@ through the plain path in arrival order on one worker (latency = completion - ar
 repeats served and plain throughput and latency over the served config's frame-size mix. `roofline`
 is the kernel family with the most device time in the plain timed region (HIP events on the
 context stream, where kernels run one at a time); `roofline_served` the same for the served
-region, whose overlapping slot streams stretch each launch.
+region, whose overlapping slot streams stretch each launch; `roofline_served_alone` the same for
+the served stream on one slot (--alone-steps requests after the resident lenses), where each
+launch runs alone.
 """
 from __future__ import annotations
 
@@ -134,15 +136,19 @@ def traffic_of(name):
     return None if rec is None else rec["hbm_bytes_per_launch"]
 
 
-def dominant(stats):
-    """Roofline of the kernel family with the most device time that has a byte model."""
+def dominant(stats, with_traffic=True):
+    """Roofline of the kernel family with the most device time that has a byte model. `traffic`
+    (profiles/traffic.json) is recorded for the plain pass's launch shapes; other passes launch the
+    same family at other shapes and report it as null (their measured bytes are in
+    profiles/uhd31_hbm_traffic_pmc.md)."""
     fams = {k: v for k, v in stats.items() if v[0] > 0 and v[2] > 0}
     name, (launches, total_ms, nbytes) = max(fams.items(), key=lambda kv: kv[1][1])
     avg_s = total_ms / 1e3 / launches
     achieved = nbytes / launches / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(name), "kernel": name,
-            "launches": launches, "avg_launch_ms": round(avg_s * 1e3, 4), "bytes_per_launch": nbytes / launches}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(name) if with_traffic else None,
+            "kernel": name, "launches": launches, "avg_launch_ms": round(avg_s * 1e3, 4),
+            "bytes_per_launch": nbytes / launches}
 
 
 def families(stats):
@@ -185,6 +191,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=60)
     ap.add_argument("--latency-steps", type=int, default=240, help="requests of each latency stream")
     ap.add_argument("--mixed-steps", type=int, default=200, help="requests of each c4 frame-size-mix stream")
+    ap.add_argument("--alone-steps", type=int, default=48,
+                    help="requests of the one-slot served stream timed per kernel family")
     ap.add_argument("--no-mixed", action="store_true", help="skip the c4 frame-size mix")
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -367,6 +375,21 @@ def main():
     plain_lat, plain_svc = latency_plain(ctx, [(L(sd), d_src[fi], w, h) for sd, fi in lat_reqs], arrivals, d_dst)
     progress("latency pass done")
 
+    # ---- the served stream on one slot: each launch runs alone (kernel families vs roofline) ----
+    ctx.server_start(1, 64 << 30)
+    for k, seed in enumerate(RESIDENT):
+        ctx.wait(ctx.submit(L(seed), d_src[0], d_dst[k % N_DST], w, h))
+    alone_reqs = stream(args.alone_steps, np.random.default_rng(5), 4000, len(uhd))
+    ctx.profile_reset()
+    ctx.profiling(True)
+    tickets = [ctx.submit(L(seed), d_src[fi], d_dst[i % N_DST], w, h) for i, (seed, fi) in enumerate(alone_reqs)]
+    for t in tickets:
+        ctx.wait(t)
+    ctx.profiling(False)
+    alone_stats = ctx.profile_read()
+    ctx.server_stop()
+    progress("one-slot served pass done")
+
     out = {
         "metric": METRIC,
         "value": round(served_fps, 3),
@@ -403,9 +426,12 @@ def main():
         # plain timed pass: one stream, every kernel runs alone
         "roofline": dominant(plain_stats),
         # served timed pass: slot streams overlap, so launch durations include sharing the device
-        "roofline_served": dominant(served_stats),
+        "roofline_served": dominant(served_stats, False),
+        # the served stream on one slot: every launch runs alone
+        "roofline_served_alone": dominant(alone_stats, False),
         "kernel_families_plain": families(plain_stats),
         "kernel_families_served": families(served_stats),
+        "kernel_families_served_alone": families(alone_stats),
     }
     for d in d_src + [d_a, d_b]:
         ctx.free(d)
