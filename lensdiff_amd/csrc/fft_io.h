@@ -40,6 +40,13 @@ struct MulSpectrumLoad {
     size_t chan = 0;
     __device__ MulSpectrumLoad at(int c) const { return MulSpectrumLoad{k + size_t(c) * chan, chan}; }
     __device__ float2 operator()(const float2* d, size_t idx) const { return cmul(d[idx], k[idx]); }
+    // the product of elements a (two, as pair() loads them) with the spectrum at idx, idx + 1
+    __device__ float4 mul_pair(float4 a, size_t idx) const {
+        const float4 b = *reinterpret_cast<const float4*>(k + idx);
+        const float2 lo = cmul(make_float2(a.x, a.y), make_float2(b.x, b.y));
+        const float2 hi = cmul(make_float2(a.z, a.w), make_float2(b.z, b.w));
+        return make_float4(lo.x, lo.y, hi.x, hi.y);
+    }
     __device__ float4 pair(const float2* d, size_t idx) const {
         const float4 a = *reinterpret_cast<const float4*>(d + idx);
         const float4 b = *reinterpret_cast<const float4*>(k + idx);
@@ -54,6 +61,12 @@ struct MulSpectrumHalfLoad {
     const __half2* k;
     size_t chan = 0;
     __device__ MulSpectrumHalfLoad at(int c) const { return MulSpectrumHalfLoad{k + size_t(c) * chan, chan}; }
+    __device__ float4 mul_pair(float4 a, size_t idx) const {
+        const __half2 k0 = k[idx], k1 = k[idx + 1];
+        const float2 lo = cmul(make_float2(a.x, a.y), __half22float2(k0));
+        const float2 hi = cmul(make_float2(a.z, a.w), __half22float2(k1));
+        return make_float4(lo.x, lo.y, hi.x, hi.y);
+    }
     __device__ float2 operator()(const float2* d, size_t idx) const {
         const float2 kk = __half22float2(k[idx]);
         return cmul(d[idx], kk);
@@ -272,6 +285,184 @@ void cols_inverse_channels(hipStream_t s, const float2* tw, const float2* src, f
     }
     col_pass<-1>(s, tw, src, data, width, n2, n1, n2, 1, height, op, ColRows{}, true, nchan, 0, dst_chan);
     col_pass<0>(s, tw, data, data, width, n1, n2, 1, n2, height, PlainLoad{}, last, true, nchan, dst_chan, dst_chan);
+}
+
+// ---- forward and inverse column passes fused for a convolution -----------------------------------
+// With a split N1 x N2, the forward pass F2 and the inverse pass I2 transform the same row groups
+// (rows N2 * k1 + j, j < N2), so one tile can run F2, then for every channel c of a product the
+// multiply by c's spectrum, I2 and I2's conjugate twiddle: the forward result stays in registers
+// and LDS. The operations are those of the two passes run apart (the spectrum product as
+// MulSpectrumLoad forms it, the twiddles as the tile kernel forms them); complex products may
+// contract to FMA differently in another kernel (cmul), so results agree to rounding (5.5e-9
+// rel L2 on a UHD frame).
+struct FwdMulInvGeom {
+    const float2* src;   // the forward pass's input (F1's output)
+    float2* data;        // channel c's I2 output at data + c * dst_chan
+    size_t dst_chan;
+    int nchan;
+    int pitch, ncolgroups;   // ncolgroups = width / kColWidth
+    int groups, gcount;      // line groups per tile; line groups in the pass (N1)
+    int g_row_mul;           // N2 (i_row_mul 1)
+    int tw_stride;           // table length / height
+    const float2* twn;       // the height's twiddle table (W_height^m at m * tw_stride)
+};
+
+template <int N>
+constexpr bool fwd_mul_inv_ok() {
+    using S = TileShape<N, false>;
+    constexpr int NP = S::NL / 2;
+    return (S::ELEMS / 2) % S::NT == 0 && S::NT % NP == 0;
+}
+
+template <int N, class Load>
+__global__ void __launch_bounds__((TileShape<N, false>::NT))
+fft_fwd_mul_inv_kernel(FwdMulInvGeom gm, Load op, const float2* __restrict__ tw, int ntiles) {
+    using S = TileShape<N, false>;
+    constexpr int NL = S::NL, NT = S::NT, E = S::ELEMS;
+    constexpr int NP = NL / 2;
+    constexpr int IP = E / 2 / NT;
+    static_assert(fwd_mul_inv_ok<N>(), "column pairs");
+    extern __shared__ float2 lds[];
+    const int tile = xcd_tile(blockIdx.x, ntiles);
+    const int tid = threadIdx.x;
+    float2 tw_mine = make_float2(1.0f, 0.0f);
+    if constexpr (S::TWL) {
+        if (tid < N) tw_mine = tw[tw_offset(N) + tid * (tw_table(N) / N)];
+    }
+    const int cg = tile % gm.ncolgroups;
+    const int lp = (tid % NP) * 2, i0 = tid / NP;
+    const int g = (tile / gm.ncolgroups) * gm.groups + lp / kColWidth;
+    const int col = cg * kColWidth + lp % kColWidth;
+    const bool live = g < gm.gcount;
+    size_t idx[IP];
+#pragma unroll
+    for (int c = 0; c < IP; ++c) idx[c] = size_t(g * gm.g_row_mul + i0 + c * (NT / NP)) * size_t(gm.pitch) + size_t(col);
+    // F2's loads (a tile past the last line group reads element 0 and is dropped)
+    float4 x[IP];
+#pragma unroll
+    for (int c = 0; c < IP; ++c) x[c] = *reinterpret_cast<const float4*>(gm.src + (live ? idx[c] : 0));
+#pragma unroll
+    for (int c = 0; c < IP; ++c) {
+        const int at = lp + (i0 + c * (NT / NP)) * NL;
+        lds[lds_pad(at)] = live ? make_float2(x[c].x, x[c].y) : make_float2(0.0f, 0.0f);
+        lds[lds_pad(at + 1)] = live ? make_float2(x[c].z, x[c].w) : make_float2(0.0f, 0.0f);
+    }
+    float2* twl = lds + S::DATA_WORDS;
+    if constexpr (S::TWL) {
+        if (tid < N) twl[tid] = tw_mine;
+    }
+    // I2's store twiddles conj(W_height^(g i)), the same for every channel, formed as the tile
+    // kernel forms them for column pairs: conj W^(g i0) times powers of conj W^(g D), D = NT / NP
+    auto twd = [&](int i) {
+        float2 t = gm.twn[(live ? g : 0) * i * gm.tw_stride];
+        t.y = -t.y;
+        return t;
+    };
+    float2 w[IP];
+    {
+        constexpr int D = NT / NP;
+        const float2 base = twd(i0);
+        float2 pw[IP];
+        pw[0] = make_float2(1.0f, 0.0f);
+        if constexpr (IP > 1) pw[1] = twd(D);
+#pragma unroll
+        for (int c = 2; c < IP; ++c) {
+            const int h = c >= 8 ? 8 : c >= 4 ? 4 : c >= 2 ? 2 : 1;
+            pw[c] = c == h ? cmul(pw[h / 2], pw[h / 2]) : cmul(pw[c - h], pw[h]);
+        }
+#pragma unroll
+        for (int c = 0; c < IP; ++c) w[c] = c == 0 ? base : cmul(base, pw[c]);
+    }
+    __syncthreads();
+    fft_stages<N, 1, NL, NT, false, 1, NL, S::TWL>(lds, tw, twl, tid);
+#pragma unroll
+    for (int c = 0; c < IP; ++c) {
+        const int at = lp + (i0 + c * (NT / NP)) * NL;
+        const float2 a = lds[lds_pad(at)], b = lds[lds_pad(at + 1)];
+        x[c] = make_float4(a.x, a.y, b.x, b.y);
+    }
+    for (int ch = 0; ch < gm.nchan; ++ch) {
+        const Load opc = op.at(ch);
+        float4 y[IP];
+#pragma unroll
+        for (int c = 0; c < IP; ++c) y[c] = opc.mul_pair(x[c], live ? idx[c] : 0);
+        __syncthreads();  // the previous readers of the tile are done
+#pragma unroll
+        for (int c = 0; c < IP; ++c) {
+            const int at = lp + (i0 + c * (NT / NP)) * NL;
+            lds[lds_pad(at)] = make_float2(y[c].x, y[c].y);
+            lds[lds_pad(at + 1)] = make_float2(y[c].z, y[c].w);
+        }
+        __syncthreads();
+        fft_stages<N, 1, NL, NT, true, 1, NL, S::TWL>(lds, tw, twl, tid);
+        if (live) {
+            float2* out = gm.data + size_t(ch) * gm.dst_chan;
+#pragma unroll
+            for (int c = 0; c < IP; ++c) {
+                const int at = lp + (i0 + c * (NT / NP)) * NL;
+                const float2 a = cmul(lds[lds_pad(at)], w[c]), b = cmul(lds[lds_pad(at + 1)], w[c]);
+                *reinterpret_cast<float4*>(out + idx[c]) = make_float4(a.x, a.y, b.x, b.y);
+            }
+        }
+    }
+}
+
+template <int N, class Load>
+bool launch_fwd_mul_inv_if(hipStream_t s, int n, const FwdMulInvGeom& gm, const Load& op, const float2* tw,
+                           int ntiles) {
+    if constexpr (!fwd_mul_inv_ok<N>()) {
+        return false;
+    } else {
+        if (n != N) return false;
+        using S = TileShape<N, false>;
+        allow_large_lds(reinterpret_cast<const void*>(&fft_fwd_mul_inv_kernel<N, Load>), S::LDS_BYTES);
+        hipLaunchKernelGGL((fft_fwd_mul_inv_kernel<N, Load>), dim3(ntiles), dim3(S::NT), S::LDS_BYTES, s, gm, op, tw,
+                           ntiles);
+        return true;
+    }
+}
+
+// The column work of a convolution of one transformed source with nchan spectra: the forward
+// transform of `drv` (whose rows [first.zero_lo, first.zero_hi) read as zero), the product with
+// op.at(c)'s spectrum and the inverse transform for every channel c, stored at data + c * dst_chan
+// (rows < store_rows of the last pass). With a split whose N2 tile moves column pairs, F2 and I2
+// run fused (fft_fwd_mul_inv_kernel) and the forward result is never stored; otherwise the passes
+// run apart and `drv` holds the source's spectrum afterwards.
+template <class Load>
+void cols_convolve_channels(hipStream_t s, const float2* tw, float2* drv, ColRows first, float2* data,
+                            size_t dst_chan, int nchan, int width, int height, const Load& op, int store_rows) {
+    check_cols(width, height);
+    const ColSplit sp = col_plan(s, tw, width, height);
+    const int n1 = sp.n1, n2 = sp.n2;
+    if (n1 != 0 && nchan >= 1) {
+        const int nl = col_tile_lines(n2);
+        FwdMulInvGeom gm{drv, data, dst_chan, nchan, width, width / kColWidth, nl / kColWidth, n1, n2,
+                         tw_table(height) / height, tw + tw_offset(height)};
+        const int ntiles = (width / kColWidth) * int(div_up(n1, nl / kColWidth));
+        first.store_rows = 1 << 30;
+        col_pass<1>(s, tw, drv, drv, width, n1, n2, 1, n2, height, PlainLoad{}, first, false);
+        const double cells = double(width) * double(height);
+        ProfileScope ps("fft-col-pass-fwd-inv", 8.0 * cells + (Load::extra_bytes + 8.0) * cells * nchan);
+        if (launch_fwd_mul_inv_if<8>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<12>(s, n2, gm, op, tw, ntiles) ||
+            launch_fwd_mul_inv_if<16>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<24>(s, n2, gm, op, tw, ntiles) ||
+            launch_fwd_mul_inv_if<32>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<36>(s, n2, gm, op, tw, ntiles) ||
+            launch_fwd_mul_inv_if<48>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<64>(s, n2, gm, op, tw, ntiles) ||
+            launch_fwd_mul_inv_if<72>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<96>(s, n2, gm, op, tw, ntiles) ||
+            launch_fwd_mul_inv_if<128>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<144>(s, n2, gm, op, tw, ntiles)) {
+            LDH_CHECK_LAUNCH("fft-col-fwd-inv");
+            ColRows last;
+            last.store_rows = store_rows;
+            col_pass<0>(s, tw, data, data, width, n1, n2, 1, n2, height, PlainLoad{}, last, true, nchan, dst_chan,
+                        dst_chan);
+            return;
+        }
+        // no fused tile for this N2: F2 apart (F1 is done)
+        col_pass<0>(s, tw, drv, drv, width, n2, n1, n2, 1, height, PlainLoad{}, ColRows{}, false);
+        cols_inverse_channels(s, tw, drv, data, dst_chan, nchan, width, height, op, store_rows);
+        return;
+    }
+    cols_forward_split(s, tw, drv, width, height, PlainLoad{}, first, sp);
+    cols_inverse_channels(s, tw, drv, data, dst_chan, nchan, width, height, op, store_rows);
 }
 
 // ---- row passes ---------------------------------------------------------------------------------

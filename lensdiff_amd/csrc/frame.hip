@@ -1052,14 +1052,16 @@ struct FinalRowsHalf : ExtractRowsHalf {
 // when styled), on half spectra: the driver is real, so its transform is stored for columns
 // 0 .. pw/2 only (DriverRowsHalf, two rows per row transform); each combined kernel likewise;
 // every channel's product is inverted over the half-width columns (one set of launches for all
-// channels), and channels are grouped in pairs (an odd count starts with one alone) for the last
-// row pass (ExtractRowsHalf), which yields two real planes per row transform. The planes then go
-// through the pre-combined spectral map into effect9[0..3) or, with `fin`, the last group's pass
-// maps and composites into fin->dst.
+// channels; unless keep_image asks for the driver's spectrum to stay in fw.image_spec for later
+// zones, its last forward column pass runs fused with the first inverse one), and channels are
+// grouped in pairs (an odd count starts with one alone) for the last row pass (ExtractRowsHalf),
+// which yields two real planes per row transform. The planes then go through the pre-combined
+// spectral map into effect9[0..3) or, with `fin`, the last group's pass maps and composites into
+// fin->dst.
 void effect_precombined(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_params& p, const DeviceBank& bank,
                         const float4* src, int w, int h, const Grid& g, const SpectrumConfig& cfg, bool with_style,
                         float force, float redist_scale, const KernelSpectra* cached, float* effect9,
-                        bool image_ready, const FusedFinal* fin) {
+                        bool image_ready, bool keep_image, const FusedFinal* fin) {
     const int64_t npx = int64_t(w) * h;
     const size_t hc = g.half_cells();
     const float2* tw = t.twiddle.as<float2>();
@@ -1067,13 +1069,16 @@ void effect_precombined(hipStream_t s, const FftTables& t, FrameWork& fw, const 
     // 2. the driver's half spectrum
     fw.image_spec.reset(sizeof(float2) * hc);
     float2* ispec = fw.image_spec.as<float2>();
+    // rows from h on are zero; the column transform runs here when the spectrum is kept for later
+    // zones, else with the inverse passes below (cols_convolve_channels)
+    ColRows zero;
+    zero.zero_lo = h;
+    zero.zero_hi = g.ph;
+    const bool fuse_columns = !image_ready && !keep_image;
     if (!image_ready) {
         DriverRowsHalf op{src, select_params(p), ispec, w, h, g.hw, p.extraction_mode, redist_scale};
         rows_pass(s, tw, g.pw, (h + 1) / 2, op, false);
-        ColRows zero;
-        zero.zero_lo = h;
-        zero.zero_hi = g.ph;
-        cols_forward(s, tw, ispec, g.hw, g.ph, PlainLoad{}, zero);
+        if (!fuse_columns) cols_forward(s, tw, ispec, g.hw, g.ph, PlainLoad{}, zero);
     }
     // 3. half spectra of the combined kernels (cached with the lens, or built for this frame)
     const KernelSpectra* ks = cached;
@@ -1094,10 +1099,18 @@ void effect_precombined(hipStream_t s, const FftTables& t, FrameWork& fw, const 
     fw.out_spec.reset(sizeof(float2) * hc * nk);
     float2* outs = fw.out_spec.as<float2>();
     // only rows < h reach the crop
-    if (ks->fp16)
+    if (fuse_columns) {
+        if (ks->fp16)
+            cols_convolve_channels(s, tw, ispec, zero, outs, hc, nk, g.hw, g.ph,
+                                   MulSpectrumHalfLoad{ks->data.as<__half2>(), hc}, h);
+        else
+            cols_convolve_channels(s, tw, ispec, zero, outs, hc, nk, g.hw, g.ph,
+                                   MulSpectrumLoad{ks->data.as<float2>(), hc}, h);
+    } else if (ks->fp16) {
         cols_inverse_channels(s, tw, ispec, outs, hc, nk, g.hw, g.ph, MulSpectrumHalfLoad{ks->data.as<__half2>(), hc}, h);
-    else
+    } else {
         cols_inverse_channels(s, tw, ispec, outs, hc, nk, g.hw, g.ph, MulSpectrumLoad{ks->data.as<float2>(), hc}, h);
+    }
     fw.planes.reset(sizeof(float) * npx * 6);
     float* planes = fw.planes.as<float>();
     const float inv_cells = 1.0f / float(g.cells());
@@ -1150,7 +1163,7 @@ bool effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
 
     if (mode == EffectMode::Precombined) {
         effect_precombined(s, t, fw, p, bank, src, w, h, g, cfg, with_style, force, redist_scale, cached, effect9,
-                           image_ready, fin);
+                           image_ready, shared_grid != nullptr, fin);
         return fin != nullptr;
     }
 
