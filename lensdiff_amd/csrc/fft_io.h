@@ -442,13 +442,17 @@ void cols_convolve_channels(hipStream_t s, const float2* tw, float2* drv, ColRow
         first.store_rows = 1 << 30;
         col_pass<1>(s, tw, drv, drv, width, n1, n2, 1, n2, height, PlainLoad{}, first, false);
         const double cells = double(width) * double(height);
-        ProfileScope ps("fft-col-pass-fwd-inv", 8.0 * cells + (Load::extra_bytes + 8.0) * cells * nchan);
-        if (launch_fwd_mul_inv_if<8>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<12>(s, n2, gm, op, tw, ntiles) ||
-            launch_fwd_mul_inv_if<16>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<24>(s, n2, gm, op, tw, ntiles) ||
-            launch_fwd_mul_inv_if<32>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<36>(s, n2, gm, op, tw, ntiles) ||
-            launch_fwd_mul_inv_if<48>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<64>(s, n2, gm, op, tw, ntiles) ||
-            launch_fwd_mul_inv_if<72>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<96>(s, n2, gm, op, tw, ntiles) ||
-            launch_fwd_mul_inv_if<128>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<144>(s, n2, gm, op, tw, ntiles)) {
+        bool fused;
+        {
+            ProfileScope ps("fft-col-pass-fwd-inv", 8.0 * cells + (Load::extra_bytes + 8.0) * cells * nchan);
+            fused = launch_fwd_mul_inv_if<8>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<12>(s, n2, gm, op, tw, ntiles) ||
+                    launch_fwd_mul_inv_if<16>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<24>(s, n2, gm, op, tw, ntiles) ||
+                    launch_fwd_mul_inv_if<32>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<36>(s, n2, gm, op, tw, ntiles) ||
+                    launch_fwd_mul_inv_if<48>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<64>(s, n2, gm, op, tw, ntiles) ||
+                    launch_fwd_mul_inv_if<72>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<96>(s, n2, gm, op, tw, ntiles) ||
+                    launch_fwd_mul_inv_if<128>(s, n2, gm, op, tw, ntiles) || launch_fwd_mul_inv_if<144>(s, n2, gm, op, tw, ntiles);
+        }
+        if (fused) {
             LDH_CHECK_LAUNCH("fft-col-fwd-inv");
             ColRows last;
             last.store_rows = store_rows;

@@ -1001,11 +1001,9 @@ void select_stage(hipStream_t s, FrameWork& fw, const ldh_params& p, const float
     // 1. selection mask
     fw.mask.reset(sizeof(float) * npx);
     const SelectParams sp = select_params(p);
-{
     ProfileScope ps_("frame-select", 20.0 * double(npx));
-        hipLaunchKernelGGL(selection_kernel, grid1(npx), dim3(256), 0, s, src, npx, sp, fw.mask.as<float>());
+    hipLaunchKernelGGL(selection_kernel, grid1(npx), dim3(256), 0, s, src, npx, sp, fw.mask.as<float>());
     LDH_CHECK_LAUNCH("selection");
-    }
 }
 
 // The final composite folded into the effect's last pass: the output frame and the composite's
@@ -1456,38 +1454,39 @@ void finish_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
         float* d_eff = fw.display.as<float>();
         float* d_red = d_eff + n * 3;
         float* d_mask = d_eff + n * 6;
-        ProfileScope ps_("frame-display-resample", 4.0 * 7.0 * double(n));
-        if (p.debug_view == LDH_VIEW_SELECTION) {
-            hipLaunchKernelGGL(resample_planes_kernel, grid1(n), dim3(256), 0, s, fw.mask.as<float>(), w, h, 1, d_mask,
-                               disp->w, disp->h);
-            fw.max_buf.reset(sizeof(unsigned));
-            LDH_HIP(hipMemsetAsync(fw.max_buf.get(), 0, sizeof(unsigned), s), "mask-max");
-            hipLaunchKernelGGL(max_kernel, dim3(256), dim3(256), 0, s, d_mask, n, fw.max_buf.as<unsigned>());
-            mask_max = fw.max_buf.as<unsigned>();
-        } else {
-            hipLaunchKernelGGL(resample_planes_kernel, grid1(n), dim3(256), 0, s, eff_out, w, h, 3, d_eff, disp->w,
-                               disp->h);
+        {
+            ProfileScope ps_("frame-display-resample", 4.0 * 7.0 * double(n));
+            if (p.debug_view == LDH_VIEW_SELECTION) {
+                hipLaunchKernelGGL(resample_planes_kernel, grid1(n), dim3(256), 0, s, fw.mask.as<float>(), w, h, 1,
+                                   d_mask, disp->w, disp->h);
+                fw.max_buf.reset(sizeof(unsigned));
+                LDH_HIP(hipMemsetAsync(fw.max_buf.get(), 0, sizeof(unsigned), s), "mask-max");
+                hipLaunchKernelGGL(max_kernel, dim3(256), dim3(256), 0, s, d_mask, n, fw.max_buf.as<unsigned>());
+                mask_max = fw.max_buf.as<unsigned>();
+            } else {
+                hipLaunchKernelGGL(resample_planes_kernel, grid1(n), dim3(256), 0, s, eff_out, w, h, 3, d_eff, disp->w,
+                                   disp->h);
+            }
+            if (p.debug_view == LDH_VIEW_FINAL) {
+                fw.redist.reset(sizeof(float) * npx * 3);
+                hipLaunchKernelGGL(redist_planes_kernel, grid1(npx), dim3(256), 0, s, src, fw.mask.as<float>(),
+                                   redist_scale, npx, fw.redist.as<float>());
+                hipLaunchKernelGGL(resample_planes_kernel, grid1(n), dim3(256), 0, s, fw.redist.as<float>(), w, h, 3,
+                                   d_red, disp->w, disp->h);
+            }
+            LDH_CHECK_LAUNCH("display-resample");
         }
-        if (p.debug_view == LDH_VIEW_FINAL) {
-            fw.redist.reset(sizeof(float) * npx * 3);
-            hipLaunchKernelGGL(redist_planes_kernel, grid1(npx), dim3(256), 0, s, src, fw.mask.as<float>(), redist_scale,
-                               npx, fw.redist.as<float>());
-            hipLaunchKernelGGL(resample_planes_kernel, grid1(n), dim3(256), 0, s, fw.redist.as<float>(), w, h, 3, d_red,
-                               disp->w, disp->h);
-        }
-        LDH_CHECK_LAUNCH("display-resample");
         cp.tf = disp->tf;
+        ProfileScope ps_("frame-composite", 56.0 * double(n));
         hipLaunchKernelGGL(composite_kernel, grid1(n), dim3(256), 0, s, disp->src, d_mask, d_eff,
                            static_cast<const float*>(nullptr), mask_max, n, cp, dst, d_red);
         LDH_CHECK_LAUNCH("composite");
         return;
     }
-{
     ProfileScope ps_("frame-composite", 56.0 * double(npx));
-        hipLaunchKernelGGL(composite_kernel, grid1(npx), dim3(256), 0, s, src, fw.mask.as<float>(), eff_out, factor,
+    hipLaunchKernelGGL(composite_kernel, grid1(npx), dim3(256), 0, s, src, fw.mask.as<float>(), eff_out, factor,
                        mask_max, npx, cp, dst, static_cast<const float*>(nullptr));
     LDH_CHECK_LAUNCH("composite");
-    }
 }
 
 }  // namespace
