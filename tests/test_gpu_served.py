@@ -115,6 +115,37 @@ def test_bank_kernel_sets_follow_the_look(ctx):
     assert st.bank_misses == 2 and st.bank_hits == 1
 
 
+@pytest.mark.parametrize("w,h,style", [(400, 200, False), (1920, 1080, False), (1920, 1080, True), (97, 63, False)],
+                         ids=["grid256-fused", "grid1152", "grid1152-styled", "grid128-one-pass"])
+def test_served_column_passes_per_grid_height(ctx, w, h, style):
+    """A served pre-combined frame's column work takes one of three forms by grid height and its
+    tuned split: the driver's last forward pass fused with every channel's first inverse pass
+    (256 rows: 16 x 16), the passes apart after F1 where the split's second factor has no fused
+    tile (1152 rows when the tuner keeps 32 x 36; it may take 48 x 24, fused), and one pass each
+    way (128 rows). Each against the plain per-bin path."""
+    p = lens(11, n=8, style=style)
+    img = workloads.frame(w, h, 0.002, seed=60)
+    plain = ctx.render_image(p, img)
+    s = ctx.alloc(img.nbytes)
+    d = ctx.alloc(img.nbytes)
+    ctx.upload(s, img)
+    ctx.server_start(1, 4 << 30)
+    try:
+        ctx.wait(ctx.submit(p, s, d, w, h))
+        out = np.empty_like(img)
+        ctx.download(d, out)
+    finally:
+        ctx.server_stop()
+        ctx.free(s)
+        ctx.free(d)
+    # the split the tuner kept (it keeps the balanced one unless another is more than 5% faster)
+    for key, rec in ctx.tuning_report().items():
+        if key.startswith("fft-cols/"):
+            cands = key[key.index("[") + 1:key.index("]")].split(",")
+            print("%s: %s" % (key.split("[")[0], cands[rec["variant"]]))
+    assert rel_l2(out, plain) <= 1e-5
+
+
 def test_served_rejects_unsupported_and_reports(ctx):
     p = lens(4)
     p.aperture_mode = _abi.APERTURE_CUSTOM
