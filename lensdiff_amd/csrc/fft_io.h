@@ -267,6 +267,22 @@ void cols_inverse(hipStream_t s, const float2* tw, const float2* src, float2* da
     cols_inverse_split(s, tw, src, data, width, height, op, store_rows, col_plan(s, tw, width, height));
 }
 
+// The forward column transforms of nchan channels of one shape in place (channel c at
+// data + c * chan) in the same launches, their rows [first.zero_lo, first.zero_hi) read as zero.
+inline void cols_forward_channels(hipStream_t s, const float2* tw, float2* data, size_t chan, int nchan, int width,
+                                  int height, ColRows first) {
+    check_cols(width, height);
+    const ColSplit sp = col_plan(s, tw, width, height);
+    first.store_rows = 1 << 30;
+    const int n1 = sp.n1, n2 = sp.n2;
+    if (n1 == 0) {
+        col_pass<0>(s, tw, data, data, width, height, 1, 0, 1, height, PlainLoad{}, first, false, nchan, chan, chan);
+        return;
+    }
+    col_pass<1>(s, tw, data, data, width, n1, n2, 1, n2, height, PlainLoad{}, first, false, nchan, chan, chan);
+    col_pass<0>(s, tw, data, data, width, n2, n1, n2, 1, height, PlainLoad{}, ColRows{}, false, nchan, chan, chan);
+}
+
 // The inverse column transforms of nchan channels in the same launches, all from one source:
 // channel c is the transform of src through op.at(c), stored at data + c * dst_chan (only rows
 // < store_rows of the last pass). Neighbouring tiles carry a column group's channels, so the

@@ -850,6 +850,10 @@ void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<K
         const size_t hc = g.half_cells();
         out.data.reset((fp16 ? sizeof(__half2) : sizeof(float2)) * hc * n);
         if (fp16) scratch.reset(sizeof(float2) * hc * 2);
+        // only the rows [0, r] and [ph - r, ph) of the wrapped kernels are non-zero
+        ColRows zero;
+        zero.zero_lo = radius + 1;
+        zero.zero_hi = g.ph - radius;
         for (int p = 0; p < pairs; ++p) {
             const bool two = 2 * p + 1 < n;
             float2* da = fp16 ? scratch.as<float2>() : out.data.as<float2>() + hc * (2 * p);
@@ -863,19 +867,16 @@ void build_kernel_spectra(hipStream_t s, const FftTables& t, const std::vector<K
             op.hw = g.hw;
             const int band = 2 * radius + 1;
             rows_pass(s, tw, g.pw, band, op, false, radius + 1, g.ph - band);
-            ColRows zero;
-            zero.zero_lo = radius + 1;
-            zero.zero_hi = g.ph - radius;
+            if (!fp16) continue;  // FP32 spectra: every kernel's columns below, in the same launches
             for (int j = 0; j < (two ? 2 : 1); ++j) {
                 float2* d = j == 0 ? da : db;
                 cols_forward(s, tw, d, g.hw, g.ph, PlainLoad{}, zero);
-                if (fp16) {
-                    hipLaunchKernelGGL(to_half_kernel, grid1(int64_t(hc)), dim3(256), 0, s, d,
-                                       out.data.as<__half2>() + hc * (2 * p + j), int64_t(hc));
-                    LDH_CHECK_LAUNCH("spectra-to-half");
-                }
+                hipLaunchKernelGGL(to_half_kernel, grid1(int64_t(hc)), dim3(256), 0, s, d,
+                                   out.data.as<__half2>() + hc * (2 * p + j), int64_t(hc));
+                LDH_CHECK_LAUNCH("spectra-to-half");
             }
         }
+        if (!fp16) cols_forward_channels(s, tw, out.data.as<float2>(), hc, n, g.hw, g.ph, zero);
         return;
     }
     out.data.reset((fp16 ? sizeof(__half2) : sizeof(float2)) * cells * pairs);
