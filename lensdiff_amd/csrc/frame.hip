@@ -1182,7 +1182,7 @@ bool effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
     }
 
     // 3-4. per kernel set: spectra x image spectra -> planes -> 3 effect planes
-    fw.out_spec.reset(sizeof(float2) * cells);
+    fw.out_spec.reset(sizeof(float2) * cells * nimg);
     const int nplanes = pl.spectral ? n : 3;
     fw.planes.reset(sizeof(float) * npx * std::max(nplanes, 3));
     fw.matrices.reset(sizeof(float) * 6 * n);
@@ -1210,28 +1210,36 @@ bool effect_stage(hipStream_t s, const FftTables& t, FrameWork& fw, const ldh_pa
         if (use_cache && (cached->grid.pw != g.pw || cached->grid.ph != g.ph || cached->pairs != pairs))
             fail(LDH_ERR_INVALID_ARGUMENT, "cached-spectra-mismatch");
         for (int pi = 0; pi < pairs; ++pi) {
-            // kernel spectrum of this pair
-            const void* kspec;
-            bool half = false;
+            // kernel spectrum of this pair: cached, or its row pass here and its columns fused
+            // with the products below (cols_convolve_channels: the kernel's transform never
+            // stored); every product with an image spectrum inverted (only rows < h reach the crop)
             if (use_cache) {
-                half = cached->fp16;
-                kspec = half ? static_cast<const void*>(cached->data.as<__half2>() + cells * pi)
-                              : static_cast<const void*>(cached->data.as<float2>() + cells * pi);
+                const bool half = cached->fp16;
+                for (int j = 0; j < nimg; ++j) {
+                    const float2* ispec = fw.image_spec.as<float2>() + cells * j;
+                    float2* o = fw.out_spec.as<float2>() + cells * j;
+                    if (half)
+                        cols_inverse(s, tw, ispec, o, g.pw, g.ph, MulSpectrumHalfLoad{cached->data.as<__half2>() + cells * pi},
+                                     h);
+                    else
+                        cols_inverse(s, tw, ispec, o, g.pw, g.ph, MulSpectrumLoad{cached->data.as<float2>() + cells * pi}, h);
+                }
             } else {
-                std::vector<KernelRef> pair{ks[size_t(pl.spectral ? 2 * pi : 0)]};
-                if (pl.spectral && size_t(2 * pi + 1) < ks.size()) pair.push_back(ks[size_t(2 * pi + 1)]);
-                build_kernel_spectra(s, t, pair, bank.kernel, bank.support, g, false, fw.kspec, fw.kspec_tmp);
-                kspec = fw.kspec.data.get();
+                const KernelRef ka = ks[size_t(pl.spectral ? 2 * pi : 0)];
+                const KernelRef kb = pl.spectral && size_t(2 * pi + 1) < ks.size() ? ks[size_t(2 * pi + 1)] : KernelRef{};
+                fw.kspec_tmp.reset(sizeof(float2) * cells);
+                float2* krow = fw.kspec_tmp.as<float2>();
+                const int r = bank.support;
+                KernelRows op{ka.data, kb.data, ka.scale, kb.scale, krow, bank.kernel, bank.kernel / 2, r, g.pw, g.ph};
+                rows_pass(s, tw, g.pw, 2 * r + 1, op, false, r + 1, g.ph - (2 * r + 1));
+                ColRows kz;  // only the rows [0, r] and [ph - r, ph) of the wrapped kernels are non-zero
+                kz.zero_lo = r + 1;
+                kz.zero_hi = g.ph - r;
+                cols_convolve_channels(s, tw, krow, kz, fw.out_spec.as<float2>(), cells, nimg, g.pw, g.ph,
+                                       MulSpectrumLoad{fw.image_spec.as<float2>(), cells}, h);
             }
-            // convolutions against the image spectra
             for (int j = 0; j < nimg; ++j) {
-                const float2* ispec = fw.image_spec.as<float2>() + cells * j;
-                float2* o = fw.out_spec.as<float2>();
-                // only rows < h reach the crop
-                if (half)
-                    cols_inverse(s, tw, ispec, o, g.pw, g.ph, MulSpectrumHalfLoad{static_cast<const __half2*>(kspec)}, h);
-                else
-                    cols_inverse(s, tw, ispec, o, g.pw, g.ph, MulSpectrumLoad{static_cast<const float2*>(kspec)}, h);
+                float2* o = fw.out_spec.as<float2>() + cells * j;
                 float* oa;
                 float* ob;
                 if (!pl.spectral) {
