@@ -204,6 +204,9 @@ def main():
     ap.add_argument("--batch-probe", action="store_true", help="only run the batched form and exit")
     ap.add_argument("--field-probe", action="store_true",
                     help="only time a UHD frame with field-dependent phase (nine zone banks) and exit")
+    ap.add_argument("--plain-probe", action="store_true",
+                    help="only render plain UHD frames with the bank cached (the launch shapes of the plain "
+                         "pass, for counter runs) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -235,6 +238,29 @@ def main():
         fams = families(ctx.profile_read())
         print(json.dumps({"bank_build_ms": ms, "kernel_sides": ctx.bank_kernel_sizes(lens(5002)),
                           "autotune": ctx.tuning_report(), "kernel_families": fams}))
+        ctx.close()
+        return
+    if args.plain_probe:
+        ctx = Context(0)
+        f = workloads.c2_frame()
+        h, w = f.shape[:2]
+        d_src, d_dst = ctx.alloc(f.nbytes), ctx.alloc(f.nbytes)
+        ctx.upload(d_src, f)
+        p = lens(RESIDENT[0])
+        ctx.render_device(p, d_src, d_dst, w, h)  # the bank built and cached
+        ctx.synchronize()
+        ctx.profile_reset()
+        ctx.profiling(True)
+        ms = []
+        for _ in range(4):
+            t0 = time.perf_counter()
+            ctx.render_device(p, d_src, d_dst, w, h)
+            ctx.synchronize()
+            ms.append(round((time.perf_counter() - t0) * 1e3, 2))
+        ctx.profiling(False)
+        stats = ctx.profile_read()
+        print(json.dumps({"plain_frame_cached_bank_ms": ms, "roofline": dominant(stats),
+                          "kernel_families": families(stats)}))
         ctx.close()
         return
     if args.field_probe:

@@ -330,7 +330,8 @@ constexpr bool fwd_mul_inv_ok() {
     return (S::ELEMS / 2) % S::NT == 0 && S::NT % NP == 0;
 }
 
-template <int N, class Load>
+// ONE: a single channel (gm.nchan == 1), so the forward result is dead after its product.
+template <int N, bool ONE, class Load>
 __global__ void __launch_bounds__((TileShape<N, false>::NT))
 fft_fwd_mul_inv_kernel(FwdMulInvGeom gm, Load op, const float2* __restrict__ tw, int ntiles) {
     using S = TileShape<N, false>;
@@ -397,7 +398,8 @@ fft_fwd_mul_inv_kernel(FwdMulInvGeom gm, Load op, const float2* __restrict__ tw,
         const float2 a = lds[lds_pad(at)], b = lds[lds_pad(at + 1)];
         x[c] = make_float4(a.x, a.y, b.x, b.y);
     }
-    for (int ch = 0; ch < gm.nchan; ++ch) {
+    const int nchan = ONE ? 1 : gm.nchan;
+    for (int ch = 0; ch < nchan; ++ch) {
         const Load opc = op.at(ch);
         float4 y[IP];
 #pragma unroll
@@ -431,9 +433,15 @@ bool launch_fwd_mul_inv_if(hipStream_t s, int n, const FwdMulInvGeom& gm, const 
     } else {
         if (n != N) return false;
         using S = TileShape<N, false>;
-        allow_large_lds(reinterpret_cast<const void*>(&fft_fwd_mul_inv_kernel<N, Load>), S::LDS_BYTES);
-        hipLaunchKernelGGL((fft_fwd_mul_inv_kernel<N, Load>), dim3(ntiles), dim3(S::NT), S::LDS_BYTES, s, gm, op, tw,
-                           ntiles);
+        if (gm.nchan == 1) {
+            allow_large_lds(reinterpret_cast<const void*>(&fft_fwd_mul_inv_kernel<N, true, Load>), S::LDS_BYTES);
+            hipLaunchKernelGGL((fft_fwd_mul_inv_kernel<N, true, Load>), dim3(ntiles), dim3(S::NT), S::LDS_BYTES, s, gm,
+                               op, tw, ntiles);
+        } else {
+            allow_large_lds(reinterpret_cast<const void*>(&fft_fwd_mul_inv_kernel<N, false, Load>), S::LDS_BYTES);
+            hipLaunchKernelGGL((fft_fwd_mul_inv_kernel<N, false, Load>), dim3(ntiles), dim3(S::NT), S::LDS_BYTES, s, gm,
+                               op, tw, ntiles);
+        }
         return true;
     }
 }
