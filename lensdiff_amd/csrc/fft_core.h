@@ -281,6 +281,12 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
 // An IO may store a tile's finished lines itself, reading any positions of the transform from
 // LDS (IO::kStoreLds and IO::store_lds<N, NL, NT>(tile, tid, lds); element i of line l is at
 // lds[lds_pad(l * N + i)] for row tiles).
+// Loads a thread issues before its first finish (IO::kFetchBatch, default 16).
+template <class T, class = void>
+struct io_fetch_batch : std::integral_constant<int, 16> {};
+template <class T>
+struct io_fetch_batch<T, std::void_t<decltype(T::kFetchBatch)>> : std::integral_constant<int, T::kFetchBatch> {};
+
 template <class T, class = void>
 struct io_stores_lds : std::false_type {};
 template <class T>
@@ -327,7 +333,7 @@ fft_tile_kernel(IO io, const float2* __restrict__ tw, int ntiles) {
         // loads in batches of up to kBatch per thread: every fetch of a batch (memory only) is issued
         // before the first finish (the arithmetic, which may branch on launch-uniform settings) and
         // the LDS writes
-        constexpr int kBatch = IT < 16 ? IT : 16;
+        constexpr int kBatch = IT < io_fetch_batch<IO>::value ? IT : io_fetch_batch<IO>::value;
 #pragma unroll
         for (int c0 = 0; c0 < IT; c0 += kBatch) {
             typename IO::Raw t[kBatch];

@@ -524,6 +524,7 @@ struct RowIO {
     }
     static constexpr bool kStoreTwiddle = false;
     static constexpr bool kPairs = false;
+    static constexpr int kFetchBatch = io_fetch_batch<RowOp>::value;
     __device__ float4 fetch_pair(int, int, int) const { return make_float4(0.0f, 0.0f, 0.0f, 0.0f); }
     __device__ void store_pair(int, int, int, float2, float2, float2) const {}
     __device__ float2 twiddle(int, int, int) const { return make_float2(1.0f, 0.0f); }

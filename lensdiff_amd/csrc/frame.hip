@@ -161,6 +161,7 @@ struct DriverRowsHalf {
     int w, h, hw, extraction;
     float scale;
     static constexpr bool kStoreLds = true;
+    static constexpr int kFetchBatch = 6;  // two pixels a load: fewer in flight, more waves
     struct Raw {
         float4 qa, qb;
         bool ina, inb;
