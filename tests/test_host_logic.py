@@ -85,3 +85,18 @@ def test_reference_render_error_tags(reference):
     p = _abi.default_params()
     ok, err = reference.render(p, _abi.RenderRequest())
     assert not ok and err == "missing-source-or-destination-buffer"
+
+
+@pytest.mark.parametrize("n", [2, 8])
+def test_bench_refuses_more_than_one_gpu(n):
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says so, before any
+    device work (nothing is printed on stdout, so no result line can be mistaken for one)."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", str(n)],
+                       capture_output=True, text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert r.stdout == ""
+    assert "one GPU" in r.stderr
